@@ -1,0 +1,210 @@
+"""LenseFlow throughput on one MI355X.
+
+Workload: forward LenseFlow of Q/U map pairs, N = 256, 2 arcmin pixels, float32,
+7 RK4 steps (the 256^2 QU configuration).  One "step" is one batched pass over
+`--batch` requests; request r uses phi number r % 32 of 32 seeded lensing
+potentials whose operators are prepared once and stay resident in HBM (the
+serving form).  The plain path is one request per call with its phi prepared
+in the same call, as the reference's lense_flow does.
+
+Prints one JSON line (rank 0).  Extra keys: plain maps/s and the batched /
+plain ratio, p50/p99 latency of plain single requests, relative L2 error of a
+batched result against the FP64 oracle, the live roofline of the dominant
+kernel and the CPU baseline (the FP64 oracle on this host's cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def model_bytes_per_map(N, C, n, s):
+    """Algorithmic bytes of one LenseFlow of one request under the canonical
+    model: 4n [C (12H + 5P) + 5P] s."""
+    P, H = N * N, N * (N // 2 + 1)
+    return 4 * n * (C * (12 * H + 5 * P) + 5 * P) * s
+
+
+def row_bytes_per_launch(B, N, C, s):
+    """Algorithmic bytes of one fused row-pass launch over B requests: per
+    request 2C packed half planes read (N^2 s each), C written, the 2 maps of
+    p(t), and on average 2.5 map transfers of y/acc per component."""
+    return B * (5.5 * C + 2) * N * N * s
+
+
+def col_bytes_per_launch(B, N, C, s):
+    """Fused column pass: per component one packed half plane read, two written."""
+    return B * C * 3 * N * N * s
+
+
+def cpu_baseline(f, phi, th, n, budget_s=10.0):
+    from oracle import lenseflow as olf
+    pw = olf.pix_width_from_theta(th)
+    t0 = time.perf_counter()
+    k = 0
+    while True:
+        olf.lense_flow(f, phi, pw, n)
+        k += 1
+        el = time.perf_counter() - t0
+        if el > budget_s:
+            break
+    return k / el, k, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--plain-requests", type=int, default=40)
+    ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the LenseFlow path shards by independent "
+              "requests with no communication, and only --gpus 1 is supported here.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import __graft_entry__ as ge
+    pkg = ge.load_package()
+    from cmblensing_amd import _lib
+    sims = ge._load_module("cmbl_sims_b", ROOT / "cmblensing.py_amd" / "sims.py")
+
+    dev = torch.device("cuda:0")
+    N, th, n, C, B = 256, 2.0, 7, 2, args.batch
+    NPHI = 32
+    dtype = torch.float32
+    print(f"[bench] workload QU N={N} theta={th} n={n} fp32 batch={B}", file=sys.stderr, flush=True)
+
+    # synthetic inputs: 32 phi, B distinct Q/U requests (cycled from 8 draws)
+    phis = [sims.phi_map(N, th, 1000 + i) for i in range(NPHI)]
+    fq = [sims.qu_map(N, th, i) for i in range(8)]
+    x = torch.as_tensor(np.stack([fq[r % 8] for r in range(B)]), dtype=dtype, device=dev)
+    phi_t = [torch.as_tensor(p, device=dev) for p in phis]
+
+    plan = _lib.get_plan(N, th, dtype, B * C, dev)
+    handles = [_lib.get_phi(plan, p, n) for p in phi_t]
+    req_phi = [handles[r % NPHI] for r in range(B)]
+    out = torch.empty_like(x)
+
+    def step():
+        _lib.lenseflow(plan, req_phi, x, out, B, C, n, 1, False)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_per_step = dt / args.steps * 1e3
+    maps_s = B * args.steps / dt
+
+    # error of one batched result against the FP64 oracle
+    from oracle import lenseflow as olf
+    r_chk = 5
+    ref = olf.lense_flow(fq[r_chk % 8], phis[r_chk % NPHI], olf.pix_width_from_theta(th), n)
+    got = out[r_chk].double().cpu().numpy()
+    err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+
+    # live per-kernel timing (HIP events on the launch stream) over a few steps
+    plan.set_timing(True)
+    plan.get_timing(reset=True)
+    ksteps = max(2, min(args.steps, 5))
+    for _ in range(ksteps):
+        step()
+    tim = plan.get_timing(reset=True)
+    plan.set_timing(False)
+    row_avg_ms = tim["row_ms"] / tim["row_n"]
+    col_avg_ms = tim["col_ms"] / tim["col_n"]
+    row_bytes = row_bytes_per_launch(B, N, C, 4)
+    col_bytes = col_bytes_per_launch(B, N, C, 4)
+    row_gbs = row_bytes / (row_avg_ms * 1e-3) / 1e9
+    col_gbs = col_bytes / (col_avg_ms * 1e-3) / 1e9
+
+    # plain single-request path: phi prepared per request, batch of one
+    lat = []
+    xs = x[:1].contiguous()
+    o1 = torch.empty_like(xs)
+    for r in range(args.plain_requests + 3):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        h = _lib.PhiHandle(plan, phi_t[r % NPHI], n)
+        _lib.lenseflow(plan, [h], xs, o1, 1, C, n, 1, False)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t1)
+        h.close()
+    lat = np.array(lat[3:]) * 1e3
+    plain_maps_s = 1e3 / lat.mean()
+    # plain with a resident phi (latency of the stage pipeline alone)
+    lat_res = []
+    for r in range(args.plain_requests):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        _lib.lenseflow(plan, [handles[0]], xs, o1, 1, C, n, 1, False)
+        torch.cuda.synchronize()
+        lat_res.append(time.perf_counter() - t1)
+    lat_res = np.array(lat_res) * 1e3
+
+    model_b = model_bytes_per_map(N, C, n, 4)
+    result = {
+        "metric": "LenseFlow maps/s @256^2 QU batched vs plain; p50/p99 latency; err vs FP64",
+        "value": round(maps_s, 2),
+        "unit": "maps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Gaussian random fields, 32 resident phi)",
+        "config": {"workload": "lenseflow_fwd_QU_256_2arcmin_n7_fp32", "N": N, "theta_pix": th,
+                   "nsteps": n, "components": "QU", "batch": B, "phi_resident": NPHI},
+        "plain_maps_s": round(plain_maps_s, 2),
+        "batched_over_plain": round(maps_s / plain_maps_s, 2),
+        "plain_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
+                             "p99": round(float(np.percentile(lat, 99)), 4)},
+        "plain_resident_phi_latency_ms": {"p50": round(float(np.percentile(lat_res, 50)), 4),
+                                          "p99": round(float(np.percentile(lat_res, 99)), 4)},
+        "rel_l2_vs_fp64_oracle": err,
+        "model_gbs": round(model_b * maps_s / 1e9, 1),
+        "model_frac_hbm": round(model_b * maps_s / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": {"bound": "hbm", "kernel": "k_row_lf_std<float,256>",
+                     "achieved": round(row_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(row_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "avg_launch_us": round(row_avg_ms * 1e3, 3),
+                     "algorithmic_bytes_per_launch": row_bytes},
+        "column_pass": {"kernel": "k_col_lf_grad<float,256,16>",
+                        "achieved_gbs": round(col_gbs, 1),
+                        "avg_launch_us": round(col_avg_ms * 1e3, 3),
+                        "algorithmic_bytes_per_launch": col_bytes},
+    }
+    if not args.no_cpu:
+        fcpu = fq[0]
+        cps, k, el = cpu_baseline(fcpu, phis[0], th, n, args.cpu_budget)
+        result["cpu_baseline"] = {"value": round(cps, 4), "unit": "maps/s", "cores": 1,
+                                  "kind": "port",
+                                  "sample": f"{k} forward LenseFlows of one 256^2 QU pair, float64 "
+                                            f"numpy oracle, {el:.1f} s"}
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,249 @@
+"""ctypes binding of libcmbl.so (the C ABI in include/cmbl.h).
+
+The library is the only compute path: if it is missing, or no HIP device is
+visible, every entry point raises -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+from collections import OrderedDict
+from pathlib import Path
+
+import torch
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libcmbl.so"
+
+CMBL_F32, CMBL_F64 = 0, 1
+CMBL_PHI_MAP, CMBL_PHI_FOURIER = 0, 1
+
+_lib = None
+_lock = threading.Lock()
+
+c_int, c_double, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+_SIGS = {
+    "cmbl_version": (ctypes.c_char_p, []),
+    "cmbl_last_error": (ctypes.c_char_p, []),
+    "cmbl_plan_create": (c_int, [ctypes.POINTER(c_void_p), c_int, c_double, c_int, c_int]),
+    "cmbl_plan_destroy": (c_int, [c_void_p]),
+    "cmbl_phi_prepare": (c_int, [c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(c_void_p),
+                                 ctypes.POINTER(c_double), c_void_p]),
+    "cmbl_phi_release": (c_int, [c_void_p]),
+    "cmbl_phi_derivatives": (c_void_p, [c_void_p]),
+    "cmbl_lenseflow": (c_int, [c_void_p, ctypes.POINTER(c_void_p), c_void_p, c_void_p, c_int,
+                               c_int, c_int, c_int, c_int, c_void_p]),
+    "cmbl_rfft2": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "cmbl_irfft2": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "cmbl_plan_set_timing": (c_int, [c_void_p, c_int]),
+    "cmbl_plan_get_timing": (c_int, [c_void_p, ctypes.POINTER(c_double), c_int]),
+}
+
+
+class CmblError(RuntimeError):
+    pass
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def load(require_device: bool = True):
+    """Load libcmbl.so once.  Raises if the library is not built or, with
+    require_device, if no HIP device is visible."""
+    global _lib
+    if require_device and not torch.cuda.is_available():
+        raise CmblError("no HIP device visible: the LenseFlow path runs only on the GPU")
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise CmblError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+            lib = ctypes.CDLL(str(LIB_PATH))
+            for name, (res, args) in _SIGS.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+    return _lib
+
+
+def check(status: int, what: str):
+    if status != 0:
+        msg = load(False).cmbl_last_error().decode()
+        raise CmblError(f"{what} failed (status {status}): {msg}")
+
+
+def stream_ptr(device=None) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def dtype_code(dtype: torch.dtype) -> int:
+    if dtype == torch.float32:
+        return CMBL_F32
+    if dtype == torch.float64:
+        return CMBL_F64
+    raise CmblError(f"unsupported dtype {dtype} (float32 or float64)")
+
+
+class Plan:
+    """Owns a cmbl_plan: geometry, dtype and stage scratch for max_maps maps."""
+
+    def __init__(self, N: int, theta_pix: float, dtype: torch.dtype, max_maps: int, device):
+        self.lib = load()
+        self.N, self.theta_pix, self.dtype, self.max_maps = N, float(theta_pix), dtype, max_maps
+        self.device = torch.device(device)
+        h = c_void_p()
+        with torch.cuda.device(self.device):
+            check(self.lib.cmbl_plan_create(ctypes.byref(h), N, self.theta_pix, dtype_code(dtype),
+                                            max_maps), "cmbl_plan_create")
+        self.handle = h
+
+    def set_timing(self, enable: bool):
+        check(self.lib.cmbl_plan_set_timing(self.handle, int(bool(enable))), "set_timing")
+
+    def get_timing(self, reset: bool = True) -> dict:
+        out = (c_double * 6)()
+        check(self.lib.cmbl_plan_get_timing(self.handle, out, int(bool(reset))), "get_timing")
+        return {"col_ms": out[0], "col_n": int(out[1]), "row_ms": out[2], "row_n": int(out[3]),
+                "other_ms": out[4], "other_n": int(out[5])}
+
+    def close(self):
+        if self.handle:
+            torch.cuda.synchronize(self.device)
+            self.lib.cmbl_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PhiHandle:
+    """A prepared phi: float64 derivatives and the p(t) table for nsteps."""
+
+    def __init__(self, plan: Plan, phi: torch.Tensor, nsteps: int):
+        self.lib = plan.lib
+        self.plan = plan
+        self.nsteps = nsteps
+        N = plan.N
+        phi = phi.to(device=plan.device, dtype=torch.complex128 if phi.is_complex() else torch.float64)
+        phi = phi.contiguous()
+        if phi.shape == (N, N):
+            basis = CMBL_PHI_MAP
+        elif phi.shape == (N, N // 2 + 1):
+            basis = CMBL_PHI_FOURIER
+        else:
+            raise CmblError(f"phi shape {tuple(phi.shape)} does not match N={N}")
+        h = c_void_p()
+        md = c_double()
+        with torch.cuda.device(plan.device):
+            check(self.lib.cmbl_phi_prepare(plan.handle, phi.data_ptr(), basis, nsteps,
+                                            ctypes.byref(h), ctypes.byref(md),
+                                            stream_ptr(plan.device)), "cmbl_phi_prepare")
+        self.handle = h
+        self.min_det = md.value
+
+    def derivatives(self) -> torch.Tensor:
+        """The five float64 phi derivative maps as a [5, N, N] tensor (a copy)."""
+        N = self.plan.N
+        ptr = self.lib.cmbl_phi_derivatives(self.handle)
+        out = torch.empty((5, N, N), dtype=torch.float64, device=self.plan.device)
+        torch.cuda.current_stream(self.plan.device).synchronize()
+        _memcpy_d2d(out.data_ptr(), ptr, out.numel() * 8)
+        return out
+
+    def close(self):
+        if self.handle:
+            torch.cuda.synchronize(self.plan.device)
+            self.lib.cmbl_phi_release(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _memcpy_d2d(dst: int, src: int, nbytes: int):
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [c_void_p, c_void_p, ctypes.c_size_t, c_int]
+    rc = hip.hipMemcpy(c_void_p(dst), c_void_p(src), nbytes, 3)  # DeviceToDevice
+    if rc != 0:
+        raise CmblError(f"hipMemcpy failed ({rc})")
+
+
+# --------------------------------------------------------------------------
+# caches
+# --------------------------------------------------------------------------
+_plans: dict = {}
+_phis: "OrderedDict" = OrderedDict()
+PHI_CACHE_SIZE = 64
+
+
+def get_plan(N: int, theta_pix: float, dtype: torch.dtype, nmaps: int, device) -> Plan:
+    device = torch.device(device)
+    if device.index is None:
+        device = torch.device("cuda", torch.cuda.current_device())
+    key = (device.index, N, round(float(theta_pix), 12), dtype)
+    plan = _plans.get(key)
+    if plan is None or plan.max_maps < nmaps:
+        cap = max(nmaps, 2 * plan.max_maps if plan else nmaps)
+        if plan is not None:
+            plan.close()
+        plan = Plan(N, theta_pix, dtype, cap, device)
+        _plans[key] = plan
+        # phi handles prepared under the old plan object are still valid: the
+        # plan geometry and dtype are unchanged; re-key them to the new plan.
+        for k, v in list(_phis.items()):
+            if k[0] == key:
+                v.plan = plan
+    return plan
+
+
+def get_phi(plan: Plan, phi: torch.Tensor, nsteps: int) -> PhiHandle:
+    """Prepared phi for (plan geometry, tensor identity and version, nsteps)."""
+    pkey = (plan.device.index, plan.N, round(plan.theta_pix, 12), plan.dtype)
+    key = (pkey, phi.data_ptr(), phi._version, tuple(phi.shape), str(phi.dtype), nsteps)
+    h = _phis.get(key)
+    if h is not None:
+        _phis.move_to_end(key)
+        return h
+    h = PhiHandle(plan, phi, nsteps)
+    _phis[key] = h
+    while len(_phis) > PHI_CACHE_SIZE:
+        _, old = _phis.popitem(last=False)
+        old.close()
+    return h
+
+
+def clear_caches():
+    for v in _phis.values():
+        v.close()
+    _phis.clear()
+    for p in _plans.values():
+        p.close()
+    _plans.clear()
+
+
+def lenseflow(plan: Plan, phis, x: torch.Tensor, out: torch.Tensor, batch: int, ncomp: int,
+              nsteps: int, direction: int, adjoint: bool):
+    arr = (c_void_p * batch)(*[p.handle for p in phis])
+    with torch.cuda.device(plan.device):
+        check(plan.lib.cmbl_lenseflow(plan.handle, arr, x.data_ptr(), out.data_ptr(), batch, ncomp,
+                                      nsteps, int(direction), int(bool(adjoint)),
+                                      stream_ptr(plan.device)), "cmbl_lenseflow")
+
+
+def rfft2(plan: Plan, x: torch.Tensor, X: torch.Tensor, nmaps: int):
+    with torch.cuda.device(plan.device):
+        check(plan.lib.cmbl_rfft2(plan.handle, x.data_ptr(), X.data_ptr(), nmaps,
+                                  stream_ptr(plan.device)), "cmbl_rfft2")
+
+
+def irfft2(plan: Plan, X: torch.Tensor, x: torch.Tensor, nmaps: int):
+    with torch.cuda.device(plan.device):
+        check(plan.lib.cmbl_irfft2(plan.handle, X.data_ptr(), x.data_ptr(), nmaps,
+                                   stream_ptr(plan.device)), "cmbl_irfft2")

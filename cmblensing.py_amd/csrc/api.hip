@@ -1,0 +1,442 @@
+// C ABI of libcmbl.so (declared in include/cmbl.h): plans, phi handles and the
+// RK4 stage sequencing of LenseFlow on top of the kernels in kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cmbl.h"
+#include "kernels.hpp"
+
+using cmbl::cplx;
+
+namespace {
+
+thread_local std::string g_err;
+
+cmbl_status fail(cmbl_status code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(CMBL_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+constexpr int kRing = 8;
+
+bool valid_n(int N) {
+  return N >= 16 && N <= 512 && (N & (N - 1)) == 0;
+}
+
+// tw[k] = exp(-2 pi i k / N) in double, then rounded once to T.
+template <typename T>
+std::vector<cplx<T>> twiddles(int N) {
+  std::vector<cplx<T>> tw(N);
+  for (int k = 0; k < N; ++k) {
+    const double a = -2.0 * M_PI * (double)k / (double)N;
+    tw[k].x = (T)std::cos(a);
+    tw[k].y = (T)std::sin(a);
+  }
+  return tw;
+}
+
+}  // namespace
+
+struct cmbl_plan {
+  int N = 0;
+  double theta_pix = 0, pix_width = 0, dk = 0;
+  int dtype = CMBL_F32;
+  int max_maps = 0;
+  size_t tsize = 4;
+  void* tw = nullptr;       // twiddles in the plan dtype
+  void* tw64 = nullptr;     // double twiddles (phi preparation)
+  void* acc = nullptr;      // [max_maps][N][N] T
+  void* s0 = nullptr;       // [max_maps][N][N/2] complex T
+  void* s1 = nullptr;
+  // per-call phi table pointer arrays: device [kRing][max_maps], pinned host
+  // staging [kRing][max_maps], and an event per slot marking the end of the
+  // last call that used it.
+  void** ptab_dev = nullptr;
+  void** ptab_host = nullptr;
+  hipEvent_t ring_ev[kRing] = {};
+  bool ring_used[kRing] = {};
+  int ring_next = 0;
+  // optional per-kernel timing (HIP events around every stage launch)
+  bool timing = false;
+  std::vector<hipEvent_t> tev;   // pool, 2 per launch
+  std::vector<int> tkind;        // kernel class per launch: 0 column, 1 row, 2 other
+  double t_ms[3] = {0, 0, 0};
+  long t_n[3] = {0, 0, 0};
+};
+
+struct cmbl_phi {
+  int N = 0;
+  int nsteps = 0;
+  int dtype = CMBL_F32;
+  const cmbl_plan* plan = nullptr;
+  double* deriv = nullptr;  // [5][N][N] double
+  void* ptab = nullptr;     // [(2n+1)][2][N][N] T
+  double min_det = 0;
+};
+
+extern "C" {
+
+const char* cmbl_version(void) { return "cmbl 0.1 (gfx950)"; }
+
+const char* cmbl_last_error(void) { return g_err.c_str(); }
+
+cmbl_status cmbl_plan_create(cmbl_plan** out, int N, double theta_pix_arcmin, int dtype,
+                             int max_maps) {
+  if (!out) return fail(CMBL_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (!valid_n(N)) return fail(CMBL_EINVAL, "N must be a power of two in [16, 512]");
+  if (dtype != CMBL_F32 && dtype != CMBL_F64) return fail(CMBL_EINVAL, "dtype must be F32 or F64");
+  if (max_maps < 1) return fail(CMBL_EINVAL, "max_maps must be >= 1");
+  if (!(theta_pix_arcmin > 0)) return fail(CMBL_EINVAL, "theta_pix must be > 0");
+  cmbl_plan* p = new cmbl_plan();
+  p->N = N;
+  p->theta_pix = theta_pix_arcmin;
+  p->pix_width = theta_pix_arcmin / 60.0 * M_PI / 180.0;
+  p->dk = 2.0 * M_PI / ((double)N * p->pix_width);
+  p->dtype = dtype;
+  p->max_maps = max_maps;
+  p->tsize = dtype == CMBL_F32 ? 4 : 8;
+  const size_t NN = (size_t)N * N, NH = (size_t)N * (N / 2);
+  auto cleanup = [&](cmbl_status st) {
+    cmbl_plan_destroy(p);
+    return st;
+  };
+  hipError_t e;
+  if (dtype == CMBL_F32) {
+    auto t = twiddles<float>(N);
+    if ((e = hipMalloc(&p->tw, N * sizeof(cplx<float>))) != hipSuccess ||
+        (e = hipMemcpy(p->tw, t.data(), N * sizeof(cplx<float>), hipMemcpyHostToDevice)) != hipSuccess)
+      return cleanup(fail(CMBL_EHIP, std::string("twiddles: ") + hipGetErrorString(e)));
+  }
+  {
+    auto t = twiddles<double>(N);
+    if ((e = hipMalloc(&p->tw64, N * sizeof(cplx<double>))) != hipSuccess ||
+        (e = hipMemcpy(p->tw64, t.data(), N * sizeof(cplx<double>), hipMemcpyHostToDevice)) != hipSuccess)
+      return cleanup(fail(CMBL_EHIP, std::string("twiddles: ") + hipGetErrorString(e)));
+    if (dtype == CMBL_F64) p->tw = p->tw64;
+  }
+  if ((e = hipMalloc(&p->acc, max_maps * NN * p->tsize)) != hipSuccess ||
+      (e = hipMalloc(&p->s0, max_maps * NH * 2 * p->tsize)) != hipSuccess ||
+      (e = hipMalloc(&p->s1, max_maps * NH * 2 * p->tsize)) != hipSuccess ||
+      (e = hipMalloc((void**)&p->ptab_dev, kRing * max_maps * sizeof(void*))) != hipSuccess ||
+      (e = hipHostMalloc((void**)&p->ptab_host, kRing * max_maps * sizeof(void*))) != hipSuccess)
+    return cleanup(fail(CMBL_ENOMEM, std::string("plan scratch: ") + hipGetErrorString(e)));
+  for (int i = 0; i < kRing; ++i)
+    if ((e = hipEventCreateWithFlags(&p->ring_ev[i], hipEventDisableTiming)) != hipSuccess)
+      return cleanup(fail(CMBL_EHIP, std::string("event: ") + hipGetErrorString(e)));
+  *out = p;
+  return CMBL_OK;
+}
+
+cmbl_status cmbl_plan_destroy(cmbl_plan* p) {
+  if (!p) return CMBL_OK;
+  for (hipEvent_t e : p->tev) (void)hipEventDestroy(e);
+  for (int i = 0; i < kRing; ++i)
+    if (p->ring_ev[i]) {
+      (void)hipEventSynchronize(p->ring_ev[i]);
+      (void)hipEventDestroy(p->ring_ev[i]);
+    }
+  if (p->tw && p->tw != p->tw64) (void)hipFree(p->tw);
+  if (p->tw64) (void)hipFree(p->tw64);
+  if (p->acc) (void)hipFree(p->acc);
+  if (p->s0) (void)hipFree(p->s0);
+  if (p->s1) (void)hipFree(p->s1);
+  if (p->ptab_dev) (void)hipFree(p->ptab_dev);
+  if (p->ptab_host) (void)hipHostFree(p->ptab_host);
+  delete p;
+  return CMBL_OK;
+}
+
+cmbl_status cmbl_rfft2(cmbl_plan* p, const void* x, void* X, int nmaps, void* stream) {
+  if (!p || !x || !X) return fail(CMBL_EINVAL, "NULL argument");
+  if (nmaps < 0) return fail(CMBL_EINVAL, "nmaps < 0");
+  hipStream_t s = (hipStream_t)stream;
+  const int N = p->N;
+  const size_t NN = (size_t)N * N, NH1 = (size_t)N * (N / 2 + 1);
+  for (int m0 = 0; m0 < nmaps; m0 += p->max_maps) {
+    const int nm = std::min(p->max_maps, nmaps - m0);
+    if (p->dtype == CMBL_F32)
+      HIP_TRY(cmbl::launch_rfft2<float>(N, (const float*)x + m0 * NN, (cplx<float>*)p->s0,
+                                        (cplx<float>*)X + m0 * NH1, nm,
+                                        (const cplx<float>*)p->tw, s));
+    else
+      HIP_TRY(cmbl::launch_rfft2<double>(N, (const double*)x + m0 * NN, (cplx<double>*)p->s0,
+                                         (cplx<double>*)X + m0 * NH1, nm,
+                                         (const cplx<double>*)p->tw, s));
+  }
+  return CMBL_OK;
+}
+
+cmbl_status cmbl_irfft2(cmbl_plan* p, const void* X, void* x, int nmaps, void* stream) {
+  if (!p || !x || !X) return fail(CMBL_EINVAL, "NULL argument");
+  if (nmaps < 0) return fail(CMBL_EINVAL, "nmaps < 0");
+  hipStream_t s = (hipStream_t)stream;
+  const int N = p->N;
+  const size_t NN = (size_t)N * N, NH1 = (size_t)N * (N / 2 + 1);
+  for (int m0 = 0; m0 < nmaps; m0 += p->max_maps) {
+    const int nm = std::min(p->max_maps, nmaps - m0);
+    if (p->dtype == CMBL_F32)
+      HIP_TRY(cmbl::launch_irfft2<float>(N, (const cplx<float>*)X + m0 * NH1, (cplx<float>*)p->s0,
+                                         (float*)x + m0 * NN, nm, (const cplx<float>*)p->tw, s));
+    else
+      HIP_TRY(cmbl::launch_irfft2<double>(N, (const cplx<double>*)X + m0 * NH1,
+                                          (cplx<double>*)p->s0, (double*)x + m0 * NN, nm,
+                                          (const cplx<double>*)p->tw, s));
+  }
+  return CMBL_OK;
+}
+
+cmbl_status cmbl_phi_prepare(cmbl_plan* p, const void* phi, int phi_basis, int nsteps,
+                             cmbl_phi** out, double* min_det, void* stream) {
+  if (!p || !phi || !out) return fail(CMBL_EINVAL, "NULL argument");
+  *out = nullptr;
+  if (phi_basis != CMBL_PHI_MAP && phi_basis != CMBL_PHI_FOURIER)
+    return fail(CMBL_EINVAL, "phi_basis must be MAP or FOURIER");
+  if (nsteps < 1) return fail(CMBL_EINVAL, "nsteps must be >= 1");
+  hipStream_t s = (hipStream_t)stream;
+  const int N = p->N;
+  const size_t NN = (size_t)N * N, NH1 = (size_t)N * (N / 2 + 1), NH = (size_t)N * (N / 2);
+  cmbl_phi* h = new cmbl_phi();
+  h->N = N;
+  h->nsteps = nsteps;
+  h->dtype = p->dtype;
+  h->plan = p;
+  cplx<double>* F = nullptr;
+  cplx<double>* D = nullptr;
+  cplx<double>* Pk = nullptr;
+  double* minpart = nullptr;
+  auto release = [&]() {
+    if (F) (void)hipFree(F);
+    if (D) (void)hipFree(D);
+    if (Pk) (void)hipFree(Pk);
+    if (minpart) (void)hipFree(minpart);
+  };
+  auto bail = [&](cmbl_status st) {
+    release();
+    cmbl_phi_release(h);
+    return st;
+  };
+  hipError_t e;
+  const size_t ntab = (size_t)(2 * nsteps + 1) * 2 * NN;
+  const int nblk = (int)((NN + 255) / 256);
+  if ((e = hipMalloc(&F, NH1 * sizeof(cplx<double>))) != hipSuccess ||
+      (e = hipMalloc(&D, 5 * NH1 * sizeof(cplx<double>))) != hipSuccess ||
+      (e = hipMalloc(&Pk, 5 * NH * sizeof(cplx<double>))) != hipSuccess ||
+      (e = hipMalloc(&minpart, nblk * sizeof(double))) != hipSuccess ||
+      (e = hipMalloc(&h->deriv, 5 * NN * sizeof(double))) != hipSuccess ||
+      (e = hipMalloc(&h->ptab, ntab * p->tsize)) != hipSuccess)
+    return bail(fail(CMBL_ENOMEM, std::string("phi_prepare: ") + hipGetErrorString(e)));
+  const cplx<double>* tw64 = (const cplx<double>*)p->tw64;
+  if (phi_basis == CMBL_PHI_MAP) {
+    if ((e = cmbl::launch_rfft2<double>(N, (const double*)phi, Pk, F, 1, tw64, s)) != hipSuccess)
+      return bail(fail(CMBL_EHIP, std::string("phi rfft2: ") + hipGetErrorString(e)));
+  } else {
+    if ((e = hipMemcpyAsync(F, phi, NH1 * sizeof(cplx<double>), hipMemcpyDeviceToDevice, s)) !=
+        hipSuccess)
+      return bail(fail(CMBL_EHIP, std::string("phi copy: ") + hipGetErrorString(e)));
+  }
+  if ((e = cmbl::launch_phi_mult(N, F, D, p->dk, s)) != hipSuccess ||
+      (e = cmbl::launch_irfft2<double>(N, D, Pk, h->deriv, 5, tw64, s)) != hipSuccess)
+    return bail(fail(CMBL_EHIP, std::string("phi derivatives: ") + hipGetErrorString(e)));
+  int nb = 0;
+  if (p->dtype == CMBL_F32)
+    e = cmbl::launch_ptab<float>(N, h->deriv, (float*)h->ptab, nsteps, minpart, &nb, s);
+  else
+    e = cmbl::launch_ptab<double>(N, h->deriv, (double*)h->ptab, nsteps, minpart, &nb, s);
+  if (e != hipSuccess) return bail(fail(CMBL_EHIP, std::string("phi table: ") + hipGetErrorString(e)));
+  std::vector<double> mp(nb);
+  if ((e = hipMemcpyAsync(mp.data(), minpart, nb * sizeof(double), hipMemcpyDeviceToHost, s)) !=
+          hipSuccess ||
+      (e = hipStreamSynchronize(s)) != hipSuccess)
+    return bail(fail(CMBL_EHIP, std::string("phi min det: ") + hipGetErrorString(e)));
+  double md = 1.0;
+  for (double v : mp) md = std::min(md, v);
+  h->min_det = md;
+  if (min_det) *min_det = md;
+  release();
+  *out = h;
+  return CMBL_OK;
+}
+
+cmbl_status cmbl_phi_release(cmbl_phi* h) {
+  if (!h) return CMBL_OK;
+  if (h->deriv) (void)hipFree(h->deriv);
+  if (h->ptab) (void)hipFree(h->ptab);
+  delete h;
+  return CMBL_OK;
+}
+
+const void* cmbl_phi_derivatives(const cmbl_phi* h) { return h ? h->deriv : nullptr; }
+
+}  // extern "C"
+
+namespace {
+
+// Timing: begin_launch/end_launch bracket one kernel launch with events when
+// the plan's timing mode is on; flush_timing waits and accumulates.
+struct Timer {
+  cmbl_plan* p;
+  hipStream_t s;
+  size_t used = 0;
+  cmbl_status begin(int kind) {
+    if (!p->timing) return CMBL_OK;
+    while (p->tev.size() < 2 * (used + 1)) {
+      hipEvent_t e;
+      HIP_TRY(hipEventCreate(&e));
+      p->tev.push_back(e);
+    }
+    if (p->tkind.size() < used + 1) p->tkind.resize(used + 1);
+    p->tkind[used] = kind;
+    HIP_TRY(hipEventRecord(p->tev[2 * used], s));
+    return CMBL_OK;
+  }
+  cmbl_status end() {
+    if (!p->timing) return CMBL_OK;
+    HIP_TRY(hipEventRecord(p->tev[2 * used + 1], s));
+    ++used;
+    return CMBL_OK;
+  }
+  cmbl_status flush() {
+    if (!p->timing || used == 0) return CMBL_OK;
+    HIP_TRY(hipEventSynchronize(p->tev[2 * used - 1]));
+    for (size_t i = 0; i < used; ++i) {
+      float ms = 0;
+      HIP_TRY(hipEventElapsedTime(&ms, p->tev[2 * i], p->tev[2 * i + 1]));
+      p->t_ms[p->tkind[i]] += ms;
+      p->t_n[p->tkind[i]] += 1;
+    }
+    used = 0;
+    return CMBL_OK;
+  }
+};
+
+#define TIMED(kind, expr)                 \
+  do {                                    \
+    cmbl_status ts_ = tm.begin(kind);     \
+    if (ts_ != CMBL_OK) return ts_;       \
+    HIP_TRY(expr);                        \
+    ts_ = tm.end();                       \
+    if (ts_ != CMBL_OK) return ts_;       \
+  } while (0)
+
+template <typename T>
+cmbl_status run_lenseflow(cmbl_plan* p, const void* const* ptabs, T* y, long nmaps, int C,
+                          int n, int direction, int adjoint, hipStream_t s) {
+  const int N = p->N;
+  const cplx<T>* tw = (const cplx<T>*)p->tw;
+  cplx<T>* S0 = (cplx<T>*)p->s0;
+  cplx<T>* S1 = (cplx<T>*)p->s1;
+  T* acc = (T*)p->acc;
+  const T* const* pt = (const T* const*)ptabs;
+  // truth table of lense_flow.py:317-322
+  const bool fwd_time = ((direction == CMBL_FORWARD_LENSE) != (adjoint != 0));
+  const double t0 = fwd_time ? 0.0 : 1.0, t1 = fwd_time ? 1.0 : 0.0;
+  const T h = (T)((t1 - t0) / n);
+  // time index j of t = t0 + (i + c) h on the grid j/(2n), c in {0, 1/2, 1}
+  auto tindex = [&](int i, int half_steps) {  // half_steps = 2c
+    const int j = 2 * i + half_steps;
+    return fwd_time ? j : 2 * n - j;
+  };
+  static const int stage_half[5] = {0, 0, 1, 1, 2};
+  const bool adj = adjoint != 0;
+  Timer tm{p, s};
+  if (!adj) {
+    TIMED(2, cmbl::launch_lf_row_r2c<T>(N, y, S0, nmaps, tw, s));
+  } else {
+    TIMED(2, cmbl::launch_lf_row<T>(N, true, S0, S1, y, acc, pt, tw, C, nmaps, 1, h,
+                                    tindex(0, 0), 1, 1, s));
+  }
+  for (int i = 0; i < n; ++i) {
+    for (int st = 1; st <= 4; ++st) {
+      const bool last = (i == n - 1) && (st == 4);
+      TIMED(0, cmbl::launch_lf_col<T>(N, adj, S0, S1, nmaps, tw, (T)p->dk, s));
+      int tidx;
+      if (!adj) {
+        tidx = tindex(i, stage_half[st]);
+      } else {
+        // time of the next stage (stage 4 of step i and stage 1 of step i+1
+        // share t + h)
+        tidx = st < 4 ? tindex(i, stage_half[st + 1]) : tindex(i + 1, 0);
+        if (last) tidx = 0;
+      }
+      TIMED(1, cmbl::launch_lf_row<T>(N, adj, S0, S1, y, acc, pt, tw, C, nmaps, st, h, tidx,
+                                      last ? 0 : 1, 0, s));
+    }
+  }
+  return tm.flush();
+}
+
+}  // namespace
+
+extern "C" cmbl_status cmbl_lenseflow(cmbl_plan* p, const cmbl_phi* const* phis, const void* in,
+                                      void* out, int batch, int ncomp, int nsteps, int direction,
+                                      int adjoint, void* stream) {
+  if (!p || !phis || !in || !out) return fail(CMBL_EINVAL, "NULL argument");
+  if (ncomp != 1 && ncomp != 2) return fail(CMBL_EINVAL, "ncomp must be 1 (T) or 2 (QU)");
+  if (batch < 0) return fail(CMBL_EINVAL, "batch < 0");
+  if (direction != CMBL_FORWARD_LENSE && direction != CMBL_INVERSE_LENSE)
+    return fail(CMBL_EINVAL, "direction must be +1 or -1");
+  if (nsteps < 1) return fail(CMBL_EINVAL, "nsteps must be >= 1");
+  if ((long)batch * ncomp > p->max_maps)
+    return fail(CMBL_EINVAL, "batch*ncomp exceeds the plan's max_maps");
+  for (int b = 0; b < batch; ++b) {
+    if (!phis[b]) return fail(CMBL_EINVAL, "phi handle is NULL");
+    if (phis[b]->N != p->N || phis[b]->dtype != p->dtype)
+      return fail(CMBL_EINVAL, "phi handle was prepared for another geometry or dtype");
+    if (phis[b]->nsteps != nsteps) return fail(CMBL_EINVAL, "phi handle prepared for another nsteps");
+  }
+  if (batch == 0) return CMBL_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t bytes = (size_t)batch * ncomp * p->N * p->N * p->tsize;
+  if (in != out) HIP_TRY(hipMemcpyAsync(out, in, bytes, hipMemcpyDeviceToDevice, s));
+  // stage the per-batch phi table pointers through the ring
+  const int slot = p->ring_next;
+  p->ring_next = (slot + 1) % kRing;
+  if (p->ring_used[slot]) HIP_TRY(hipEventSynchronize(p->ring_ev[slot]));
+  void** host = p->ptab_host + (size_t)slot * p->max_maps;
+  void** dev = p->ptab_dev + (size_t)slot * p->max_maps;
+  for (int b = 0; b < batch; ++b) host[b] = phis[b]->ptab;
+  HIP_TRY(hipMemcpyAsync(dev, host, batch * sizeof(void*), hipMemcpyHostToDevice, s));
+  cmbl_status st;
+  if (p->dtype == CMBL_F32)
+    st = run_lenseflow<float>(p, (const void* const*)dev, (float*)out, (long)batch * ncomp, ncomp,
+                              nsteps, direction, adjoint, s);
+  else
+    st = run_lenseflow<double>(p, (const void* const*)dev, (double*)out, (long)batch * ncomp,
+                               ncomp, nsteps, direction, adjoint, s);
+  if (st != CMBL_OK) return st;
+  HIP_TRY(hipEventRecord(p->ring_ev[slot], s));
+  p->ring_used[slot] = true;
+  return CMBL_OK;
+}
+
+extern "C" cmbl_status cmbl_plan_set_timing(cmbl_plan* p, int enable) {
+  if (!p) return fail(CMBL_EINVAL, "NULL plan");
+  p->timing = enable != 0;
+  return CMBL_OK;
+}
+
+extern "C" cmbl_status cmbl_plan_get_timing(cmbl_plan* p, double* out, int reset) {
+  if (!p || !out) return fail(CMBL_EINVAL, "NULL argument");
+  for (int k = 0; k < 3; ++k) {
+    out[2 * k] = p->t_ms[k];
+    out[2 * k + 1] = (double)p->t_n[k];
+    if (reset) {
+      p->t_ms[k] = 0;
+      p->t_n[k] = 0;
+    }
+  }
+  return CMBL_OK;
+}

@@ -1,0 +1,207 @@
+// Device-side building blocks: complex arithmetic and a Stockham radix-8 line
+// FFT staged through LDS, for gfx950 (CDNA4, 64-wide wavefronts).
+//
+// A "line" of N complex points (N = 2^m, 16 <= N <= 512) is transformed by a
+// group of N/8 threads.  On entry and on exit thread t of the group holds the
+// 8 points   v[q] = line[t + q*N/8],  q = 0..7   in registers, so a caller can
+// load/store a line with fully coalesced accesses (consecutive threads touch
+// consecutive points), and chain a forward transform, a pointwise multiply and
+// an inverse transform without a trip through memory.  Between radix passes
+// the line goes through an LDS buffer of lds_len<N>() complex values.
+//
+// Pass plan: floor(m/3) radix-8 passes, then one radix-2 or radix-4 pass when
+// m is not a multiple of 3 (each thread then runs 4 or 2 small butterflies).
+// Twiddles come from a table tw[k] = exp(-2 pi i k / N) held in LDS, computed
+// on the host in double precision and rounded once to T.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace cmbl {
+
+template <typename T>
+struct cplx {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cplx<T> mk(T a, T b) {
+  cplx<T> r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> operator+(cplx<T> a, cplx<T> b) {
+  return mk(a.x + b.x, a.y + b.y);
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> operator-(cplx<T> a, cplx<T> b) {
+  return mk(a.x - b.x, a.y - b.y);
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+template <typename T>
+__device__ __forceinline__ cplx<T> cmulc(cplx<T> a, cplx<T> b) {
+  return mk(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> conj(cplx<T> a) {
+  return mk(a.x, -a.y);
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> scale(cplx<T> a, T s) {
+  return mk(a.x * s, a.y * s);
+}
+// multiply by -i (forward) or +i (inverse)
+template <bool INV, typename T>
+__device__ __forceinline__ cplx<T> rot90(cplx<T> a) {
+  return INV ? mk(-a.y, a.x) : mk(a.y, -a.x);
+}
+
+// LDS padding: one slot every 8 keeps the stride-8 writes of the first pass
+// off a single bank group.
+template <int N>
+__host__ __device__ constexpr int lds_len() {
+  return N + N / 8;
+}
+__device__ __forceinline__ int pad_idx(int i) { return i + (i >> 3); }
+
+template <int N>
+__host__ __device__ constexpr int ilog2() {
+  return N <= 1 ? 0 : 1 + ilog2<N / 2>();
+}
+
+template <bool INV, typename T>
+__device__ __forceinline__ void dft2(cplx<T>& a, cplx<T>& b) {
+  cplx<T> t = a - b;
+  a = a + b;
+  b = t;
+}
+
+// In-place 4-point DFT, natural order in and out.
+template <bool INV, typename T>
+__device__ __forceinline__ void dft4(cplx<T>& y0, cplx<T>& y1, cplx<T>& y2, cplx<T>& y3) {
+  cplx<T> c0 = y0 + y2, c1 = y0 - y2, c2 = y1 + y3, c3 = rot90<INV>(y1 - y3);
+  y0 = c0 + c2;
+  y2 = c0 - c2;
+  y1 = c1 + c3;
+  y3 = c1 - c3;
+}
+
+// In-place 8-point DFT, natural order in and out.
+template <bool INV, typename T>
+__device__ __forceinline__ void dft8(cplx<T>* v) {
+  const T r = T(0.70710678118654752440084436210485);
+  cplx<T> a0 = v[0] + v[4], a1 = v[1] + v[5], a2 = v[2] + v[6], a3 = v[3] + v[7];
+  cplx<T> b0 = v[0] - v[4], b1 = v[1] - v[5], b2 = v[2] - v[6], b3 = v[3] - v[7];
+  // b_k *= W8^k, W8 = exp(-+2 pi i / 8)
+  b1 = INV ? mk(r * (b1.x - b1.y), r * (b1.x + b1.y)) : mk(r * (b1.x + b1.y), r * (b1.y - b1.x));
+  b2 = rot90<INV>(b2);
+  b3 = INV ? mk(-r * (b3.x + b3.y), r * (b3.x - b3.y)) : mk(r * (b3.y - b3.x), -r * (b3.x + b3.y));
+  dft4<INV>(a0, a1, a2, a3);
+  dft4<INV>(b0, b1, b2, b3);
+  v[0] = a0;
+  v[2] = a1;
+  v[4] = a2;
+  v[6] = a3;
+  v[1] = b0;
+  v[3] = b1;
+  v[5] = b2;
+  v[7] = b3;
+}
+
+template <bool INV, typename T>
+__device__ __forceinline__ cplx<T> twiddle(const cplx<T>* tw, int idx) {
+  cplx<T> w = tw[idx];
+  return INV ? conj(w) : w;
+}
+
+// NL independent lines, each handled by the same group of N/8 threads
+// (thread index t within the group).  bufs[l] points to that line's LDS
+// buffer (lds_len<N>() entries), tw to the LDS twiddle table (N entries).
+// Every thread of the workgroup must call this the same number of times:
+// the passes synchronise with __syncthreads().
+template <int N, bool INV, int NL, typename T>
+__device__ __forceinline__ void fft_lines(cplx<T> (&v)[NL][8], cplx<T>* const (&bufs)[NL],
+                                          const cplx<T>* tw, int t) {
+  constexpr int M = ilog2<N>();
+  constexpr int P8 = M / 3;      // radix-8 passes
+  constexpr int REM = M % 3;     // trailing radix-2^REM pass (0: none)
+  constexpr int G = N / 8;       // threads per line
+  static_assert(N >= 16 && N <= 512 && (N & (N - 1)) == 0, "N must be 2^m, 16..512");
+
+  int Ns = 1;
+#pragma unroll
+  for (int p = 0; p < P8; ++p) {
+    // radix-8 butterfly j = t on points t + q*N/8
+    const int k = t & (Ns - 1);
+    if (p > 0) {
+#pragma unroll
+      for (int q = 1; q < 8; ++q) {
+        cplx<T> w = twiddle<INV>(tw, (q * k) * (N / (Ns * 8)));
+#pragma unroll
+        for (int l = 0; l < NL; ++l) v[l][q] = cmul(v[l][q], w);
+      }
+    }
+#pragma unroll
+    for (int l = 0; l < NL; ++l) dft8<INV>(v[l]);
+    const bool last = (p == P8 - 1) && (REM == 0);
+    if (!last) {
+      const int base = (t / Ns) * Ns * 8 + k;
+      __syncthreads();
+#pragma unroll
+      for (int l = 0; l < NL; ++l)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) bufs[l][pad_idx(base + q * Ns)] = v[l][q];
+      __syncthreads();
+      Ns *= 8;
+      if (p < P8 - 1) {
+#pragma unroll
+        for (int l = 0; l < NL; ++l)
+#pragma unroll
+          for (int q = 0; q < 8; ++q) v[l][q] = bufs[l][pad_idx(t + q * G)];
+      }
+    }
+  }
+  if constexpr (REM != 0) {
+    constexpr int R = 1 << REM;        // 2 or 4
+    constexpr int NB = 8 / R;          // butterflies per thread
+    // Ns == N / R here.  Butterfly b handles j = t + b*G on points
+    // j + q*N/R = t + (b + q*NB)*G -> register index b + q*NB.
+#pragma unroll
+    for (int l = 0; l < NL; ++l)
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[l][m] = bufs[l][pad_idx(t + m * G)];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int j = t + b * G;
+      const int k = j & (Ns - 1);
+#pragma unroll
+      for (int q = 1; q < R; ++q) {
+        cplx<T> w = twiddle<INV>(tw, (q * k) * (N / (Ns * R)));
+#pragma unroll
+        for (int l = 0; l < NL; ++l) v[l][b + q * NB] = cmul(v[l][b + q * NB], w);
+      }
+#pragma unroll
+      for (int l = 0; l < NL; ++l) {
+        if constexpr (R == 2) {
+          dft2<INV>(v[l][b], v[l][b + NB]);
+        } else {
+          dft4<INV>(v[l][b], v[l][b + NB], v[l][b + 2 * NB], v[l][b + 3 * NB]);
+        }
+      }
+    }
+    // outputs of butterfly j land on j + q*N/R: the same register slots.
+  }
+}
+
+// Cooperative load of the twiddle table into LDS (whole workgroup).
+template <int N, typename T>
+__device__ __forceinline__ void load_twiddles(cplx<T>* tw_lds, const cplx<T>* tw_glob) {
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw_lds[i] = tw_glob[i];
+}
+
+}  // namespace cmbl

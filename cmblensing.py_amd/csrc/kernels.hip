@@ -1,0 +1,668 @@
+// HIP kernels for the LenseFlow hot path on gfx950.
+//
+// Array layouts (row-major, contiguous):
+//   map      x[m][i][j]          m = batch*C + component, i = row (KX axis),
+//                                 j = column (KY axis, the rfft axis)
+//   natural  X[m][kx][ky]        ky in [0, N/2], numpy rfft2 layout
+//   packed   P[m][kx][ky]        ky in [0, N/2): slot ky=0 holds the DC column
+//                                 in .x and the ky=N/2 (Nyquist) column in .y.
+//                                 Used between the row and column passes: after
+//                                 the row R2C both columns are real per row, so
+//                                 one complex column carries both and the half
+//                                 plane becomes a power-of-two N x N/2.
+//
+// Conventions are numpy's: forward unnormalised, inverse 1/N^2, the C2R along
+// rows ignores the imaginary parts of the DC and Nyquist columns, KX from
+// fftfreq (kx = N/2 maps to -N/2), KY from rfftfreq with the Nyquist negated.
+#include <hip/hip_runtime.h>
+
+#include "fft_lds.hpp"
+#include "kernels.hpp"
+
+namespace cmbl {
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+template <int N>
+__device__ __forceinline__ int kx_signed(int kx) {
+  return kx < N / 2 ? kx : kx - N;
+}
+
+// i * s * F
+template <typename T>
+__device__ __forceinline__ cplx<T> imul(cplx<T> F, T s) {
+  return mk(-s * F.y, s * F.x);
+}
+
+// Row pairing used by the fused stage kernels.  Pair index p covers two
+// map-rows: for C == 2 the Q and U rows of the same pixel row (they share the
+// deflection operator rows), for C == 1 the pixel rows 2r and 2r+1.
+struct RowPair {
+  int b;       // batch element
+  int ia, ib;  // pixel rows
+  long ra, rb; // flat row indices into [B*C*N]
+};
+template <int N>
+__device__ __forceinline__ RowPair row_pair(int p, int C) {
+  RowPair r;
+  if (C == 2) {
+    r.b = p / N;
+    int i = p - r.b * N;
+    r.ia = r.ib = i;
+    r.ra = (long)(2 * r.b) * N + i;
+    r.rb = r.ra + N;
+  } else {
+    r.b = p / (N / 2);
+    int i = 2 * (p - r.b * (N / 2));
+    r.ia = i;
+    r.ib = i + 1;
+    r.ra = (long)r.b * N + i;
+    r.rb = r.ra + 1;
+  }
+  return r;
+}
+
+// Build the full complex line Z = A + i B of two packed Hermitian half rows A
+// and B into LDS, then load the 8 register points of thread t.  The packed
+// slot 0 supplies the DC (.x) and Nyquist (.y) values; their imaginary parts
+// are dropped, which is the C2R convention.
+template <int N, typename T>
+__device__ __forceinline__ void herm_pair_to_line(const cplx<T>* A, const cplx<T>* B, cplx<T>* buf,
+                                                  cplx<T> (&v)[8], int t) {
+  constexpr int G = N / 8;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int k = t + q * G;
+    cplx<T> a = A[k], b = B[k];
+    if (k == 0) {
+      buf[pad_idx(0)] = mk(a.x, b.x);
+      buf[pad_idx(N / 2)] = mk(a.y, b.y);
+    } else {
+      buf[pad_idx(k)] = mk(a.x - b.y, a.y + b.x);          // a + i b
+      buf[pad_idx(N - k)] = mk(a.x + b.y, b.x - a.y);      // conj(a) + i conj(b)
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 8; ++q) v[q] = buf[pad_idx(t + q * G)];
+}
+
+// After a forward FFT of z = a + i b (v[m] = Z[t + m*G]) split into the two
+// packed half spectra and store them to A and B.
+template <int N, typename T>
+__device__ __forceinline__ void split_store(const cplx<T> (&v)[8], cplx<T>* buf, cplx<T>* A,
+                                            cplx<T>* B, int t, bool active) {
+  constexpr int G = N / 8;
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < 8; ++m) buf[pad_idx(t + m * G)] = v[m];
+  __syncthreads();
+  if (!active) return;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int k = t + q * G;
+    cplx<T> zk = buf[pad_idx(k)];
+    cplx<T> a, b;
+    if (k == 0) {
+      cplx<T> zn = buf[pad_idx(N / 2)];
+      a = mk(zk.x, zn.x);
+      b = mk(zk.y, zn.y);
+    } else {
+      cplx<T> zc = conj(buf[pad_idx(N - k)]);
+      const T h = T(0.5);
+      a = scale(zk + zc, h);
+      cplx<T> d = zk - zc;
+      b = mk(h * d.y, -h * d.x);  // -i/2 * d
+    }
+    A[k] = a;
+    B[k] = b;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// generic rfft2 / irfft2 passes
+// ---------------------------------------------------------------------------
+
+// Row R2C of row pairs (2p, 2p+1): x[nrows][N] -> packed P[nrows][N/2].
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_row_r2c(const T* __restrict__ x, cplx<T>* __restrict__ P,
+                                                 const cplx<T>* __restrict__ twg, long npairs) {
+  constexpr int G = N / 8, PAIRS = 256 / G, H = N / 2;
+  __shared__ cplx<T> tw[N];
+  __shared__ cplx<T> lbuf[PAIRS][lds_len<N>()];
+  load_twiddles<N>(tw, twg);
+  const int grp = threadIdx.x / G, t = threadIdx.x % G;
+  const long p = (long)blockIdx.x * PAIRS + grp;
+  const bool active = p < npairs;
+  const long pp = active ? p : npairs - 1;
+  const T* xa = x + (2 * pp) * N;
+  const T* xb = xa + N;
+  cplx<T> v[1][8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) v[0][q] = mk(xa[t + q * G], xb[t + q * G]);
+  __syncthreads();
+  cplx<T>* const bufs[1] = {lbuf[grp]};
+  fft_lines<N, false, 1>(v, bufs, tw, t);
+  split_store<N>(v[0], lbuf[grp], P + (2 * pp) * H, P + (2 * pp + 1) * H, t, active);
+}
+
+// Row C2R of row pairs: packed P[nrows][N/2] -> x[nrows][N] (no scaling).
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_row_c2r(const cplx<T>* __restrict__ P, T* __restrict__ x,
+                                                 const cplx<T>* __restrict__ twg, long npairs) {
+  constexpr int G = N / 8, PAIRS = 256 / G, H = N / 2;
+  __shared__ cplx<T> tw[N];
+  __shared__ cplx<T> lbuf[PAIRS][lds_len<N>()];
+  load_twiddles<N>(tw, twg);
+  const int grp = threadIdx.x / G, t = threadIdx.x % G;
+  const long p = (long)blockIdx.x * PAIRS + grp;
+  const bool active = p < npairs;
+  const long pp = active ? p : npairs - 1;
+  cplx<T> v[1][8];
+  __syncthreads();
+  herm_pair_to_line<N>(P + (2 * pp) * H, P + (2 * pp + 1) * H, lbuf[grp], v[0], t);
+  cplx<T>* const bufs[1] = {lbuf[grp]};
+  fft_lines<N, true, 1>(v, bufs, tw, t);
+  if (!active) return;
+  T* xa = x + (2 * pp) * N;
+  T* xb = xa + N;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    xa[t + m * G] = v[0][m].x;
+    xb[t + m * G] = v[0][m].y;
+  }
+}
+
+// Column forward C2C of packed P[m][N][N/2] -> natural X[m][N][N/2+1].
+template <typename T, int N, int KC>
+__global__ __launch_bounds__(KC* N / 8) void k_col_fwd_unpack(const cplx<T>* __restrict__ P,
+                                                              cplx<T>* __restrict__ X,
+                                                              const cplx<T>* __restrict__ twg) {
+  constexpr int G = N / 8, H = N / 2, TILES = H / KC;
+  __shared__ cplx<T> tw[N];
+  __shared__ cplx<T> lbuf[KC][lds_len<N>()];
+  load_twiddles<N>(tw, twg);
+  const int g = threadIdx.x % KC, t = threadIdx.x / KC;
+  const long m = blockIdx.x / TILES;
+  const int k = (blockIdx.x % TILES) * KC + g;
+  const cplx<T>* Pm = P + m * N * H;
+  cplx<T>* Xm = X + m * N * (H + 1);
+  cplx<T> v[1][8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) v[0][q] = Pm[(long)(t + q * G) * H + k];
+  __syncthreads();
+  cplx<T>* const bufs[1] = {lbuf[g]};
+  fft_lines<N, false, 1>(v, bufs, tw, t);
+  // the packed column needs G[-kx]: go through LDS (uniform for all groups)
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 8; ++q) lbuf[g][pad_idx(t + q * G)] = v[0][q];
+  __syncthreads();
+  if (k == 0) {
+    const T h = T(0.5);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int kx = t + q * G;
+      cplx<T> z = v[0][q];
+      cplx<T> zc = conj(lbuf[g][pad_idx((N - kx) & (N - 1))]);
+      cplx<T> a = scale(z + zc, h);
+      cplx<T> d = z - zc;
+      cplx<T> b = mk(h * d.y, -h * d.x);
+      Xm[(long)kx * (H + 1) + 0] = a;
+      Xm[(long)kx * (H + 1) + H] = b;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) Xm[(long)(t + q * G) * (H + 1) + k] = v[0][q];
+  }
+}
+
+// Column inverse C2C of natural X[m][N][N/2+1] -> packed P[m][N][N/2], times
+// `scale`.  The DC and Nyquist columns enter as Herm(X0) + i Herm(Xn), with
+// Herm(Y)[kx] = (Y[kx] + conj(Y[-kx]))/2: exactly the part the row C2R keeps.
+template <typename T, int N, int KC>
+__global__ __launch_bounds__(KC* N / 8) void k_col_inv_pack(const cplx<T>* __restrict__ X,
+                                                            cplx<T>* __restrict__ P,
+                                                            const cplx<T>* __restrict__ twg,
+                                                            T scale_) {
+  constexpr int G = N / 8, H = N / 2, TILES = H / KC;
+  __shared__ cplx<T> tw[N];
+  __shared__ cplx<T> lbuf[KC][lds_len<N>()];
+  load_twiddles<N>(tw, twg);
+  const int g = threadIdx.x % KC, t = threadIdx.x / KC;
+  const long m = blockIdx.x / TILES;
+  const int k = (blockIdx.x % TILES) * KC + g;
+  const cplx<T>* Xm = X + m * N * (H + 1);
+  cplx<T>* Pm = P + m * N * H;
+  cplx<T> v[1][8];
+  if (k == 0) {
+    const T h = T(0.5);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int kx = t + q * G;
+      const long r0 = (long)kx * (H + 1), r1 = (long)((N - kx) & (N - 1)) * (H + 1);
+      cplx<T> d0 = scale(Xm[r0] + conj(Xm[r1]), h);
+      cplx<T> n0 = scale(Xm[r0 + H] + conj(Xm[r1 + H]), h);
+      v[0][q] = mk(d0.x - n0.y, d0.y + n0.x);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[0][q] = Xm[(long)(t + q * G) * (H + 1) + k];
+  }
+  __syncthreads();
+  cplx<T>* const bufs[1] = {lbuf[g]};
+  fft_lines<N, true, 1>(v, bufs, tw, t);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) Pm[(long)(t + q * G) * H + k] = scale(v[0][q], scale_);
+}
+
+// ---------------------------------------------------------------------------
+// phi preparation (double precision)
+// ---------------------------------------------------------------------------
+
+// Five derivative spectra of F (natural layout): i KX F, i KY F, -KX^2 F,
+// -KX KY F, -KY^2 F, with KY's Nyquist entry negated (util.py:103-107).
+__global__ void k_phi_mult(const cplx<double>* __restrict__ F, cplx<double>* __restrict__ D, int N,
+                           double dk) {
+  const int H1 = N / 2 + 1;
+  const long n = (long)N * H1;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < n;
+       idx += (long)gridDim.x * blockDim.x) {
+    const int kx = (int)(idx / H1), ky = (int)(idx % H1);
+    const double KX = (kx < N / 2 ? kx : kx - N) * dk;
+    const double KY = (ky == N / 2 ? -ky : ky) * dk;
+    const cplx<double> f = F[idx];
+    D[0 * n + idx] = mk(-KX * f.y, KX * f.x);
+    D[1 * n + idx] = mk(-KY * f.y, KY * f.x);
+    D[2 * n + idx] = scale(f, -(KX * KX));
+    D[3 * n + idx] = scale(f, -(KX * KY));
+    D[4 * n + idx] = scale(f, -(KY * KY));
+  }
+}
+
+// Deflection-operator table: for t_j = j/(2n), j = 0..2n,
+//   p(t_j) = (I + t_j Hess phi)^-1 grad phi        (lense_flow.py:372-381, 397-415)
+// stored as T[(2n+1)][2][N*N].  Also the per-block minimum over pixels and
+// t in [0,1] of det(I + t Hess phi) = 1 + t tr + t^2 det(Hess phi).
+template <typename T>
+__global__ __launch_bounds__(256) void k_ptab(const double* __restrict__ Dm, T* __restrict__ ptab,
+                                              int N, int nsteps, double* __restrict__ minpart) {
+  const long P = (long)N * N;
+  const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  double mdet = 1.0;
+  if (idx < P) {
+    const double px = Dm[idx], py = Dm[P + idx], hxx = Dm[2 * P + idx], hxy = Dm[3 * P + idx],
+                 hyy = Dm[4 * P + idx];
+    const int nt = 2 * nsteps;
+    for (int j = 0; j <= nt; ++j) {
+      const double t = (double)j / (double)nt;
+      const double mxx = 1 + t * hxx, mxy = t * hxy, myy = 1 + t * hyy;
+      const double det = mxx * myy - mxy * mxy;
+      const double inv = 1.0 / (det + 1e-12);
+      const double ixx = myy * inv, ixy = -mxy * inv, iyy = mxx * inv;
+      ptab[(2L * j) * P + idx] = (T)(px * ixx + py * ixy);
+      ptab[(2L * j + 1) * P + idx] = (T)(px * ixy + py * iyy);
+    }
+    const double tr = hxx + hyy, dh = hxx * hyy - hxy * hxy;
+    mdet = fmin(1.0, 1.0 + tr + dh);
+    if (dh > 0) {
+      const double ts = -tr / (2 * dh);
+      if (ts > 0 && ts < 1) mdet = fmin(mdet, 1.0 - tr * tr / (4 * dh));
+    }
+    if (!(mdet == mdet)) mdet = -1.0;  // NaN phi is rejected like a caustic
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = mdet;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) minpart[blockIdx.x] = red[0];
+}
+
+// ---------------------------------------------------------------------------
+// LenseFlow fused stage kernels
+// ---------------------------------------------------------------------------
+
+// Column pass, standard (forward/inverse) rate: S0 holds the row-R2C spectrum
+// of the stage input z.  Column FFT, multiply by i KX / N^2 and i KY / N^2,
+// inverse column FFTs; d/dx goes back to S0, d/dy to S1 (packed layout).
+template <typename T, int N, int KC>
+__global__ __launch_bounds__(KC* N / 8) void k_col_lf_grad(cplx<T>* __restrict__ S0,
+                                                           cplx<T>* __restrict__ S1,
+                                                           const cplx<T>* __restrict__ twg, T dk) {
+  constexpr int G = N / 8, H = N / 2, TILES = H / KC;
+  __shared__ cplx<T> tw[N];
+  __shared__ cplx<T> lbuf[KC][lds_len<N>()];
+  load_twiddles<N>(tw, twg);
+  const int g = threadIdx.x % KC, t = threadIdx.x / KC;
+  const long m = blockIdx.x / TILES;
+  const int k = (blockIdx.x % TILES) * KC + g;
+  cplx<T>* s0 = S0 + m * N * H + k;
+  cplx<T>* s1 = S1 + m * N * H + k;
+  cplx<T> v[1][8], w[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) v[0][q] = s0[(long)(t + q * G) * H];
+  __syncthreads();
+  cplx<T>* const bufs[1] = {lbuf[g]};
+  fft_lines<N, false, 1>(v, bufs, tw, t);
+  const T sc = T(1) / (T(N) * T(N));
+  const T ky = (T)k * dk * sc;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int kx = t + q * G;
+    const T kxv = (T)kx_signed<N>(kx) * dk * sc;
+    cplx<T> f = v[0][q];
+    if (k == 0) {
+      // packed DC|Nyquist column: only the x-derivative survives the C2R and
+      // the self-conjugate kx = N/2 bin drops out.
+      v[0][q] = (kx == N / 2) ? mk(T(0), T(0)) : imul(f, kxv);
+      w[q] = mk(T(0), T(0));
+    } else {
+      v[0][q] = imul(f, kxv);
+      w[q] = imul(f, ky);
+    }
+  }
+  fft_lines<N, true, 1>(v, bufs, tw, t);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) s0[(long)(t + q * G) * H] = v[0][q];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) v[0][q] = w[q];
+  fft_lines<N, true, 1>(v, bufs, tw, t);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) s1[(long)(t + q * G) * H] = v[0][q];
+}
+
+// Column pass, adjoint (divergence) rate: S0, S1 hold the row-R2C spectra of
+// u = p_x z and v = p_y z.  W = (i KX U + i KY V) / N^2 back to S0.
+template <typename T, int N, int KC>
+__global__ __launch_bounds__(KC* N / 8) void k_col_lf_div(cplx<T>* __restrict__ S0,
+                                                          const cplx<T>* __restrict__ S1,
+                                                          const cplx<T>* __restrict__ twg, T dk) {
+  constexpr int G = N / 8, H = N / 2, TILES = H / KC;
+  __shared__ cplx<T> tw[N];
+  __shared__ cplx<T> lbuf[KC][lds_len<N>()];
+  load_twiddles<N>(tw, twg);
+  const int g = threadIdx.x % KC, t = threadIdx.x / KC;
+  const long m = blockIdx.x / TILES;
+  const int k = (blockIdx.x % TILES) * KC + g;
+  cplx<T>* s0 = S0 + m * N * H + k;
+  const cplx<T>* s1 = S1 + m * N * H + k;
+  cplx<T> u[1][8], v[1][8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    u[0][q] = s0[(long)(t + q * G) * H];
+    v[0][q] = s1[(long)(t + q * G) * H];
+  }
+  __syncthreads();
+  cplx<T>* const bufs[1] = {lbuf[g]};
+  fft_lines<N, false, 1>(u, bufs, tw, t);
+  fft_lines<N, false, 1>(v, bufs, tw, t);
+  const T sc = T(1) / (T(N) * T(N));
+  const T ky = (T)k * dk * sc;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int kx = t + q * G;
+    const T kxv = (T)kx_signed<N>(kx) * dk * sc;
+    if (k == 0) {
+      u[0][q] = (kx == N / 2) ? mk(T(0), T(0)) : imul(u[0][q], kxv);
+    } else {
+      u[0][q] = imul(u[0][q], kxv) + imul(v[0][q], ky);
+    }
+  }
+  fft_lines<N, true, 1>(u, bufs, tw, t);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) s0[(long)(t + q * G) * H] = u[0][q];
+}
+
+// RK4 bookkeeping for one stage (util.py:184-199), per pixel:
+//   stage 1: acc = y + h/6 k ; z = y + h/2 k
+//   stage 2: acc += h/3 k    ; z = y + h/2 k
+//   stage 3: acc += h/3 k    ; z = y + h k
+//   stage 4: y = acc + h/6 k ; z = y
+template <typename T>
+__device__ __forceinline__ T rk4_update(int stage, T h, T k, T* yp, T* ap, bool active) {
+  T z;
+  if (stage == 1) {
+    const T y = *yp;
+    if (active) *ap = y + (h / T(6)) * k;
+    z = y + (h / T(2)) * k;
+  } else if (stage == 4) {
+    z = *ap + (h / T(6)) * k;
+    if (active) *yp = z;
+  } else {
+    const T y = *yp;
+    if (active) *ap = *ap + (h / T(3)) * k;
+    z = y + (stage == 2 ? h / T(2) : h) * k;
+  }
+  return z;
+}
+
+// Row pass, standard rate.  For each row pair: C2R of (dz/dx + i dz/dy) per
+// row, k = p(t_s) . grad z, RK4 update of y/acc, then (unless this is the last
+// stage of the last step) the row R2C of the next stage input into S0.
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_row_lf_std(cplx<T>* __restrict__ S0,
+                                                    const cplx<T>* __restrict__ S1,
+                                                    T* __restrict__ y, T* __restrict__ acc,
+                                                    const T* const* __restrict__ ptabs,
+                                                    const cplx<T>* __restrict__ twg, int C,
+                                                    long npairs, int stage, T h, int tidx,
+                                                    int do_r2c) {
+  constexpr int G = N / 8, PAIRS = 256 / G, H = N / 2;
+  constexpr long NN = (long)N * N;
+  __shared__ cplx<T> tw[N];
+  __shared__ cplx<T> lbuf[PAIRS][2][lds_len<N>()];
+  load_twiddles<N>(tw, twg);
+  const int grp = threadIdx.x / G, t = threadIdx.x % G;
+  const long p = (long)blockIdx.x * PAIRS + grp;
+  const bool active = p < npairs;
+  const RowPair rp = row_pair<N>((int)(active ? p : npairs - 1), C);
+  cplx<T>* bufa = lbuf[grp][0];
+  cplx<T>* bufb = lbuf[grp][1];
+  cplx<T> v[2][8];
+  __syncthreads();
+  herm_pair_to_line<N>(S0 + rp.ra * H, S1 + rp.ra * H, bufa, v[0], t);
+  herm_pair_to_line<N>(S0 + rp.rb * H, S1 + rp.rb * H, bufb, v[1], t);
+  cplx<T>* const bufs[2] = {bufa, bufb};
+  fft_lines<N, true, 2>(v, bufs, tw, t);
+
+  const T* pt = ptabs[rp.b] + (long)tidx * 2 * NN;
+  cplx<T> z[1][8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int j = t + m * G;
+    const long ja = (long)rp.ia * N + j, jb = (long)rp.ib * N + j;
+    const T ka = pt[ja] * v[0][m].x + pt[NN + ja] * v[0][m].y;
+    const T kb = pt[jb] * v[1][m].x + pt[NN + jb] * v[1][m].y;
+    const T za = rk4_update(stage, h, ka, y + rp.ra * N + j, acc + rp.ra * N + j, active);
+    const T zb = rk4_update(stage, h, kb, y + rp.rb * N + j, acc + rp.rb * N + j, active);
+    z[0][m] = mk(za, zb);
+  }
+  if (!do_r2c) return;  // uniform over the grid
+  cplx<T>* const bufz[1] = {bufa};
+  fft_lines<N, false, 1>(z, bufz, tw, t);
+  split_store<N>(z[0], bufa, S0 + rp.ra * H, S0 + rp.rb * H, t, active);
+}
+
+// Row pass, adjoint rate.  init != 0: no rate, z = y (first stage input).
+// Otherwise C2R of (W_a + i W_b) gives the two rates, then RK4 update.  Then,
+// unless this is the very last stage, u = p_x(t_next) z and v = p_y(t_next) z
+// are row-R2C'd per row (as u + i v) into S0 (U) and S1 (V).
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_row_lf_adj(cplx<T>* __restrict__ S0,
+                                                    cplx<T>* __restrict__ S1,
+                                                    T* __restrict__ y, T* __restrict__ acc,
+                                                    const T* const* __restrict__ ptabs,
+                                                    const cplx<T>* __restrict__ twg, int C,
+                                                    long npairs, int stage, T h, int tidx_next,
+                                                    int do_r2c, int init) {
+  constexpr int G = N / 8, PAIRS = 256 / G, H = N / 2;
+  constexpr long NN = (long)N * N;
+  __shared__ cplx<T> tw[N];
+  __shared__ cplx<T> lbuf[PAIRS][2][lds_len<N>()];
+  load_twiddles<N>(tw, twg);
+  const int grp = threadIdx.x / G, t = threadIdx.x % G;
+  const long p = (long)blockIdx.x * PAIRS + grp;
+  const bool active = p < npairs;
+  const RowPair rp = row_pair<N>((int)(active ? p : npairs - 1), C);
+  cplx<T>* bufa = lbuf[grp][0];
+  cplx<T>* bufb = lbuf[grp][1];
+  T za[8], zb[8];
+  __syncthreads();
+  if (init) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      za[m] = y[rp.ra * N + t + m * G];
+      zb[m] = y[rp.rb * N + t + m * G];
+    }
+  } else {
+    cplx<T> v[1][8];
+    herm_pair_to_line<N>(S0 + rp.ra * H, S0 + rp.rb * H, bufa, v[0], t);
+    cplx<T>* const bufs[1] = {bufa};
+    fft_lines<N, true, 1>(v, bufs, tw, t);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int j = t + m * G;
+      za[m] = rk4_update(stage, h, v[0][m].x, y + rp.ra * N + j, acc + rp.ra * N + j, active);
+      zb[m] = rk4_update(stage, h, v[0][m].y, y + rp.rb * N + j, acc + rp.rb * N + j, active);
+    }
+  }
+  if (!do_r2c) return;  // uniform over the grid
+  const T* pt = ptabs[rp.b] + (long)tidx_next * 2 * NN;
+  cplx<T> uv[2][8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int j = t + m * G;
+    const long ja = (long)rp.ia * N + j, jb = (long)rp.ib * N + j;
+    uv[0][m] = mk(pt[ja] * za[m], pt[NN + ja] * za[m]);
+    uv[1][m] = mk(pt[jb] * zb[m], pt[NN + jb] * zb[m]);
+  }
+  cplx<T>* const bufs2[2] = {bufa, bufb};
+  fft_lines<N, false, 2>(uv, bufs2, tw, t);
+  split_store<N>(uv[0], bufa, S0 + rp.ra * H, S1 + rp.ra * H, t, active);
+  split_store<N>(uv[1], bufb, S0 + rp.rb * H, S1 + rp.rb * H, t, active);
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers (dispatch on N)
+// ---------------------------------------------------------------------------
+template <typename T>
+constexpr int col_kc(int N) {
+  return (sizeof(T) == 4 ? 16 : 8) < N / 2 ? (sizeof(T) == 4 ? 16 : 8) : N / 2;
+}
+
+#define CMBL_DISPATCH_N(N, ...)                                \
+  switch (N) {                                                 \
+    case 16: { constexpr int NN_ = 16; __VA_ARGS__; } break;   \
+    case 32: { constexpr int NN_ = 32; __VA_ARGS__; } break;   \
+    case 64: { constexpr int NN_ = 64; __VA_ARGS__; } break;   \
+    case 128: { constexpr int NN_ = 128; __VA_ARGS__; } break; \
+    case 256: { constexpr int NN_ = 256; __VA_ARGS__; } break; \
+    case 512: { constexpr int NN_ = 512; __VA_ARGS__; } break; \
+    default: return hipErrorInvalidValue;                      \
+  }
+
+static inline unsigned blocks_for(long n, long per) { return (unsigned)((n + per - 1) / per); }
+
+template <typename T>
+hipError_t launch_rfft2(int N, const T* x, cplx<T>* P, cplx<T>* X, long nmaps,
+                        const cplx<T>* tw, hipStream_t s) {
+  CMBL_DISPATCH_N(N, {
+    constexpr int KC = col_kc<T>(NN_);
+    const long npairs = nmaps * NN_ / 2;
+    k_row_r2c<T, NN_><<<blocks_for(npairs, 256 / (NN_ / 8)), 256, 0, s>>>(x, P, tw, npairs);
+    k_col_fwd_unpack<T, NN_, KC><<<(unsigned)(nmaps * (NN_ / 2 / KC)), KC * NN_ / 8, 0, s>>>(P, X, tw);
+  });
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_irfft2(int N, const cplx<T>* X, cplx<T>* P, T* x, long nmaps,
+                         const cplx<T>* tw, hipStream_t s) {
+  CMBL_DISPATCH_N(N, {
+    constexpr int KC = col_kc<T>(NN_);
+    const long npairs = nmaps * NN_ / 2;
+    const T sc = T(1) / (T(NN_) * T(NN_));
+    k_col_inv_pack<T, NN_, KC><<<(unsigned)(nmaps * (NN_ / 2 / KC)), KC * NN_ / 8, 0, s>>>(X, P, tw, sc);
+    k_row_c2r<T, NN_><<<blocks_for(npairs, 256 / (NN_ / 8)), 256, 0, s>>>(P, x, tw, npairs);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_phi_mult(int N, const cplx<double>* F, cplx<double>* D, double dk,
+                           hipStream_t s) {
+  const long n = (long)N * (N / 2 + 1);
+  k_phi_mult<<<blocks_for(n, 256), 256, 0, s>>>(F, D, N, dk);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_ptab(int N, const double* D, T* ptab, int nsteps, double* minpart,
+                       int* nblocks, hipStream_t s) {
+  const long P = (long)N * N;
+  *nblocks = (int)blocks_for(P, 256);
+  k_ptab<T><<<*nblocks, 256, 0, s>>>(D, ptab, N, nsteps, minpart);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_lf_row_r2c(int N, const T* y, cplx<T>* S0, long nmaps, const cplx<T>* tw,
+                             hipStream_t s) {
+  CMBL_DISPATCH_N(N, {
+    const long npairs = nmaps * NN_ / 2;
+    k_row_r2c<T, NN_><<<blocks_for(npairs, 256 / (NN_ / 8)), 256, 0, s>>>(y, S0, tw, npairs);
+  });
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_lf_col(int N, bool adjoint, cplx<T>* S0, cplx<T>* S1, long nmaps,
+                         const cplx<T>* tw, T dk, hipStream_t s) {
+  CMBL_DISPATCH_N(N, {
+    constexpr int KC = col_kc<T>(NN_);
+    const unsigned nb = (unsigned)(nmaps * (NN_ / 2 / KC));
+    if (adjoint)
+      k_col_lf_div<T, NN_, KC><<<nb, KC * NN_ / 8, 0, s>>>(S0, S1, tw, dk);
+    else
+      k_col_lf_grad<T, NN_, KC><<<nb, KC * NN_ / 8, 0, s>>>(S0, S1, tw, dk);
+  });
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_lf_row(int N, bool adjoint, cplx<T>* S0, cplx<T>* S1, T* y, T* acc,
+                         const T* const* ptabs, const cplx<T>* tw, int C, long nmaps, int stage,
+                         T h, int tidx, int do_r2c, int init, hipStream_t s) {
+  CMBL_DISPATCH_N(N, {
+    const long npairs = nmaps * NN_ / 2;
+    const unsigned nb = blocks_for(npairs, 256 / (NN_ / 8));
+    if (adjoint)
+      k_row_lf_adj<T, NN_><<<nb, 256, 0, s>>>(S0, S1, y, acc, ptabs, tw, C, npairs, stage, h, tidx,
+                                              do_r2c, init);
+    else
+      k_row_lf_std<T, NN_><<<nb, 256, 0, s>>>(S0, S1, y, acc, ptabs, tw, C, npairs, stage, h, tidx,
+                                              do_r2c);
+  });
+  return hipGetLastError();
+}
+
+#define CMBL_INST(T)                                                                            \
+  template hipError_t launch_rfft2<T>(int, const T*, cplx<T>*, cplx<T>*, long, const cplx<T>*,  \
+                                      hipStream_t);                                             \
+  template hipError_t launch_irfft2<T>(int, const cplx<T>*, cplx<T>*, T*, long, const cplx<T>*, \
+                                       hipStream_t);                                            \
+  template hipError_t launch_ptab<T>(int, const double*, T*, int, double*, int*, hipStream_t);  \
+  template hipError_t launch_lf_row_r2c<T>(int, const T*, cplx<T>*, long, const cplx<T>*,       \
+                                           hipStream_t);                                        \
+  template hipError_t launch_lf_col<T>(int, bool, cplx<T>*, cplx<T>*, long, const cplx<T>*, T,  \
+                                       hipStream_t);                                            \
+  template hipError_t launch_lf_row<T>(int, bool, cplx<T>*, cplx<T>*, T*, T*, const T* const*,  \
+                                       const cplx<T>*, int, long, int, T, int, int, int,        \
+                                       hipStream_t);
+CMBL_INST(float)
+CMBL_INST(double)
+
+}  // namespace cmbl

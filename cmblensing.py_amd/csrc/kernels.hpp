@@ -1,0 +1,38 @@
+// Host-side launchers of the HIP kernels in kernels.hip (internal to the
+// library; the public C ABI is include/cmbl.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fft_lds.hpp"
+
+namespace cmbl {
+
+// rfft2 of nmaps real N x N maps -> natural half planes X[nmaps][N][N/2+1].
+// P is packed scratch [nmaps][N][N/2].
+template <typename T>
+hipError_t launch_rfft2(int N, const T* x, cplx<T>* P, cplx<T>* X, long nmaps,
+                        const cplx<T>* tw, hipStream_t s);
+// irfft2 (numpy semantics, 1/N^2) of natural half planes -> real maps.
+template <typename T>
+hipError_t launch_irfft2(int N, const cplx<T>* X, cplx<T>* P, T* x, long nmaps,
+                         const cplx<T>* tw, hipStream_t s);
+// Five derivative spectra of F (double precision, natural layout).
+hipError_t launch_phi_mult(int N, const cplx<double>* F, cplx<double>* D, double dk,
+                           hipStream_t s);
+// Deflection-operator table and per-block minimum of det(I + t Hess phi).
+template <typename T>
+hipError_t launch_ptab(int N, const double* D, T* ptab, int nsteps, double* minpart,
+                       int* nblocks, hipStream_t s);
+// LenseFlow stage passes.
+template <typename T>
+hipError_t launch_lf_row_r2c(int N, const T* y, cplx<T>* S0, long nmaps, const cplx<T>* tw,
+                             hipStream_t s);
+template <typename T>
+hipError_t launch_lf_col(int N, bool adjoint, cplx<T>* S0, cplx<T>* S1, long nmaps,
+                         const cplx<T>* tw, T dk, hipStream_t s);
+template <typename T>
+hipError_t launch_lf_row(int N, bool adjoint, cplx<T>* S0, cplx<T>* S1, T* y, T* acc,
+                         const T* const* ptabs, const cplx<T>* tw, int C, long nmaps, int stage,
+                         T h, int tidx, int do_r2c, int init, hipStream_t s);
+
+}  // namespace cmbl

@@ -1,0 +1,94 @@
+/*
+ * cmbl.h -- C ABI of the MI355X-native LenseFlow library (libcmbl.so).
+ *
+ * The reference (CMBLensing.py, a JAX package) has no FFI: its boundary is the
+ * Python call surface.  Each entry point below replaces the compute behind one
+ * reference function; the Python package cmblensing.py_amd binds them with
+ * ctypes and keeps the reference's names, argument order and defaults.
+ *
+ * Conventions
+ *   - Pointers named in/out/x/X are DEVICE pointers owned by the caller; the
+ *     library never frees caller memory.  Plans and phi handles are owned by
+ *     the library and freed explicitly.
+ *   - MAP arrays are row-major [batch][comp][N][N] real; FOURIER arrays are
+ *     [batch][comp][N][N/2+1] interleaved complex (numpy rfft2 layout, axis 0
+ *     carries KX, axis 1 the rfft axis KY).  dtype is CMBL_F32 or CMBL_F64 as
+ *     fixed by the plan (phi input is always float64).
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *     Work is stream-ordered; calls return after enqueueing unless stated.
+ *   - Every call returns 0 on success or a nonzero status; cmbl_last_error()
+ *     gives a message (thread-local).  No C++ exception crosses the ABI.
+ *   - A plan is not re-entrant: concurrent calls on one plan from different
+ *     streams race on its scratch.  phi handles are read-only after prepare.
+ */
+#ifndef CMBL_H
+#define CMBL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cmbl_plan cmbl_plan;
+typedef struct cmbl_phi cmbl_phi;
+typedef int cmbl_status;
+
+enum { CMBL_OK = 0, CMBL_EINVAL = 1, CMBL_EHIP = 2, CMBL_ENOMEM = 3, CMBL_ECAUSTIC = 4 };
+enum { CMBL_F32 = 0, CMBL_F64 = 1 };
+enum { CMBL_PHI_MAP = 0, CMBL_PHI_FOURIER = 1 };
+enum { CMBL_FORWARD_LENSE = 1, CMBL_INVERSE_LENSE = -1 }; /* constants.py:7-8 */
+
+/* Library version string. */
+const char* cmbl_version(void);
+/* Message of the last failing call on this thread. */
+const char* cmbl_last_error(void);
+
+/* Plan: geometry (N, pixel size in arcmin), arithmetic type and the largest
+ * batch*ncomp it will be asked to process; allocates the stage scratch.
+ * Replaces the static field metadata of Field (fields.py:15-22) plus the
+ * per-call setup of rk4_solve (util.py:184-199).  N in {16,...,512}, 2^m. */
+cmbl_status cmbl_plan_create(cmbl_plan** out, int N, double theta_pix_arcmin, int dtype,
+                             int max_maps);
+cmbl_status cmbl_plan_destroy(cmbl_plan* plan);
+
+/* phi preparation: the five derivatives of phi (util.py:110-130, in float64)
+ * and the table p(t_j) = (I + t_j Hess phi)^-1 grad phi for t_j = j/(2n),
+ * j = 0..2n (lense_flow.py:372-381, 397-415), stored in the plan dtype.
+ * phi is float64: MAP [N][N] or FOURIER [N][N/2+1] complex (lense_flow.py:299).
+ * min_det (host out, may be NULL) receives min over pixels and t in [0,1] of
+ * det(I + t Hess phi).  Synchronises `stream`. */
+cmbl_status cmbl_phi_prepare(cmbl_plan* plan, const void* phi, int phi_basis, int nsteps,
+                             cmbl_phi** out, double* min_det, void* stream);
+cmbl_status cmbl_phi_release(cmbl_phi* phi);
+/* Device pointer of the phi derivative maps, float64 [5][N][N]
+ * (phi_x, phi_y, phi_xx, phi_xy, phi_yy). */
+const void* cmbl_phi_derivatives(const cmbl_phi* phi);
+
+/* LenseFlow of `batch` fields of `ncomp` components each (1: T, 2: Q,U).
+ * phis[b] (host array of handles) is the phi of batch element b; all must be
+ * prepared with this plan and nsteps.  direction = +1 / -1, adjoint = 0 / 1
+ * follow the truth table of lense_flow.py:317-322.  in may equal out.
+ * Replaces lense_flow / primal_lense_flow (lense_flow.py:289-333). */
+cmbl_status cmbl_lenseflow(cmbl_plan* plan, const cmbl_phi* const* phis, const void* in,
+                           void* out, int batch, int ncomp, int nsteps, int direction,
+                           int adjoint, void* stream);
+
+/* rfft2 / irfft2 of nmaps maps in the plan dtype with numpy semantics
+ * (fields.py:158-172: fourier / map). */
+cmbl_status cmbl_rfft2(cmbl_plan* plan, const void* x, void* X, int nmaps, void* stream);
+cmbl_status cmbl_irfft2(cmbl_plan* plan, const void* X, void* x, int nmaps, void* stream);
+
+/* Timing mode (no counterpart in the reference; measurement aid): when on,
+ * every LenseFlow stage launch is bracketed by HIP events on the call's stream
+ * and cmbl_lenseflow waits for its own work before returning.
+ * cmbl_plan_get_timing writes out[6] = {column-pass ms, launches, row-pass ms,
+ * launches, other ms, launches}, accumulated since the last reset. */
+cmbl_status cmbl_plan_set_timing(cmbl_plan* plan, int enable);
+cmbl_status cmbl_plan_get_timing(cmbl_plan* plan, double* out, int reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CMBL_H */

@@ -1,0 +1,140 @@
+"""FP64 numpy restatement of CMBLensing.py's gradf_logpdf, logpdf and the
+preconditioned-CG Wiener filter (oracle).
+
+TEST INFRASTRUCTURE ONLY -- imported by tests/ and bench.py's cpu_baseline leg.
+
+Fields are FOURIER arrays [C, N, N//2+1] complex128 with C=1 (T, FlatS0) or
+C=2 (E, B of a FlatS2 in the EB parametrisation).  Diagonal operators
+(DiagonalScalar / DiagonalEB, matrix_operators.py:45-90) are real arrays of
+the same shape, applied per mode.  ``theta_pix`` is in arcmin.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import lenseflow as lf
+
+
+def pinv(op):
+    """Nonzero-safe reciprocal (util.py:45-47; matrix_operators.py:208-219)."""
+    op = np.asarray(op)
+    out = np.zeros_like(op)
+    nz = op != 0
+    out[nz] = 1.0 / op[nz]
+    return out
+
+
+def log_det(op, N):
+    """Weighted log-determinant of a diagonal operator (util.py:39-42;
+    matrix_operators.py:179-190: DiagonalEB sums both blocks)."""
+    w = lf.fourier_weights(N)
+    total = 0.0
+    for m in np.asarray(op).reshape(-1, *np.shape(op)[-2:]):
+        val = np.sum(np.where(m != 0, np.log(np.abs(np.where(m != 0, m, 1))), 0) * w)
+        sign = np.prod(np.sign(np.where(m != 0, m, 1)))
+        total = total + sign * val
+    return total
+
+
+def eb2qu_field(f, N, theta_pix):
+    """fields.py:182-199: T passes through, (E,B) -> (Q,U)."""
+    if f.shape[0] == 1:
+        return f
+    q, u = lf.eb2qu(f[0], f[1], N, theta_pix)
+    return np.stack([q, u])
+
+
+def qu2eb_field(f, N, theta_pix):
+    """fields.py:209-226."""
+    if f.shape[0] == 1:
+        return f
+    e, b = lf.qu2eb(f[0], f[1], N, theta_pix)
+    return np.stack([e, b])
+
+
+def to_map(f):
+    """fields.py:167-172 (irfft2 of every component)."""
+    return np.fft.irfft2(f)
+
+
+def to_fourier(f):
+    """fields.py:158-163."""
+    return np.fft.rfft2(f)
+
+
+def phi_map(phi):
+    """map(phi) for a FOURIER phi (gradients.py:17); MAP phi passes through."""
+    return phi if phi.shape[-2] == phi.shape[-1] else np.fft.irfft2(phi)
+
+
+def gradf_logpdf(f, phi, d, Cf, Cn, M, B, theta_pix, n=10):
+    """d logpdf / d f = L^T B M Cn^-1 (d - M B L f) - Cf^-1 f (gradients.py:12-26)."""
+    N = f.shape[-2]
+    pw = lf.pix_width_from_theta(theta_pix)
+    fq = to_map(eb2qu_field(f, N, theta_pix))                       # :16
+    phim = phi_map(phi)                                             # :17
+    lensed = qu2eb_field(to_fourier(lf.lense_flow(fq, phim, pw, n)), N, theta_pix)  # :18
+    g = B * M * pinv(Cn) * (d - M * B * lensed)                     # :20
+    g = to_map(eb2qu_field(g, N, theta_pix))                        # :22
+    g = lf.lense_flow(g, phim, pw, n, adjoint=True)                 # :23
+    return (qu2eb_field(to_fourier(g), N, theta_pix)
+            - pinv(Cf) * qu2eb_field(to_fourier(fq), N, theta_pix))  # :25
+
+
+def logpdf(f, phi, d, Cn, Cphi, Cf, M, B, theta_pix, n=10):
+    """statistics.py:138-165 (note the argument order differs from gradf_logpdf)."""
+    N = f.shape[-2]
+    pw = lf.pix_width_from_theta(theta_pix)
+    logdets = log_det(Cphi, N) + log_det(Cn, N) + log_det(Cf, N)    # :143-145
+    field_product = lf.field_dot(f, pinv(Cf) * f, N)                # :148
+    phi_product = lf.primal_dot(phi, pinv(Cphi) * phi, N)           # :149 / :185-186
+    fq = to_map(eb2qu_field(f, N, theta_pix))                       # :155
+    lensed = qu2eb_field(to_fourier(lf.lense_flow(fq, phi, pw, n)), N, theta_pix)  # :156
+    diff = d - M * B * lensed                                       # :159
+    data_product = lf.field_dot(diff, pinv(Cn) * diff, N)           # :160
+    return -np.real(data_product + field_product + phi_product + logdets) / 2   # :164-165
+
+
+def preconditioner(Cf, B, M, Cn):
+    """P = Cf^-1 + B M Cn^-1 M B (wiener_filter.py:35-43)."""
+    return pinv(Cf) + B * M * pinv(Cn) * M * B
+
+
+def conjugate_gradient(x0, d, phi, M, B, Cn, Cf, b, P, theta_pix, maxiter=500, tol=1e-1,
+                       n=10, history=None):
+    """PCG exactly as wiener_filter.py:51-105 (absolute stop on dot(r, P^-1 r))."""
+    N = x0.shape[-2]
+
+    def A(x):                                                       # :45-49, :57-59
+        return gradf_logpdf(x, phi, 0 * x, Cf, Cn, M, B, theta_pix, n)
+
+    Pinv = pinv(P)                                                  # :63
+    r = b - A(x0)                                                   # :61-62
+    z = Pinv * r                                                    # :65
+    p = z
+    res = lf.field_dot(r, z, N)                                     # :69
+    res_curr = res
+    x = x0
+    k = 0
+    while res_curr >= tol and k < maxiter:                          # :75-77
+        Ap = A(p)                                                   # :82
+        alpha = res / lf.field_dot(p, Ap, N)                        # :84
+        x = x + alpha * p                                           # :86
+        r = r - alpha * Ap                                          # :88
+        z = Pinv * r                                                # :90
+        res_curr = lf.field_dot(r, z, N)                            # :92
+        p = z + (res_curr / res) * p                                # :94-95
+        res = res_curr
+        k += 1
+        if history is not None:
+            history.append(res_curr)
+    return x
+
+
+def wiener_filter(x0, phi, d, Cf, Cn, M, B, theta_pix, maxiter=500, tol=1e-1, n=10,
+                  history=None):
+    """wiener_filter.py:10-26 (b = -gradf(0, d); P as above; PCG)."""
+    b = -gradf_logpdf(0 * d, phi, d, Cf, Cn, M, B, theta_pix, n)    # :28-33
+    P = preconditioner(Cf, B, M, Cn)
+    return conjugate_gradient(x0, d, phi, M, B, Cn, Cf, b, P, theta_pix, maxiter, tol, n,
+                              history)

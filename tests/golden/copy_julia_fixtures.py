@@ -1,0 +1,43 @@
+"""Copy the subset of the reference's Julia (CMBLensing.jl) ground-truth data
+used by these tests into tests/golden/julia/ as plain .npy files.
+
+The reference stores them as .npy payloads with a .npz name
+(tests/ground_truth_data/*.npz, written by tests/generate_julia_data/*.py with
+seed 67, Nside 256, theta_pix 2.5 arcmin, Float64).  They are data, loaded with
+numpy.load(allow_pickle=False).
+
+    python tests/golden/copy_julia_fixtures.py [/path/to/reference]
+"""
+import shutil
+import sys
+from pathlib import Path
+
+import numpy as np
+
+NAMES = [
+    # lensing, intensity and polarisation (test_lensing.py)
+    "phi_I", "t_field_I", "t_lensed_field_I", "t_adjoint_lensed_field_I",
+    "phi_P", "e_field_P", "b_field_P", "e_lensed_field_P", "b_lensed_field_P",
+    "e_adjoint_lensed_field_P", "b_adjoint_lensed_field_P",
+    # gradients / logpdf / Wiener filter, intensity (test_gradients.py, test_logpdf.py,
+    # test_wiener_filter.py)
+    "data_t_field_I", "cn_I", "cf_I", "m_I", "b_I", "cphi_I", "gradf_I", "grad_phi_I",
+    "f_wf_ffpp_I", "f_wf_f0pp_I", "f_wf_f0p0_I",
+]
+TEXT = ["logpdf_I.txt", "logpdf_P.txt", "n_side.txt", "theta_pix.txt"]
+
+
+def main(ref_root="/root/reference"):
+    src = Path(ref_root) / "tests" / "ground_truth_data"
+    dst = Path(__file__).resolve().parent / "julia"
+    dst.mkdir(exist_ok=True)
+    for n in NAMES:
+        a = np.load(src / f"{n}.npz", allow_pickle=False)
+        np.save(dst / f"{n}.npy", a, allow_pickle=False)
+    for t in TEXT:
+        shutil.copyfile(src / t, dst / t)
+    print(f"copied {len(NAMES) + len(TEXT)} files to {dst}")
+
+
+if __name__ == "__main__":
+    main(*sys.argv[1:])

@@ -1,0 +1,220 @@
+"""Pins of the FP64 oracle (CPU): the reference's Julia fixtures, closed forms,
+brute force and invariants, each chosen so a plausible mistake (a sign, an
+index, a dropped term) fails one of them."""
+import numpy as np
+import pytest
+
+from oracle import lenseflow as olf
+from oracle import wiener as owf
+
+from conftest import GOLDEN
+
+J = GOLDEN / "julia"
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def jl(name):
+    return np.load(J / f"{name}.npy", allow_pickle=False)
+
+
+# ---------------------------------------------------------------- FFT pins
+def test_rfft2_matches_bruteforce_dft():
+    N = 8
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((N, N))
+    i = np.arange(N)
+    W = np.exp(-2j * np.pi * np.outer(i, i) / N)
+    full = W @ x @ W.T
+    assert np.allclose(np.fft.rfft2(x), full[:, : N // 2 + 1], atol=1e-12)
+    # irfft2 ignores the imaginary parts of self-conjugate bins
+    X = np.fft.rfft2(x)
+    X2 = X.copy()
+    X2[0, 0] += 3j
+    X2[0, N // 2] += 2j
+    assert np.allclose(np.fft.irfft2(X2), x, atol=1e-12)
+
+
+# ------------------------------------------------------- derivative pins
+def test_derivatives_of_plane_wave():
+    N, th = 32, 2.0
+    pw = olf.pix_width_from_theta(th)
+    a, b = 3, 5
+    x = np.arange(N)[:, None] * pw
+    y = np.arange(N)[None, :] * pw
+    kx, ky = 2 * np.pi * a / (N * pw), 2 * np.pi * b / (N * pw)
+    f = np.cos(kx * x + ky * y)
+    s = np.sin(kx * x + ky * y)
+    fx, fy, fxx, fxy, fyy = olf.primal_derivatives(f, pw)
+    assert np.allclose(fx, -kx * s, atol=1e-9 * kx)
+    assert np.allclose(fy, -ky * s, atol=1e-9 * ky)
+    assert np.allclose(fxx, -kx * kx * f, atol=1e-9 * kx * kx)
+    assert np.allclose(fxy, -kx * ky * f, atol=1e-9 * kx * ky)
+    assert np.allclose(fyy, -ky * ky * f, atol=1e-9 * ky * ky)
+
+
+def test_nyquist_ky_sign_from_fourier():
+    """util.py:106 negates the last KY entry; a purely imaginary coefficient on
+    the self-conjugate (0, N/2) bin then has d/dy = +(pi/Delta) (-1)^y / N^2."""
+    N, th = 16, 2.0
+    pw = olf.pix_width_from_theta(th)
+    F = np.zeros((N, N // 2 + 1), complex)
+    F[0, N // 2] = 1j
+    fy = olf.primal_derivatives_from_fourier(F, pw)[1]
+    expect = (np.pi / pw) * (-1.0) ** np.arange(N)[None, :] / N ** 2 * np.ones((N, 1))
+    assert np.allclose(fy, expect, rtol=1e-12, atol=0)
+
+
+def test_inverse_matrix_closed_form():
+    rng = np.random.default_rng(1)
+    a, b, c = rng.uniform(0.5, 1.5, 3)
+    ixx, ixy, iyy = olf.inverse_matrix(a, b, c, eps=0.0)
+    M = np.array([[a, b], [b, c]])
+    assert np.allclose(np.array([[ixx, ixy], [ixy, iyy]]) @ M, np.eye(2), atol=1e-14)
+
+
+# ------------------------------------------------------------- RK4 pins
+def _small_inputs(N=32, th=3.0, seed=0):
+    import importlib.util
+    from conftest import ROOT
+    spec = importlib.util.spec_from_file_location("s", ROOT / "cmblensing.py_amd" / "sims.py")
+    sims = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(sims)
+    return sims.t_map(N, th, seed)[0], sims.phi_map(N, th, seed + 1), th
+
+
+def test_phi_zero_is_identity_and_constant_unchanged():
+    f, phi, th = _small_inputs()
+    pw = olf.pix_width_from_theta(th)
+    for d, adj in [(1, False), (-1, False), (1, True), (-1, True)]:
+        assert np.array_equal(olf.lense_flow(f, np.zeros_like(phi), pw, 5, d, adj), f)
+        c = np.full_like(f, 3.25)
+        if not adj:  # the divergence form does not preserve constants
+            assert np.allclose(olf.lense_flow(c, phi, pw, 5, d, adj), c, atol=1e-12)
+
+
+def test_rk4_fourth_order_round_trip():
+    f, phi, th = _small_inputs()
+    pw = olf.pix_width_from_theta(th)
+    errs = []
+    for n in (4, 8, 16):
+        lf_ = olf.lense_flow(f, phi, pw, n)
+        back = olf.lense_flow(lf_, phi, pw, n, direction=-1)
+        errs.append(rel(back, f))
+    r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
+    # 4th order: ~16x per halving (pre-asymptotic up to ~32x); order <= 3 gives <= 8x
+    assert 10 < r1 < 64 and 10 < r2 < 64, errs
+
+
+def test_adjoint_is_exact_discrete_transpose():
+    """RK4 with stage times (0, 1/2, 1/2, 1) applied backwards to the adjoint
+    rate div(p g) is the exact transpose of the forward RK4 map (the stage
+    polynomials match term by term and the spectral derivative is exactly
+    antisymmetric), so <L f, g> = <f, L^T g> to rounding for every n."""
+    f, phi, th = _small_inputs(seed=3)
+    g, _, _ = _small_inputs(seed=7)
+    pw = olf.pix_width_from_theta(th)
+    for n in (2, 5, 7):
+        for d in (1, -1):
+            lhs = np.sum(olf.lense_flow(f, phi, pw, n, d) * g)
+            rhs = np.sum(f * olf.lense_flow(g, phi, pw, n, d, adjoint=True))
+            assert abs(lhs - rhs) < 1e-13 * abs(lhs), (n, d, lhs, rhs)
+    # and a wrong-sign adjoint rate is far from it
+    bad = olf.rk4_solve(lambda t, y, a: -olf.adjoint_term(y, olf.phi_derivatives(phi, pw), t, pw),
+                        g, 1.0, 0.0, 7)
+    assert abs(np.sum(olf.lense_flow(f, phi, pw, 7) * g) - np.sum(f * bad)) > 1e-3 * abs(lhs)
+
+
+def test_continuum_theorem_nonuniform_dft():
+    """Lf(x) = f(x + grad phi(x)) for band-limited f and smooth phi, evaluated
+    by a brute-force non-uniform DFT of f's Fourier series."""
+    N, th = 32, 3.0
+    pw = olf.pix_width_from_theta(th)
+    rng = np.random.default_rng(4)
+    F = np.zeros((N, N // 2 + 1), complex)
+    for kx in range(-3, 4):
+        for ky in range(0, 4):
+            F[kx % N, ky] = rng.standard_normal() + 1j * rng.standard_normal()
+    f = np.fft.irfft2(F)
+    P = np.zeros((N, N // 2 + 1), complex)
+    P[1, 1], P[N - 2, 1], P[2, 0] = 1.0, 0.6j, 0.4
+    phi = np.fft.irfft2(P)
+    gx, gy = olf.primal_gradient(phi, pw)
+    phi *= 0.5 * pw / np.sqrt(np.mean(gx ** 2 + gy ** 2))   # 0.5 pixel RMS deflection
+    gx, gy = olf.primal_gradient(phi, pw)
+    Lf = olf.lense_flow(f, phi, pw, 32)
+    # exact: evaluate the Fourier series of f at displaced points
+    Ff = np.fft.fft2(f) / N ** 2
+    k = np.fft.fftfreq(N, pw) * 2 * np.pi
+    X = np.arange(N)[:, None] * pw + gx
+    Y = np.arange(N)[None, :] * pw + gy
+    exact = np.zeros((N, N))
+    for a in range(N):
+        for b in range(N):
+            if Ff[a, b] != 0 and abs(Ff[a, b]) > 1e-14:
+                exact += np.real(Ff[a, b] * np.exp(1j * (k[a] * X + k[b] * Y)))
+    assert rel(Lf, exact) < 2e-3
+
+
+# --------------------------------------------------- Julia fixture pins
+def test_julia_forward_and_adjoint_T():
+    phi = np.fft.irfft2(jl("phi_I"))
+    t = np.fft.irfft2(jl("t_field_I"))
+    pw = olf.pix_width_from_theta(2.5)
+    assert rel(olf.lense_flow(t, phi, pw, 7), np.fft.irfft2(jl("t_lensed_field_I"))) < 1e-12
+    assert rel(olf.lense_flow(t, phi, pw, 7, adjoint=True),
+               np.fft.irfft2(jl("t_adjoint_lensed_field_I"))) < 1e-12
+
+
+def test_julia_forward_and_adjoint_P():
+    N = 256
+    phi = np.fft.irfft2(jl("phi_P"))
+    pw = olf.pix_width_from_theta(2.5)
+
+    def qu(e, b):
+        q, u = olf.eb2qu(jl(e), jl(b), N, 2.5)
+        return np.fft.irfft2(np.stack([q, u]))
+    f = qu("e_field_P", "b_field_P")
+    assert rel(olf.lense_flow(f, phi, pw, 7), qu("e_lensed_field_P", "b_lensed_field_P")) < 1e-12
+    assert rel(olf.lense_flow(f, phi, pw, 7, adjoint=True),
+               qu("e_adjoint_lensed_field_P", "b_adjoint_lensed_field_P")) < 1e-12
+
+
+def _ops_I():
+    return dict(d=jl("data_t_field_I")[None], Cn=jl("cn_I")[None], Cf=jl("cf_I")[None],
+                M=jl("m_I")[None], B=jl("b_I")[None])
+
+
+def test_julia_logpdf_I():
+    o = _ops_I()
+    lp = owf.logpdf(jl("t_field_I")[None], jl("phi_I"), o["d"], o["Cn"], jl("cphi_I"), o["Cf"],
+                    o["M"], o["B"], 2.5, n=7)
+    ref = float((J / "logpdf_I.txt").read_text())
+    assert abs(lp - ref) < 1e-6 * abs(ref)
+
+
+def test_julia_gradf_I_interior_modes():
+    """Interior modes (0 < ky < N/2) agree to ~1e-11; the self-conjugate ky
+    columns differ by CMBLensing.jl's real-FFT gradient convention."""
+    o = _ops_I()
+    g = owf.gradf_logpdf(jl("t_field_I")[None], jl("phi_I"), o["d"], o["Cf"], o["Cn"], o["M"],
+                         o["B"], 2.5, n=7)[0]
+    gj = jl("gradf_I")
+    assert rel(g[:, 1:-1], gj[:, 1:-1]) < 1e-9
+
+
+def test_julia_grad_phi_I():
+    """Continuous-adjoint gradient ODE vs CMBLensing.jl (reference Python
+    agrees with Julia at the ~1e-3 level on this quantity)."""
+    o = _ops_I()
+    phi = jl("phi_I")
+    pw = olf.pix_width_from_theta(2.5)
+    fq = np.fft.irfft2(jl("t_field_I")[None])
+    Lf = olf.lense_flow(fq, phi, pw, 7)
+    diff = o["d"] - o["M"] * o["B"] * np.fft.rfft2(Lf)
+    cot = np.fft.irfft2(o["M"] * o["B"] * owf.pinv(o["Cn"]) * diff)
+    _, _, dphi = olf.lensing_operator_gradients(phi, list(Lf), list(cot), -1, 7, pw)
+    g = dphi - owf.pinv(jl("cphi_I")) * phi
+    assert rel(g, jl("grad_phi_I")) < 2e-3
