@@ -44,8 +44,9 @@ def row_bytes_per_launch(B, N, C, s):
 
 
 def col_bytes_per_launch(B, N, C, s):
-    """Fused column pass: per component one packed half plane read, two written."""
-    return B * C * 3 * N * N * s
+    """x-derivative column pass: per component one packed half plane (N^2 s
+    bytes) read and one written."""
+    return B * C * 2 * N * N * s
 
 
 def cpu_baseline(f, phi, th, n, budget_s=10.0):
@@ -67,7 +68,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--batch", type=int, default=128)
     ap.add_argument("--plain-requests", type=int, default=40)
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -191,7 +192,7 @@ def main():
                      "frac": round(row_gbs / HBM_PEAK_GBS, 4), "traffic": None,
                      "avg_launch_us": round(row_avg_ms * 1e3, 3),
                      "algorithmic_bytes_per_launch": row_bytes},
-        "column_pass": {"kernel": "k_col_lf_grad<float,256,16>",
+        "column_pass": {"kernel": "k_col_lf_dx<float,256,16>",
                         "achieved_gbs": round(col_gbs, 1),
                         "avg_launch_us": round(col_avg_ms * 1e3, 3),
                         "algorithmic_bytes_per_launch": col_bytes},

@@ -6,6 +6,7 @@ visible, every entry point raises -- there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes
+import os
 import threading
 from collections import OrderedDict
 from pathlib import Path
@@ -13,7 +14,8 @@ from pathlib import Path
 import torch
 
 PKG = Path(__file__).resolve().parent
-LIB_PATH = PKG / "libcmbl.so"
+# CMBL_LIBRARY may point at another build of the same library (A/B builds)
+LIB_PATH = Path(os.environ.get("CMBL_LIBRARY", str(PKG / "libcmbl.so")))
 
 CMBL_F32, CMBL_F64 = 0, 1
 CMBL_PHI_MAP, CMBL_PHI_FOURIER = 0, 1

@@ -356,12 +356,14 @@ cmbl_status run_lenseflow(cmbl_plan* p, const void* const* ptabs, T* y, long nma
     TIMED(2, cmbl::launch_lf_row_r2c<T>(N, y, S0, nmaps, tw, s));
   } else {
     TIMED(2, cmbl::launch_lf_row<T>(N, true, S0, S1, y, acc, pt, tw, C, nmaps, 1, h,
-                                    tindex(0, 0), 1, 1, s));
+                                    tindex(0, 0), 1, 1, (T)p->dk, s));
   }
   for (int i = 0; i < n; ++i) {
     for (int st = 1; st <= 4; ++st) {
       const bool last = (i == n - 1) && (st == 4);
-      TIMED(0, cmbl::launch_lf_col<T>(N, adj, S0, S1, nmaps, tw, (T)p->dk, s));
+      // standard rate: d/dx of S0 into S1 (S0 is kept for d/dy); adjoint:
+      // d/dx of U in place (V in S1 supplies the y part)
+      TIMED(0, cmbl::launch_lf_col<T>(N, S0, adj ? S0 : S1, nmaps, tw, (T)p->dk, s));
       int tidx;
       if (!adj) {
         tidx = tindex(i, stage_half[st]);
@@ -372,7 +374,7 @@ cmbl_status run_lenseflow(cmbl_plan* p, const void* const* ptabs, T* y, long nma
         if (last) tidx = 0;
       }
       TIMED(1, cmbl::launch_lf_row<T>(N, adj, S0, S1, y, acc, pt, tw, C, nmaps, st, h, tidx,
-                                      last ? 0 : 1, 0, s));
+                                      last ? 0 : 1, 0, (T)p->dk, s));
     }
   }
   return tm.flush();

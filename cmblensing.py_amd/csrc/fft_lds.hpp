@@ -11,8 +11,9 @@
 //
 // Pass plan: floor(m/3) radix-8 passes, then one radix-2 or radix-4 pass when
 // m is not a multiple of 3 (each thread then runs 4 or 2 small butterflies).
-// Twiddles come from a table tw[k] = exp(-2 pi i k / N) held in LDS, computed
-// on the host in double precision and rounded once to T.
+// Twiddles come from a table tw[k] = exp(-2 pi i k / N) computed on the host
+// in double precision and rounded once to T; each thread keeps the ones it
+// needs in registers (Twiddles<N, T>).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -69,6 +70,19 @@ __host__ __device__ constexpr int lds_len() {
 }
 __device__ __forceinline__ int pad_idx(int i) { return i + (i >> 3); }
 
+// Stride between the LDS buffers of neighbouring lines.  Kernels whose
+// consecutive lanes work on different lines (the column passes) would put all
+// lines on one bank if the stride were a multiple of the bank count; rounding
+// lds_len up to 32 complex and adding 1 spreads 16 float2 (or 8 double2) lines
+// over distinct banks.
+#ifndef CMBL_LINE_PAD
+#define CMBL_LINE_PAD 1
+#endif
+template <int N>
+__host__ __device__ constexpr int line_stride() {
+  return ((lds_len<N>() + 31) / 32) * 32 + CMBL_LINE_PAD;
+}
+
 template <int N>
 __host__ __device__ constexpr int ilog2() {
   return N <= 1 ? 0 : 1 + ilog2<N / 2>();
@@ -113,27 +127,66 @@ __device__ __forceinline__ void dft8(cplx<T>* v) {
   v[7] = b3;
 }
 
+// Pass plan of a length-N line.
+template <int N>
+struct Passes {
+  static constexpr int M = ilog2<N>();
+  static constexpr int P8 = M / 3;                       // radix-8 passes
+  static constexpr int REM = M % 3;                      // trailing radix-2^REM pass
+  static constexpr int R = 1 << REM;                     // its radix (1: none)
+  static constexpr int NB = REM ? 8 / R : 0;             // its butterflies per thread
+  static constexpr int G = N / 8;                        // threads per line
+  static constexpr int NTW = 7 * (P8 - 1) + NB * (R - 1);  // twiddles per thread
+};
+
+// The twiddles one thread needs across all passes of a forward transform
+// (an inverse transform uses their conjugates), read once from the table
+// tw[k] = exp(-2 pi i k / N) in global memory and kept in registers, so
+// every transform the thread runs reuses them.
+template <int N, typename T>
+struct Twiddles {
+  cplx<T> w[Passes<N>::NTW > 0 ? Passes<N>::NTW : 1];
+  __device__ __forceinline__ void load(const cplx<T>* __restrict__ tw, int t) {
+    using P = Passes<N>;
+    int i = 0;
+    int Ns = 8;
+#pragma unroll
+    for (int p = 1; p < P::P8; ++p) {
+      const int k = t & (Ns - 1);
+#pragma unroll
+      for (int q = 1; q < 8; ++q) w[i++] = tw[(q * k) * (N / (Ns * 8))];
+      Ns *= 8;
+    }
+    if constexpr (P::REM != 0) {
+#pragma unroll
+      for (int b = 0; b < P::NB; ++b) {
+        const int k = (t + b * P::G) & (Ns - 1);
+#pragma unroll
+        for (int q = 1; q < P::R; ++q) w[i++] = tw[(q * k) * (N / (Ns * P::R))];
+      }
+    }
+  }
+};
+
 template <bool INV, typename T>
-__device__ __forceinline__ cplx<T> twiddle(const cplx<T>* tw, int idx) {
-  cplx<T> w = tw[idx];
-  return INV ? conj(w) : w;
+__device__ __forceinline__ cplx<T> cmul_tw(cplx<T> a, cplx<T> w) {
+  return INV ? cmulc(a, w) : cmul(a, w);
 }
 
 // NL independent lines, each handled by the same group of N/8 threads
 // (thread index t within the group).  bufs[l] points to that line's LDS
-// buffer (lds_len<N>() entries), tw to the LDS twiddle table (N entries).
-// Every thread of the workgroup must call this the same number of times:
-// the passes synchronise with __syncthreads().
+// buffer (lds_len<N>() entries), tw to the thread's twiddles.  Every thread
+// of the workgroup must call this the same number of times: the passes
+// synchronise with __syncthreads().
 template <int N, bool INV, int NL, typename T>
 __device__ __forceinline__ void fft_lines(cplx<T> (&v)[NL][8], cplx<T>* const (&bufs)[NL],
-                                          const cplx<T>* tw, int t) {
-  constexpr int M = ilog2<N>();
-  constexpr int P8 = M / 3;      // radix-8 passes
-  constexpr int REM = M % 3;     // trailing radix-2^REM pass (0: none)
-  constexpr int G = N / 8;       // threads per line
+                                          const Twiddles<N, T>& tw, int t) {
+  using P = Passes<N>;
+  constexpr int P8 = P::P8, REM = P::REM, G = P::G;
   static_assert(N >= 16 && N <= 512 && (N & (N - 1)) == 0, "N must be 2^m, 16..512");
 
   int Ns = 1;
+  int wi = 0;
 #pragma unroll
   for (int p = 0; p < P8; ++p) {
     // radix-8 butterfly j = t on points t + q*N/8
@@ -141,9 +194,9 @@ __device__ __forceinline__ void fft_lines(cplx<T> (&v)[NL][8], cplx<T>* const (&
     if (p > 0) {
 #pragma unroll
       for (int q = 1; q < 8; ++q) {
-        cplx<T> w = twiddle<INV>(tw, (q * k) * (N / (Ns * 8)));
+        const cplx<T> w = tw.w[wi++];
 #pragma unroll
-        for (int l = 0; l < NL; ++l) v[l][q] = cmul(v[l][q], w);
+        for (int l = 0; l < NL; ++l) v[l][q] = cmul_tw<INV>(v[l][q], w);
       }
     }
 #pragma unroll
@@ -167,8 +220,8 @@ __device__ __forceinline__ void fft_lines(cplx<T> (&v)[NL][8], cplx<T>* const (&
     }
   }
   if constexpr (REM != 0) {
-    constexpr int R = 1 << REM;        // 2 or 4
-    constexpr int NB = 8 / R;          // butterflies per thread
+    constexpr int R = P::R;     // 2 or 4
+    constexpr int NB = P::NB;   // butterflies per thread
     // Ns == N / R here.  Butterfly b handles j = t + b*G on points
     // j + q*N/R = t + (b + q*NB)*G -> register index b + q*NB.
 #pragma unroll
@@ -177,13 +230,11 @@ __device__ __forceinline__ void fft_lines(cplx<T> (&v)[NL][8], cplx<T>* const (&
       for (int m = 0; m < 8; ++m) v[l][m] = bufs[l][pad_idx(t + m * G)];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      const int j = t + b * G;
-      const int k = j & (Ns - 1);
 #pragma unroll
       for (int q = 1; q < R; ++q) {
-        cplx<T> w = twiddle<INV>(tw, (q * k) * (N / (Ns * R)));
+        const cplx<T> w = tw.w[wi++];
 #pragma unroll
-        for (int l = 0; l < NL; ++l) v[l][b + q * NB] = cmul(v[l][b + q * NB], w);
+        for (int l = 0; l < NL; ++l) v[l][b + q * NB] = cmul_tw<INV>(v[l][b + q * NB], w);
       }
 #pragma unroll
       for (int l = 0; l < NL; ++l) {

@@ -63,10 +63,23 @@ __device__ __forceinline__ RowPair row_pair(int p, int C) {
   return r;
 }
 
-// Build the full complex line Z = A + i B of two packed Hermitian half rows A
-// and B into LDS, then load the 8 register points of thread t.  The packed
-// slot 0 supplies the DC (.x) and Nyquist (.y) values; their imaginary parts
-// are dropped, which is the C2R convention.
+// Place the values a = A[k], b = B[k] of two packed Hermitian half rows into
+// the full complex line Z = A + i B held in LDS.  The packed slot k = 0 carries
+// the DC (.x) and Nyquist (.y) values; their imaginary parts are dropped,
+// which is the C2R convention.
+template <int N, typename T>
+__device__ __forceinline__ void put_herm(cplx<T>* buf, int k, cplx<T> a, cplx<T> b) {
+  if (k == 0) {
+    buf[pad_idx(0)] = mk(a.x, b.x);
+    buf[pad_idx(N / 2)] = mk(a.y, b.y);
+  } else {
+    buf[pad_idx(k)] = mk(a.x - b.y, a.y + b.x);      // a + i b
+    buf[pad_idx(N - k)] = mk(a.x + b.y, b.x - a.y);  // conj(a) + i conj(b)
+  }
+}
+
+// Z = A + i B of two packed half rows into LDS, then the 8 register points of
+// thread t.
 template <int N, typename T>
 __device__ __forceinline__ void herm_pair_to_line(const cplx<T>* A, const cplx<T>* B, cplx<T>* buf,
                                                   cplx<T> (&v)[8], int t) {
@@ -74,14 +87,7 @@ __device__ __forceinline__ void herm_pair_to_line(const cplx<T>* A, const cplx<T
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int k = t + q * G;
-    cplx<T> a = A[k], b = B[k];
-    if (k == 0) {
-      buf[pad_idx(0)] = mk(a.x, b.x);
-      buf[pad_idx(N / 2)] = mk(a.y, b.y);
-    } else {
-      buf[pad_idx(k)] = mk(a.x - b.y, a.y + b.x);          // a + i b
-      buf[pad_idx(N - k)] = mk(a.x + b.y, b.x - a.y);      // conj(a) + i conj(b)
-    }
+    put_herm<N>(buf, k, A[k], B[k]);
   }
   __syncthreads();
 #pragma unroll
@@ -129,10 +135,10 @@ template <typename T, int N>
 __global__ __launch_bounds__(256) void k_row_r2c(const T* __restrict__ x, cplx<T>* __restrict__ P,
                                                  const cplx<T>* __restrict__ twg, long npairs) {
   constexpr int G = N / 8, PAIRS = 256 / G, H = N / 2;
-  __shared__ cplx<T> tw[N];
   __shared__ cplx<T> lbuf[PAIRS][lds_len<N>()];
-  load_twiddles<N>(tw, twg);
   const int grp = threadIdx.x / G, t = threadIdx.x % G;
+  Twiddles<N, T> tw;
+  tw.load(twg, t);
   const long p = (long)blockIdx.x * PAIRS + grp;
   const bool active = p < npairs;
   const long pp = active ? p : npairs - 1;
@@ -152,10 +158,10 @@ template <typename T, int N>
 __global__ __launch_bounds__(256) void k_row_c2r(const cplx<T>* __restrict__ P, T* __restrict__ x,
                                                  const cplx<T>* __restrict__ twg, long npairs) {
   constexpr int G = N / 8, PAIRS = 256 / G, H = N / 2;
-  __shared__ cplx<T> tw[N];
   __shared__ cplx<T> lbuf[PAIRS][lds_len<N>()];
-  load_twiddles<N>(tw, twg);
   const int grp = threadIdx.x / G, t = threadIdx.x % G;
+  Twiddles<N, T> tw;
+  tw.load(twg, t);
   const long p = (long)blockIdx.x * PAIRS + grp;
   const bool active = p < npairs;
   const long pp = active ? p : npairs - 1;
@@ -180,10 +186,10 @@ __global__ __launch_bounds__(KC* N / 8) void k_col_fwd_unpack(const cplx<T>* __r
                                                               cplx<T>* __restrict__ X,
                                                               const cplx<T>* __restrict__ twg) {
   constexpr int G = N / 8, H = N / 2, TILES = H / KC;
-  __shared__ cplx<T> tw[N];
-  __shared__ cplx<T> lbuf[KC][lds_len<N>()];
-  load_twiddles<N>(tw, twg);
+  __shared__ cplx<T> lbuf[KC][line_stride<N>()];
   const int g = threadIdx.x % KC, t = threadIdx.x / KC;
+  Twiddles<N, T> tw;
+  tw.load(twg, t);
   const long m = blockIdx.x / TILES;
   const int k = (blockIdx.x % TILES) * KC + g;
   const cplx<T>* Pm = P + m * N * H;
@@ -227,10 +233,10 @@ __global__ __launch_bounds__(KC* N / 8) void k_col_inv_pack(const cplx<T>* __res
                                                             const cplx<T>* __restrict__ twg,
                                                             T scale_) {
   constexpr int G = N / 8, H = N / 2, TILES = H / KC;
-  __shared__ cplx<T> tw[N];
-  __shared__ cplx<T> lbuf[KC][lds_len<N>()];
-  load_twiddles<N>(tw, twg);
+  __shared__ cplx<T> lbuf[KC][line_stride<N>()];
   const int g = threadIdx.x % KC, t = threadIdx.x / KC;
+  Twiddles<N, T> tw;
+  tw.load(twg, t);
   const long m = blockIdx.x / TILES;
   const int k = (blockIdx.x % TILES) * KC + g;
   const cplx<T>* Xm = X + m * N * (H + 1);
@@ -326,95 +332,41 @@ __global__ __launch_bounds__(256) void k_ptab(const double* __restrict__ Dm, T* 
 // LenseFlow fused stage kernels
 // ---------------------------------------------------------------------------
 
-// Column pass, standard (forward/inverse) rate: S0 holds the row-R2C spectrum
-// of the stage input z.  Column FFT, multiply by i KX / N^2 and i KY / N^2,
-// inverse column FFTs; d/dx goes back to S0, d/dy to S1 (packed layout).
+// Column pass: the x-derivative of a packed row spectrum.  src holds the row
+// R2C of a real map; column FFT, multiply by i KX / N^2, inverse column FFT,
+// store to dst (may alias src).  The y-derivative needs no column transform
+// (it is diagonal in the row transform) and is applied by the row pass.
+// In the packed DC|Nyquist column only the DC part of d/dx survives the C2R
+// (the Nyquist part's d/dy drops out) and the self-conjugate kx = N/2 bin of
+// d/dx drops out, so that bin is zeroed.
 template <typename T, int N, int KC>
-__global__ __launch_bounds__(KC* N / 8) void k_col_lf_grad(cplx<T>* __restrict__ S0,
-                                                           cplx<T>* __restrict__ S1,
-                                                           const cplx<T>* __restrict__ twg, T dk) {
+__global__ __launch_bounds__(KC* N / 8) void k_col_lf_dx(const cplx<T>* src, cplx<T>* dst,
+                                                         const cplx<T>* __restrict__ twg, T dk) {
   constexpr int G = N / 8, H = N / 2, TILES = H / KC;
-  __shared__ cplx<T> tw[N];
-  __shared__ cplx<T> lbuf[KC][lds_len<N>()];
-  load_twiddles<N>(tw, twg);
+  __shared__ cplx<T> lbuf[KC][line_stride<N>()];
   const int g = threadIdx.x % KC, t = threadIdx.x / KC;
+  Twiddles<N, T> tw;
+  tw.load(twg, t);
   const long m = blockIdx.x / TILES;
   const int k = (blockIdx.x % TILES) * KC + g;
-  cplx<T>* s0 = S0 + m * N * H + k;
-  cplx<T>* s1 = S1 + m * N * H + k;
-  cplx<T> v[1][8], w[8];
+  const cplx<T>* s0 = src + m * N * H + k;
+  cplx<T>* d0 = dst + m * N * H + k;
+  cplx<T> v[1][8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) v[0][q] = s0[(long)(t + q * G) * H];
   __syncthreads();
   cplx<T>* const bufs[1] = {lbuf[g]};
   fft_lines<N, false, 1>(v, bufs, tw, t);
   const T sc = T(1) / (T(N) * T(N));
-  const T ky = (T)k * dk * sc;
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     const int kx = t + q * G;
     const T kxv = (T)kx_signed<N>(kx) * dk * sc;
-    cplx<T> f = v[0][q];
-    if (k == 0) {
-      // packed DC|Nyquist column: only the x-derivative survives the C2R and
-      // the self-conjugate kx = N/2 bin drops out.
-      v[0][q] = (kx == N / 2) ? mk(T(0), T(0)) : imul(f, kxv);
-      w[q] = mk(T(0), T(0));
-    } else {
-      v[0][q] = imul(f, kxv);
-      w[q] = imul(f, ky);
-    }
+    v[0][q] = (k == 0 && kx == N / 2) ? mk(T(0), T(0)) : imul(v[0][q], kxv);
   }
   fft_lines<N, true, 1>(v, bufs, tw, t);
 #pragma unroll
-  for (int q = 0; q < 8; ++q) s0[(long)(t + q * G) * H] = v[0][q];
-#pragma unroll
-  for (int q = 0; q < 8; ++q) v[0][q] = w[q];
-  fft_lines<N, true, 1>(v, bufs, tw, t);
-#pragma unroll
-  for (int q = 0; q < 8; ++q) s1[(long)(t + q * G) * H] = v[0][q];
-}
-
-// Column pass, adjoint (divergence) rate: S0, S1 hold the row-R2C spectra of
-// u = p_x z and v = p_y z.  W = (i KX U + i KY V) / N^2 back to S0.
-template <typename T, int N, int KC>
-__global__ __launch_bounds__(KC* N / 8) void k_col_lf_div(cplx<T>* __restrict__ S0,
-                                                          const cplx<T>* __restrict__ S1,
-                                                          const cplx<T>* __restrict__ twg, T dk) {
-  constexpr int G = N / 8, H = N / 2, TILES = H / KC;
-  __shared__ cplx<T> tw[N];
-  __shared__ cplx<T> lbuf[KC][lds_len<N>()];
-  load_twiddles<N>(tw, twg);
-  const int g = threadIdx.x % KC, t = threadIdx.x / KC;
-  const long m = blockIdx.x / TILES;
-  const int k = (blockIdx.x % TILES) * KC + g;
-  cplx<T>* s0 = S0 + m * N * H + k;
-  const cplx<T>* s1 = S1 + m * N * H + k;
-  cplx<T> u[1][8], v[1][8];
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    u[0][q] = s0[(long)(t + q * G) * H];
-    v[0][q] = s1[(long)(t + q * G) * H];
-  }
-  __syncthreads();
-  cplx<T>* const bufs[1] = {lbuf[g]};
-  fft_lines<N, false, 1>(u, bufs, tw, t);
-  fft_lines<N, false, 1>(v, bufs, tw, t);
-  const T sc = T(1) / (T(N) * T(N));
-  const T ky = (T)k * dk * sc;
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int kx = t + q * G;
-    const T kxv = (T)kx_signed<N>(kx) * dk * sc;
-    if (k == 0) {
-      u[0][q] = (kx == N / 2) ? mk(T(0), T(0)) : imul(u[0][q], kxv);
-    } else {
-      u[0][q] = imul(u[0][q], kxv) + imul(v[0][q], ky);
-    }
-  }
-  fft_lines<N, true, 1>(u, bufs, tw, t);
-#pragma unroll
-  for (int q = 0; q < 8; ++q) s0[(long)(t + q * G) * H] = u[0][q];
+  for (int q = 0; q < 8; ++q) d0[(long)(t + q * G) * H] = v[0][q];
 }
 
 // RK4 bookkeeping for one stage (util.py:184-199), per pixel:
@@ -422,27 +374,27 @@ __global__ __launch_bounds__(KC* N / 8) void k_col_lf_div(cplx<T>* __restrict__ 
 //   stage 2: acc += h/3 k    ; z = y + h/2 k
 //   stage 3: acc += h/3 k    ; z = y + h k
 //   stage 4: y = acc + h/6 k ; z = y
+// Register form: y, a (acc) in, a updated in place, z out.  Stage 4 returns
+// the new y in z.
 template <typename T>
-__device__ __forceinline__ T rk4_update(int stage, T h, T k, T* yp, T* ap, bool active) {
-  T z;
+__device__ __forceinline__ void rk4_stage(int stage, T h, T k, T y, T& a, T& z) {
   if (stage == 1) {
-    const T y = *yp;
-    if (active) *ap = y + (h / T(6)) * k;
+    a = y + (h / T(6)) * k;
     z = y + (h / T(2)) * k;
   } else if (stage == 4) {
-    z = *ap + (h / T(6)) * k;
-    if (active) *yp = z;
+    z = a + (h / T(6)) * k;
   } else {
-    const T y = *yp;
-    if (active) *ap = *ap + (h / T(3)) * k;
+    a = a + (h / T(3)) * k;
     z = y + (stage == 2 ? h / T(2) : h) * k;
   }
-  return z;
 }
 
 // Row pass, standard rate.  For each row pair: C2R of (dz/dx + i dz/dy) per
 // row, k = p(t_s) . grad z, RK4 update of y/acc, then (unless this is the last
 // stage of the last step) the row R2C of the next stage input into S0.
+// S0 holds the row spectra S of the stage input, S1 their x-derivative DX;
+// the line is Z = DX + i DY with DY = i ky S / N (DY's packed slot is zero:
+// ky = 0 at DC, and the real Nyquist value times i drops out of the C2R).
 template <typename T, int N>
 __global__ __launch_bounds__(256) void k_row_lf_std(cplx<T>* __restrict__ S0,
                                                     const cplx<T>* __restrict__ S1,
@@ -450,35 +402,80 @@ __global__ __launch_bounds__(256) void k_row_lf_std(cplx<T>* __restrict__ S0,
                                                     const T* const* __restrict__ ptabs,
                                                     const cplx<T>* __restrict__ twg, int C,
                                                     long npairs, int stage, T h, int tidx,
-                                                    int do_r2c) {
+                                                    int do_r2c, T cy) {
   constexpr int G = N / 8, PAIRS = 256 / G, H = N / 2;
   constexpr long NN = (long)N * N;
-  __shared__ cplx<T> tw[N];
   __shared__ cplx<T> lbuf[PAIRS][2][lds_len<N>()];
-  load_twiddles<N>(tw, twg);
   const int grp = threadIdx.x / G, t = threadIdx.x % G;
+  Twiddles<N, T> tw;
+  tw.load(twg, t);
   const long p = (long)blockIdx.x * PAIRS + grp;
   const bool active = p < npairs;
   const RowPair rp = row_pair<N>((int)(active ? p : npairs - 1), C);
   cplx<T>* bufa = lbuf[grp][0];
   cplx<T>* bufb = lbuf[grp][1];
-  cplx<T> v[2][8];
-  __syncthreads();
-  herm_pair_to_line<N>(S0 + rp.ra * H, S1 + rp.ra * H, bufa, v[0], t);
-  herm_pair_to_line<N>(S0 + rp.rb * H, S1 + rp.rb * H, bufb, v[1], t);
-  cplx<T>* const bufs[2] = {bufa, bufb};
-  fft_lines<N, true, 2>(v, bufs, tw, t);
-
+  // loads in the order they are consumed (vmcnt retires in order): the row
+  // spectra for the transforms first, then the per-pixel operands
+  cplx<T> dxa[4], sa[4], dxb[4], sb[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int k = t + q * G;
+    dxa[q] = S1[rp.ra * H + k];
+    sa[q] = S0[rp.ra * H + k];
+    dxb[q] = S1[rp.rb * H + k];
+    sb[q] = S0[rp.rb * H + k];
+  }
   const T* pt = ptabs[rp.b] + (long)tidx * 2 * NN;
-  cplx<T> z[1][8];
+  T pxa[8], pya[8], pxb[8], pyb[8], ya[8], yb[8], aa[8], ab[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     const int j = t + m * G;
     const long ja = (long)rp.ia * N + j, jb = (long)rp.ib * N + j;
-    const T ka = pt[ja] * v[0][m].x + pt[NN + ja] * v[0][m].y;
-    const T kb = pt[jb] * v[1][m].x + pt[NN + jb] * v[1][m].y;
-    const T za = rk4_update(stage, h, ka, y + rp.ra * N + j, acc + rp.ra * N + j, active);
-    const T zb = rk4_update(stage, h, kb, y + rp.rb * N + j, acc + rp.rb * N + j, active);
+    pxa[m] = pt[ja];
+    pya[m] = pt[NN + ja];
+    pxb[m] = pt[jb];
+    pyb[m] = pt[NN + jb];
+    ya[m] = stage == 4 ? T(0) : y[rp.ra * N + j];
+    yb[m] = stage == 4 ? T(0) : y[rp.rb * N + j];
+    aa[m] = stage == 1 ? T(0) : acc[rp.ra * N + j];
+    ab[m] = stage == 1 ? T(0) : acc[rp.rb * N + j];
+  }
+  cplx<T> v[2][8];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int k = t + q * G;
+    const T c = (T)k * cy;
+    put_herm<N>(bufa, k, dxa[q], k == 0 ? mk(T(0), T(0)) : imul(sa[q], c));
+    put_herm<N>(bufb, k, dxb[q], k == 0 ? mk(T(0), T(0)) : imul(sb[q], c));
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    v[0][q] = bufa[pad_idx(t + q * G)];
+    v[1][q] = bufb[pad_idx(t + q * G)];
+  }
+  cplx<T>* const bufs[2] = {bufa, bufb};
+  fft_lines<N, true, 2>(v, bufs, tw, t);
+
+  cplx<T> z[1][8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int j = t + m * G;
+    const T ka = pxa[m] * v[0][m].x + pya[m] * v[0][m].y;
+    const T kb = pxb[m] * v[1][m].x + pyb[m] * v[1][m].y;
+    T za, zb;
+    rk4_stage(stage, h, ka, ya[m], aa[m], za);
+    rk4_stage(stage, h, kb, yb[m], ab[m], zb);
+    if (active) {
+      if (stage == 4) {
+        y[rp.ra * N + j] = za;
+        y[rp.rb * N + j] = zb;
+      } else {
+        acc[rp.ra * N + j] = aa[m];
+        acc[rp.rb * N + j] = ab[m];
+      }
+    }
     z[0][m] = mk(za, zb);
   }
   if (!do_r2c) return;  // uniform over the grid
@@ -488,7 +485,9 @@ __global__ __launch_bounds__(256) void k_row_lf_std(cplx<T>* __restrict__ S0,
 }
 
 // Row pass, adjoint rate.  init != 0: no rate, z = y (first stage input).
-// Otherwise C2R of (W_a + i W_b) gives the two rates, then RK4 update.  Then,
+// Otherwise, with S0 = d/dx of U (column pass) and S1 = V, the divergence
+// spectrum is W = S0 + i ky V / N per row (V's packed slot drops out as in
+// grad_line); C2R of (W_a + i W_b) gives the two rates, then RK4 update.  Then,
 // unless this is the very last stage, u = p_x(t_next) z and v = p_y(t_next) z
 // are row-R2C'd per row (as u + i v) into S0 (U) and S1 (V).
 template <typename T, int N>
@@ -498,47 +497,93 @@ __global__ __launch_bounds__(256) void k_row_lf_adj(cplx<T>* __restrict__ S0,
                                                     const T* const* __restrict__ ptabs,
                                                     const cplx<T>* __restrict__ twg, int C,
                                                     long npairs, int stage, T h, int tidx_next,
-                                                    int do_r2c, int init) {
+                                                    int do_r2c, int init, T cy) {
   constexpr int G = N / 8, PAIRS = 256 / G, H = N / 2;
   constexpr long NN = (long)N * N;
-  __shared__ cplx<T> tw[N];
   __shared__ cplx<T> lbuf[PAIRS][2][lds_len<N>()];
-  load_twiddles<N>(tw, twg);
   const int grp = threadIdx.x / G, t = threadIdx.x % G;
+  Twiddles<N, T> tw;
+  tw.load(twg, t);
   const long p = (long)blockIdx.x * PAIRS + grp;
   const bool active = p < npairs;
   const RowPair rp = row_pair<N>((int)(active ? p : npairs - 1), C);
   cplx<T>* bufa = lbuf[grp][0];
   cplx<T>* bufb = lbuf[grp][1];
+  // loads in the order they are consumed (vmcnt retires in order): the row
+  // spectra first, then the per-pixel operands
+  cplx<T> wa[4], wb[4], va[4], vb[4];
+  if (!init) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = t + q * G;
+      wa[q] = S0[rp.ra * H + k];
+      wb[q] = S0[rp.rb * H + k];
+      va[q] = S1[rp.ra * H + k];
+      vb[q] = S1[rp.rb * H + k];
+    }
+  }
+  const T* pt = ptabs[rp.b] + (long)(do_r2c ? tidx_next : 0) * 2 * NN;
+  T pxa[8], pya[8], pxb[8], pyb[8], ya[8], yb[8], aa[8], ab[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int j = t + m * G;
+    const long ja = (long)rp.ia * N + j, jb = (long)rp.ib * N + j;
+    pxa[m] = do_r2c ? pt[ja] : T(0);
+    pya[m] = do_r2c ? pt[NN + ja] : T(0);
+    pxb[m] = do_r2c ? pt[jb] : T(0);
+    pyb[m] = do_r2c ? pt[NN + jb] : T(0);
+    ya[m] = (stage == 4 && !init) ? T(0) : y[rp.ra * N + j];
+    yb[m] = (stage == 4 && !init) ? T(0) : y[rp.rb * N + j];
+    aa[m] = (stage == 1 || init) ? T(0) : acc[rp.ra * N + j];
+    ab[m] = (stage == 1 || init) ? T(0) : acc[rp.rb * N + j];
+  }
   T za[8], zb[8];
   __syncthreads();
   if (init) {
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-      za[m] = y[rp.ra * N + t + m * G];
-      zb[m] = y[rp.rb * N + t + m * G];
+      za[m] = ya[m];
+      zb[m] = yb[m];
     }
   } else {
     cplx<T> v[1][8];
-    herm_pair_to_line<N>(S0 + rp.ra * H, S0 + rp.rb * H, bufa, v[0], t);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = t + q * G;
+      cplx<T> a = wa[q], b = wb[q];
+      if (k != 0) {
+        a = a + imul(va[q], (T)k * cy);
+        b = b + imul(vb[q], (T)k * cy);
+      }
+      put_herm<N>(bufa, k, a, b);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[0][q] = bufa[pad_idx(t + q * G)];
     cplx<T>* const bufs[1] = {bufa};
     fft_lines<N, true, 1>(v, bufs, tw, t);
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
       const int j = t + m * G;
-      za[m] = rk4_update(stage, h, v[0][m].x, y + rp.ra * N + j, acc + rp.ra * N + j, active);
-      zb[m] = rk4_update(stage, h, v[0][m].y, y + rp.rb * N + j, acc + rp.rb * N + j, active);
+      rk4_stage(stage, h, v[0][m].x, ya[m], aa[m], za[m]);
+      rk4_stage(stage, h, v[0][m].y, yb[m], ab[m], zb[m]);
+      if (active) {
+        if (stage == 4) {
+          y[rp.ra * N + j] = za[m];
+          y[rp.rb * N + j] = zb[m];
+        } else {
+          acc[rp.ra * N + j] = aa[m];
+          acc[rp.rb * N + j] = ab[m];
+        }
+      }
     }
   }
   if (!do_r2c) return;  // uniform over the grid
-  const T* pt = ptabs[rp.b] + (long)tidx_next * 2 * NN;
   cplx<T> uv[2][8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
-    const int j = t + m * G;
-    const long ja = (long)rp.ia * N + j, jb = (long)rp.ib * N + j;
-    uv[0][m] = mk(pt[ja] * za[m], pt[NN + ja] * za[m]);
-    uv[1][m] = mk(pt[jb] * zb[m], pt[NN + jb] * zb[m]);
+    uv[0][m] = mk(pxa[m] * za[m], pya[m] * za[m]);
+    uv[1][m] = mk(pxb[m] * zb[m], pyb[m] * zb[m]);
   }
   cplx<T>* const bufs2[2] = {bufa, bufb};
   fft_lines<N, false, 2>(uv, bufs2, tw, t);
@@ -619,15 +664,12 @@ hipError_t launch_lf_row_r2c(int N, const T* y, cplx<T>* S0, long nmaps, const c
 }
 
 template <typename T>
-hipError_t launch_lf_col(int N, bool adjoint, cplx<T>* S0, cplx<T>* S1, long nmaps,
-                         const cplx<T>* tw, T dk, hipStream_t s) {
+hipError_t launch_lf_col(int N, const cplx<T>* src, cplx<T>* dst, long nmaps, const cplx<T>* tw,
+                         T dk, hipStream_t s) {
   CMBL_DISPATCH_N(N, {
     constexpr int KC = col_kc<T>(NN_);
     const unsigned nb = (unsigned)(nmaps * (NN_ / 2 / KC));
-    if (adjoint)
-      k_col_lf_div<T, NN_, KC><<<nb, KC * NN_ / 8, 0, s>>>(S0, S1, tw, dk);
-    else
-      k_col_lf_grad<T, NN_, KC><<<nb, KC * NN_ / 8, 0, s>>>(S0, S1, tw, dk);
+    k_col_lf_dx<T, NN_, KC><<<nb, KC * NN_ / 8, 0, s>>>(src, dst, tw, dk);
   });
   return hipGetLastError();
 }
@@ -635,16 +677,17 @@ hipError_t launch_lf_col(int N, bool adjoint, cplx<T>* S0, cplx<T>* S1, long nma
 template <typename T>
 hipError_t launch_lf_row(int N, bool adjoint, cplx<T>* S0, cplx<T>* S1, T* y, T* acc,
                          const T* const* ptabs, const cplx<T>* tw, int C, long nmaps, int stage,
-                         T h, int tidx, int do_r2c, int init, hipStream_t s) {
+                         T h, int tidx, int do_r2c, int init, T dk, hipStream_t s) {
   CMBL_DISPATCH_N(N, {
     const long npairs = nmaps * NN_ / 2;
     const unsigned nb = blocks_for(npairs, 256 / (NN_ / 8));
+    const T cy = dk / T(NN_);
     if (adjoint)
       k_row_lf_adj<T, NN_><<<nb, 256, 0, s>>>(S0, S1, y, acc, ptabs, tw, C, npairs, stage, h, tidx,
-                                              do_r2c, init);
+                                              do_r2c, init, cy);
     else
       k_row_lf_std<T, NN_><<<nb, 256, 0, s>>>(S0, S1, y, acc, ptabs, tw, C, npairs, stage, h, tidx,
-                                              do_r2c);
+                                              do_r2c, cy);
   });
   return hipGetLastError();
 }
@@ -657,10 +700,10 @@ hipError_t launch_lf_row(int N, bool adjoint, cplx<T>* S0, cplx<T>* S1, T* y, T*
   template hipError_t launch_ptab<T>(int, const double*, T*, int, double*, int*, hipStream_t);  \
   template hipError_t launch_lf_row_r2c<T>(int, const T*, cplx<T>*, long, const cplx<T>*,       \
                                            hipStream_t);                                        \
-  template hipError_t launch_lf_col<T>(int, bool, cplx<T>*, cplx<T>*, long, const cplx<T>*, T,  \
+  template hipError_t launch_lf_col<T>(int, const cplx<T>*, cplx<T>*, long, const cplx<T>*, T, \
                                        hipStream_t);                                            \
   template hipError_t launch_lf_row<T>(int, bool, cplx<T>*, cplx<T>*, T*, T*, const T* const*,  \
-                                       const cplx<T>*, int, long, int, T, int, int, int,        \
+                                       const cplx<T>*, int, long, int, T, int, int, int, T,     \
                                        hipStream_t);
 CMBL_INST(float)
 CMBL_INST(double)

@@ -27,12 +27,13 @@ hipError_t launch_ptab(int N, const double* D, T* ptab, int nsteps, double* minp
 template <typename T>
 hipError_t launch_lf_row_r2c(int N, const T* y, cplx<T>* S0, long nmaps, const cplx<T>* tw,
                              hipStream_t s);
+// x-derivative column pass (src may alias dst).
 template <typename T>
-hipError_t launch_lf_col(int N, bool adjoint, cplx<T>* S0, cplx<T>* S1, long nmaps,
-                         const cplx<T>* tw, T dk, hipStream_t s);
+hipError_t launch_lf_col(int N, const cplx<T>* src, cplx<T>* dst, long nmaps, const cplx<T>* tw,
+                         T dk, hipStream_t s);
 template <typename T>
 hipError_t launch_lf_row(int N, bool adjoint, cplx<T>* S0, cplx<T>* S1, T* y, T* acc,
                          const T* const* ptabs, const cplx<T>* tw, int C, long nmaps, int stage,
-                         T h, int tidx, int do_r2c, int init, hipStream_t s);
+                         T h, int tidx, int do_r2c, int init, T dk, hipStream_t s);
 
 }  // namespace cmbl
