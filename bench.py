@@ -63,6 +63,28 @@ def cpu_baseline(f, phi, th, n, budget_s=10.0):
     return k / el, k, el
 
 
+def copy_probe(torch):
+    """A plain copy of known size: FETCH_SIZE / WRITE_SIZE of this kernel give
+    the scale factors of the HBM counters, and its time the achievable copy
+    bandwidth."""
+    n = 64 * 1024 * 1024
+    x = torch.ones(n, device="cuda", dtype=torch.float32)
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        y.copy_(x)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 20
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4,
+                      "copy_us": round(ms * 1e3, 2),
+                      "copy_gbs": round(2 * n * 4 / (ms * 1e-3) / 1e9, 1)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +94,8 @@ def main():
     ap.add_argument("--plain-requests", type=int, default=40)
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 256 MiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the LenseFlow path shards by independent "
@@ -80,6 +104,9 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_probe:
+        copy_probe(torch)
+        return
     import __graft_entry__ as ge
     pkg = ge.load_package()
     from cmblensing_amd import _lib
@@ -162,6 +189,13 @@ def main():
         lat_res.append(time.perf_counter() - t1)
     lat_res = np.array(lat_res) * 1e3
 
+    # HBM traffic per launch of the row pass, from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated on a copy)
+    traffic = None
+    tfile = ROOT / "profiles" / "traffic.json"
+    if tfile.exists() and B == 128:
+        traffic = json.loads(tfile.read_text()).get("k_row_lf_std<float,256>", {}).get(
+            "bytes_per_launch")
     model_b = model_bytes_per_map(N, C, n, 4)
     result = {
         "metric": "LenseFlow maps/s @256^2 QU batched vs plain; p50/p99 latency; err vs FP64",
@@ -189,7 +223,7 @@ def main():
         "model_frac_hbm": round(model_b * maps_s / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "kernel": "k_row_lf_std<float,256>",
                      "achieved": round(row_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(row_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(row_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "avg_launch_us": round(row_avg_ms * 1e3, 3),
                      "algorithmic_bytes_per_launch": row_bytes},
         "column_pass": {"kernel": "k_col_lf_dx<float,256,16>",
