@@ -37,6 +37,14 @@ _SIGS = {
                                c_int, c_int, c_int, c_int, c_void_p]),
     "cmbl_rfft2": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "cmbl_irfft2": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "cmbl_gradf_logpdf": (c_int, [c_void_p, ctypes.POINTER(c_void_p), c_void_p, c_void_p, c_void_p,
+                                  c_void_p, c_int, c_int, c_int, c_void_p]),
+    "cmbl_dot_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p,
+                             c_void_p]),
+    "cmbl_cg_update_xrz": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                   c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "cmbl_cg_update_p": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
+                                 c_void_p]),
     "cmbl_plan_set_timing": (c_int, [c_void_p, c_int]),
     "cmbl_plan_get_timing": (c_int, [c_void_p, ctypes.POINTER(c_double), c_int]),
 }
@@ -249,3 +257,43 @@ def irfft2(plan: Plan, X: torch.Tensor, x: torch.Tensor, nmaps: int):
     with torch.cuda.device(plan.device):
         check(plan.lib.cmbl_irfft2(plan.handle, X.data_ptr(), x.data_ptr(), nmaps,
                                    stream_ptr(plan.device)), "cmbl_irfft2")
+
+
+class WfOps(ctypes.Structure):
+    """cmbl_wf_ops: device pointers of the diagonal operators (plan dtype)."""
+    _fields_ = [("mb", c_void_p), ("bmcn", c_void_p), ("cfinv", c_void_p), ("beam", c_void_p),
+                ("cninv", c_void_p), ("mask_pix", c_void_p)]
+
+
+def _ptr(t):
+    return None if t is None else t.data_ptr()
+
+
+def gradf_logpdf(plan: Plan, phis, f, d, ops: "WfOps", out, batch: int, ncomp: int, nsteps: int):
+    arr = (c_void_p * batch)(*[p.handle for p in phis])
+    with torch.cuda.device(plan.device):
+        check(plan.lib.cmbl_gradf_logpdf(plan.handle, arr, _ptr(f), _ptr(d), ctypes.byref(ops),
+                                         out.data_ptr(), batch, ncomp, nsteps,
+                                         stream_ptr(plan.device)), "cmbl_gradf_logpdf")
+
+
+def dot_f64(plan: Plan, a, b, nrhs: int, ncomp: int, out):
+    b_f64 = int(b.dtype in (torch.complex128, torch.float64))
+    with torch.cuda.device(plan.device):
+        check(plan.lib.cmbl_dot_f64(plan.handle, a.data_ptr(), b.data_ptr(), b_f64, nrhs, ncomp,
+                                    out.data_ptr(), stream_ptr(plan.device)), "cmbl_dot_f64")
+
+
+def cg_update_xrz(plan: Plan, x, r, z, p, Ap, pinv, alpha, nrhs: int, ncomp: int, rz):
+    with torch.cuda.device(plan.device):
+        check(plan.lib.cmbl_cg_update_xrz(plan.handle, x.data_ptr(), r.data_ptr(), z.data_ptr(),
+                                          p.data_ptr(), Ap.data_ptr(), pinv.data_ptr(),
+                                          alpha.data_ptr(), nrhs, ncomp, rz.data_ptr(),
+                                          stream_ptr(plan.device)), "cmbl_cg_update_xrz")
+
+
+def cg_update_p(plan: Plan, p, z, beta, p_op, nrhs: int, ncomp: int):
+    with torch.cuda.device(plan.device):
+        check(plan.lib.cmbl_cg_update_p(plan.handle, p.data_ptr(), z.data_ptr(), beta.data_ptr(),
+                                        _ptr(p_op), nrhs, ncomp, stream_ptr(plan.device)),
+              "cmbl_cg_update_p")

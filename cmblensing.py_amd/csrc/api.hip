@@ -10,6 +10,7 @@
 
 #include "../../include/cmbl.h"
 #include "kernels.hpp"
+#include "wf.hpp"
 
 using cmbl::cplx;
 
@@ -74,6 +75,14 @@ struct cmbl_plan {
   std::vector<int> tkind;        // kernel class per launch: 0 column, 1 row, 2 other
   double t_ms[3] = {0, 0, 0};
   long t_n[3] = {0, 0, 0};
+  // Wiener-filter scratch, allocated on first use: two map stacks and two
+  // natural half-plane stacks of max_maps each, and reduction partials.
+  void* wy1 = nullptr;
+  void* wy2 = nullptr;
+  void* wx1 = nullptr;
+  void* wx2 = nullptr;
+  double* red = nullptr;
+  int red_rhs = 0;
 };
 
 struct cmbl_phi {
@@ -154,6 +163,8 @@ cmbl_status cmbl_plan_destroy(cmbl_plan* p) {
   if (p->s0) (void)hipFree(p->s0);
   if (p->s1) (void)hipFree(p->s1);
   if (p->ptab_dev) (void)hipFree(p->ptab_dev);
+  for (void* q : {p->wy1, p->wy2, p->wx1, p->wx2, (void*)p->red})
+    if (q) (void)hipFree(q);
   if (p->ptab_host) (void)hipHostFree(p->ptab_host);
   delete p;
   return CMBL_OK;
@@ -440,5 +451,169 @@ extern "C" cmbl_status cmbl_plan_get_timing(cmbl_plan* p, double* out, int reset
       p->t_n[k] = 0;
     }
   }
+  return CMBL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// lensed Wiener filter
+// ---------------------------------------------------------------------------
+namespace {
+
+cmbl_status ensure_wf_scratch(cmbl_plan* p) {
+  if (p->wy1) return CMBL_OK;
+  const size_t NN = (size_t)p->N * p->N, NH1 = (size_t)p->N * (p->N / 2 + 1);
+  const size_t mb = p->max_maps * NN * p->tsize, xb = p->max_maps * NH1 * 2 * p->tsize;
+  hipError_t e;
+  if ((e = hipMalloc(&p->wy1, mb)) != hipSuccess || (e = hipMalloc(&p->wy2, mb)) != hipSuccess ||
+      (e = hipMalloc(&p->wx1, xb)) != hipSuccess || (e = hipMalloc(&p->wx2, xb)) != hipSuccess)
+    return fail(CMBL_ENOMEM, std::string("Wiener-filter scratch: ") + hipGetErrorString(e));
+  return CMBL_OK;
+}
+
+cmbl_status ensure_red(cmbl_plan* p, int nrhs) {
+  if (p->red && p->red_rhs >= nrhs) return CMBL_OK;
+  if (p->red) {
+    HIP_TRY(hipDeviceSynchronize());
+    (void)hipFree(p->red);
+    p->red = nullptr;
+  }
+  HIP_TRY(hipMalloc(&p->red, (size_t)nrhs * cmbl::red_parts() * sizeof(double)));
+  p->red_rhs = nrhs;
+  return CMBL_OK;
+}
+
+#define CMBL_TRY(expr)                  \
+  do {                                  \
+    cmbl_status st_ = (expr);           \
+    if (st_ != CMBL_OK) return st_;     \
+  } while (0)
+
+template <typename T>
+cmbl_status run_gradf(cmbl_plan* p, const cmbl_phi* const* phis, const void* f, const void* d,
+                      const cmbl_wf_ops* ops, void* out, int batch, int C, int n, hipStream_t s) {
+  const int N = p->N;
+  const long nm = (long)batch * C;
+  const long NN = (long)N * N;
+  cplx<T>* X1 = (cplx<T>*)p->wx1;
+  cplx<T>* X2 = (cplx<T>*)p->wx2;
+  T* Y1 = (T*)p->wy1;
+  T* Y2 = (T*)p->wy2;
+  const bool pix = ops->mask_pix != nullptr;
+  const T* mask = (const T*)ops->mask_pix;
+  void* st = (void*)s;
+  // 1. forward model on f (skipped when f = 0: L 0 = 0)
+  if (f) {
+    HIP_TRY(cmbl::launch_rotate<T>((const cplx<T>*)f, X1, nullptr, N, C, batch, 1, s));
+    CMBL_TRY(cmbl_irfft2(p, X1, Y1, (int)nm, st));
+    CMBL_TRY(cmbl_lenseflow(p, phis, Y1, Y2, batch, C, n, CMBL_FORWARD_LENSE, 0, st));
+    CMBL_TRY(cmbl_rfft2(p, Y2, X2, (int)nm, st));
+  }
+  if (!pix) {
+    // 2. g = B M Cn^-1 (d - M B L f), as eb2qu(g) in X2
+    HIP_TRY(cmbl::launch_wf_data<T>(f ? X2 : nullptr, (const cplx<T>*)d, (const T*)ops->mb,
+                                    (const T*)ops->bmcn, X2, N, C, batch, s));
+  } else {
+    // 2'. pixel mask: r = d - M map(B L f) in pixel space, then
+    //     g = B fourier(M map(Cn^-1 fourier(r)))
+    if (f) {
+      HIP_TRY(cmbl::launch_rotate<T>(X2, X2, nullptr, N, C, batch, 0, s));
+      HIP_TRY(cmbl::launch_rotate<T>(X2, X2, (const T*)ops->beam, N, C, batch, 1, s));
+      CMBL_TRY(cmbl_irfft2(p, X2, Y2, (int)nm, st));
+      HIP_TRY(cmbl::launch_pixel_resid<T>((const T*)d, mask, Y2, NN, nm, s));
+    } else if (d) {
+      HIP_TRY(hipMemcpyAsync(Y2, d, nm * NN * sizeof(T), hipMemcpyDeviceToDevice, s));
+    } else {
+      HIP_TRY(cmbl::launch_fill_zero<T>(Y2, nm * NN, s));
+    }
+    CMBL_TRY(cmbl_rfft2(p, Y2, X2, (int)nm, st));
+    HIP_TRY(cmbl::launch_rotate<T>(X2, X2, nullptr, N, C, batch, 0, s));
+    HIP_TRY(cmbl::launch_rotate<T>(X2, X2, (const T*)ops->cninv, N, C, batch, 1, s));
+    CMBL_TRY(cmbl_irfft2(p, X2, Y2, (int)nm, st));
+    HIP_TRY(cmbl::launch_pixel_mul<T>(Y2, mask, NN, nm, s));
+    CMBL_TRY(cmbl_rfft2(p, Y2, X2, (int)nm, st));
+    HIP_TRY(cmbl::launch_rotate<T>(X2, X2, nullptr, N, C, batch, 0, s));
+    HIP_TRY(cmbl::launch_rotate<T>(X2, X2, (const T*)ops->beam, N, C, batch, 1, s));
+  }
+  // 3. adjoint LenseFlow of map(eb2qu(g))
+  CMBL_TRY(cmbl_irfft2(p, X2, Y2, (int)nm, st));
+  CMBL_TRY(cmbl_lenseflow(p, phis, Y2, Y2, batch, C, n, CMBL_FORWARD_LENSE, 1, st));
+  CMBL_TRY(cmbl_rfft2(p, Y2, X2, (int)nm, st));
+  // 4. out = qu2eb(fourier(L^T g)) - Cf^-1 qu2eb(fourier(map(eb2qu f))); the
+  //    round trip of f is a Hermitian projection applied per mode in-kernel
+  const bool prior = f && ops->cfinv;
+  HIP_TRY(cmbl::launch_wf_final<T>(X2, prior ? (const cplx<T>*)f : nullptr,
+                                   prior ? (const T*)ops->cfinv : nullptr, (cplx<T>*)out, N, C,
+                                   batch, s));
+  return CMBL_OK;
+}
+
+}  // namespace
+
+extern "C" cmbl_status cmbl_gradf_logpdf(cmbl_plan* p, const cmbl_phi* const* phis, const void* f,
+                                         const void* d, const cmbl_wf_ops* ops, void* out,
+                                         int batch, int ncomp, int nsteps, void* stream) {
+  if (!p || !phis || !ops || !out) return fail(CMBL_EINVAL, "NULL argument");
+  if (ncomp != 1 && ncomp != 2) return fail(CMBL_EINVAL, "ncomp must be 1 (T) or 2 (EB)");
+  if ((long)batch * ncomp > p->max_maps)
+    return fail(CMBL_EINVAL, "batch*ncomp exceeds the plan's max_maps");
+  if (ops->mask_pix) {
+    if (!ops->beam || !ops->cninv) return fail(CMBL_EINVAL, "pixel-mask mode needs beam and cninv");
+  } else if (!ops->mb || !ops->bmcn) {
+    return fail(CMBL_EINVAL, "Fourier-mask mode needs mb and bmcn");
+  }
+  if (batch == 0) return CMBL_OK;
+  CMBL_TRY(ensure_wf_scratch(p));
+  hipStream_t s = (hipStream_t)stream;
+  if (p->dtype == CMBL_F32) return run_gradf<float>(p, phis, f, d, ops, out, batch, ncomp, nsteps, s);
+  return run_gradf<double>(p, phis, f, d, ops, out, batch, ncomp, nsteps, s);
+}
+
+extern "C" cmbl_status cmbl_dot_f64(cmbl_plan* p, const void* a, const void* b, int b_f64, int nrhs,
+                                    int ncomp, double* out, void* stream) {
+  if (!p || !a || !b || !out) return fail(CMBL_EINVAL, "NULL argument");
+  if (nrhs < 1 || ncomp < 1) return fail(CMBL_EINVAL, "nrhs and ncomp must be >= 1");
+  CMBL_TRY(ensure_red(p, nrhs));
+  const long len = (long)ncomp * p->N * (p->N / 2 + 1);
+  hipStream_t s = (hipStream_t)stream;
+  if (b_f64 || p->dtype == CMBL_F64)
+    HIP_TRY(cmbl::launch_dot<double>((const cplx<double>*)a, (const cplx<double>*)b, p->N, len,
+                                     nrhs, p->red, out, s));
+  else
+    HIP_TRY(cmbl::launch_dot<float>((const cplx<double>*)a, (const cplx<float>*)b, p->N, len, nrhs,
+                                    p->red, out, s));
+  return CMBL_OK;
+}
+
+extern "C" cmbl_status cmbl_cg_update_xrz(cmbl_plan* p, void* x, void* r, void* z, const void* pv,
+                                          const void* Ap, const double* pinv, const double* alpha,
+                                          int nrhs, int ncomp, double* rz, void* stream) {
+  if (!p || !x || !r || !z || !pv || !Ap || !pinv || !alpha || !rz)
+    return fail(CMBL_EINVAL, "NULL argument");
+  if (nrhs < 1 || ncomp < 1) return fail(CMBL_EINVAL, "nrhs and ncomp must be >= 1");
+  CMBL_TRY(ensure_red(p, nrhs));
+  const long len = (long)ncomp * p->N * (p->N / 2 + 1);
+  hipStream_t s = (hipStream_t)stream;
+  if (p->dtype == CMBL_F32)
+    HIP_TRY(cmbl::launch_cg_xrz<float>((cplx<double>*)x, (cplx<double>*)r, (cplx<double>*)z,
+                                       (const cplx<double>*)pv, (const cplx<float>*)Ap, pinv, alpha,
+                                       p->N, ncomp, len, nrhs, p->red, rz, s));
+  else
+    HIP_TRY(cmbl::launch_cg_xrz<double>((cplx<double>*)x, (cplx<double>*)r, (cplx<double>*)z,
+                                        (const cplx<double>*)pv, (const cplx<double>*)Ap, pinv,
+                                        alpha, p->N, ncomp, len, nrhs, p->red, rz, s));
+  return CMBL_OK;
+}
+
+extern "C" cmbl_status cmbl_cg_update_p(cmbl_plan* p, void* pv, const void* z, const double* beta,
+                                        void* p_op, int nrhs, int ncomp, void* stream) {
+  if (!p || !pv || !z || !beta) return fail(CMBL_EINVAL, "NULL argument");
+  const long len = (long)ncomp * p->N * (p->N / 2 + 1);
+  hipStream_t s = (hipStream_t)stream;
+  if (p->dtype == CMBL_F32)
+    HIP_TRY(cmbl::launch_cg_p<float>((cplx<double>*)pv, (const cplx<double>*)z, beta,
+                                     (cplx<float>*)p_op, len, nrhs, s));
+  else
+    HIP_TRY(cmbl::launch_cg_p<double>((cplx<double>*)pv, (const cplx<double>*)z, beta,
+                                      (cplx<double>*)p_op, len, nrhs, s));
   return CMBL_OK;
 }

@@ -1,0 +1,402 @@
+// Fourier-domain kernels of the lensed Wiener filter (gradients.py:12-26,
+// wiener_filter.py:28-105) on natural-layout half planes X[m][kx][ky],
+// ky in [0, N/2], m = batch*C + component (C = 1: T, C = 2: E,B or Q,U).
+//
+// EB <-> QU (util.py:140-161): psi = arctan2(lx, ly) with lx the half-axis
+// frequency (its Nyquist column negated for rows kx <= N/2) and ly the full
+// axis; cos 2psi = (ly^2 - lx^2) / l^2, sin 2psi = 2 lx ly / l^2 (psi = 0 at
+// l = 0).  The common 2 pi / (N d) factor cancels, so integer frequencies give
+// the same angles.
+//   eb2qu:  Q = -E c + B s,  U = -E s - B c
+//   qu2eb:  E = -Q c - U s,  B =  Q s - U c
+#include <hip/hip_runtime.h>
+
+#include "fft_lds.hpp"
+#include "wf.hpp"
+
+namespace cmbl {
+
+template <typename T>
+__device__ __forceinline__ void rot2psi(int N, long idx, T& c, T& s) {
+  const int H1 = N / 2 + 1;
+  const int i = (int)(idx / H1), j = (int)(idx % H1);
+  const double ly = (i < N / 2) ? i : i - N;
+  double lx = j;
+  if (j == N / 2 && i <= N / 2) lx = -lx;
+  const double l2 = lx * lx + ly * ly;
+  if (l2 == 0) {
+    c = T(1);
+    s = T(0);
+  } else {
+    c = (T)((ly * ly - lx * lx) / l2);
+    s = (T)(2.0 * lx * ly / l2);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void eb2qu_mode(cplx<T> e, cplx<T> b, T c, T s, cplx<T>& q,
+                                           cplx<T>& u) {
+  q = mk(-e.x * c + b.x * s, -e.y * c + b.y * s);
+  u = mk(-e.x * s - b.x * c, -e.y * s - b.y * c);
+}
+
+template <typename T>
+__device__ __forceinline__ void qu2eb_mode(cplx<T> q, cplx<T> u, T c, T s, cplx<T>& e,
+                                           cplx<T>& b) {
+  e = mk(-q.x * c - u.x * s, -q.y * c - u.y * s);
+  b = mk(q.x * s - u.x * c, q.y * s - u.y * c);
+}
+
+// out = rot(diag .* in): per-mode diagonal (per component, may be NULL), then
+// EB->QU (to_qu) or QU->EB.  C = 1 skips the rotation.  in may alias out.
+template <typename T>
+__global__ void k_rotate(const cplx<T>* in, cplx<T>* out, const T* __restrict__ diag, int N,
+                         int C, long nfields, int to_qu) {
+  const long P = (long)N * (N / 2 + 1);
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < nfields * P;
+       g += (long)gridDim.x * blockDim.x) {
+    const long f = g / P, idx = g % P;
+    const long base = f * C * P + idx;
+    if (C == 1) {
+      cplx<T> a = in[base];
+      if (diag) a = scale(a, diag[idx]);
+      out[base] = a;
+      continue;
+    }
+    cplx<T> a = in[base], b = in[base + P];
+    if (diag) {
+      a = scale(a, diag[idx]);
+      b = scale(b, diag[P + idx]);
+    }
+    T c, s;
+    rot2psi<T>(N, idx, c, s);
+    cplx<T> o0, o1;
+    if (to_qu)
+      eb2qu_mode(a, b, c, s, o0, o1);
+    else
+      qu2eb_mode(a, b, c, s, o0, o1);
+    out[base] = o0;
+    out[base + P] = o1;
+  }
+}
+
+// Data term of gradf_logpdf (gradients.py:18-22) from the rfft2 of the lensed
+// QU maps X:  g = bmcn .* (d - mb .* qu2eb(X)) in EB, returned as eb2qu(g).
+// mb = M B and bmcn = B M Cn^-1 are per-mode diagonals [C][N][N/2+1]; d may be
+// NULL (zero data, the CG operator), X may be NULL (zero field).  out may
+// alias X.
+template <typename T>
+__global__ void k_wf_data(const cplx<T>* X, const cplx<T>* __restrict__ d,
+                          const T* __restrict__ mb, const T* __restrict__ bmcn, cplx<T>* out,
+                          int N, int C, long nfields) {
+  const long P = (long)N * (N / 2 + 1);
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < nfields * P;
+       g += (long)gridDim.x * blockDim.x) {
+    const long f = g / P, idx = g % P;
+    const long base = f * C * P + idx;
+    const cplx<T> zero = mk(T(0), T(0));
+    if (C == 1) {
+      cplx<T> r = X ? scale(X[base], -mb[idx]) : zero;
+      if (d) r = r + d[base];
+      out[base] = scale(r, bmcn[idx]);
+      continue;
+    }
+    T c, s;
+    rot2psi<T>(N, idx, c, s);
+    cplx<T> e = zero, b = zero;
+    if (X) qu2eb_mode(X[base], X[base + P], c, s, e, b);
+    cplx<T> re = scale(e, -mb[idx]), rb = scale(b, -mb[P + idx]);
+    if (d) {
+      re = re + d[base];
+      rb = rb + d[base + P];
+    }
+    re = scale(re, bmcn[idx]);
+    rb = scale(rb, bmcn[P + idx]);
+    cplx<T> q, u;
+    eb2qu_mode(re, rb, c, s, q, u);
+    out[base] = q;
+    out[base + P] = u;
+  }
+}
+
+// Last line of gradf_logpdf (gradients.py:25):
+//   out = qu2eb(XL) - cf_inv .* qu2eb(fourier(map(eb2qu(F))))
+// with XL the rfft2 of the adjoint-lensed maps and F the EB (or T) input
+// field.  fourier(map(.)) of a half plane only replaces the two
+// self-conjugate columns (ky = 0 and ky = N/2) by their Hermitian parts
+// (Y[kx] + conj(Y[-kx]))/2, and the EB<->QU rotation is the same at kx and -kx
+// on those columns, so the prior term is cf_inv .* Herm(F) computed per mode:
+// identical to the reference's round trip, without leaking FFT rounding of
+// large modes into small ones (which cf_inv, spanning ~1e10, would amplify).
+// F and cf_inv may be NULL (no prior term).  out may alias XL.
+template <typename T>
+__device__ __forceinline__ cplx<T> herm_col(const cplx<T>* F, long base_f, int N, int i, int j) {
+  const int H1 = N / 2 + 1;
+  const cplx<T> a = F[base_f + (long)i * H1 + j];
+  if (j != 0 && j != N / 2) return a;
+  const cplx<T> b = F[base_f + (long)((N - i) & (N - 1)) * H1 + j];
+  return mk(T(0.5) * (a.x + b.x), T(0.5) * (a.y - b.y));
+}
+
+template <typename T>
+__global__ void k_wf_final(const cplx<T>* XL, const cplx<T>* __restrict__ F,
+                           const T* __restrict__ cfinv, cplx<T>* out, int N, int C, long nfields) {
+  const long P = (long)N * (N / 2 + 1);
+  const int H1 = N / 2 + 1;
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < nfields * P;
+       g += (long)gridDim.x * blockDim.x) {
+    const long f = g / P, idx = g % P;
+    const long base = f * C * P + idx;
+    const int i = (int)(idx / H1), j = (int)(idx % H1);
+    const bool prior = F && cfinv;
+    if (C == 1) {
+      cplx<T> o = XL[base];
+      if (prior) o = o - scale(herm_col(F, f * C * P, N, i, j), cfinv[idx]);
+      out[base] = o;
+      continue;
+    }
+    T c, s;
+    rot2psi<T>(N, idx, c, s);
+    cplx<T> e, b;
+    qu2eb_mode(XL[base], XL[base + P], c, s, e, b);
+    if (prior) {
+      e = e - scale(herm_col(F, f * C * P, N, i, j), cfinv[idx]);
+      b = b - scale(herm_col(F, f * C * P + P, N, i, j), cfinv[P + idx]);
+    }
+    out[base] = e;
+    out[base + P] = b;
+  }
+}
+
+// Pixel-space multiply x[m][p] *= w[p] (a pixel mask shared by all maps).
+template <typename T>
+__global__ void k_pixel_mul(T* x, const T* __restrict__ w, long npix, long nmaps) {
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < nmaps * npix;
+       g += (long)gridDim.x * blockDim.x)
+    x[g] *= w[g % npix];
+}
+
+template <typename T>
+__global__ void k_fill_zero(T* x, long n) {
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x)
+    x[g] = T(0);
+}
+
+// residual in pixel space: r = d - w .* x
+template <typename T>
+__global__ void k_pixel_resid(const T* __restrict__ d, const T* __restrict__ w, T* x, long npix,
+                              long nmaps) {
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < nmaps * npix;
+       g += (long)gridDim.x * blockDim.x) {
+    const T v = x[g] * w[g % npix];
+    x[g] = (d ? d[g] : T(0)) - v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 CG vector kernels.  Vectors are complex double half planes
+// [nrhs][C][N][N/2+1]; the inner product is fields.py:136-142 /
+// util.py:100-101: Re sum conj(a) b w / N^2, w = 1 on the ky = 0 and ky = N/2
+// columns, 2 elsewhere.  Reductions are two-pass in a fixed order, so results
+// do not depend on scheduling.
+// ---------------------------------------------------------------------------
+constexpr int kRedBlock = 256;
+constexpr int kRedParts = 64;  // partial sums per right-hand side
+
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double red[kRedBlock];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kRedBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ double wgt(long idx, int N) {
+  const int j = (int)(idx % (N / 2 + 1));
+  return (j == 0 || j == N / 2) ? 1.0 : 2.0;
+}
+
+// partial[rhs][part] of Re sum conj(a) b w; b is complex<Tb> (double or float)
+template <typename Tb>
+__global__ __launch_bounds__(kRedBlock) void k_dot_partial(const cplx<double>* __restrict__ a,
+                                                           const cplx<Tb>* __restrict__ b, int N,
+                                                           long len, double* __restrict__ part) {
+  const int rhs = blockIdx.y;
+  const long P = (long)N * (N / 2 + 1);
+  const cplx<double>* ar = a + rhs * len;
+  const cplx<Tb>* br = b + rhs * len;
+  double acc = 0.0;
+  for (long i = blockIdx.x * (long)kRedBlock + threadIdx.x; i < len; i += (long)kRedParts * kRedBlock) {
+    const cplx<double> x = ar[i];
+    const cplx<Tb> y = br[i];
+    acc += (x.x * (double)y.x + x.y * (double)y.y) * wgt(i % P, N);
+  }
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) part[rhs * kRedParts + blockIdx.x] = s;
+}
+
+__global__ void k_dot_final(const double* __restrict__ part, double scale_, double* __restrict__ out,
+                            int nrhs) {
+  const int rhs = blockIdx.x * blockDim.x + threadIdx.x;
+  if (rhs >= nrhs) return;
+  double s = 0.0;
+  for (int k = 0; k < kRedParts; ++k) s += part[rhs * kRedParts + k];
+  out[rhs] = s * scale_;
+}
+
+// x += alpha p ; r -= alpha Ap ; z = pinv .* r ; partial of Re sum conj(r) z w
+template <typename Tb>
+__global__ __launch_bounds__(kRedBlock) void k_cg_xrz(cplx<double>* __restrict__ x,
+                                                      cplx<double>* __restrict__ r,
+                                                      cplx<double>* __restrict__ z,
+                                                      const cplx<double>* __restrict__ p,
+                                                      const cplx<Tb>* __restrict__ Ap,
+                                                      const double* __restrict__ pinv,
+                                                      const double* __restrict__ alpha, int N,
+                                                      int C, long len, double* __restrict__ part) {
+  const int rhs = blockIdx.y;
+  const long P = (long)N * (N / 2 + 1);
+  const double al = alpha[rhs];
+  const long off = rhs * len;
+  double acc = 0.0;
+  for (long i = blockIdx.x * (long)kRedBlock + threadIdx.x; i < len; i += (long)kRedParts * kRedBlock) {
+    const long o = off + i;
+    cplx<double> xi = x[o], ri = r[o];
+    const cplx<double> pi = p[o];
+    const cplx<Tb> api = Ap[o];
+    xi = mk(xi.x + al * pi.x, xi.y + al * pi.y);
+    ri = mk(ri.x - al * (double)api.x, ri.y - al * (double)api.y);
+    const double w = pinv[i % (C * P)];
+    const cplx<double> zi = mk(w * ri.x, w * ri.y);
+    x[o] = xi;
+    r[o] = ri;
+    z[o] = zi;
+    acc += (ri.x * zi.x + ri.y * zi.y) * wgt(i % P, N);
+  }
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) part[rhs * kRedParts + blockIdx.x] = s;
+}
+
+// p = z + beta p, and p rounded to the operator precision (pT)
+template <typename T>
+__global__ void k_cg_p(cplx<double>* __restrict__ p, const cplx<double>* __restrict__ z,
+                       const double* __restrict__ beta, cplx<T>* __restrict__ pT, long len,
+                       int nrhs) {
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < len * nrhs;
+       g += (long)gridDim.x * blockDim.x) {
+    const double be = beta[g / len];
+    const cplx<double> zi = z[g], pi = p[g];
+    const cplx<double> np = mk(zi.x + be * pi.x, zi.y + be * pi.y);
+    p[g] = np;
+    if (pT) pT[g] = mk((T)np.x, (T)np.y);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline unsigned grid_for(long n) {
+  long b = (n + 255) / 256;
+  return (unsigned)(b > 65536 ? 65536 : (b < 1 ? 1 : b));
+}
+
+template <typename T>
+hipError_t launch_rotate(const cplx<T>* in, cplx<T>* out, const T* diag, int N, int C,
+                         long nfields, int to_qu, hipStream_t s) {
+  const long n = nfields * (long)N * (N / 2 + 1);
+  k_rotate<T><<<grid_for(n), 256, 0, s>>>(in, out, diag, N, C, nfields, to_qu);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_wf_data(const cplx<T>* X, const cplx<T>* d, const T* mb, const T* bmcn,
+                          cplx<T>* out, int N, int C, long nfields, hipStream_t s) {
+  const long n = nfields * (long)N * (N / 2 + 1);
+  k_wf_data<T><<<grid_for(n), 256, 0, s>>>(X, d, mb, bmcn, out, N, C, nfields);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_wf_final(const cplx<T>* XL, const cplx<T>* F, const T* cfinv, cplx<T>* out,
+                           int N, int C, long nfields, hipStream_t s) {
+  const long n = nfields * (long)N * (N / 2 + 1);
+  k_wf_final<T><<<grid_for(n), 256, 0, s>>>(XL, F, cfinv, out, N, C, nfields);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_pixel_mul(T* x, const T* w, long npix, long nmaps, hipStream_t s) {
+  k_pixel_mul<T><<<grid_for(npix * nmaps), 256, 0, s>>>(x, w, npix, nmaps);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_fill_zero(T* x, long n, hipStream_t s) {
+  k_fill_zero<T><<<grid_for(n), 256, 0, s>>>(x, n);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_pixel_resid(const T* d, const T* w, T* x, long npix, long nmaps,
+                              hipStream_t s) {
+  k_pixel_resid<T><<<grid_for(npix * nmaps), 256, 0, s>>>(d, w, x, npix, nmaps);
+  return hipGetLastError();
+}
+
+template <typename Tb>
+hipError_t launch_dot(const cplx<double>* a, const cplx<Tb>* b, int N, long len, int nrhs,
+                      double* part, double* out, hipStream_t s) {
+  const double sc = 1.0 / ((double)N * (double)N);
+  k_dot_partial<Tb><<<dim3(kRedParts, nrhs), kRedBlock, 0, s>>>(a, b, N, len, part);
+  k_dot_final<<<(nrhs + 63) / 64, 64, 0, s>>>(part, sc, out, nrhs);
+  return hipGetLastError();
+}
+
+template <typename Tb>
+hipError_t launch_cg_xrz(cplx<double>* x, cplx<double>* r, cplx<double>* z,
+                         const cplx<double>* p, const cplx<Tb>* Ap, const double* pinv,
+                         const double* alpha, int N, int C, long len, int nrhs, double* part,
+                         double* rz, hipStream_t s) {
+  const double sc = 1.0 / ((double)N * (double)N);
+  k_cg_xrz<Tb><<<dim3(kRedParts, nrhs), kRedBlock, 0, s>>>(x, r, z, p, Ap, pinv, alpha, N, C, len,
+                                                          part);
+  k_dot_final<<<(nrhs + 63) / 64, 64, 0, s>>>(part, sc, rz, nrhs);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_cg_p(cplx<double>* p, const cplx<double>* z, const double* beta, cplx<T>* pT,
+                       long len, int nrhs, hipStream_t s) {
+  k_cg_p<T><<<grid_for(len * nrhs), 256, 0, s>>>(p, z, beta, pT, len, nrhs);
+  return hipGetLastError();
+}
+
+int red_parts() { return kRedParts; }
+
+#define CMBL_WF_INST(T)                                                                         \
+  template hipError_t launch_rotate<T>(const cplx<T>*, cplx<T>*, const T*, int, int, long, int, \
+                                       hipStream_t);                                            \
+  template hipError_t launch_wf_data<T>(const cplx<T>*, const cplx<T>*, const T*, const T*,     \
+                                        cplx<T>*, int, int, long, hipStream_t);                 \
+  template hipError_t launch_wf_final<T>(const cplx<T>*, const cplx<T>*, const T*, cplx<T>*,    \
+                                         int, int, long, hipStream_t);                          \
+  template hipError_t launch_pixel_mul<T>(T*, const T*, long, long, hipStream_t);               \
+  template hipError_t launch_fill_zero<T>(T*, long, hipStream_t);                               \
+  template hipError_t launch_pixel_resid<T>(const T*, const T*, T*, long, long, hipStream_t);   \
+  template hipError_t launch_dot<T>(const cplx<double>*, const cplx<T>*, int, long, int,        \
+                                    double*, double*, hipStream_t);                             \
+  template hipError_t launch_cg_xrz<T>(cplx<double>*, cplx<double>*, cplx<double>*,             \
+                                       const cplx<double>*, const cplx<T>*, const double*,      \
+                                       const double*, int, int, long, int, double*, double*,    \
+                                       hipStream_t);                                            \
+  template hipError_t launch_cg_p<T>(cplx<double>*, const cplx<double>*, const double*,         \
+                                     cplx<T>*, long, int, hipStream_t);
+CMBL_WF_INST(float)
+CMBL_WF_INST(double)
+
+}  // namespace cmbl

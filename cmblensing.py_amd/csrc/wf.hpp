@@ -1,0 +1,37 @@
+// Launchers of the Wiener-filter kernels in wf.hip (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fft_lds.hpp"
+
+namespace cmbl {
+
+template <typename T>
+hipError_t launch_rotate(const cplx<T>* in, cplx<T>* out, const T* diag, int N, int C,
+                         long nfields, int to_qu, hipStream_t s);
+template <typename T>
+hipError_t launch_wf_data(const cplx<T>* X, const cplx<T>* d, const T* mb, const T* bmcn,
+                          cplx<T>* out, int N, int C, long nfields, hipStream_t s);
+template <typename T>
+hipError_t launch_wf_final(const cplx<T>* XL, const cplx<T>* XF, const T* cfinv, cplx<T>* out,
+                           int N, int C, long nfields, hipStream_t s);
+template <typename T>
+hipError_t launch_pixel_mul(T* x, const T* w, long npix, long nmaps, hipStream_t s);
+template <typename T>
+hipError_t launch_fill_zero(T* x, long n, hipStream_t s);
+template <typename T>
+hipError_t launch_pixel_resid(const T* d, const T* w, T* x, long npix, long nmaps, hipStream_t s);
+template <typename Tb>
+hipError_t launch_dot(const cplx<double>* a, const cplx<Tb>* b, int N, long len, int nrhs,
+                      double* part, double* out, hipStream_t s);
+template <typename Tb>
+hipError_t launch_cg_xrz(cplx<double>* x, cplx<double>* r, cplx<double>* z,
+                         const cplx<double>* p, const cplx<Tb>* Ap, const double* pinv,
+                         const double* alpha, int N, int C, long len, int nrhs, double* part,
+                         double* rz, hipStream_t s);
+template <typename T>
+hipError_t launch_cg_p(cplx<double>* p, const cplx<double>* z, const double* beta, cplx<T>* pT,
+                       long len, int nrhs, hipStream_t s);
+int red_parts();
+
+}  // namespace cmbl

@@ -141,3 +141,150 @@ def lenseflow_inputs(config: int, seed: int = 0):
         N, th = 256, 2.0
         return qu_map(N, th, seed), phi_map(N, th, seed + 1), th, 7
     raise ValueError(config)
+
+
+# ---------------------------------------------------------------------------
+# Wiener-filter / likelihood problems
+# ---------------------------------------------------------------------------
+def covariance_diag(N, theta_pix_arcmin, cl_fn) -> np.ndarray:
+    """Fourier-diagonal covariance C_l(|l|) / Delta^2 on the half plane with the
+    (0,0) mode zeroed (simulate.py:70-97 with rescale)."""
+    ell = ell_grid(N, theta_pix_arcmin)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        C = np.where(ell > 0, cl_fn(np.where(ell > 0, ell, 1.0)), 0.0)
+    C = C / pix_width(theta_pix_arcmin) ** 2
+    C[0, 0] = 0.0
+    return C
+
+
+def noise_cl(uk_arcmin, ell_knee=None, alpha_knee=3.0):
+    """N_l = (uK-arcmin in rad)^2 (1 + (l_knee/l)^alpha) (simulate.py:54-65 form);
+    the stated level is used directly for T and for each of Q, U."""
+    w = (uk_arcmin * ARCMIN) ** 2
+
+    def fn(ell):
+        if ell_knee is None:
+            return w * np.ones_like(ell)
+        return w * (1.0 + (ell_knee / ell) ** alpha_knee)
+    return fn
+
+
+def beam_diag(N, theta_pix_arcmin, fwhm_arcmin) -> np.ndarray:
+    """b_l = exp(-l^2 sigma^2 / 2), sigma = FWHM / sqrt(8 ln 2) (simulate.py:50-51,135)."""
+    ell = ell_grid(N, theta_pix_arcmin)
+    sig = fwhm_arcmin * ARCMIN / math.sqrt(8 * math.log(2))
+    return np.exp(-0.5 * (ell * sig) ** 2)
+
+
+def lowpass_mask_diag(N, theta_pix_arcmin, ell_cut=3000.0, delta_ell=50.0) -> np.ndarray:
+    """Fourier-diagonal low-pass 'mask' like the reference's (simulate.py:117-130):
+    1 below ell_cut - delta_ell, a cosine taper to 0 at ell_cut, 1 at l = 0."""
+    ell = ell_grid(N, theta_pix_arcmin)
+    x = np.clip((ell - (ell_cut - delta_ell)) / delta_ell, 0.0, 1.0)
+    m = 0.5 * (1 + np.cos(np.pi * x))
+    m[0, 0] = 1.0
+    return m
+
+
+def square_mask(N, frac_area=0.8, taper_px=8) -> np.ndarray:
+    """Centred square covering `frac_area` of the map with a cosine taper."""
+    side = math.sqrt(frac_area) * N
+    c = (N - 1) / 2.0
+    d = np.abs(np.arange(N) - c)
+    inner = side / 2 - taper_px / 2
+
+    def prof(v):
+        t = np.clip((v - inner) / taper_px, 0.0, 1.0)
+        return 0.5 * (1 + np.cos(np.pi * t))
+    return np.outer(prof(d), prof(d))
+
+
+def point_source_mask(N, theta_pix_arcmin, nholes=200, radius_arcmin=5.0, seed=4) -> np.ndarray:
+    """1 everywhere except `nholes` discs of `radius_arcmin` at uniform random
+    centres (periodic distances)."""
+    rng = np.random.default_rng(seed)
+    m = np.ones((N, N))
+    r_px = radius_arcmin / theta_pix_arcmin
+    ii, jj = np.meshgrid(np.arange(N), np.arange(N), indexing="ij")
+    for cx, cy in rng.uniform(0, N, size=(nholes, 2)):
+        dx = np.abs(ii - cx)
+        dy = np.abs(jj - cy)
+        dx = np.minimum(dx, N - dx)
+        dy = np.minimum(dy, N - dy)
+        m[dx * dx + dy * dy <= r_px * r_px] = 0.0
+    return m
+
+
+def wf_problem(N, theta_pix_arcmin, pol=True, seed=0, mask="fourier", noise_uk=5.0,
+               ell_knee=None, beam_fwhm=1.0):
+    """A lensed Wiener-filter problem as numpy arrays (EB / T FOURIER fields,
+    per-mode operators [C, N, N//2+1]):
+
+      f (truth), phi (MAP), Cf, Cn, B, and either M (Fourier-diagonal) with
+      d = M B L f + n in Fourier, or mask_pix with d = m map(B L f) + n as maps.
+    The lensing of the truth uses the FP64 numpy FFTs of this module's
+    helpers and 7 RK4 steps of the same LenseFlow (imported lazily)."""
+    rng = np.random.default_rng(seed)
+    th = theta_pix_arcmin
+    C = 2 if pol else 1
+    cl = cl_ee if pol else cl_tt
+    Cf1 = covariance_diag(N, th, cl)
+    Cn1 = covariance_diag(N, th, noise_cl(noise_uk, ell_knee))
+    # The monopole gets the variance of the fundamental mode: with C[0,0] = 0
+    # the nonzero-safe pinv (util.py:45-47) leaves it without prior, noise
+    # weight or preconditioner, and a pixel mask couples it to every mode, so
+    # PCG could never reduce the residual there.
+    for A in (Cf1, Cn1):
+        A[0, 0] = A[0, 1]
+    Cn = np.stack([Cn1] * C)
+    Cf = np.stack([Cf1, 1e-3 * Cf1])[:C] if pol else Cf1[None]   # small B-mode power
+    B = np.stack([beam_diag(N, th, beam_fwhm)] * C)
+    shape = (C, N, N // 2 + 1)
+    f = np.sqrt(Cf / 2) * (rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+    f = np.fft.rfft2(np.fft.irfft2(f))           # Hermitian-consistent half plane
+    phi = phi_map(N, th, seed + 1)
+    return dict(N=N, theta_pix=th, C=C, f=f, phi=phi, Cf=Cf, Cn=Cn, B=B, mask_kind=mask,
+                rng_seed=seed)
+
+
+def wf_data(prob, lensed_qu_maps, seed=None):
+    """Attach data to a problem given the lensed T or Q/U maps [C, N, N] of its
+    truth: Fourier mask -> d = M B L f + n (EB Fourier); pixel mask ->
+    d = m map(B L f) + n (maps)."""
+    N, th, C = prob["N"], prob["theta_pix"], prob["C"]
+    rng = np.random.default_rng(prob["rng_seed"] + 7 if seed is None else seed)
+    shape = (C, N, N // 2 + 1)
+    nF = np.sqrt(prob["Cn"] / 2) * (rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+    LF = np.fft.rfft2(lensed_qu_maps)
+    if C == 2:
+        e, b = _qu2eb_np(LF[0], LF[1], N, th)
+        LF = np.stack([e, b])
+    if prob["mask_kind"] == "fourier":
+        M = np.stack([lowpass_mask_diag(N, th, ell_cut=0.8 * np.pi / pix_width(th))] * C)
+        prob["M"] = M
+        prob["mask_pix"] = None
+        prob["d"] = M * prob["B"] * LF + nF
+    else:
+        m = square_mask(N) if prob["mask_kind"] == "square" else point_source_mask(N, th)
+        prob["M"] = None
+        prob["mask_pix"] = m
+        BLF = prob["B"] * LF
+        if C == 2:
+            q, u = eb_to_qu_fourier(BLF[0], BLF[1], N, th)
+            BLF = np.stack([q, u])
+            ne, nb = eb_to_qu_fourier(nF[0], nF[1], N, th)
+            nF = np.stack([ne, nb])
+        prob["d"] = m * np.fft.irfft2(BLF) + np.fft.irfft2(nF)
+    return prob
+
+
+def _qu2eb_np(q, u, N, th):
+    d = pix_width(th)
+    lx = 2 * np.pi * np.fft.rfftfreq(N, d)
+    ly = 2 * np.pi * np.fft.fftfreq(N, d)
+    LX, LY = np.meshgrid(lx, ly)
+    LX = LX.copy()
+    LX[: N // 2 + 1, -1] *= -1
+    a = np.arctan2(LX, LY)
+    c2, s2 = np.cos(2 * a), np.sin(2 * a)
+    return -q * c2 - u * s2, q * s2 - u * c2

@@ -1,0 +1,224 @@
+"""Lensed Wiener filter by preconditioned conjugate gradient on the GPU
+(wiener_filter.py:10-105 of the reference).
+
+Precision: the operator A (two LenseFlows and the FFT/rotation chain,
+``cmbl_gradf_logpdf``) runs in the field's dtype (float32 for the mixed
+configuration); the CG vectors x, r, z, p, the inner products, alpha and beta
+and the residual recomputation are float64 (``cmbl_dot_f64``,
+``cmbl_cg_update_xrz``, ``cmbl_cg_update_p``).  Several right-hand sides (a
+leading batch axis) are solved together, each with its own alpha, beta and
+stopping test.
+
+Stopping: the reference stops when dot(r, P^-1 r) < tol (absolute, tol = 0.1)
+or after maxiter = 500 iterations (wiener_filter.py:21-22, 75-77); that is the
+default here.  ``rtol`` adds a relative test sqrt(dot(r, r) / dot(b, b)) <
+rtol, and ``recompute_every`` replaces the recursive residual by b - A x every
+k iterations.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field as dc_field
+from typing import List, Optional
+
+import torch
+
+from . import _lib
+from .constants import DEFAULT_NSTEPS
+from .fields import Field
+from .gradients import (WfOperators, _ncomp, gradf_apply, phi_handles, stack_matrices,
+                        unstack_matrices)
+from .matrix_operators import DiagonalEB, DiagonalScalar, PixelMask, pinv
+
+
+@dataclass
+class CGInfo:
+    iterations: List[int]
+    converged: List[bool]
+    res_history: List[List[float]] = dc_field(default_factory=list)   # dot(r, z) per iteration
+    rel_history: List[List[float]] = dc_field(default_factory=list)   # ||r|| / ||b||
+    final_true_rel_residual: Optional[List[float]] = None
+
+
+class _Problem:
+    """Geometry, plan, prepared phi and operator arrays of one solve."""
+
+    def __init__(self, field: Field, phi, field_covariance, noise_covariance, mask, beam, n: int,
+                 batch: int):
+        self.C = _ncomp(field)
+        m0 = getattr(field, field._matrix_names()[0])
+        self.N = int(m0.shape[-2])
+        self.B = batch
+        self.n = n
+        self.device = m0.device
+        self.cdt = m0.dtype if m0.is_complex() else (
+            torch.complex64 if m0.dtype == torch.float32 else torch.complex128)
+        self.rdt = torch.float32 if self.cdt == torch.complex64 else torch.float64
+        self.plan = _lib.get_plan(self.N, field.theta_pix, self.rdt, self.B * self.C, self.device)
+        self.ops = WfOperators(field_covariance, noise_covariance, mask, beam, self.rdt,
+                               self.device)
+        self.phis = phi_handles(self.plan, phi, self.B, n)
+
+    def A(self, X_op: torch.Tensor) -> torch.Tensor:
+        """A x = gradf(x, d = 0) (wiener_filter.py:45-49), operator precision."""
+        return gradf_apply(self.plan, self.phis, X_op, None, self.ops, self.B, self.C, self.n)
+
+    def dot(self, a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+        out = torch.empty(self.B, dtype=torch.float64, device=self.device)
+        _lib.dot_f64(self.plan, torch.view_as_real(a), torch.view_as_real(b), self.B, self.C, out)
+        return out
+
+
+def get_b_vector(phi, data, field_covariance, noise_covariance, mask, beam,
+                 n: int = DEFAULT_NSTEPS, like: Optional[Field] = None) -> Field:
+    """b = -gradf_logpdf(0, d) (wiener_filter.py:28-33).  The forward LenseFlow
+    of the zero field is skipped (it is exactly zero)."""
+    tmpl = like if like is not None else data
+    X, lead = stack_matrices(tmpl)
+    prob = _Problem(tmpl, phi, field_covariance, noise_covariance, mask, beam, n, X.shape[0])
+    D, _ = stack_matrices(data)
+    D = D.to(prob.cdt if not prob.ops.pixel else prob.rdt)
+    if D.shape[0] == 1 and prob.B > 1:
+        D = D.expand(prob.B, *D.shape[1:]).contiguous()
+    out = gradf_apply(prob.plan, prob.phis, None, D, prob.ops, prob.B, prob.C, n)
+    return unstack_matrices(tmpl, -out, lead) if like is not None else \
+        _as_fourier_like(data, -out, lead)
+
+
+def _as_fourier_like(data: Field, X: torch.Tensor, lead) -> Field:
+    from .constants import Basis
+    f = unstack_matrices(data, X, lead)
+    return f.replace(basis=Basis.FOURIER)
+
+
+def hessian_logpdf_preconditioner(field_covariance, beam, mask, noise_covariance):
+    """P = Cf^-1 + B M Cn^-1 M B (wiener_filter.py:35-43).  For a PixelMask the
+    mask is replaced by its mean square (a Fourier-diagonal approximation)."""
+    if isinstance(mask, PixelMask):
+        return pinv(field_covariance) + beam * beam * pinv(noise_covariance) * mask.mean_square()
+    return pinv(field_covariance) + beam * mask * pinv(noise_covariance) * mask * beam
+
+
+def A_matrix_operator(field, phi, data, field_covariance, noise_covariance, mask, beam,
+                      n: int = DEFAULT_NSTEPS) -> Field:
+    """gradf_logpdf at d = 0 (wiener_filter.py:45-49)."""
+    X, lead = stack_matrices(field)
+    prob = _Problem(field, phi, field_covariance, noise_covariance, mask, beam, n, X.shape[0])
+    return unstack_matrices(field, prob.A(X.to(prob.cdt)), lead)
+
+
+def conjugate_gradient(x0, data, phi, mask, beam, noise_covariance, field_covariance, b_vector,
+                       preconditioner, maxiter: int = 500, tol: float = 1e-1, *,
+                       n: int = DEFAULT_NSTEPS, rtol: Optional[float] = None,
+                       recompute_every: Optional[int] = None, return_info: bool = False):
+    """PCG of wiener_filter.py:51-105, batched over right-hand sides, float64
+    vectors and scalars around a float32 (or float64) operator."""
+    X0, lead = stack_matrices(x0)
+    B = X0.shape[0]
+    prob = _Problem(x0, phi, field_covariance, noise_covariance, mask, beam, n, B)
+    dev = prob.device
+    x = X0.to(torch.complex128).contiguous().clone()
+    bvec, _ = stack_matrices(b_vector)
+    bvec = bvec.to(torch.complex128)
+    if bvec.shape[0] == 1 and B > 1:
+        bvec = bvec.expand(B, *bvec.shape[1:])
+    bvec = bvec.contiguous()
+    pinv_P = pinv(preconditioner)
+    if isinstance(pinv_P, (DiagonalScalar, DiagonalEB)):
+        pinv_arr = pinv_P.per_mode().to(device=dev, dtype=torch.float64).contiguous()
+    else:
+        raise NotImplementedError("preconditioner must be DiagonalScalar or DiagonalEB")
+
+    r = torch.empty_like(x)
+    z = torch.empty_like(x)
+    scratch = torch.empty_like(x)
+    ones = torch.ones(B, dtype=torch.float64, device=dev)
+
+    def residual(xv):
+        """r = b - A x, z = P^-1 r, returns dot(r, z)  (FP64 arithmetic)."""
+        Ax = prob.A(xv.to(prob.cdt))
+        r.copy_(bvec)
+        scratch.zero_()
+        rz = torch.empty(B, dtype=torch.float64, device=dev)
+        _lib.cg_update_xrz(prob.plan, torch.view_as_real(scratch), torch.view_as_real(r),
+                           torch.view_as_real(z), torch.view_as_real(xv), torch.view_as_real(Ax),
+                           pinv_arr, ones, B, prob.C, rz)
+        return rz
+
+    res = residual(x)                                         # :61-69
+    p = z.clone()
+    p_op = p.to(prob.cdt).contiguous()
+    bb = prob.dot(bvec, bvec).cpu().double()
+    res_h = res.cpu().double()
+    res_curr = res_h.clone()
+    k = torch.zeros(B, dtype=torch.long)
+
+    def rel_now():
+        return torch.sqrt(prob.dot(r, r).cpu().double() / torch.where(bb > 0, bb, torch.ones_like(bb)))
+
+    rel = rel_now() if rtol is not None else None
+
+    def active_mask():
+        act = (res_curr >= tol) & (k < maxiter)               # :75-77
+        if rtol is not None:
+            act &= rel >= rtol
+        return act
+
+    info = CGInfo(iterations=[0] * B, converged=[False] * B,
+                  res_history=[[float(v)] for v in res_h],
+                  rel_history=[[float(v)] for v in rel] if rel is not None else [])
+    act = active_mask()
+    it = 0
+    while bool(act.any()):
+        Ap = prob.A(p_op)                                     # :82
+        pAp = prob.dot(p, Ap).cpu().double()
+        alpha = torch.where(act, res_h / torch.where(pAp != 0, pAp, torch.ones_like(pAp)),
+                            torch.zeros_like(res_h))          # :84
+        rz = torch.empty(B, dtype=torch.float64, device=dev)
+        _lib.cg_update_xrz(prob.plan, torch.view_as_real(x), torch.view_as_real(r),
+                           torch.view_as_real(z), torch.view_as_real(p), torch.view_as_real(Ap),
+                           pinv_arr, alpha.to(dev), B, prob.C, rz)       # :86-92
+        rz_h = rz.cpu().double()
+        beta = torch.where(act, rz_h / torch.where(res_h != 0, res_h, torch.ones_like(res_h)),
+                           torch.zeros_like(res_h))       # :94
+        _lib.cg_update_p(prob.plan, torch.view_as_real(p), torch.view_as_real(z), beta.to(dev),
+                         torch.view_as_real(p_op), B, prob.C)            # :95
+        res_curr = torch.where(act, rz_h, res_curr)
+        res_h = torch.where(act, rz_h, res_h)
+        k += act.long()
+        it += 1
+        if recompute_every and it % recompute_every == 0:
+            res_new = residual(x).cpu().double()
+            res_curr = torch.where(act, res_new, res_curr)
+            res_h = torch.where(act, res_new, res_h)
+        if rtol is not None:
+            rel = rel_now()
+        for i in range(B):
+            if bool(act[i]):
+                info.res_history[i].append(float(res_curr[i]))
+                if rel is not None:
+                    info.rel_history[i].append(float(rel[i]))
+        act = active_mask()
+
+    info.iterations = [int(v) for v in k]
+    conv = res_curr < tol
+    if rtol is not None:
+        conv = conv | (rel < rtol)
+    info.converged = [bool(v) for v in conv]
+    if return_info:
+        residual(x)
+        info.final_true_rel_residual = [float(v) for v in rel_now()]
+    out = unstack_matrices(x0, x.to(X0.dtype if X0.is_complex() else torch.complex128), lead)
+    return (out, info) if return_info else out
+
+
+def wiener_filter(field, phi, data, field_covariance, noise_covariance, mask, beam, *,
+                  n: int = DEFAULT_NSTEPS, maxiter: int = 500, tol: float = 1e-1,
+                  rtol: Optional[float] = None, recompute_every: Optional[int] = None,
+                  return_info: bool = False):
+    """wiener_filter.py:10-26: b = -gradf(0, d), P = Cf^-1 + B M Cn^-1 M B,
+    PCG from `field`."""
+    b = get_b_vector(phi, data, field_covariance, noise_covariance, mask, beam, n=n, like=field)
+    P = hessian_logpdf_preconditioner(field_covariance, beam, mask, noise_covariance)
+    return conjugate_gradient(field, data, phi, mask, beam, noise_covariance, field_covariance, b, P,
+                              maxiter, tol, n=n, rtol=rtol, recompute_every=recompute_every,
+                              return_info=return_info)

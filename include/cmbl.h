@@ -80,6 +80,49 @@ cmbl_status cmbl_lenseflow(cmbl_plan* plan, const cmbl_phi* const* phis, const v
 cmbl_status cmbl_rfft2(cmbl_plan* plan, const void* x, void* X, int nmaps, void* stream);
 cmbl_status cmbl_irfft2(cmbl_plan* plan, const void* X, void* x, int nmaps, void* stream);
 
+/* ---- lensed Wiener filter -------------------------------------------------
+ * Diagonal operators are real arrays [ncomp][N][N/2+1] in the plan dtype,
+ * shared by all batch elements (DiagonalScalar / DiagonalEB,
+ * matrix_operators.py:45-90).  Fields are EB (or T) FOURIER arrays. */
+typedef struct {
+  const void* mb;       /* M B, per mode (Fourier-mask mode) */
+  const void* bmcn;     /* B M Cn^-1, per mode (Fourier-mask mode) */
+  const void* cfinv;    /* Cf^-1 per mode; NULL drops the prior term */
+  const void* beam;     /* B per mode (pixel-mask mode) */
+  const void* cninv;    /* Cn^-1 per mode (pixel-mask mode) */
+  const void* mask_pix; /* real [N][N] pixel mask; NULL selects the Fourier-mask mode */
+} cmbl_wf_ops;
+
+/* gradf_logpdf (gradients.py:12-26):
+ *   out = L^T B M^T Cn^-1 (d - M B L f) - Cf^-1 f       (EB FOURIER in and out)
+ * with L = LenseFlow(phi, nsteps) forward and L^T its adjoint, and the EB<->QU /
+ * map / fourier conversions of the reference.  f == NULL means f = 0 (the
+ * forward LenseFlow of zeros is skipped); d == NULL means d = 0 (the CG
+ * operator A, wiener_filter.py:45-49).  d is EB FOURIER in Fourier-mask mode,
+ * a [batch][ncomp][N][N] MAP (T or Q,U) in pixel-mask mode.  out may not alias
+ * f or d. */
+cmbl_status cmbl_gradf_logpdf(cmbl_plan* plan, const cmbl_phi* const* phis, const void* f,
+                              const void* d, const cmbl_wf_ops* ops, void* out, int batch,
+                              int ncomp, int nsteps, void* stream);
+
+/* FP64 inner products for nrhs right-hand sides (fields.py:136-142): out[k] =
+ * Re sum conj(a_k) b_k w / N^2 over [ncomp][N][N/2+1].  a is complex double;
+ * b is complex double (b_f64 = 1) or complex in the plan dtype.  out is a
+ * device array of nrhs doubles. */
+cmbl_status cmbl_dot_f64(cmbl_plan* plan, const void* a, const void* b, int b_f64, int nrhs,
+                         int ncomp, double* out, void* stream);
+/* One PCG update (wiener_filter.py:84-92) for nrhs systems, FP64 vectors:
+ *   x += alpha p ; r -= alpha Ap ; z = pinv .* r ; rz = dot(r, z)
+ * Ap is complex in the plan dtype (the operator's output); pinv is a double
+ * [ncomp][N][N/2+1] diagonal; alpha and rz are device arrays of nrhs doubles. */
+cmbl_status cmbl_cg_update_xrz(cmbl_plan* plan, void* x, void* r, void* z, const void* p,
+                               const void* Ap, const double* pinv, const double* alpha, int nrhs,
+                               int ncomp, double* rz, void* stream);
+/* p = z + beta p (wiener_filter.py:94-95); also writes p rounded to the plan
+ * dtype into p_op when p_op is not NULL. */
+cmbl_status cmbl_cg_update_p(cmbl_plan* plan, void* p, const void* z, const double* beta,
+                             void* p_op, int nrhs, int ncomp, void* stream);
+
 /* Timing mode (no counterpart in the reference; measurement aid): when on,
  * every LenseFlow stage launch is bracketed by HIP events on the call's stream
  * and cmbl_lenseflow waits for its own work before returning.

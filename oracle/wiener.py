@@ -67,14 +67,25 @@ def phi_map(phi):
     return phi if phi.shape[-2] == phi.shape[-1] else np.fft.irfft2(phi)
 
 
-def gradf_logpdf(f, phi, d, Cf, Cn, M, B, theta_pix, n=10):
-    """d logpdf / d f = L^T B M Cn^-1 (d - M B L f) - Cf^-1 f (gradients.py:12-26)."""
+def gradf_logpdf(f, phi, d, Cf, Cn, M, B, theta_pix, n=10, mask_pix=None):
+    """d logpdf / d f = L^T B M Cn^-1 (d - M B L f) - Cf^-1 f (gradients.py:12-26).
+
+    With ``mask_pix`` (a real [N, N] map; an extension, the reference's mask is
+    Fourier-diagonal) M acts on the T or Q/U maps, ``d`` is a MAP array
+    [C, N, N] and ``M`` is ignored:
+        g = B qu2eb(fourier(m map(eb2qu(Cn^-1 qu2eb(fourier(d - m map(eb2qu(B L f))))))))"""
     N = f.shape[-2]
     pw = lf.pix_width_from_theta(theta_pix)
     fq = to_map(eb2qu_field(f, N, theta_pix))                       # :16
     phim = phi_map(phi)                                             # :17
     lensed = qu2eb_field(to_fourier(lf.lense_flow(fq, phim, pw, n)), N, theta_pix)  # :18
-    g = B * M * pinv(Cn) * (d - M * B * lensed)                     # :20
+    if mask_pix is None:
+        g = B * M * pinv(Cn) * (d - M * B * lensed)                 # :20
+    else:
+        q = to_map(eb2qu_field(B * lensed, N, theta_pix))
+        r = d - mask_pix * q
+        s_ = to_map(eb2qu_field(pinv(Cn) * qu2eb_field(to_fourier(r), N, theta_pix), N, theta_pix))
+        g = B * qu2eb_field(to_fourier(mask_pix * s_), N, theta_pix)
     g = to_map(eb2qu_field(g, N, theta_pix))                        # :22
     g = lf.lense_flow(g, phim, pw, n, adjoint=True)                 # :23
     return (qu2eb_field(to_fourier(g), N, theta_pix)
@@ -95,18 +106,28 @@ def logpdf(f, phi, d, Cn, Cphi, Cf, M, B, theta_pix, n=10):
     return -np.real(data_product + field_product + phi_product + logdets) / 2   # :164-165
 
 
-def preconditioner(Cf, B, M, Cn):
-    """P = Cf^-1 + B M Cn^-1 M B (wiener_filter.py:35-43)."""
+def preconditioner(Cf, B, M, Cn, mask_pix=None):
+    """P = Cf^-1 + B M Cn^-1 M B (wiener_filter.py:35-43); with a pixel mask M
+    is replaced by its mean square."""
+    if mask_pix is not None:
+        return pinv(Cf) + B * B * pinv(Cn) * float(np.mean(mask_pix ** 2))
     return pinv(Cf) + B * M * pinv(Cn) * M * B
 
 
 def conjugate_gradient(x0, d, phi, M, B, Cn, Cf, b, P, theta_pix, maxiter=500, tol=1e-1,
-                       n=10, history=None):
-    """PCG exactly as wiener_filter.py:51-105 (absolute stop on dot(r, P^-1 r))."""
+                       n=10, history=None, mask_pix=None, rtol=None):
+    """PCG exactly as wiener_filter.py:51-105 (absolute stop on dot(r, P^-1 r));
+    ``rtol`` adds the relative stop sqrt(dot(r,r)/dot(b,b)) < rtol."""
     N = x0.shape[-2]
 
     def A(x):                                                       # :45-49, :57-59
-        return gradf_logpdf(x, phi, 0 * x, Cf, Cn, M, B, theta_pix, n)
+        dz = 0 * x if mask_pix is None else np.zeros(x.shape[:-1] + (N,))
+        return gradf_logpdf(x, phi, dz, Cf, Cn, M, B, theta_pix, n, mask_pix)
+
+    bb = lf.field_dot(b, b, N)
+
+    def rel_ok(r):
+        return rtol is not None and np.sqrt(lf.field_dot(r, r, N) / bb) < rtol
 
     Pinv = pinv(P)                                                  # :63
     r = b - A(x0)                                                   # :61-62
@@ -116,7 +137,7 @@ def conjugate_gradient(x0, d, phi, M, B, Cn, Cf, b, P, theta_pix, maxiter=500, t
     res_curr = res
     x = x0
     k = 0
-    while res_curr >= tol and k < maxiter:                          # :75-77
+    while res_curr >= tol and k < maxiter and not rel_ok(r):        # :75-77
         Ap = A(p)                                                   # :82
         alpha = res / lf.field_dot(p, Ap, N)                        # :84
         x = x + alpha * p                                           # :86
@@ -132,9 +153,9 @@ def conjugate_gradient(x0, d, phi, M, B, Cn, Cf, b, P, theta_pix, maxiter=500, t
 
 
 def wiener_filter(x0, phi, d, Cf, Cn, M, B, theta_pix, maxiter=500, tol=1e-1, n=10,
-                  history=None):
+                  history=None, mask_pix=None, rtol=None):
     """wiener_filter.py:10-26 (b = -gradf(0, d); P as above; PCG)."""
-    b = -gradf_logpdf(0 * d, phi, d, Cf, Cn, M, B, theta_pix, n)    # :28-33
-    P = preconditioner(Cf, B, M, Cn)
+    b = -gradf_logpdf(0 * x0, phi, d, Cf, Cn, M, B, theta_pix, n, mask_pix)   # :28-33
+    P = preconditioner(Cf, B, M, Cn, mask_pix)
     return conjugate_gradient(x0, d, phi, M, B, Cn, Cf, b, P, theta_pix, maxiter, tol, n,
-                              history)
+                              history, mask_pix, rtol)

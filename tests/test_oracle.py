@@ -218,3 +218,72 @@ def test_julia_grad_phi_I():
     _, _, dphi = olf.lensing_operator_gradients(phi, list(Lf), list(cot), -1, 7, pw)
     g = dphi - owf.pinv(jl("cphi_I")) * phi
     assert rel(g, jl("grad_phi_I")) < 2e-3
+
+
+# ------------------------------------------------------- Wiener-filter pins
+def _sims():
+    import importlib.util
+    from conftest import ROOT
+    spec = importlib.util.spec_from_file_location("s2", ROOT / "cmblensing.py_amd" / "sims.py")
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return m
+
+
+@pytest.mark.parametrize("mask", ["fourier", "square"])
+def test_wiener_filter_dense_direct_solve(mask):
+    """At N = 16 the operator A is built densely on maps (A_m = irfft2 A rfft2):
+    it is symmetric to rounding (the adjoint LenseFlow is the exact transpose)
+    and negative definite, and PCG reaches the direct solution."""
+    sims = _sims()
+    N, th, n = 16, 3.0, 4
+    prob = sims.wf_problem(N, th, pol=False, seed=2, mask=mask)
+    fq = owf.to_map(prob["f"])
+    lensed = olf.lense_flow(fq, prob["phi"], olf.pix_width_from_theta(th), n)
+    prob = sims.wf_data(prob, lensed)
+    args = (prob["Cf"], prob["Cn"], prob["M"], prob["B"], th)
+
+    def A_map(m):
+        F = np.fft.rfft2(m)[None]
+        dz = np.zeros((1, N, N)) if prob["mask_pix"] is not None else 0 * F
+        return np.fft.irfft2(owf.gradf_logpdf(F, prob["phi"], dz, *args, n=n,
+                                              mask_pix=prob["mask_pix"]))[0]
+    cols = []
+    for j in range(N * N):
+        e = np.zeros(N * N)
+        e[j] = 1.0
+        cols.append(A_map(e.reshape(N, N)).ravel())
+    Am = np.array(cols).T
+    assert np.abs(Am - Am.T).max() < 1e-10 * np.abs(Am).max()
+    assert np.linalg.eigvalsh(0.5 * (Am + Am.T)).max() < 0
+    b = -owf.gradf_logpdf(0 * prob["f"], prob["phi"], prob["d"], *args, n=n,
+                          mask_pix=prob["mask_pix"])
+    direct = np.fft.rfft2(np.linalg.solve(Am, np.fft.irfft2(b)[0].ravel()).reshape(N, N))
+    x = owf.wiener_filter(0 * prob["f"], prob["phi"], prob["d"], *args, n=n, tol=0.0,
+                          rtol=1e-12, maxiter=500, mask_pix=prob["mask_pix"])
+    assert rel(x[0], direct) < 1e-9
+
+
+def test_wiener_filter_julia_I():
+    """The reference's CG (tol 0.1 absolute) at n = 7 reproduces CMBLensing.jl's
+    Wiener filter of the I problem from the truth start."""
+    o = _ops_I()
+    x = owf.wiener_filter(jl("t_field_I")[None], jl("phi_I"), o["d"], o["Cf"], o["Cn"], o["M"],
+                          o["B"], 2.5, n=7)
+    wj = jl("f_wf_ffpp_I")
+    assert rel(x[0], wj) < 1e-5
+    assert rel(x[0][:, 1:-1], wj[:, 1:-1]) < 1e-6
+
+
+def test_prior_round_trip_is_hermitian_projection():
+    """qu2eb(fourier(map(eb2qu(f)))) replaces only the ky = 0 and ky = N/2
+    columns by their Hermitian parts (the GPU prior term uses this identity)."""
+    N, th = 16, 2.0
+    rng = np.random.default_rng(5)
+    f = rng.standard_normal((2, N, N // 2 + 1)) + 1j * rng.standard_normal((2, N, N // 2 + 1))
+    rt = owf.qu2eb_field(owf.to_fourier(owf.to_map(owf.eb2qu_field(f, N, th))), N, th)
+    proj = f.copy()
+    for j in (0, N // 2):
+        col = f[:, :, j]
+        proj[:, :, j] = 0.5 * (col + np.conj(col[:, (-np.arange(N)) % N]))
+    assert np.allclose(rt, proj, atol=1e-12)
