@@ -45,6 +45,8 @@ _SIGS = {
                                    c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "cmbl_cg_update_p": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
                                  c_void_p]),
+    "cmbl_lenseflow_vjp": (c_int, [c_void_p, ctypes.POINTER(c_void_p), c_void_p, c_void_p,
+                                   c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p]),
     "cmbl_plan_set_timing": (c_int, [c_void_p, c_int]),
     "cmbl_plan_get_timing": (c_int, [c_void_p, ctypes.POINTER(c_double), c_int]),
 }
@@ -209,29 +211,37 @@ def get_plan(N: int, theta_pix: float, dtype: torch.dtype, nmaps: int, device) -
         # plan geometry and dtype are unchanged; re-key them to the new plan.
         for k, v in list(_phis.items()):
             if k[0] == key:
-                v.plan = plan
+                v[2].plan = plan
     return plan
 
 
-def get_phi(plan: Plan, phi: torch.Tensor, nsteps: int) -> PhiHandle:
-    """Prepared phi for (plan geometry, tensor identity and version, nsteps)."""
+def get_phi(plan: Plan, phi: torch.Tensor, nsteps: int, index=None) -> PhiHandle:
+    """Prepared phi for (plan geometry, the tensor object and its version,
+    nsteps) -- or for ``phi[index]`` of a batched phi tensor.  Entries are
+    keyed by object identity (checked through a weak reference), never by
+    address: torch's caching allocator hands freed addresses to new tensors."""
+    import weakref
     pkey = (plan.device.index, plan.N, round(plan.theta_pix, 12), plan.dtype)
-    key = (pkey, phi.data_ptr(), phi._version, tuple(phi.shape), str(phi.dtype), nsteps)
-    h = _phis.get(key)
-    if h is not None:
-        _phis.move_to_end(key)
-        return h
-    h = PhiHandle(plan, phi, nsteps)
-    _phis[key] = h
+    key = (pkey, id(phi), index, nsteps)
+    ent = _phis.get(key)
+    if ent is not None:
+        ref, version, h = ent
+        if ref() is phi and version == phi._version:
+            _phis.move_to_end(key)
+            return h
+        del _phis[key]
+        h.close()
+    h = PhiHandle(plan, phi if index is None else phi[index], nsteps)
+    _phis[key] = (weakref.ref(phi), phi._version, h)
     while len(_phis) > PHI_CACHE_SIZE:
-        _, old = _phis.popitem(last=False)
+        _, (_, _, old) = _phis.popitem(last=False)
         old.close()
     return h
 
 
 def clear_caches():
     for v in _phis.values():
-        v.close()
+        v[2].close()
     _phis.clear()
     for p in _plans.values():
         p.close()
@@ -297,3 +307,13 @@ def cg_update_p(plan: Plan, p, z, beta, p_op, nrhs: int, ncomp: int):
         check(plan.lib.cmbl_cg_update_p(plan.handle, p.data_ptr(), z.data_ptr(), beta.data_ptr(),
                                         _ptr(p_op), nrhs, ncomp, stream_ptr(plan.device)),
               "cmbl_cg_update_p")
+
+
+def lenseflow_vjp(plan: Plan, phis, lensed, cot, d_field, d_phi, batch: int, ncomp: int,
+                  nsteps: int, direction: int):
+    arr = (c_void_p * batch)(*[p.handle for p in phis])
+    with torch.cuda.device(plan.device):
+        check(plan.lib.cmbl_lenseflow_vjp(plan.handle, arr, lensed.data_ptr(), cot.data_ptr(),
+                                          d_field.data_ptr(), d_phi.data_ptr(), batch, ncomp,
+                                          nsteps, int(direction), stream_ptr(plan.device)),
+              "cmbl_lenseflow_vjp")

@@ -83,6 +83,14 @@ struct cmbl_plan {
   void* wx2 = nullptr;
   double* red = nullptr;
   int red_rhs = 0;
+  // gradient-ODE scratch, allocated on first use
+  void* gy = nullptr;    // f state copy [max_maps][N][N]
+  void* gacc = nullptr;  // delta-f accumulator [max_maps][N][N]
+  void* gzd = nullptr;   // delta-f stage input [max_maps][N][N]
+  void* gu = nullptr;    // U, V packed spectra [max_maps][N][N/2]
+  void* gv = nullptr;
+  void* gsrc = nullptr;  // delta-phi sources [max_maps][5][N][N]
+  void* gF = nullptr;    // their rfft2 [max_maps][5][N][N/2+1]
 };
 
 struct cmbl_phi {
@@ -92,6 +100,7 @@ struct cmbl_phi {
   const cmbl_plan* plan = nullptr;
   double* deriv = nullptr;  // [5][N][N] double
   void* ptab = nullptr;     // [(2n+1)][2][N][N] T
+  void* gtab = nullptr;     // [(2n+1)][3][N][N] T, M^-1 for the gradient ODE (lazy)
   double min_det = 0;
 };
 
@@ -139,8 +148,8 @@ cmbl_status cmbl_plan_create(cmbl_plan** out, int N, double theta_pix_arcmin, in
   if ((e = hipMalloc(&p->acc, max_maps * NN * p->tsize)) != hipSuccess ||
       (e = hipMalloc(&p->s0, max_maps * NH * 2 * p->tsize)) != hipSuccess ||
       (e = hipMalloc(&p->s1, max_maps * NH * 2 * p->tsize)) != hipSuccess ||
-      (e = hipMalloc((void**)&p->ptab_dev, kRing * max_maps * sizeof(void*))) != hipSuccess ||
-      (e = hipHostMalloc((void**)&p->ptab_host, kRing * max_maps * sizeof(void*))) != hipSuccess)
+      (e = hipMalloc((void**)&p->ptab_dev, kRing * 2 * max_maps * sizeof(void*))) != hipSuccess ||
+      (e = hipHostMalloc((void**)&p->ptab_host, kRing * 2 * max_maps * sizeof(void*))) != hipSuccess)
     return cleanup(fail(CMBL_ENOMEM, std::string("plan scratch: ") + hipGetErrorString(e)));
   for (int i = 0; i < kRing; ++i)
     if ((e = hipEventCreateWithFlags(&p->ring_ev[i], hipEventDisableTiming)) != hipSuccess)
@@ -163,7 +172,8 @@ cmbl_status cmbl_plan_destroy(cmbl_plan* p) {
   if (p->s0) (void)hipFree(p->s0);
   if (p->s1) (void)hipFree(p->s1);
   if (p->ptab_dev) (void)hipFree(p->ptab_dev);
-  for (void* q : {p->wy1, p->wy2, p->wx1, p->wx2, (void*)p->red})
+  for (void* q : {p->wy1, p->wy2, p->wx1, p->wx2, (void*)p->red, p->gy, p->gacc, p->gzd, p->gu,
+                  p->gv, p->gsrc, p->gF})
     if (q) (void)hipFree(q);
   if (p->ptab_host) (void)hipHostFree(p->ptab_host);
   delete p;
@@ -285,6 +295,7 @@ cmbl_status cmbl_phi_release(cmbl_phi* h) {
   if (!h) return CMBL_OK;
   if (h->deriv) (void)hipFree(h->deriv);
   if (h->ptab) (void)hipFree(h->ptab);
+  if (h->gtab) (void)hipFree(h->gtab);
   delete h;
   return CMBL_OK;
 }
@@ -418,8 +429,8 @@ extern "C" cmbl_status cmbl_lenseflow(cmbl_plan* p, const cmbl_phi* const* phis,
   const int slot = p->ring_next;
   p->ring_next = (slot + 1) % kRing;
   if (p->ring_used[slot]) HIP_TRY(hipEventSynchronize(p->ring_ev[slot]));
-  void** host = p->ptab_host + (size_t)slot * p->max_maps;
-  void** dev = p->ptab_dev + (size_t)slot * p->max_maps;
+  void** host = p->ptab_host + (size_t)slot * 2 * p->max_maps;
+  void** dev = p->ptab_dev + (size_t)slot * 2 * p->max_maps;
   for (int b = 0; b < batch; ++b) host[b] = phis[b]->ptab;
   HIP_TRY(hipMemcpyAsync(dev, host, batch * sizeof(void*), hipMemcpyHostToDevice, s));
   cmbl_status st;
@@ -615,5 +626,154 @@ extern "C" cmbl_status cmbl_cg_update_p(cmbl_plan* p, void* pv, const void* z, c
   else
     HIP_TRY(cmbl::launch_cg_p<double>((cplx<double>*)pv, (const cplx<double>*)z, beta,
                                       (cplx<double>*)p_op, len, nrhs, s));
+  return CMBL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// gradient ODE (lense_flow_wrapper backward)
+// ---------------------------------------------------------------------------
+namespace {
+
+cmbl_status ensure_grad_scratch(cmbl_plan* p) {
+  if (p->gy) return CMBL_OK;
+  const size_t NN = (size_t)p->N * p->N, NH = (size_t)p->N * (p->N / 2),
+               NH1 = (size_t)p->N * (p->N / 2 + 1);
+  const size_t M = p->max_maps, ts = p->tsize;
+  hipError_t e;
+  if ((e = hipMalloc(&p->gy, M * NN * ts)) != hipSuccess ||
+      (e = hipMalloc(&p->gacc, M * NN * ts)) != hipSuccess ||
+      (e = hipMalloc(&p->gzd, M * NN * ts)) != hipSuccess ||
+      (e = hipMalloc(&p->gu, M * NH * 2 * ts)) != hipSuccess ||
+      (e = hipMalloc(&p->gv, M * NH * 2 * ts)) != hipSuccess ||
+      (e = hipMalloc(&p->gsrc, M * 5 * NN * ts)) != hipSuccess ||
+      (e = hipMalloc(&p->gF, M * 5 * NH1 * 2 * ts)) != hipSuccess)
+    return fail(CMBL_ENOMEM, std::string("gradient scratch: ") + hipGetErrorString(e));
+  return CMBL_OK;
+}
+
+cmbl_status ensure_gtab(cmbl_plan* p, cmbl_phi* h, hipStream_t s) {
+  if (h->gtab) return CMBL_OK;
+  const size_t NN = (size_t)p->N * p->N;
+  HIP_TRY(hipMalloc(&h->gtab, (size_t)(2 * h->nsteps + 1) * 3 * NN * p->tsize));
+  if (p->dtype == CMBL_F32)
+    HIP_TRY(cmbl::launch_gtab<float>(p->N, h->deriv, (float*)h->gtab, h->nsteps, s));
+  else
+    HIP_TRY(cmbl::launch_gtab<double>(p->N, h->deriv, (double*)h->gtab, h->nsteps, s));
+  return CMBL_OK;
+}
+
+template <typename T>
+cmbl_status run_vjp(cmbl_plan* p, const void* const* ptabs, const void* const* gtabs,
+                    const T* lensed, const T* cot, T* dfield, cplx<T>* dphi, int batch, int C,
+                    int n, int direction, hipStream_t s) {
+  const int N = p->N;
+  const long nm = (long)batch * C;
+  const long NN = (long)N * N;
+  const cplx<T>* tw = (const cplx<T>*)p->tw;
+  cplx<T>* Sf = (cplx<T>*)p->s0;
+  cplx<T>* DXf = (cplx<T>*)p->s1;
+  cplx<T>* Ud = (cplx<T>*)p->gu;
+  cplx<T>* Vd = (cplx<T>*)p->gv;
+  T* yf = (T*)p->gy;
+  T* accf = (T*)p->acc;
+  T* accd = (T*)p->gacc;
+  T* zd = (T*)p->gzd;
+  T* src = (T*)p->gsrc;
+  const T* const* pt = (const T* const*)ptabs;
+  // lense_flow.py:30 integrates with -direction; :62-74 picks t 1 -> 0 for -1
+  const bool fwd_time = (-direction) != CMBL_INVERSE_LENSE;
+  const double t0 = fwd_time ? 0.0 : 1.0, t1 = fwd_time ? 1.0 : 0.0;
+  const double hd = (t1 - t0) / n;
+  const T h = (T)hd;
+  auto tindex = [&](int i, int half_steps) {
+    const int j = 2 * i + half_steps;
+    return fwd_time ? j : 2 * n - j;
+  };
+  static const int stage_half[5] = {0, 0, 1, 1, 2};
+  static const double bw[5] = {0, 1.0 / 6, 1.0 / 3, 1.0 / 3, 1.0 / 6};
+  HIP_TRY(hipMemcpyAsync(yf, lensed, nm * NN * sizeof(T), hipMemcpyDeviceToDevice, s));
+  if (dfield != cot)
+    HIP_TRY(hipMemcpyAsync(dfield, cot, nm * NN * sizeof(T), hipMemcpyDeviceToDevice, s));
+  HIP_TRY(hipMemcpyAsync(zd, cot, nm * NN * sizeof(T), hipMemcpyDeviceToDevice, s));
+  HIP_TRY(cmbl::launch_fill_zero<T>(src, (long)batch * 5 * NN, s));
+  HIP_TRY(cmbl::launch_lf_row_r2c<T>(N, yf, Sf, nm, tw, s));
+  HIP_TRY(cmbl::launch_lf_row<T>(N, true, Ud, Vd, dfield, accd, pt, tw, C, nm, 1, h, tindex(0, 0),
+                                 1, 1, (T)p->dk, s));
+  for (int i = 0; i < n; ++i) {
+    for (int st = 1; st <= 4; ++st) {
+      const bool last = (i == n - 1) && (st == 4);
+      HIP_TRY(cmbl::launch_lf_col<T>(N, Sf, DXf, nm, tw, (T)p->dk, s));
+      HIP_TRY(cmbl::launch_lf_col<T>(N, Ud, Ud, nm, tw, (T)p->dk, s));
+      const int tidx = tindex(i, stage_half[st]);
+      cmbl::GradArgs<T> ga;
+      ga.zd = zd;
+      ga.gtabs = (const T* const*)gtabs;
+      ga.sacc = src;
+      ga.cs = (T)(hd * bw[st]);
+      ga.tval = (T)((double)tidx / (2.0 * n));
+      HIP_TRY(cmbl::launch_lf_row<T>(N, false, Sf, DXf, yf, accf, pt, tw, C, nm, st, h, tidx,
+                                     last ? 0 : 1, 0, (T)p->dk, s, nullptr, &ga));
+      int tn = st < 4 ? tindex(i, stage_half[st + 1]) : tindex(i + 1, 0);
+      if (last) tn = 0;
+      HIP_TRY(cmbl::launch_lf_row<T>(N, true, Ud, Vd, dfield, accd, pt, tw, C, nm, st, h, tn,
+                                     last ? 0 : 1, 0, (T)p->dk, s, last ? nullptr : zd));
+    }
+  }
+  // delta phi = sum of multipliers times rfft2 of the accumulated sources
+  cplx<T>* F = (cplx<T>*)p->gF;
+  for (long m0 = 0; m0 < (long)batch * 5; m0 += p->max_maps) {
+    const long cnt = std::min<long>(p->max_maps, (long)batch * 5 - m0);
+    HIP_TRY(cmbl::launch_rfft2<T>(N, src + m0 * NN, Sf, F + m0 * (long)N * (N / 2 + 1), cnt, tw,
+                                  s));
+  }
+  HIP_TRY(cmbl::launch_dphi_combine<T>(N, F, dphi, batch, p->dk, s));
+  return CMBL_OK;
+}
+
+}  // namespace
+
+extern "C" cmbl_status cmbl_lenseflow_vjp(cmbl_plan* p, const cmbl_phi* const* phis,
+                                          const void* lensed, const void* cotangent,
+                                          void* d_field, void* d_phi, int batch, int ncomp,
+                                          int nsteps, int direction, void* stream) {
+  if (!p || !phis || !lensed || !cotangent || !d_field || !d_phi)
+    return fail(CMBL_EINVAL, "NULL argument");
+  if (ncomp != 1 && ncomp != 2) return fail(CMBL_EINVAL, "ncomp must be 1 (T) or 2 (QU)");
+  if (direction != CMBL_FORWARD_LENSE && direction != CMBL_INVERSE_LENSE)
+    return fail(CMBL_EINVAL, "direction must be +1 or -1");
+  if ((long)batch * ncomp > p->max_maps)
+    return fail(CMBL_EINVAL, "batch*ncomp exceeds the plan's max_maps");
+  for (int b = 0; b < batch; ++b) {
+    if (!phis[b]) return fail(CMBL_EINVAL, "phi handle is NULL");
+    if (phis[b]->N != p->N || phis[b]->dtype != p->dtype)
+      return fail(CMBL_EINVAL, "phi handle was prepared for another geometry or dtype");
+    if (phis[b]->nsteps != nsteps) return fail(CMBL_EINVAL, "phi handle prepared for another nsteps");
+  }
+  if (batch == 0) return CMBL_OK;
+  hipStream_t s = (hipStream_t)stream;
+  CMBL_TRY(ensure_grad_scratch(p));
+  for (int b = 0; b < batch; ++b) CMBL_TRY(ensure_gtab(p, const_cast<cmbl_phi*>(phis[b]), s));
+  const int slot = p->ring_next;
+  p->ring_next = (slot + 1) % kRing;
+  if (p->ring_used[slot]) HIP_TRY(hipEventSynchronize(p->ring_ev[slot]));
+  void** host = p->ptab_host + (size_t)slot * 2 * p->max_maps;
+  void** dev = p->ptab_dev + (size_t)slot * 2 * p->max_maps;
+  for (int b = 0; b < batch; ++b) {
+    host[b] = phis[b]->ptab;
+    host[p->max_maps + b] = phis[b]->gtab;
+  }
+  HIP_TRY(hipMemcpyAsync(dev, host, 2 * p->max_maps * sizeof(void*), hipMemcpyHostToDevice, s));
+  cmbl_status st;
+  if (p->dtype == CMBL_F32)
+    st = run_vjp<float>(p, (const void* const*)dev, (const void* const*)(dev + p->max_maps),
+                        (const float*)lensed, (const float*)cotangent, (float*)d_field,
+                        (cplx<float>*)d_phi, batch, ncomp, nsteps, direction, s);
+  else
+    st = run_vjp<double>(p, (const void* const*)dev, (const void* const*)(dev + p->max_maps),
+                         (const double*)lensed, (const double*)cotangent, (double*)d_field,
+                         (cplx<double>*)d_phi, batch, ncomp, nsteps, direction, s);
+  if (st != CMBL_OK) return st;
+  HIP_TRY(hipEventRecord(p->ring_ev[slot], s));
+  p->ring_used[slot] = true;
   return CMBL_OK;
 }

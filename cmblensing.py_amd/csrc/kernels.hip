@@ -16,6 +16,8 @@
 // fftfreq (kx = N/2 maps to -N/2), KY from rfftfreq with the Nyquist negated.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "fft_lds.hpp"
 #include "kernels.hpp"
 
@@ -328,6 +330,49 @@ __global__ __launch_bounds__(256) void k_ptab(const double* __restrict__ Dm, T* 
   if (threadIdx.x == 0) minpart[blockIdx.x] = red[0];
 }
 
+// Inverse-magnification table for the gradient ODE: for t_j = j/(2n),
+// (M^-1_xx, M^-1_xy, M^-1_yy)(t_j) (lense_flow.py:397-415), T[(2n+1)][3][N*N].
+template <typename T>
+__global__ __launch_bounds__(256) void k_gtab(const double* __restrict__ Dm, T* __restrict__ gtab,
+                                              int N, int nsteps) {
+  const long P = (long)N * N;
+  const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= P) return;
+  const double hxx = Dm[2 * P + idx], hxy = Dm[3 * P + idx], hyy = Dm[4 * P + idx];
+  const int nt = 2 * nsteps;
+  for (int j = 0; j <= nt; ++j) {
+    const double t = (double)j / (double)nt;
+    const double mxx = 1 + t * hxx, mxy = t * hxy, myy = 1 + t * hyy;
+    const double inv = 1.0 / (mxx * myy - mxy * mxy + 1e-12);
+    gtab[(3L * j) * P + idx] = (T)(myy * inv);
+    gtab[(3L * j + 1) * P + idx] = (T)(-mxy * inv);
+    gtab[(3L * j + 2) * P + idx] = (T)(mxx * inv);
+  }
+}
+
+// delta-phi spectrum from the five accumulated source maps' rfft2 (natural
+// layout, F[b][5][N][N/2+1]):
+//   dphi = i KX F1 + i KY F2 - KX^2 F3 - KX KY F4 - KY^2 F5
+// with KY's Nyquist entry negated (lense_flow.py:233-251, util.py:103-107,133-137).
+template <typename T>
+__global__ void k_dphi_combine(const cplx<T>* __restrict__ F, cplx<T>* __restrict__ out, int N,
+                               long nb, double dk) {
+  const int H1 = N / 2 + 1;
+  const long P = (long)N * H1;
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < nb * P;
+       g += (long)gridDim.x * blockDim.x) {
+    const long b = g / P, idx = g % P;
+    const int kx = (int)(idx / H1), ky = (int)(idx % H1);
+    const T KX = (T)((kx < N / 2 ? kx : kx - N) * dk);
+    const T KY = (T)((ky == N / 2 ? -ky : ky) * dk);
+    const cplx<T>* f = F + b * 5 * P + idx;
+    const cplx<T> f1 = f[0], f2 = f[P], f3 = f[2 * P], f4 = f[3 * P], f5 = f[4 * P];
+    cplx<T> r = imul(f1, KX) + imul(f2, KY);
+    r = r - scale(f3, KX * KX) - scale(f4, KX * KY) - scale(f5, KY * KY);
+    out[b * P + idx] = r;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // LenseFlow fused stage kernels
 // ---------------------------------------------------------------------------
@@ -389,20 +434,38 @@ __device__ __forceinline__ void rk4_stage(int stage, T h, T k, T y, T& a, T& z) 
   }
 }
 
+// Extra operands of the standard-rate row pass inside the gradient ODE
+// (lense_flow.py:156-252): the stage inputs of delta f, the M^-1 tables, and
+// the five pixel-space source maps of the delta-phi rate, accumulated with the
+// RK4 weight of each stage (the rate is linear in them, so their rfft2 and the
+// derivative multipliers are applied once, after the last stage).
+template <typename T>
+__device__ __forceinline__ void add_sources(T* S, long NN, T cs, T tval, T fx, T fy, T px, T py,
+                                            T ixx, T ixy, T iyy) {
+  const T mx = ixx * fx + ixy * fy;   // lense_flow.py:221-222
+  const T my = ixy * fx + iyy * fy;
+  const T wxx = px * mx, wxy = px * my, wyx = py * mx, wyy = py * my;  // :226-229
+  S[0] += cs * mx;
+  S[NN] += cs * my;
+  S[2 * NN] += cs * (tval * wxx);
+  S[3 * NN] += cs * (tval * wxy + tval * wyx);
+  S[4 * NN] += cs * (tval * wyy);
+}
+
 // Row pass, standard rate.  For each row pair: C2R of (dz/dx + i dz/dy) per
 // row, k = p(t_s) . grad z, RK4 update of y/acc, then (unless this is the last
 // stage of the last step) the row R2C of the next stage input into S0.
 // S0 holds the row spectra S of the stage input, S1 their x-derivative DX;
 // the line is Z = DX + i DY with DY = i ky S / N (DY's packed slot is zero:
 // ky = 0 at DC, and the real Nyquist value times i drops out of the C2R).
-template <typename T, int N>
+template <typename T, int N, bool GRAD>
 __global__ __launch_bounds__(256) void k_row_lf_std(cplx<T>* __restrict__ S0,
                                                     const cplx<T>* __restrict__ S1,
                                                     T* __restrict__ y, T* __restrict__ acc,
                                                     const T* const* __restrict__ ptabs,
                                                     const cplx<T>* __restrict__ twg, int C,
                                                     long npairs, int stage, T h, int tidx,
-                                                    int do_r2c, T cy) {
+                                                    int do_r2c, T cy, GradArgs<T> ga) {
   constexpr int G = N / 8, PAIRS = 256 / G, H = N / 2;
   constexpr long NN = (long)N * N;
   __shared__ cplx<T> lbuf[PAIRS][2][lds_len<N>()];
@@ -458,6 +521,31 @@ __global__ __launch_bounds__(256) void k_row_lf_std(cplx<T>* __restrict__ S0,
   cplx<T>* const bufs[2] = {bufa, bufb};
   fft_lines<N, true, 2>(v, bufs, tw, t);
 
+  if constexpr (GRAD) {
+    // delta-phi sources at this stage: sum_c df_c grad f_c (lense_flow.py:201-213)
+    if (active) {
+      const T* gt = ga.gtabs[rp.b] + (long)tidx * 3 * NN;
+      T* Sb = ga.sacc + (long)rp.b * 5 * NN;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const int j = t + m * G;
+        const long ja = (long)rp.ia * N + j, jb = (long)rp.ib * N + j;
+        const T zda = ga.zd[rp.ra * N + j], zdb = ga.zd[rp.rb * N + j];
+        if (C == 2) {
+          const T fx = zda * v[0][m].x + zdb * v[1][m].x;
+          const T fy = zda * v[0][m].y + zdb * v[1][m].y;
+          add_sources(Sb + ja, NN, ga.cs, ga.tval, fx, fy, pxa[m], pya[m], gt[ja], gt[NN + ja],
+                      gt[2 * NN + ja]);
+        } else {
+          add_sources(Sb + ja, NN, ga.cs, ga.tval, zda * v[0][m].x, zda * v[0][m].y, pxa[m], pya[m],
+                      gt[ja], gt[NN + ja], gt[2 * NN + ja]);
+          add_sources(Sb + jb, NN, ga.cs, ga.tval, zdb * v[1][m].x, zdb * v[1][m].y, pxb[m], pyb[m],
+                      gt[jb], gt[NN + jb], gt[2 * NN + jb]);
+        }
+      }
+    }
+  }
+
   cplx<T> z[1][8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
@@ -497,7 +585,8 @@ __global__ __launch_bounds__(256) void k_row_lf_adj(cplx<T>* __restrict__ S0,
                                                     const T* const* __restrict__ ptabs,
                                                     const cplx<T>* __restrict__ twg, int C,
                                                     long npairs, int stage, T h, int tidx_next,
-                                                    int do_r2c, int init, T cy) {
+                                                    int do_r2c, int init, T cy,
+                                                    T* __restrict__ zstore) {
   constexpr int G = N / 8, PAIRS = 256 / G, H = N / 2;
   constexpr long NN = (long)N * N;
   __shared__ cplx<T> lbuf[PAIRS][2][lds_len<N>()];
@@ -579,6 +668,13 @@ __global__ __launch_bounds__(256) void k_row_lf_adj(cplx<T>* __restrict__ S0,
     }
   }
   if (!do_r2c) return;  // uniform over the grid
+  if (zstore && active) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      zstore[rp.ra * N + t + m * G] = za[m];
+      zstore[rp.rb * N + t + m * G] = zb[m];
+    }
+  }
   cplx<T> uv[2][8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
@@ -677,18 +773,40 @@ hipError_t launch_lf_col(int N, const cplx<T>* src, cplx<T>* dst, long nmaps, co
 template <typename T>
 hipError_t launch_lf_row(int N, bool adjoint, cplx<T>* S0, cplx<T>* S1, T* y, T* acc,
                          const T* const* ptabs, const cplx<T>* tw, int C, long nmaps, int stage,
-                         T h, int tidx, int do_r2c, int init, T dk, hipStream_t s) {
+                         T h, int tidx, int do_r2c, int init, T dk, hipStream_t s, T* zstore,
+                         const GradArgs<T>* ga) {
   CMBL_DISPATCH_N(N, {
     const long npairs = nmaps * NN_ / 2;
     const unsigned nb = blocks_for(npairs, 256 / (NN_ / 8));
     const T cy = dk / T(NN_);
-    if (adjoint)
+    if (adjoint) {
       k_row_lf_adj<T, NN_><<<nb, 256, 0, s>>>(S0, S1, y, acc, ptabs, tw, C, npairs, stage, h, tidx,
-                                              do_r2c, init, cy);
-    else
-      k_row_lf_std<T, NN_><<<nb, 256, 0, s>>>(S0, S1, y, acc, ptabs, tw, C, npairs, stage, h, tidx,
-                                              do_r2c, cy);
+                                              do_r2c, init, cy, zstore);
+    } else if (ga) {
+      k_row_lf_std<T, NN_, true><<<nb, 256, 0, s>>>(S0, S1, y, acc, ptabs, tw, C, npairs, stage, h,
+                                                    tidx, do_r2c, cy, *ga);
+    } else {
+      GradArgs<T> none{};
+      k_row_lf_std<T, NN_, false><<<nb, 256, 0, s>>>(S0, S1, y, acc, ptabs, tw, C, npairs, stage,
+                                                     h, tidx, do_r2c, cy, none);
+    }
   });
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_gtab(int N, const double* D, T* gtab, int nsteps, hipStream_t s) {
+  const long P = (long)N * N;
+  k_gtab<T><<<blocks_for(P, 256), 256, 0, s>>>(D, gtab, N, nsteps);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_dphi_combine(int N, const cplx<T>* F, cplx<T>* out, long nb, double dk,
+                               hipStream_t s) {
+  const long n = nb * (long)N * (N / 2 + 1);
+  k_dphi_combine<T><<<(unsigned)std::min<long>((n + 255) / 256, 65536), 256, 0, s>>>(F, out, N, nb,
+                                                                                     dk);
   return hipGetLastError();
 }
 
@@ -704,7 +822,10 @@ hipError_t launch_lf_row(int N, bool adjoint, cplx<T>* S0, cplx<T>* S1, T* y, T*
                                        hipStream_t);                                            \
   template hipError_t launch_lf_row<T>(int, bool, cplx<T>*, cplx<T>*, T*, T*, const T* const*,  \
                                        const cplx<T>*, int, long, int, T, int, int, int, T,     \
-                                       hipStream_t);
+                                       hipStream_t, T*, const GradArgs<T>*);                    \
+  template hipError_t launch_gtab<T>(int, const double*, T*, int, hipStream_t);                 \
+  template hipError_t launch_dphi_combine<T>(int, const cplx<T>*, cplx<T>*, long, double,       \
+                                             hipStream_t);
 CMBL_INST(float)
 CMBL_INST(double)
 

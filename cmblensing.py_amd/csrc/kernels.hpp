@@ -32,8 +32,26 @@ template <typename T>
 hipError_t launch_lf_col(int N, const cplx<T>* src, cplx<T>* dst, long nmaps, const cplx<T>* tw,
                          T dk, hipStream_t s);
 template <typename T>
+struct GradArgs {
+  const T* zd;            // delta f stage inputs [B*C][N][N]
+  const T* const* gtabs;  // per batch element: M^-1 table [(2n+1)][3][N][N]
+  T* sacc;                // [B][5][N][N]: m_x, m_y, t w_xx, t (w_xy + w_yx), t w_yy
+  T cs;                   // h * b_s (b = 1/6, 1/3, 1/3, 1/6)
+  T tval;                 // stage time t_s
+};
+
+// Row pass of one stage.  adjoint: divergence rate, zstore (may be NULL)
+// receives the next stage input.  Standard rate with ga != NULL: also
+// accumulates the delta-phi sources of the gradient ODE.
+template <typename T>
 hipError_t launch_lf_row(int N, bool adjoint, cplx<T>* S0, cplx<T>* S1, T* y, T* acc,
                          const T* const* ptabs, const cplx<T>* tw, int C, long nmaps, int stage,
-                         T h, int tidx, int do_r2c, int init, T dk, hipStream_t s);
+                         T h, int tidx, int do_r2c, int init, T dk, hipStream_t s,
+                         T* zstore = nullptr, const GradArgs<T>* ga = nullptr);
+template <typename T>
+hipError_t launch_gtab(int N, const double* D, T* gtab, int nsteps, hipStream_t s);
+template <typename T>
+hipError_t launch_dphi_combine(int N, const cplx<T>* F, cplx<T>* out, long nb, double dk,
+                               hipStream_t s);
 
 }  // namespace cmbl

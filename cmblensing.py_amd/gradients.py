@@ -80,16 +80,29 @@ class WfOperators:
 def phi_handles(plan, phi, batch: int, n: int):
     """Prepared phi handles for each batch element (phi in MAP basis, as
     gradients.py:17 does)."""
+    from .lense_flow import phi_handles_for
     pm = phi.scalar_matrix if isinstance(phi, Field) else phi
     if pm.shape[-1] != pm.shape[-2]:
-        pm = to_map(FlatS0(scalar_matrix=pm, theta_pix=plan.theta_pix)).scalar_matrix
-    if pm.dim() == 2:
-        h = _lib.get_phi(plan, pm, n)
-        return [h] * batch
-    flat = pm.reshape(-1, *pm.shape[-2:])
-    if flat.shape[0] == 1:
-        return [_lib.get_phi(plan, flat[0], n)] * batch
-    return [_lib.get_phi(plan, flat[i], n) for i in range(batch)]
+        pm = _map_cache(pm, plan.theta_pix)
+    return phi_handles_for(plan, pm, batch, n)
+
+
+_map_cache_d: dict = {}
+
+
+def _map_cache(pF: torch.Tensor, theta_pix: float) -> torch.Tensor:
+    """map(phi) of a FOURIER phi, kept per phi object (and version) so the
+    prepared handles of the MAP phi stay valid across calls."""
+    import weakref
+    key = id(pF)
+    ent = _map_cache_d.get(key)
+    if ent is not None and ent[0]() is pF and ent[1] == pF._version:
+        return ent[2]
+    pm = to_map(FlatS0(scalar_matrix=pF, theta_pix=theta_pix)).scalar_matrix
+    _map_cache_d[key] = (weakref.ref(pF), pF._version, pm)
+    if len(_map_cache_d) > 64:
+        _map_cache_d.pop(next(iter(_map_cache_d)))
+    return pm
 
 
 def gradf_apply(plan, phis, X: Optional[torch.Tensor], D: Optional[torch.Tensor], ops: WfOperators,

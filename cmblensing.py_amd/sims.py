@@ -255,6 +255,7 @@ def wf_data(prob, lensed_qu_maps, seed=None):
     rng = np.random.default_rng(prob["rng_seed"] + 7 if seed is None else seed)
     shape = (C, N, N // 2 + 1)
     nF = np.sqrt(prob["Cn"] / 2) * (rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+    nF = np.fft.rfft2(np.fft.irfft2(nF))        # the transform of a real noise map
     LF = np.fft.rfft2(lensed_qu_maps)
     if C == 2:
         e, b = _qu2eb_np(LF[0], LF[1], N, th)
@@ -288,3 +289,21 @@ def _qu2eb_np(q, u, N, th):
     a = np.arctan2(LX, LY)
     c2, s2 = np.cos(2 * a), np.sin(2 * a)
     return -q * c2 - u * s2, q * s2 - u * c2
+
+
+def phi_map_and_cov(N, theta_pix_arcmin, seed, rms_arcmin=2.5, damping_ell=500.0):
+    """phi (MAP) and the Fourier-diagonal covariance of its rfft2 coefficients
+    (same normalisation as the reference's fixtures: <|rfft2(phi)_l|^2> per
+    mode), with the monopole given the fundamental mode's variance."""
+    rng = np.random.default_rng(seed)
+    cl = lambda l: cl_phiphi(l, damping_ell)
+    F = grf_fourier(N, theta_pix_arcmin, cl, rng)
+    phi = np.fft.irfft2(F, norm="ortho")
+    s = rms_arcmin * ARCMIN / deflection_rms(phi, theta_pix_arcmin)
+    C = covariance_diag(N, theta_pix_arcmin, cl) * (s * N) ** 2
+    C[0, 0] = C[0, 1]
+    # floor the damped tail: below ~1e-10 of the peak the variance is smaller
+    # than the FFT rounding of phi itself, and dot(phi, C^-1 phi) would be
+    # dominated by that rounding
+    C = np.maximum(C, 1e-10 * C.max())
+    return phi * s, C

@@ -1,0 +1,85 @@
+"""logpdf and its phi gradient on the GPU (statistics.py:138-213 and
+gradients.py:29-37 of the reference).
+
+logpdf = -1/2 Re[ dot(r, Cn^-1 r) + dot(f, Cf^-1 f) + dot(phi, Cphi^-1 phi)
+                  + logdet Cn + logdet Cphi + logdet Cf ],   r = d - M B L f
+
+grad_phi_logpdf follows the reference's definition for a FOURIER phi: the
+gradient-ODE delta phi accumulated in Fourier (the hand-written backward,
+``cmbl_lenseflow_vjp``) minus Cphi^-1 phi (the custom VJP of the prior term,
+statistics.py:196-208).  Both accept a PixelMask (the residual is then formed
+on the maps).
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+from .constants import Basis, DEFAULT_NSTEPS, Parametrization
+from .fields import (Field, FlatS0, dot, eb2qu, fourier, map as to_map, qu2eb)
+from .lense_flow import _phi_matrix, lense_flow_matrices, lense_flow_vjp_matrices
+from .matrix_operators import PixelMask, log_det, pinv
+
+
+def _lensed_maps(field: Field, phi, n: int):
+    """(stacked unlensed QU/T maps, stacked lensed maps, group names)."""
+    fq = to_map(eb2qu(field))
+    names = fq._matrix_names()
+    m0 = getattr(fq, names[0])
+    N = m0.shape[-1]
+    x = torch.stack([getattr(fq, nm).reshape(-1, N, N) for nm in names], dim=1)
+    y = lense_flow_matrices(x, _phi_matrix(phi), field.theta_pix, n, 1, False)
+    return fq, x, y, names, m0.shape[:-2]
+
+
+def _residual(lensed_eb: Field, data: Field, mask, beam) -> Field:
+    """r = d - M B L f (statistics.py:159) in EB Fourier."""
+    if isinstance(mask, PixelMask):
+        q = to_map(eb2qu(beam * lensed_eb))
+        m = mask.matrix.to(dtype=getattr(q, q._matrix_names()[0]).dtype)
+        return qu2eb(fourier(data - q * m))
+    return data - mask * (beam * lensed_eb)
+
+
+def logpdf(field, phi, data, noise_covariance, phi_covariance, field_covariance, mask, beam,
+           n: int = DEFAULT_NSTEPS):
+    """statistics.py:138-165 (same argument order as the reference)."""
+    logdets = log_det(phi_covariance) + log_det(noise_covariance) + log_det(field_covariance)
+    field_product = dot(field, pinv(field_covariance) * field)
+    phi_product = dot(phi, pinv(phi_covariance) * phi)
+    fq, x, y, names, lead = _lensed_maps(field, phi, n)
+    lensed = qu2eb(fourier(fq.replace(**{nm: y[:, c].reshape(*lead, *y.shape[-2:])
+                                         for c, nm in enumerate(names)})))
+    r = _residual(lensed, data, mask, beam)
+    data_product = dot(r, pinv(noise_covariance) * r)
+    return -(data_product + field_product + phi_product + logdets) / 2
+
+
+def grad_phi_logpdf(unlensed_field, phi, data, noise_covariance, phi_covariance,
+                    field_covariance, mask, beam, n: int = DEFAULT_NSTEPS) -> FlatS0:
+    """d logpdf / d phi for a FOURIER phi (gradients.py:29-37 with
+    lense_flow.py:10-51 and statistics.py:183-213)."""
+    phi_m = _phi_matrix(phi)
+    if phi_m.shape[-1] == phi_m.shape[-2]:
+        raise ValueError("grad_phi_logpdf takes phi in the FOURIER basis, like the reference")
+    f = unlensed_field
+    fq, x, y, names, lead = _lensed_maps(f, phi, n)
+    N = x.shape[-1]
+    lensed = qu2eb(fourier(fq.replace(**{nm: y[:, c].reshape(*lead, N, N)
+                                         for c, nm in enumerate(names)})))
+    r = _residual(lensed, data, mask, beam)
+    if isinstance(mask, PixelMask):
+        s_ = to_map(eb2qu(pinv(noise_covariance) * r))
+        m = mask.matrix.to(dtype=getattr(s_, s_._matrix_names()[0]).dtype)
+        g = beam * qu2eb(fourier(s_ * m))
+    else:
+        g = beam * (mask * (pinv(noise_covariance) * r))
+    cot = to_map(eb2qu(g))
+    c = torch.stack([getattr(cot, nm).reshape(-1, N, N) for nm in names], dim=1)
+    _, d_phi = lense_flow_vjp_matrices(y, c, phi_m, f.theta_pix, n, 1)
+    # one gradient per batch element (each chain's own phi), or a single one
+    d_phi = d_phi.reshape(*lead, N, N // 2 + 1) if len(lead) else d_phi[0]
+    prior = pinv(phi_covariance) * (phi if isinstance(phi, Field) else FlatS0(scalar_matrix=phi_m))
+    out = d_phi.to(phi_m.dtype) - prior.scalar_matrix
+    return FlatS0(scalar_matrix=out, theta_pix=f.theta_pix, basis=Basis.FOURIER,
+                  parametrization=Parametrization.T)

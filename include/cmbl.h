@@ -75,6 +75,19 @@ cmbl_status cmbl_lenseflow(cmbl_plan* plan, const cmbl_phi* const* phis, const v
                            void* out, int batch, int ncomp, int nsteps, int direction,
                            int adjoint, void* stream);
 
+/* Backward rule of lense_flow_wrapper (lense_flow.py:10-51, 53-126, 156-252):
+ * the joint RK4 of (f, delta f, delta phi) integrated from the transformed
+ * field `lensed` (= lense_flow(f, phi, nsteps, direction)) and the cotangent
+ * `cotangent` (both MAP [batch][ncomp][N][N]) with -direction.  Writes the raw
+ * delta f (MAP, = L^T cotangent) into d_field and the delta-phi rate
+ * accumulated in Fourier, complex [batch][N][N/2+1], into d_phi.  Both are
+ * plain gradients: <d_field, w> and <irfft2(d_phi), v> are the directional
+ * derivatives of <cotangent, L f> (undo_inner_product, fields.py:146-154, is
+ * left to the caller).  d_field may equal cotangent. */
+cmbl_status cmbl_lenseflow_vjp(cmbl_plan* plan, const cmbl_phi* const* phis, const void* lensed,
+                               const void* cotangent, void* d_field, void* d_phi, int batch,
+                               int ncomp, int nsteps, int direction, void* stream);
+
 /* rfft2 / irfft2 of nmaps maps in the plan dtype with numpy semantics
  * (fields.py:158-172: fourier / map). */
 cmbl_status cmbl_rfft2(cmbl_plan* plan, const void* x, void* X, int nmaps, void* stream);

@@ -92,7 +92,17 @@ def gradf_logpdf(f, phi, d, Cf, Cn, M, B, theta_pix, n=10, mask_pix=None):
             - pinv(Cf) * qu2eb_field(to_fourier(fq), N, theta_pix))  # :25
 
 
-def logpdf(f, phi, d, Cn, Cphi, Cf, M, B, theta_pix, n=10):
+def data_residual(lensed, d, M, B, N, theta_pix, mask_pix=None):
+    """d - M B L f in EB Fourier (statistics.py:159); with a pixel mask the
+    residual is formed on the maps, d - m map(eb2qu(B L f)), and returned in EB
+    Fourier."""
+    if mask_pix is None:
+        return d - M * B * lensed
+    q = to_map(eb2qu_field(B * lensed, N, theta_pix))
+    return qu2eb_field(to_fourier(d - mask_pix * q), N, theta_pix)
+
+
+def logpdf(f, phi, d, Cn, Cphi, Cf, M, B, theta_pix, n=10, mask_pix=None):
     """statistics.py:138-165 (note the argument order differs from gradf_logpdf)."""
     N = f.shape[-2]
     pw = lf.pix_width_from_theta(theta_pix)
@@ -101,7 +111,7 @@ def logpdf(f, phi, d, Cn, Cphi, Cf, M, B, theta_pix, n=10):
     phi_product = lf.primal_dot(phi, pinv(Cphi) * phi, N)           # :149 / :185-186
     fq = to_map(eb2qu_field(f, N, theta_pix))                       # :155
     lensed = qu2eb_field(to_fourier(lf.lense_flow(fq, phi, pw, n)), N, theta_pix)  # :156
-    diff = d - M * B * lensed                                       # :159
+    diff = data_residual(lensed, d, M, B, N, theta_pix, mask_pix)   # :159
     data_product = lf.field_dot(diff, pinv(Cn) * diff, N)           # :160
     return -np.real(data_product + field_product + phi_product + logdets) / 2   # :164-165
 
@@ -159,3 +169,32 @@ def wiener_filter(x0, phi, d, Cf, Cn, M, B, theta_pix, maxiter=500, tol=1e-1, n=
     P = preconditioner(Cf, B, M, Cn, mask_pix)
     return conjugate_gradient(x0, d, phi, M, B, Cn, Cf, b, P, theta_pix, maxiter, tol, n,
                               history, mask_pix, rtol)
+
+
+def logpdf_lensed_cotangent(f, phi, d, Cn, M, B, theta_pix, n=10, mask_pix=None):
+    """(L f, d logpdf / d(L f)) on the maps: with r = d - M B L f,
+    the cotangent is map(eb2qu(B M^T Cn^-1 r)) (Fourier mask) or
+    m map(eb2qu(Cn^-1 r)) projected through B (pixel mask)."""
+    N = f.shape[-2]
+    pw = lf.pix_width_from_theta(theta_pix)
+    fq = to_map(eb2qu_field(f, N, theta_pix))
+    Lf = lf.lense_flow(fq, phi, pw, n)
+    lensed = qu2eb_field(to_fourier(Lf), N, theta_pix)
+    r = data_residual(lensed, d, M, B, N, theta_pix, mask_pix)
+    if mask_pix is None:
+        g = B * M * pinv(Cn) * r
+    else:
+        s_ = to_map(eb2qu_field(pinv(Cn) * r, N, theta_pix))
+        g = B * qu2eb_field(to_fourier(mask_pix * s_), N, theta_pix)
+    return Lf, to_map(eb2qu_field(g, N, theta_pix))
+
+
+def grad_phi_logpdf(f, phi, d, Cn, Cphi, Cf, M, B, theta_pix, n=10, mask_pix=None):
+    """jax.grad(logpdf, argnums=1) for a FOURIER phi as the reference defines it
+    (gradients.py:29-37): the gradient-ODE delta phi accumulated in Fourier
+    (lense_flow.py:23-46, undo_inner_product leaves FOURIER arrays alone)
+    minus the prior term 2 Cphi^-1 phi / 2 (statistics.py:196-208)."""
+    pw = lf.pix_width_from_theta(theta_pix)
+    Lf, cot = logpdf_lensed_cotangent(f, phi, d, Cn, M, B, theta_pix, n, mask_pix)
+    _, _, dphi = lf.lensing_operator_gradients(phi, list(Lf), list(cot), -1, n, pw)
+    return dphi - pinv(Cphi) * phi

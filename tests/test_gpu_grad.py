@@ -1,0 +1,161 @@
+"""GPU parity of the hand-written LenseFlow backward (gradient ODE), the
+autograd path, logpdf and grad_phi_logpdf."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import lenseflow as olf
+from oracle import wiener as owf
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def _case(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+@pytest.mark.parametrize("C", [1, 2])
+@pytest.mark.parametrize("basis", ["map", "fourier"])
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-10), (torch.float32, 1e-4)])
+def test_vjp_matches_oracle(pkg, dev, sims, C, basis, dtype, tol):
+    from cmblensing_amd.lense_flow import lense_flow_vjp_matrices
+    N, th, n = 64, 2.0, 7
+    _case(f"vjp N={N} C={C} phi={basis} {dtype}")
+    pw = olf.pix_width_from_theta(th)
+    f = sims.qu_map(N, th, 1) if C == 2 else sims.t_map(N, th, 1)
+    phi = sims.phi_map(N, th, 2)
+    phin = phi if basis == "map" else np.fft.rfft2(phi)
+    y = olf.lense_flow(f, phi, pw, n)
+    cot = np.random.default_rng(3).standard_normal(f.shape)
+    _, dfs, dphi = olf.lensing_operator_gradients(phin, list(y), list(cot), -1, n, pw)
+    T = lambda a: torch.as_tensor(a, device=dev)
+    d_field, d_phi = lense_flow_vjp_matrices(T(y[None]).to(dtype), T(cot[None]).to(dtype), T(phin),
+                                             th, n, 1)
+    assert rel(d_field[0].double().cpu().numpy(), np.stack(dfs)) < tol
+    got = d_phi[0].cpu().numpy()
+    if basis == "map":
+        got = np.fft.irfft2(got)
+    assert rel(got, dphi) < tol
+
+
+def test_autograd_matches_finite_differences(pkg, dev, sims):
+    """torch.autograd through LenseFlowFunction: d/dx exact (linear), d/dphi
+    within the continuous-adjoint gap of a central difference."""
+    from cmblensing_amd.lense_flow import LenseFlowFunction
+    N, th, n = 64, 2.0, 7
+    _case("autograd fd N=64 QU fp64")
+    T = lambda a: torch.as_tensor(a, device=dev)
+    x = T(sims.qu_map(N, th, 4)[None]).requires_grad_(True)
+    phi = T(sims.phi_map(N, th, 5)).requires_grad_(True)
+    c = T(np.random.default_rng(6).standard_normal((1, 2, N, N)))
+    L = (LenseFlowFunction.apply(x, phi, th, n, 1) * c).sum()
+    gx, gphi = torch.autograd.grad(L, (x, phi))
+    rng = np.random.default_rng(7)
+    with torch.no_grad():
+        for _ in range(3):
+            v = T(rng.standard_normal(x.shape))
+            e = 1.0
+            fd = ((LenseFlowFunction.apply(x + e * v, phi, th, n, 1) * c).sum()
+                  - (LenseFlowFunction.apply(x - e * v, phi, th, n, 1) * c).sum()) / (2 * e)
+            assert abs(float((gx * v).sum()) - float(fd)) < 1e-9 * abs(float(fd))
+            # a smooth, phi-like direction (white noise would be dominated by
+            # the nonlinearity of M(t)^-1 in Hess phi at the pixel scale)
+            w = T(sims.phi_map(N, th, int(rng.integers(1000))))
+            e = 1e-3
+            fd = ((LenseFlowFunction.apply(x, phi + e * w, th, n, 1) * c).sum()
+                  - (LenseFlowFunction.apply(x, phi - e * w, th, n, 1) * c).sum()) / (2 * e)
+            # continuous-adjoint gap at n = 7: ~1e-3, falling ~16x per doubling of n
+            assert abs(float((gphi * w).sum()) - float(fd)) < 5e-3 * abs(float(fd))
+
+
+def _julia_I(pkg, dev):
+    from cmblensing_amd import matrix_operators as mo
+    J = GOLDEN / "julia"
+    L = lambda n: np.load(J / f"{n}.npy")
+    T = lambda a: torch.as_tensor(a, device=dev)
+    S = lambda a: pkg.FlatS0(scalar_matrix=T(a), theta_pix=2.5)
+    D = lambda a: mo.DiagonalScalar(scalar_matrix=T(a), theta_pix=2.5)
+    return L, S, D
+
+
+def test_logpdf_julia_I(pkg, dev):
+    from cmblensing_amd import statistics
+    _case("logpdf julia I fp64")
+    L, S, D = _julia_I(pkg, dev)
+    lp = statistics.logpdf(S(L("t_field_I")), S(L("phi_I")), S(L("data_t_field_I")), D(L("cn_I")),
+                           D(L("cphi_I")), D(L("cf_I")), D(L("m_I")), D(L("b_I")), n=7)
+    ref = float((GOLDEN / "julia" / "logpdf_I.txt").read_text())
+    assert abs(float(lp) - ref) < 1e-9 * abs(ref)
+
+
+def test_grad_phi_julia_I(pkg, dev):
+    from cmblensing_amd import statistics
+    _case("grad_phi julia I fp64")
+    L, S, D = _julia_I(pkg, dev)
+    g = statistics.grad_phi_logpdf(S(L("t_field_I")), S(L("phi_I")), S(L("data_t_field_I")),
+                                   D(L("cn_I")), D(L("cphi_I")), D(L("cf_I")), D(L("m_I")),
+                                   D(L("b_I")), n=7).scalar_matrix.cpu().numpy()
+    ref = owf.grad_phi_logpdf(L("t_field_I")[None], L("phi_I"), L("data_t_field_I")[None],
+                              L("cn_I")[None], L("cphi_I"), L("cf_I")[None], L("m_I")[None],
+                              L("b_I")[None], 2.5, n=7)
+    assert rel(g, ref) < 1e-9
+    assert rel(g, L("grad_phi_I")) < 2e-3
+
+
+def test_differentiation_workload_batch16(pkg, dev, sims):
+    """QU N=256, 2 arcmin, float32 gradients of logpdf w.r.t. f and phi for 16
+    chains (square pixel mask, 1 arcmin beam, 5 uK-arcmin noise), checked
+    against float64 central differences on the GPU along random directions."""
+    from cmblensing_amd import gradients, statistics, matrix_operators as mo
+    N, th, n, B = 256, 2.0, 7, 16
+    _case(f"differentiation workload N={N} QU batch={B}")
+    prob = sims.wf_problem(N, th, pol=True, seed=21, mask="square", noise_uk=5.0, beam_fwhm=1.0)
+    fq = owf.to_map(owf.eb2qu_field(prob["f"], N, th))
+    prob = sims.wf_data(prob, olf.lense_flow(fq, prob["phi"], olf.pix_width_from_theta(th), n))
+    phi_m, Cphi = sims.phi_map_and_cov(N, th, 2)
+    rng = np.random.default_rng(22)
+
+    def objs(dt, cdt):
+        T = lambda a, d=cdt: torch.as_tensor(a, device=dev).to(d)
+        # 16 chains: the truth plus small seeded offsets
+        Fs = np.stack([prob["f"] * (1 + 0.01 * k) for k in range(B)])
+        f = pkg.FlatS2(polar_matrix_1=T(Fs[:, 0]), polar_matrix_2=T(Fs[:, 1]), theta_pix=th)
+        data = pkg.FlatS2(polar_matrix_1=T(prob["d"][0], dt), polar_matrix_2=T(prob["d"][1], dt),
+                          theta_pix=th)
+        op = lambda A: mo.DiagonalEB(matrix_EE=T(A[0], torch.float64),
+                                     matrix_BB=T(A[1], torch.float64), theta_pix=th)
+        phiF = pkg.FlatS0(scalar_matrix=T(np.fft.rfft2(phi_m)), theta_pix=th)
+        Cp = mo.DiagonalScalar(scalar_matrix=T(Cphi, torch.float64), theta_pix=th)
+        M = mo.PixelMask(matrix=torch.as_tensor(prob["mask_pix"], device=dev))
+        return f, phiF, data, op(prob["Cf"]), op(prob["Cn"]), M, op(prob["B"]), Cp
+
+    f32 = objs(torch.float32, torch.complex64)
+    f, phiF, data, Cf, Cn, M, Bm, Cp = f32
+    gf = gradients.gradf_logpdf(f, phiF, data, Cf, Cn, M, Bm, n=n)
+    gp = statistics.grad_phi_logpdf(f, phiF, data, Cn, Cp, Cf, M, Bm, n=n)
+    f64 = objs(torch.float64, torch.complex128)
+    f6, phi6, data6, Cf6, Cn6, M6, B6, Cp6 = f64
+
+    def lp(ff, pp):
+        return statistics.logpdf(ff, pp, data6, Cn6, Cp6, Cf6, M6, B6, n=n)
+    for _ in range(2):
+        v = np.fft.rfft2(rng.standard_normal((2, N, N)))
+        vt = [torch.as_tensor(v[c], device=dev) for c in range(2)]
+        dirf = f6.replace(polar_matrix_1=vt[0].expand(B, -1, -1), polar_matrix_2=vt[1].expand(B, -1, -1))
+        e = 0.1   # logpdf is quadratic in f
+        fd = (lp(f6 + dirf * e, phi6) - lp(f6 - dirf * e, phi6)) / (2 * e)
+        an = pkg.dot(gf, dirf)
+        assert float(torch.max(torch.abs(an - fd) / torch.abs(fd))) < 1e-4
+        w = np.fft.rfft2(sims.phi_map(N, th, int(rng.integers(1000))))
+        wt = phi6.replace(scalar_matrix=torch.as_tensor(w, device=dev))
+        e = 1e-3
+        fdp = (lp(f6, phi6 + wt * e) - lp(f6, phi6 - wt * e)) / (2 * e)
+        anp = pkg.dot(gp, wt.replace(scalar_matrix=wt.scalar_matrix.expand(B, -1, -1)))
+        print("phi: analytic", anp[:3].tolist(), "fd", fdp[:3].tolist(), flush=True)
+        assert float(torch.max(torch.abs(anp - fdp) / torch.abs(fdp))) < 5e-3

@@ -157,7 +157,7 @@ def test_wiener_filter_mixed_precision_batch(pkg, dev, sims):
                                 recompute_every=10, maxiter=500, return_info=True)
     print("iterations", info.iterations, "true rel residual", info.final_true_rel_residual)
     assert all(info.converged)
-    assert max(info.final_true_rel_residual) < 3e-6
+    assert max(info.final_true_rel_residual) < 1e-6
     xs = np.stack([x.polar_matrix_1.cpu().numpy(), x.polar_matrix_2.cpu().numpy()], axis=1)
     for i, p in enumerate(probs):
         ref = owf.wiener_filter(0 * p["f"], p["phi"], p["d"], p["Cf"], p["Cn"], None, p["B"], th,

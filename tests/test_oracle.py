@@ -287,3 +287,43 @@ def test_prior_round_trip_is_hermitian_projection():
         col = f[:, :, j]
         proj[:, :, j] = 0.5 * (col + np.conj(col[:, (-np.arange(N)) % N]))
     assert np.allclose(rt, proj, atol=1e-12)
+
+
+@pytest.mark.parametrize("mask", ["fourier", "square"])
+def test_logpdf_gradients_match_finite_differences(mask):
+    """gradf_logpdf and grad_phi_logpdf are the Riesz representers of the
+    logpdf derivative under `dot` (statistics.py:196-208 fixes the convention):
+    dot(g, v) = d/de logpdf(x + e v) for Hermitian-consistent v."""
+    sims = _sims()
+    N, th, n = 32, 3.0, 7
+    prob = sims.wf_problem(N, th, pol=True, seed=4, mask=mask)
+    fq = owf.to_map(owf.eb2qu_field(prob["f"], N, th))
+    prob = sims.wf_data(prob, olf.lense_flow(fq, prob["phi"], olf.pix_width_from_theta(th), n))
+    phiF = np.fft.rfft2(prob["phi"])
+    # a flat phi prior scaled to phi's norm keeps the lensing part of the
+    # gradient comparable to the prior part (a physical C_phi makes the prior
+    # term ~1e25 and the finite difference pure rounding)
+    Cphi = np.full(phiF.shape, olf.primal_dot(phiF, phiF, N))
+    a = (prob["d"], prob["Cn"], Cphi, prob["Cf"], prob["M"], prob["B"], th)
+
+    def lp(f, phi):
+        return owf.logpdf(f, phi, *a, n=n, mask_pix=prob["mask_pix"])
+    rng = np.random.default_rng(3)
+    gf = owf.gradf_logpdf(prob["f"], phiF, prob["d"], prob["Cf"], prob["Cn"], prob["M"],
+                          prob["B"], th, n=n, mask_pix=prob["mask_pix"])
+    gp = owf.grad_phi_logpdf(prob["f"], phiF, *a, n=n, mask_pix=prob["mask_pix"])
+    for _ in range(2):
+        v = np.fft.rfft2(rng.standard_normal((2, N, N)))
+        v *= np.sqrt(olf.field_dot(prob["f"], prob["f"], N) / olf.field_dot(v, v, N))
+        # logpdf is quadratic in f: central differences are exact for any step,
+        # and a large one keeps the rounding of logpdf (~1e8) negligible
+        e = 0.1
+        fd = (lp(prob["f"] + e * v, phiF) - lp(prob["f"] - e * v, phiF)) / (2 * e)
+        assert abs(olf.field_dot(gf, v, N) - fd) < 1e-6 * abs(fd) + 1e-9 * np.sqrt(
+            olf.field_dot(gf, gf, N) * olf.field_dot(v, v, N))
+        w = np.fft.rfft2(rng.standard_normal((N, N)))
+        w *= np.sqrt(olf.primal_dot(phiF, phiF, N) / olf.primal_dot(w, w, N))
+        e = 1e-3
+        fdp = (lp(prob["f"], phiF + e * w) - lp(prob["f"], phiF - e * w)) / (2 * e)
+        # continuous-adjoint gradient: agrees with the discrete FD to O(h^4)
+        assert abs(olf.primal_dot(gp, w, N) - fdp) < 2e-3 * abs(fdp)
