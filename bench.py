@@ -7,10 +7,16 @@ potentials whose operators are prepared once and stay resident in HBM (the
 serving form).  The plain path is one request per call with its phi prepared
 in the same call, as the reference's lense_flow does.
 
+The timed step replays the captured batched graph of the serving executor
+(gather, 28 RK4 stages, scatter); the same batch through the direct batched
+entry point is timed beside it.
+
 Prints one JSON line (rank 0).  Extra keys: plain maps/s and the batched /
 plain ratio, p50/p99 latency of plain single requests, relative L2 error of a
 batched result against the FP64 oracle, the live roofline of the dominant
-kernel and the CPU baseline (the FP64 oracle on this host's cores).
+kernel, the CPU baseline (the FP64 oracle on this host's cores), and the
+mixed serving stream (10k Poisson requests at 2k req/s of mixed N, T/QU,
+lense/unlense: p50/p99 latency arrival -> done, maps/s, phi hit rate).
 """
 from __future__ import annotations
 
@@ -63,6 +69,60 @@ def cpu_baseline(f, phi, th, n, budget_s=10.0):
     return k / el, k, el
 
 
+def serving_run(torch, sims, dev, n_requests, rate, seed=2, nphi=32, budget_gib=64.0):
+    """The mixed serving stream: Poisson arrivals at `rate`, N in {64, 128,
+    256, 512} with weights {.2, .4, .3, .1}, T/QU and lense/unlense 50/50,
+    phi ~ U{0..31} of 32 seeded potentials per N, float32, 2 arcmin pixels,
+    7 RK4 steps, a 64 GiB HBM admission budget.  Inputs are resident in HBM
+    (a pool of 8 seeded maps per (N, C)); phi operators start cold."""
+    import importlib
+    srv = importlib.import_module("cmblensing_amd.serving")
+    th, n = 2.0, 7
+    sizes = (64, 128, 256, 512)
+    desc = srv.mixed_workload(n_requests, seed=seed, sizes=sizes, nphi=nphi)
+    offs = srv.poisson_schedule(n_requests, rate, seed=seed)
+    pool = {}
+    for N in sizes:
+        pool[(N, 1)] = torch.as_tensor(np.stack([sims.t_map(N, th, i) for i in range(8)]),
+                                       dtype=torch.float32, device=dev)
+        pool[(N, 2)] = torch.as_tensor(np.stack([sims.qu_map(N, th, i) for i in range(8)]),
+                                       dtype=torch.float32, device=dev)
+    phi_maps = {}
+    for N in sizes:
+        for k in range(nphi):
+            phi_maps[(k, N)] = torch.as_tensor(sims.phi_map(N, th, 5000 + 100 * k + N % 97),
+                                               device=dev)
+    reqs = [srv.Request(x=pool[(N, C)][i % 8], phi_id=phi, direction=d)
+            for i, (N, C, d, phi) in enumerate(desc)]
+    be = srv.GpuBackend(dev, theta_pix=th, nsteps=n, nstreams=2, sizes=sizes)
+    t0 = time.perf_counter()
+    be.warmup()
+    warm_s = time.perf_counter() - t0
+    server = srv.Server(be, lambda k, N: phi_maps[(k, N)], hbm_budget=int(budget_gib * srv.GiB))
+    print(f"[bench] serving {n_requests} requests at {rate:.0f} req/s "
+          f"(graphs captured in {warm_s:.1f} s)", file=sys.stderr, flush=True)
+    st = server.serve(reqs, list(offs))
+    torch.cuda.synchronize()
+    per_n = {}
+    for N in sizes:
+        lat = np.array([r.latency for r in reqs if r.N == N and r.state == "done"])
+        if len(lat):
+            per_n[str(N)] = {"p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                             "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+                             "requests": int(len(lat))}
+    out = {"requests": n_requests, "offered_rate_req_s": rate,
+           "completed": st["completed"], "maps_per_s": round(st.get("maps_per_s", 0.0), 1),
+           "p50_ms": round(st.get("p50_ms", 0.0), 3), "p99_ms": round(st.get("p99_ms", 0.0), 3),
+           "max_ms": round(st.get("max_ms", 0.0), 3),
+           "batches": st["batches"], "mean_batch": round(st["mean_batch"], 2),
+           "phi_hit_rate": round(st["phi_hit_rate"], 4), "phi_evictions": st["phi_evictions"],
+           "hbm_budget_gib": round(st["hbm_budget_bytes"] / srv.GiB, 2),
+           "hbm_peak_gib": round(st["hbm_peak_bytes"] / srv.GiB, 3),
+           "graph_warmup_s": round(warm_s, 2), "per_nside": per_n}
+    be.close()
+    return out
+
+
 def copy_probe(torch):
     """A plain copy of known size: FETCH_SIZE / WRITE_SIZE of this kernel give
     the scale factors of the HBM counters, and its time the achievable copy
@@ -94,6 +154,9 @@ def main():
     ap.add_argument("--plain-requests", type=int, default=40)
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=10000)
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -129,17 +192,32 @@ def main():
     req_phi = [handles[r % NPHI] for r in range(B)]
     out = torch.empty_like(x)
 
+    # the served form: one replay of the captured batched graph per step
+    # (gather of the B request maps, the 28 RK4 stages, scatter of results)
+    ex = _lib.Executor(plan, B, C, n, 1, False)
+    in_ptrs = [x[b].data_ptr() for b in range(B)]
+    out_ptrs = [out[b].data_ptr() for b in range(B)]
+    stream = torch.cuda.current_stream(dev).cuda_stream
+
     def step():
+        ex.run(req_phi, in_ptrs, out_ptrs, stream)
+
+    def direct_step():
         _lib.lenseflow(plan, req_phi, x, out, B, C, n, 1, False)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    direct_dt = timed(direct_step, args.steps, args.warmup)
+    direct_maps_s = B * args.steps / direct_dt
+    dt = timed(step, args.steps, args.warmup)
     ms_per_step = dt / args.steps * 1e3
     maps_s = B * args.steps / dt
 
@@ -155,7 +233,7 @@ def main():
     plan.get_timing(reset=True)
     ksteps = max(2, min(args.steps, 5))
     for _ in range(ksteps):
-        step()
+        direct_step()
     tim = plan.get_timing(reset=True)
     plan.set_timing(False)
     row_avg_ms = tim["row_ms"] / tim["row_n"]
@@ -212,6 +290,7 @@ def main():
         "data": "synthetic (seeded Gaussian random fields, 32 resident phi)",
         "config": {"workload": "lenseflow_fwd_QU_256_2arcmin_n7_fp32", "N": N, "theta_pix": th,
                    "nsteps": n, "components": "QU", "batch": B, "phi_resident": NPHI},
+        "direct_batched_maps_s": round(direct_maps_s, 2),
         "plain_maps_s": round(plain_maps_s, 2),
         "batched_over_plain": round(maps_s / plain_maps_s, 2),
         "plain_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
@@ -231,6 +310,11 @@ def main():
                         "avg_launch_us": round(col_avg_ms * 1e3, 3),
                         "algorithmic_bytes_per_launch": col_bytes},
     }
+    ex.close()
+    if not args.no_serve:
+        result["serving"] = serving_run(torch, sims, dev, args.serve_requests, args.serve_rate)
+        result["latency_ms"] = {"p50": result["serving"]["p50_ms"],
+                                "p99": result["serving"]["p99_ms"]}
     if not args.no_cpu:
         fcpu = fq[0]
         cps, k, el = cpu_baseline(fcpu, phis[0], th, n, args.cpu_budget)

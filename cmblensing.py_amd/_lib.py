@@ -47,6 +47,11 @@ _SIGS = {
                                  c_void_p]),
     "cmbl_lenseflow_vjp": (c_int, [c_void_p, ctypes.POINTER(c_void_p), c_void_p, c_void_p,
                                    c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p]),
+    "cmbl_exec_create": (c_int, [ctypes.POINTER(c_void_p), c_void_p, c_int, c_int, c_int, c_int,
+                                 c_int]),
+    "cmbl_exec_run": (c_int, [c_void_p, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
+                              ctypes.POINTER(c_void_p), c_int, c_void_p]),
+    "cmbl_exec_destroy": (c_int, [c_void_p]),
     "cmbl_plan_set_timing": (c_int, [c_void_p, c_int]),
     "cmbl_plan_get_timing": (c_int, [c_void_p, ctypes.POINTER(c_double), c_int]),
 }
@@ -170,6 +175,48 @@ class PhiHandle:
         if self.handle:
             torch.cuda.synchronize(self.plan.device)
             self.lib.cmbl_phi_release(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Executor:
+    """A cmbl_exec: a batched LenseFlow of fixed capacity (max_batch requests of
+    ncomp components) captured once into a hipGraph on the plan's staging
+    buffer.  The executor keeps its plan alive; runs of executors sharing a
+    plan must be issued on one stream."""
+
+    def __init__(self, plan: Plan, max_batch: int, ncomp: int, nsteps: int, direction: int,
+                 adjoint: bool):
+        self.lib, self.plan = plan.lib, plan
+        self.max_batch, self.ncomp, self.nsteps = max_batch, ncomp, nsteps
+        self.direction, self.adjoint = int(direction), bool(adjoint)
+        h = c_void_p()
+        with torch.cuda.device(plan.device):
+            check(self.lib.cmbl_exec_create(ctypes.byref(h), plan.handle, max_batch, ncomp, nsteps,
+                                            self.direction, int(self.adjoint)), "cmbl_exec_create")
+        self.handle = h
+        self._arrs = ((c_void_p * max_batch)(), (c_void_p * max_batch)(), (c_void_p * max_batch)())
+
+    def run(self, phis, in_ptrs, out_ptrs, stream: int):
+        """Lens len(phis) requests: phis are PhiHandles, in_ptrs/out_ptrs device
+        addresses of [ncomp, N, N] maps; enqueued on `stream` (a hipStream_t)."""
+        count = len(phis)
+        a, i, o = self._arrs
+        for b in range(count):
+            a[b] = phis[b].handle.value
+        i[:count] = in_ptrs
+        o[:count] = out_ptrs
+        check(self.lib.cmbl_exec_run(self.handle, a, i, o, count, stream), "cmbl_exec_run")
+
+    def close(self):
+        if self.handle:
+            torch.cuda.synchronize(self.plan.device)
+            self.lib.cmbl_exec_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -91,6 +91,8 @@ struct cmbl_plan {
   void* gv = nullptr;
   void* gsrc = nullptr;  // delta-phi sources [max_maps][5][N][N]
   void* gF = nullptr;    // their rfft2 [max_maps][5][N][N/2+1]
+  // serving executors' staging buffer [max_maps][N][N], allocated on first use
+  void* stage = nullptr;
 };
 
 struct cmbl_phi {
@@ -173,7 +175,7 @@ cmbl_status cmbl_plan_destroy(cmbl_plan* p) {
   if (p->s1) (void)hipFree(p->s1);
   if (p->ptab_dev) (void)hipFree(p->ptab_dev);
   for (void* q : {p->wy1, p->wy2, p->wx1, p->wx2, (void*)p->red, p->gy, p->gacc, p->gzd, p->gu,
-                  p->gv, p->gsrc, p->gF})
+                  p->gv, p->gsrc, p->gF, p->stage})
     if (q) (void)hipFree(q);
   if (p->ptab_host) (void)hipHostFree(p->ptab_host);
   delete p;
@@ -775,5 +777,169 @@ extern "C" cmbl_status cmbl_lenseflow_vjp(cmbl_plan* p, const cmbl_phi* const* p
   if (st != CMBL_OK) return st;
   HIP_TRY(hipEventRecord(p->ring_ev[slot], s));
   p->ring_used[slot] = true;
+  return CMBL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// serving executor: a batched LenseFlow captured once into a hipGraph
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kExecSlots = 4;  // pointer-array slots in flight per executor
+}
+
+struct cmbl_exec {
+  cmbl_plan* plan = nullptr;
+  int max_batch = 0, ncomp = 0, nsteps = 0, direction = 1, adjoint = 0;
+  // device pointer arrays [kExecSlots][3][max_batch] (phi tables, inputs,
+  // outputs), their pinned host mirror, and an event per slot marking that the
+  // slot's host-to-device copy has run
+  void** dev_ptrs = nullptr;
+  void** host_ptrs = nullptr;
+  hipEvent_t copied[kExecSlots] = {};
+  bool used[kExecSlots] = {};
+  int next = 0;
+  hipGraph_t graph[kExecSlots] = {};
+  hipGraphExec_t gexec[kExecSlots] = {};
+};
+
+extern "C" cmbl_status cmbl_exec_destroy(cmbl_exec* ex) {
+  if (!ex) return CMBL_OK;
+  for (int i = 0; i < kExecSlots; ++i) {
+    if (ex->copied[i]) {
+      (void)hipEventSynchronize(ex->copied[i]);
+      (void)hipEventDestroy(ex->copied[i]);
+    }
+    if (ex->gexec[i]) (void)hipGraphExecDestroy(ex->gexec[i]);
+    if (ex->graph[i]) (void)hipGraphDestroy(ex->graph[i]);
+  }
+  if (ex->dev_ptrs) (void)hipFree(ex->dev_ptrs);
+  if (ex->host_ptrs) (void)hipHostFree(ex->host_ptrs);
+  delete ex;
+  return CMBL_OK;
+}
+
+namespace {
+
+// Capture gather -> LenseFlow stages -> scatter over the plan's staging
+// buffer, reading the pointer arrays of one slot.
+cmbl_status capture_exec(cmbl_exec* ex, int slot, hipStream_t cs) {
+  cmbl_plan* p = ex->plan;
+  const int B = ex->max_batch;
+  void** ptrs = ex->dev_ptrs + (size_t)slot * 3 * B;
+  const long nm = (long)B * ex->ncomp;
+  const long len = (long)ex->ncomp * p->N * p->N;
+  hipError_t e;
+  if ((e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+    return fail(CMBL_EHIP, std::string("begin capture: ") + hipGetErrorString(e));
+  const bool timing = p->timing;
+  p->timing = false;  // no events inside the graph
+  cmbl_status st = CMBL_OK;
+  auto step = [&](hipError_t r) {
+    if (st == CMBL_OK && r != hipSuccess) st = fail(CMBL_EHIP, hipGetErrorString(r));
+  };
+  if (p->dtype == CMBL_F32) {
+    float* stage = (float*)p->stage;
+    step(cmbl::launch_gather<float>((const float* const*)(ptrs + B), stage, len, B, cs));
+    if (st == CMBL_OK)
+      st = run_lenseflow<float>(p, (const void* const*)ptrs, stage, nm, ex->ncomp, ex->nsteps,
+                                ex->direction, ex->adjoint, cs);
+    if (st == CMBL_OK)
+      step(cmbl::launch_scatter<float>(stage, (float* const*)(ptrs + 2 * B), len, B, cs));
+  } else {
+    double* stage = (double*)p->stage;
+    step(cmbl::launch_gather<double>((const double* const*)(ptrs + B), stage, len, B, cs));
+    if (st == CMBL_OK)
+      st = run_lenseflow<double>(p, (const void* const*)ptrs, stage, nm, ex->ncomp, ex->nsteps,
+                                 ex->direction, ex->adjoint, cs);
+    if (st == CMBL_OK)
+      step(cmbl::launch_scatter<double>(stage, (double* const*)(ptrs + 2 * B), len, B, cs));
+  }
+  p->timing = timing;
+  e = hipStreamEndCapture(cs, &ex->graph[slot]);
+  if (st != CMBL_OK) return st;
+  if (e != hipSuccess) return fail(CMBL_EHIP, std::string("end capture: ") + hipGetErrorString(e));
+  if ((e = hipGraphInstantiate(&ex->gexec[slot], ex->graph[slot], nullptr, nullptr, 0)) != hipSuccess)
+    return fail(CMBL_EHIP, std::string("instantiate: ") + hipGetErrorString(e));
+  return CMBL_OK;
+}
+
+}  // namespace
+
+extern "C" cmbl_status cmbl_exec_create(cmbl_exec** out, cmbl_plan* p, int max_batch, int ncomp,
+                                        int nsteps, int direction, int adjoint) {
+  if (!out || !p) return fail(CMBL_EINVAL, "NULL argument");
+  *out = nullptr;
+  if (ncomp != 1 && ncomp != 2) return fail(CMBL_EINVAL, "ncomp must be 1 (T) or 2 (QU)");
+  if (max_batch < 1 || (long)max_batch * ncomp > p->max_maps)
+    return fail(CMBL_EINVAL, "max_batch*ncomp must be in [1, plan max_maps]");
+  if (direction != CMBL_FORWARD_LENSE && direction != CMBL_INVERSE_LENSE)
+    return fail(CMBL_EINVAL, "direction must be +1 or -1");
+  if (nsteps < 1) return fail(CMBL_EINVAL, "nsteps must be >= 1");
+  hipError_t e;
+  if (!p->stage) {
+    const size_t bytes = (size_t)p->max_maps * p->N * p->N * p->tsize;
+    if ((e = hipMalloc(&p->stage, bytes)) != hipSuccess)
+      return fail(CMBL_ENOMEM, std::string("staging buffer: ") + hipGetErrorString(e));
+  }
+  cmbl_exec* ex = new cmbl_exec();
+  ex->plan = p;
+  ex->max_batch = max_batch;
+  ex->ncomp = ncomp;
+  ex->nsteps = nsteps;
+  ex->direction = direction;
+  ex->adjoint = adjoint ? 1 : 0;
+  auto bail = [&](cmbl_status st) {
+    cmbl_exec_destroy(ex);
+    return st;
+  };
+  const size_t pbytes = (size_t)kExecSlots * 3 * max_batch * sizeof(void*);
+  if ((e = hipMalloc((void**)&ex->dev_ptrs, pbytes)) != hipSuccess ||
+      (e = hipHostMalloc((void**)&ex->host_ptrs, pbytes)) != hipSuccess)
+    return bail(fail(CMBL_ENOMEM, std::string("exec pointer arrays: ") + hipGetErrorString(e)));
+  for (int i = 0; i < kExecSlots; ++i)
+    if ((e = hipEventCreateWithFlags(&ex->copied[i], hipEventDisableTiming)) != hipSuccess)
+      return bail(fail(CMBL_EHIP, std::string("event: ") + hipGetErrorString(e)));
+  hipStream_t cs;
+  if ((e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking)) != hipSuccess)
+    return bail(fail(CMBL_EHIP, std::string("capture stream: ") + hipGetErrorString(e)));
+  cmbl_status st = CMBL_OK;
+  for (int i = 0; i < kExecSlots && st == CMBL_OK; ++i) st = capture_exec(ex, i, cs);
+  (void)hipStreamDestroy(cs);
+  if (st != CMBL_OK) return bail(st);
+  *out = ex;
+  return CMBL_OK;
+}
+
+extern "C" cmbl_status cmbl_exec_run(cmbl_exec* ex, const cmbl_phi* const* phis,
+                                     const void* const* in_ptrs, void* const* out_ptrs, int count,
+                                     void* stream) {
+  if (!ex || !phis || !in_ptrs || !out_ptrs) return fail(CMBL_EINVAL, "NULL argument");
+  if (count < 1 || count > ex->max_batch) return fail(CMBL_EINVAL, "count must be in [1, max_batch]");
+  cmbl_plan* p = ex->plan;
+  for (int b = 0; b < count; ++b) {
+    if (!phis[b] || phis[b]->N != p->N || phis[b]->dtype != p->dtype ||
+        phis[b]->nsteps != ex->nsteps)
+      return fail(CMBL_EINVAL, "phi handle does not match the executor");
+    if (!in_ptrs[b] || !out_ptrs[b]) return fail(CMBL_EINVAL, "NULL map pointer");
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const int slot = ex->next;
+  ex->next = (slot + 1) % kExecSlots;
+  if (ex->used[slot]) HIP_TRY(hipEventSynchronize(ex->copied[slot]));
+  const int B = ex->max_batch;
+  void** host = ex->host_ptrs + (size_t)slot * 3 * B;
+  for (int b = 0; b < B; ++b) {
+    // padding slots lens a zero map with the first request's phi and are not
+    // written back
+    const bool real = b < count;
+    host[b] = phis[real ? b : 0]->ptab;
+    host[B + b] = real ? const_cast<void*>(in_ptrs[b]) : nullptr;
+    host[2 * B + b] = real ? out_ptrs[b] : nullptr;
+  }
+  HIP_TRY(hipMemcpyAsync(ex->dev_ptrs + (size_t)slot * 3 * B, host, 3 * B * sizeof(void*),
+                         hipMemcpyHostToDevice, s));
+  HIP_TRY(hipEventRecord(ex->copied[slot], s));
+  ex->used[slot] = true;
+  HIP_TRY(hipGraphLaunch(ex->gexec[slot], s));
   return CMBL_OK;
 }

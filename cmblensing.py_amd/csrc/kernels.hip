@@ -687,6 +687,32 @@ __global__ __launch_bounds__(256) void k_row_lf_adj(cplx<T>* __restrict__ S0,
   split_store<N>(uv[1], bufb, S0 + rp.rb * H, S1 + rp.rb * H, t, active);
 }
 
+// Gather / scatter of whole maps through device pointer arrays (serving
+// executor), `len` elements per slot b = blockIdx.y.  A NULL source fills the
+// slot with zeros (padding); a NULL destination skips it.
+template <typename T>
+__global__ void k_gather(const T* const* __restrict__ src, T* __restrict__ dst, long len) {
+  const int b = blockIdx.y;
+  const T* s = src[b];
+  T* d = dst + (long)b * len;
+  const long i0 = blockIdx.x * (long)blockDim.x + threadIdx.x, di = (long)gridDim.x * blockDim.x;
+  if (s) {
+    for (long i = i0; i < len; i += di) d[i] = s[i];
+  } else {
+    for (long i = i0; i < len; i += di) d[i] = T(0);
+  }
+}
+
+template <typename T>
+__global__ void k_scatter(const T* __restrict__ src, T* const* __restrict__ dst, long len) {
+  const int b = blockIdx.y;
+  T* d = dst[b];
+  if (!d) return;
+  const T* s = src + (long)b * len;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < len; i += (long)gridDim.x * blockDim.x)
+    d[i] = s[i];
+}
+
 // ---------------------------------------------------------------------------
 // host-side launchers (dispatch on N)
 // ---------------------------------------------------------------------------
@@ -810,6 +836,22 @@ hipError_t launch_dphi_combine(int N, const cplx<T>* F, cplx<T>* out, long nb, d
   return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_gather(const T* const* src, T* dst, long len, int count, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  const unsigned bx = (unsigned)std::min<long>((len + 255) / 256, 256);
+  k_gather<T><<<dim3(bx, count), 256, 0, s>>>(src, dst, len);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_scatter(const T* src, T* const* dst, long len, int count, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  const unsigned bx = (unsigned)std::min<long>((len + 255) / 256, 256);
+  k_scatter<T><<<dim3(bx, count), 256, 0, s>>>(src, dst, len);
+  return hipGetLastError();
+}
+
 #define CMBL_INST(T)                                                                            \
   template hipError_t launch_rfft2<T>(int, const T*, cplx<T>*, cplx<T>*, long, const cplx<T>*,  \
                                       hipStream_t);                                             \
@@ -824,6 +866,8 @@ hipError_t launch_dphi_combine(int N, const cplx<T>* F, cplx<T>* out, long nb, d
                                        const cplx<T>*, int, long, int, T, int, int, int, T,     \
                                        hipStream_t, T*, const GradArgs<T>*);                    \
   template hipError_t launch_gtab<T>(int, const double*, T*, int, hipStream_t);                 \
+  template hipError_t launch_gather<T>(const T* const*, T*, long, int, hipStream_t);            \
+  template hipError_t launch_scatter<T>(const T*, T* const*, long, int, hipStream_t);           \
   template hipError_t launch_dphi_combine<T>(int, const cplx<T>*, cplx<T>*, long, double,       \
                                              hipStream_t);
 CMBL_INST(float)

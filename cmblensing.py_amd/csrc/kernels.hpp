@@ -49,6 +49,10 @@ hipError_t launch_lf_row(int N, bool adjoint, cplx<T>* S0, cplx<T>* S1, T* y, T*
                          T h, int tidx, int do_r2c, int init, T dk, hipStream_t s,
                          T* zstore = nullptr, const GradArgs<T>* ga = nullptr);
 template <typename T>
+hipError_t launch_gather(const T* const* src, T* dst, long len, int count, hipStream_t s);
+template <typename T>
+hipError_t launch_scatter(const T* src, T* const* dst, long len, int count, hipStream_t s);
+template <typename T>
 hipError_t launch_gtab(int N, const double* D, T* gtab, int nsteps, hipStream_t s);
 template <typename T>
 hipError_t launch_dphi_combine(int N, const cplx<T>* F, cplx<T>* out, long nb, double dk,

@@ -1,0 +1,473 @@
+"""Serving: continuous batching of LenseFlow requests on one GPU.
+
+The reference serves one call at a time (lense_flow, lense_flow.py:289-292);
+this module keeps that call's meaning per request -- a request is one T map or
+one Q/U pair, lensed (direction +1), unlensed (-1) or adjoint-lensed by one of
+a set of shared lensing potentials -- and runs many requests together:
+
+* admission: a request is admitted when its input and output maps, plus the
+  operators of its phi if they are not resident yet, fit the HBM budget
+  (min of the configured budget and the device's free memory at start-up);
+  admission is first come first served, and resident phi operators that no
+  queued or running request uses are evicted (least recently used) to make
+  room;
+* resident operators: the phi derivatives and the p(t) table of a (phi, N)
+  are prepared once, on first use, and shared by every later request for the
+  same phi (the hit rate is reported);
+* continuous batching: admitted requests wait in buckets of one (N, C,
+  direction, adjoint); whenever a stream has room, the bucket whose oldest
+  request arrived first is launched as one batch of up to the bucket's batch
+  size, so requests that arrive while others run are packed into the next
+  launch and batches of different buckets overlap on the device's streams;
+* each batch is one replay of a hipGraph captured once per (stream, bucket,
+  batch class), with the request maps gathered into and scattered out of a
+  staging buffer by the graph's own kernels;
+* latency is arrival -> completion, the completion time read from a HIP
+  event recorded after the batch on its stream.
+
+The scheduler (``Server``) is independent of the device: ``GpuBackend`` runs
+the batches through libcmbl.so, and any object with the same methods (tests
+use a host-side one) can stand in for it.
+"""
+from __future__ import annotations
+
+import math
+import time
+from collections import OrderedDict, deque
+from dataclasses import dataclass, field
+from typing import Callable, Dict, List, Optional
+
+import numpy as np
+import torch
+
+from .constants import FORWARD_LENSE, INVERSE_LENSE
+
+GiB = 1 << 30
+DEFAULT_MAX_BATCH = {16: 256, 32: 256, 64: 256, 128: 128, 256: 128, 512: 32}
+
+
+@dataclass(eq=False)
+class Request:
+    """One LenseFlow request: x is [C, N, N] (C = 1 for T, 2 for Q/U) on the
+    serving device; phi_id names one of the server's shared potentials."""
+    x: torch.Tensor
+    phi_id: int
+    direction: int = FORWARD_LENSE
+    adjoint: bool = False
+    arrival: Optional[float] = None  # time.perf_counter() seconds; set on submit if None
+    rid: int = -1
+    # filled in by the server
+    out: Optional[torch.Tensor] = None
+    done: Optional[float] = None
+    state: str = "new"  # new, waiting, queued, running, done, cancelled, rejected
+    error: Optional[str] = None
+    nbytes: int = 0
+
+    @property
+    def N(self) -> int:
+        return int(self.x.shape[-1])
+
+    @property
+    def C(self) -> int:
+        return int(self.x.shape[0])
+
+    @property
+    def key(self):
+        return (self.N, self.C, int(self.direction), bool(self.adjoint))
+
+    @property
+    def latency(self) -> Optional[float]:
+        if self.done is None or self.arrival is None:
+            return None
+        return self.done - self.arrival
+
+
+@dataclass(eq=False)
+class _Batch:
+    token: object
+    requests: List[Request]
+    stream: int
+    key: tuple
+
+
+@dataclass(eq=False)
+class _PhiEntry:
+    handle: object
+    nbytes: int
+    refs: int = 0
+
+
+class Server:
+    """Admission, resident phi operators and continuous batching over a
+    backend (see the module docstring).  Drive it with submit() and step(),
+    or with serve() for a timed arrival schedule."""
+
+    def __init__(self, backend, phi_source: Callable[[int, int], torch.Tensor],
+                 hbm_budget: int = 64 * GiB, max_inflight: int = 2):
+        self.backend = backend
+        self.phi_source = phi_source
+        free = backend.free_bytes()
+        self.budget = int(min(hbm_budget, free)) if free is not None else int(hbm_budget)
+        self.max_inflight = max_inflight
+        self.used = backend.fixed_bytes()
+        self.peak_used = self.used
+        self.waiting: deque = deque()
+        self.buckets: "OrderedDict[tuple, deque]" = OrderedDict()
+        self.inflight: List[_Batch] = []
+        self.per_stream = [0] * backend.nstreams
+        self.phis: "OrderedDict[tuple, _PhiEntry]" = OrderedDict()
+        self.hits = self.misses = self.evictions = 0
+        self.completed: List[Request] = []
+        self.batch_sizes: List[int] = []
+        self._next_rid = 0
+        self._by_rid: Dict[int, Request] = {}
+
+    # -- requests ----------------------------------------------------------
+    def submit(self, req: Request) -> Request:
+        if req.N not in self.backend.sizes or req.C not in (1, 2):
+            raise ValueError(f"unsupported request shape {tuple(req.x.shape)}")
+        if req.direction not in (FORWARD_LENSE, INVERSE_LENSE):
+            raise ValueError("direction must be FORWARD_LENSE (+1) or INVERSE_LENSE (-1)")
+        if req.arrival is None:
+            req.arrival = time.perf_counter()
+        if req.rid < 0:
+            req.rid = self._next_rid
+            self._next_rid += 1
+        req.nbytes = self.backend.request_bytes(req)
+        req.state = "waiting"
+        self._by_rid[req.rid] = req
+        self.waiting.append(req)
+        return req
+
+    def cancel(self, rid: int) -> bool:
+        """Cancel a request that has not completed.  A waiting or queued
+        request is dropped at once; a running one is marked and its result
+        discarded when its batch completes (a launched batch is not cut)."""
+        req = self._by_rid.get(rid)
+        if req is None or req.state in ("done", "cancelled", "rejected"):
+            return False
+        if req.state == "waiting":
+            self.waiting.remove(req)
+            req.state = "cancelled"
+        elif req.state == "queued":
+            self.buckets[req.key].remove(req)
+            self._release(req)
+            req.state = "cancelled"
+        else:  # running
+            req.state = "cancelling"
+        return True
+
+    # -- admission -----------------------------------------------------------
+    def _phi_cost(self, req: Request) -> int:
+        k = (req.phi_id, req.N)
+        return 0 if k in self.phis else self.backend.phi_bytes(req.N)
+
+    def _evict_for(self, need: int, keep) -> bool:
+        """Evict unreferenced resident phi (LRU first, never `keep`) until
+        `need` bytes fit."""
+        while self.used + need > self.budget:
+            victim = next((k for k, e in self.phis.items() if e.refs == 0 and k != keep), None)
+            if victim is None:
+                return False
+            e = self.phis.pop(victim)
+            self.backend.release_phi(e.handle)
+            self.used -= e.nbytes
+            self.evictions += 1
+        return True
+
+    def _admit(self):
+        while self.waiting:
+            req = self.waiting[0]
+            need = req.nbytes + self._phi_cost(req)
+            if need > self.budget - self.backend.fixed_bytes():
+                self.waiting.popleft()
+                req.state, req.error = "rejected", "request does not fit the HBM budget"
+                continue
+            if self.used + need > self.budget and not self._evict_for(need, (req.phi_id, req.N)):
+                return  # wait for running batches to free memory
+            self.waiting.popleft()
+            k = (req.phi_id, req.N)
+            ent = self.phis.get(k)
+            if ent is None:
+                self.misses += 1
+                nb = self.backend.phi_bytes(req.N)
+                ent = _PhiEntry(self.backend.prepare_phi(req.N, self.phi_source(req.phi_id, req.N)),
+                                nb)
+                self.phis[k] = ent
+                self.used += nb
+            else:
+                self.hits += 1
+                self.phis.move_to_end(k)
+            ent.refs += 1
+            req.out = self.backend.alloc_output(req)
+            self.used += req.nbytes
+            self.peak_used = max(self.peak_used, self.used)
+            req.state = "queued"
+            self.buckets.setdefault(req.key, deque()).append(req)
+
+    def _release(self, req: Request):
+        self.used -= req.nbytes
+        self.phis[(req.phi_id, req.N)].refs -= 1
+
+    # -- batching ------------------------------------------------------------
+    def _free_stream(self) -> Optional[int]:
+        best = None
+        for s, n in enumerate(self.per_stream):
+            if n < self.max_inflight and (best is None or n < self.per_stream[best]):
+                best = s
+        return best
+
+    def _launch(self) -> int:
+        launched = 0
+        while True:
+            s = self._free_stream()
+            if s is None:
+                break
+            key, q = None, None
+            for k, dq in self.buckets.items():
+                if dq and (q is None or dq[0].arrival < q[0].arrival):
+                    key, q = k, dq
+            if q is None:
+                break
+            nb = min(len(q), self.backend.max_batch(key))
+            reqs = [q.popleft() for _ in range(nb)]
+            handles = [self.phis[(r.phi_id, r.N)].handle for r in reqs]
+            for r in reqs:
+                r.state = "running"
+            token = self.backend.launch(key, reqs, handles, s)
+            self.inflight.append(_Batch(token, reqs, s, key))
+            self.per_stream[s] += 1
+            self.batch_sizes.append(nb)
+            launched += 1
+        return launched
+
+    def _poll(self) -> int:
+        finished = 0
+        still = []
+        for b in self.inflight:
+            if not self.backend.ready(b.token):
+                still.append(b)
+                continue
+            t = self.backend.finish_time(b.token)
+            for r in b.requests:
+                self._release(r)
+                if r.state == "cancelling":
+                    r.state, r.out = "cancelled", None
+                    continue
+                r.done, r.state = t, "done"
+                self.completed.append(r)
+                finished += 1
+            self.per_stream[b.stream] -= 1
+        self.inflight = still
+        return finished
+
+    def step(self) -> int:
+        """One scheduling pass: retire finished batches, admit, launch."""
+        n = self._poll()
+        self._admit()
+        self._launch()
+        return n
+
+    def busy(self) -> bool:
+        return bool(self.waiting or self.inflight or any(self.buckets.values()))
+
+    def drain(self):
+        while self.busy():
+            self.step()
+
+    def serve(self, requests: List[Request], offsets: List[float]) -> dict:
+        """Open-loop run: request i arrives `offsets[i]` seconds after the
+        start (its arrival time is the scheduled one, so scheduling delay
+        counts toward its latency).  Returns stats()."""
+        order = np.argsort(np.asarray(offsets), kind="stable")
+        t0 = self.backend.start_clock()
+        i = 0
+        n = len(requests)
+        while i < n or self.busy():
+            now = time.perf_counter()
+            while i < n and t0 + offsets[order[i]] <= now:
+                r = requests[order[i]]
+                r.arrival = t0 + offsets[order[i]]
+                self.submit(r)
+                i += 1
+            self.step()
+        return self.stats()
+
+    # -- reporting -------------------------------------------------------------
+    def stats(self) -> dict:
+        lat = np.array([r.latency for r in self.completed], dtype=np.float64)
+        out = {"completed": len(self.completed), "batches": len(self.batch_sizes),
+               "mean_batch": float(np.mean(self.batch_sizes)) if self.batch_sizes else 0.0,
+               "phi_hits": self.hits, "phi_misses": self.misses,
+               "phi_hit_rate": self.hits / max(1, self.hits + self.misses),
+               "phi_evictions": self.evictions,
+               "hbm_budget_bytes": self.budget, "hbm_peak_bytes": self.peak_used}
+        if len(lat):
+            first = min(r.arrival for r in self.completed)
+            last = max(r.done for r in self.completed)
+            out.update({"p50_ms": float(np.percentile(lat, 50) * 1e3),
+                        "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                        "max_ms": float(lat.max() * 1e3),
+                        "maps_per_s": len(lat) / max(last - first, 1e-12)})
+        return out
+
+
+def _batch_class(count: int, cap: int) -> int:
+    c = 1 << max(0, math.ceil(math.log2(max(1, count))))
+    return min(c, cap)
+
+
+class GpuBackend:
+    """Runs batches through libcmbl.so graph executors on `nstreams` streams
+    of default priority.  Per (stream, N) one plan (scratch and staging for
+    the N's largest batch); per (stream, bucket, power-of-two batch class) one
+    executor, created on first use or by warmup()."""
+
+    def __init__(self, device="cuda:0", dtype=torch.float32, theta_pix: float = 2.0,
+                 nsteps: int = 7, nstreams: int = 2, max_batch: Optional[Dict[int, int]] = None,
+                 sizes=(64, 128, 256, 512)):
+        from . import _lib
+        self._lib = _lib
+        _lib.load()
+        self.device = torch.device(device)
+        if self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
+        self.dtype, self.theta_pix, self.nsteps = dtype, float(theta_pix), nsteps
+        self.sizes = tuple(sizes)
+        self.mb = {N: (max_batch or DEFAULT_MAX_BATCH).get(N, 32) for N in self.sizes}
+        self.nstreams = nstreams
+        self.streams = [torch.cuda.Stream(self.device) for _ in range(nstreams)]
+        self.prep_stream = torch.cuda.Stream(self.device)
+        self.s = 4 if dtype == torch.float32 else 8
+        self.plans: dict = {}
+        self.execs: dict = {}
+        self._fixed = 0
+        self._anchor_ev = None
+        self._anchor_t = 0.0
+
+    # memory model ---------------------------------------------------------------
+    def free_bytes(self) -> Optional[int]:
+        free, _ = torch.cuda.mem_get_info(self.device)
+        return int(free * 0.9)
+
+    def fixed_bytes(self) -> int:
+        """Plan scratch + staging of every (stream, N) plan (4 map stacks of
+        the plan's max_maps), counted up front."""
+        return sum(4 * 2 * self.mb[N] * N * N * self.s for N in self.sizes) * self.nstreams
+
+    def request_bytes(self, req: Request) -> int:
+        return 2 * req.C * req.N * req.N * self.s
+
+    def phi_bytes(self, N: int) -> int:
+        return 5 * N * N * 8 + (2 * self.nsteps + 1) * 2 * N * N * self.s
+
+    def max_batch(self, key) -> int:
+        return self.mb[key[0]]
+
+    # resources --------------------------------------------------------------------
+    def _plan(self, si: int, N: int):
+        p = self.plans.get((si, N))
+        if p is None:
+            p = self._lib.Plan(N, self.theta_pix, self.dtype, 2 * self.mb[N], self.device)
+            self.plans[(si, N)] = p
+        return p
+
+    def _exec(self, si: int, key, cls: int):
+        ek = (si, key, cls)
+        ex = self.execs.get(ek)
+        if ex is None:
+            N, C, direction, adjoint = key
+            ex = self._lib.Executor(self._plan(si, N), cls, C, self.nsteps, direction, adjoint)
+            self.execs[ek] = ex
+        return ex
+
+    def warmup(self, keys=None):
+        """Create every executor (all batch classes) for `keys` (default: every
+        bucket of the served sizes), so no graph is captured while serving."""
+        if keys is None:
+            keys = [(N, C, d, False) for N in self.sizes for C in (1, 2)
+                    for d in (FORWARD_LENSE, INVERSE_LENSE)]
+        for key in keys:
+            cap = self.mb[key[0]]
+            classes = sorted({_batch_class(c, cap) for c in range(1, cap + 1)})
+            for si in range(self.nstreams):
+                for cls in classes:
+                    self._exec(si, key, cls)
+        torch.cuda.synchronize(self.device)
+
+    def prepare_phi(self, N: int, phi: torch.Tensor):
+        # on a stream of its own (preparation waits for its own work only); the
+        # handle is complete on return and serves every plan of this N
+        with torch.cuda.stream(self.prep_stream):
+            return self._lib.PhiHandle(self._plan(0, N), phi.to(self.device), self.nsteps)
+
+    def release_phi(self, handle):
+        handle.close()
+
+    def alloc_output(self, req: Request) -> torch.Tensor:
+        return torch.empty_like(req.x)
+
+    # batches ------------------------------------------------------------------------
+    def start_clock(self) -> float:
+        """Anchor the event clock: returns the host time of the anchor event."""
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record(torch.cuda.current_stream(self.device))
+        ev.synchronize()
+        self._anchor_t = time.perf_counter()
+        self._anchor_ev = ev
+        return self._anchor_t
+
+    def launch(self, key, reqs: List[Request], handles, si: int):
+        if self._anchor_ev is None:
+            self.start_clock()
+        N, C, _, _ = key
+        for r in reqs:
+            if r.x.dtype != self.dtype or not r.x.is_contiguous() or r.x.device != self.device:
+                raise ValueError("request maps must be contiguous, on the serving device, in the "
+                                 "serving dtype")
+        cls = _batch_class(len(reqs), self.mb[N])
+        ex = self._exec(si, key, cls)
+        stream = self.streams[si]
+        # inputs written on the caller's stream become visible to this one
+        stream.wait_stream(torch.cuda.current_stream(self.device))
+        ex.run(handles, [r.x.data_ptr() for r in reqs], [r.out.data_ptr() for r in reqs],
+               stream.cuda_stream)
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record(stream)
+        return ev
+
+    def ready(self, token) -> bool:
+        return token.query()
+
+    def finish_time(self, token) -> float:
+        return self._anchor_t + self._anchor_ev.elapsed_time(token) * 1e-3
+
+    def synchronize(self):
+        torch.cuda.synchronize(self.device)
+
+    def close(self):
+        torch.cuda.synchronize(self.device)
+        for ex in self.execs.values():
+            ex.close()
+        self.execs.clear()
+        for p in self.plans.values():
+            p.close()
+        self.plans.clear()
+
+
+def poisson_schedule(n: int, rate: float, seed: int) -> np.ndarray:
+    """Arrival offsets (s) of a Poisson stream: cumulative Exp(1/rate) gaps."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, size=n))
+
+
+def mixed_workload(n: int, seed: int, sizes=(64, 128, 256, 512), weights=(0.2, 0.4, 0.3, 0.1),
+                   nphi: int = 32):
+    """Request descriptors of the mixed stream: (N, C, direction, phi_id) with
+    N drawn with `weights`, T/QU and lense/unlense 50/50, phi_id ~ U{0..nphi-1}."""
+    rng = np.random.default_rng(seed)
+    Ns = rng.choice(np.asarray(sizes), size=n, p=np.asarray(weights) / np.sum(weights))
+    Cs = rng.integers(1, 3, size=n)
+    dirs = np.where(rng.random(n) < 0.5, FORWARD_LENSE, INVERSE_LENSE)
+    phis = rng.integers(0, nphi, size=n)
+    return [(int(a), int(b), int(c), int(d)) for a, b, c, d in zip(Ns, Cs, dirs, phis)]

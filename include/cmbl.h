@@ -136,6 +136,25 @@ cmbl_status cmbl_cg_update_xrz(cmbl_plan* plan, void* x, void* r, void* z, const
 cmbl_status cmbl_cg_update_p(cmbl_plan* plan, void* p, const void* z, const double* beta,
                              void* p_op, int nrhs, int ncomp, void* stream);
 
+/* ---- serving executor ------------------------------------------------------
+ * No counterpart in the reference (it serves one call at a time): a batched
+ * LenseFlow of fixed capacity (max_batch requests of ncomp components, nsteps,
+ * direction, adjoint) whose whole launch sequence is captured once into a
+ * hipGraph (kernel nodes only) over the plan's staging buffer.  cmbl_exec_run
+ * uploads the request pointers (device pointers to [ncomp][N][N] maps and
+ * phi handles prepared for nsteps) and replays the graph on `stream`: gather
+ * of the `count` inputs into staging, the RK4 stages, scatter of the results
+ * to out_ptrs; slots past `count` lens zero maps and are not written back.
+ * Executors use their plan's scratch: runs of executors that share a plan
+ * must be ordered on one stream, and every executor is destroyed before its
+ * plan.  Inputs and outputs must stay allocated until the run completes. */
+typedef struct cmbl_exec cmbl_exec;
+cmbl_status cmbl_exec_create(cmbl_exec** out, cmbl_plan* plan, int max_batch, int ncomp,
+                             int nsteps, int direction, int adjoint);
+cmbl_status cmbl_exec_run(cmbl_exec* ex, const cmbl_phi* const* phis, const void* const* in_ptrs,
+                          void* const* out_ptrs, int count, void* stream);
+cmbl_status cmbl_exec_destroy(cmbl_exec* ex);
+
 /* Timing mode (no counterpart in the reference; measurement aid): when on,
  * every LenseFlow stage launch is bracketed by HIP events on the call's stream
  * and cmbl_lenseflow waits for its own work before returning.
