@@ -123,6 +123,140 @@ def serving_run(torch, sims, dev, n_requests, rate, seed=2, nphi=32, budget_gib=
     return out
 
 
+def _gpu_problem(torch, pkg, sims, dev, prob, cdt):
+    """GPU fields and operators of a numpy problem from sims.wf_problem /
+    sims.wf_data (EB Fourier field, MAP phi, per-mode operators, pixel mask)."""
+    import importlib
+    mo = importlib.import_module("cmblensing_amd.matrix_operators")
+    th = prob["theta_pix"]
+    rdt = torch.float32 if cdt == torch.complex64 else torch.float64
+    T = lambda a, dt=cdt: torch.as_tensor(a, device=dev).to(dt)
+    op = lambda A: mo.DiagonalEB(matrix_EE=T(A[0], torch.float64), matrix_BB=T(A[1], torch.float64),
+                                 theta_pix=th)
+    f = pkg.FlatS2(polar_matrix_1=T(prob["f"][0]), polar_matrix_2=T(prob["f"][1]), theta_pix=th)
+    data = pkg.FlatS2(polar_matrix_1=T(prob["d"][0], rdt), polar_matrix_2=T(prob["d"][1], rdt),
+                      theta_pix=th)
+    mask = mo.PixelMask(matrix=torch.as_tensor(prob["mask_pix"], device=dev))
+    return f, data, op(prob["Cf"]), op(prob["Cn"]), mask, op(prob["B"])
+
+
+def _lensed_truth(torch, pkg, dev, prob, n):
+    """Q/U maps of L(phi) f for a problem's truth, lensed by the float64 GPU
+    path (the data model d = M B L f + n)."""
+    th = prob["theta_pix"]
+    f = pkg.FlatS2(polar_matrix_1=torch.as_tensor(prob["f"][0], device=dev),
+                   polar_matrix_2=torch.as_tensor(prob["f"][1], device=dev), theta_pix=th)
+    q = pkg.map(pkg.eb2qu(f))
+    x = torch.stack([q.polar_matrix_1, q.polar_matrix_2])[None]
+    y = pkg.lense_flow_matrices(x, torch.as_tensor(prob["phi"], device=dev), th, n, 1, False)
+    return y[0].cpu().numpy()
+
+
+def wf_run(torch, pkg, sims, dev, N=512, th=1.5, nrhs=8, n=7, rtol=1e-6):
+    """Lensed Wiener filter, Q/U N=512 at 1.5 arcmin, 8 right-hand sides
+    (signal, noise and phi seeded 100..107), 1 uK-arcmin white noise with a
+    l_knee = 100 1/f term, 200 point-source holes of 5 arcmin, 1 arcmin beam.
+    FP32 operator with FP64 CG vectors, dots, alpha/beta and FP64 residual
+    recomputation, to a relative residual of 1e-6; compared with the same
+    solve run entirely in FP64 to 1e-10."""
+    import importlib
+    wfm = importlib.import_module("cmblensing_amd.wiener_filter")
+    probs = []
+    for i in range(nrhs):
+        pr = sims.wf_problem(N, th, pol=True, seed=100 + i, mask="holes", noise_uk=1.0,
+                             ell_knee=100.0, beam_fwhm=1.0)
+        probs.append(sims.wf_data(pr, _lensed_truth(torch, pkg, dev, pr, n)))
+
+    def batched(cdt):
+        objs = [_gpu_problem(torch, pkg, sims, dev, p, cdt) for p in probs]
+        f0, _, Cf, Cn, M, Bm = objs[0]
+        st = lambda name, j: torch.stack([getattr(o[j], name) for o in objs])
+        data = objs[0][1].replace(polar_matrix_1=st("polar_matrix_1", 1),
+                                  polar_matrix_2=st("polar_matrix_2", 1))
+        x0 = f0.replace(polar_matrix_1=torch.zeros_like(st("polar_matrix_1", 0)),
+                        polar_matrix_2=torch.zeros_like(st("polar_matrix_2", 0)))
+        phi = pkg.FlatS0(scalar_matrix=torch.stack([torch.as_tensor(p["phi"], device=dev)
+                                                    for p in probs]), theta_pix=th)
+        return x0, phi, data, Cf, Cn, M, Bm
+
+    args32 = batched(torch.complex64)
+    wfm.wiener_filter(*args32, n=n, tol=0.0, rtol=rtol, recompute_every=10, maxiter=2)  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x32, info = wfm.wiener_filter(*args32, n=n, tol=0.0, rtol=rtol, recompute_every=10,
+                                  maxiter=1000, return_info=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    args64 = batched(torch.complex128)
+    t1 = time.perf_counter()
+    x64, info64 = wfm.wiener_filter(*args64, n=n, tol=0.0, rtol=1e-10, maxiter=3000,
+                                    return_info=True)
+    torch.cuda.synchronize()
+    dt64 = time.perf_counter() - t1
+    errs = []
+    for i in range(nrhs):
+        a = torch.stack([x32.polar_matrix_1[i], x32.polar_matrix_2[i]]).to(torch.complex128)
+        b = torch.stack([x64.polar_matrix_1[i], x64.polar_matrix_2[i]])
+        errs.append(float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)))
+    its = info.iterations
+    return {"workload": "wf_QU_512_1.5arcmin_8rhs_holes_1overf", "nrhs": nrhs,
+            "iterations": its, "converged": all(info.converged),
+            "final_true_rel_residual_fp64": max(info.final_true_rel_residual),
+            "time_s": round(dt, 3), "ms_per_iteration": round(dt / max(its) * 1e3, 3),
+            "fp64_solve": {"iterations": info64.iterations, "time_s": round(dt64, 3),
+                           "final_true_rel_residual": max(info64.final_true_rel_residual)},
+            "rel_l2_vs_fp64_solve_max": max(errs)}
+
+
+def grad_run(torch, pkg, sims, dev, N=256, th=2.0, chains=16, n=7, reps=5):
+    """d logpdf / d(f, phi) for 16 chains, Q/U N=256 at 2 arcmin, float32:
+    1 arcmin beam, 5 uK-arcmin white noise, apodised square mask of 80% area;
+    timed per evaluation of both gradients, compared with the float64 path."""
+    import importlib
+    grads = importlib.import_module("cmblensing_amd.gradients")
+    stats = importlib.import_module("cmblensing_amd.statistics")
+    mo = importlib.import_module("cmblensing_amd.matrix_operators")
+    pr = sims.wf_problem(N, th, pol=True, seed=21, mask="square", noise_uk=5.0, beam_fwhm=1.0)
+    pr = sims.wf_data(pr, _lensed_truth(torch, pkg, dev, pr, n))
+    phi_m, Cphi = sims.phi_map_and_cov(N, th, 2)
+
+    def objs(cdt):
+        f, data, Cf, Cn, M, Bm = _gpu_problem(torch, pkg, sims, dev, pr, cdt)
+        scale = torch.tensor([1 + 0.01 * k for k in range(chains)], device=dev).to(cdt)[:, None, None]
+        f = f.replace(polar_matrix_1=f.polar_matrix_1[None] * scale,
+                      polar_matrix_2=f.polar_matrix_2[None] * scale)
+        phiF = pkg.FlatS0(scalar_matrix=torch.as_tensor(np.fft.rfft2(phi_m), device=dev).to(cdt),
+                          theta_pix=th)
+        Cp = mo.DiagonalScalar(scalar_matrix=torch.as_tensor(Cphi, device=dev), theta_pix=th)
+        return f, phiF, data, Cf, Cn, M, Bm, Cp
+
+    def both(o):
+        f, phiF, data, Cf, Cn, M, Bm, Cp = o
+        gf = grads.gradf_logpdf(f, phiF, data, Cf, Cn, M, Bm, n=n)
+        gp = stats.grad_phi_logpdf(f, phiF, data, Cn, Cp, Cf, M, Bm, n=n)
+        return gf, gp
+
+    o32 = objs(torch.complex64)
+    both(o32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        gf, gp = both(o32)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    gf64, gp64 = both(objs(torch.complex128))
+
+    def rel(a, b):
+        a = a.to(torch.complex128)
+        return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+    return {"workload": "grad_logpdf_QU_256_2arcmin_16chains", "chains": chains,
+            "ms_per_evaluation": round(dt * 1e3, 3),
+            "chain_gradients_per_s": round(chains / dt, 1),
+            "rel_l2_gradf_vs_fp64": max(rel(gf.polar_matrix_1, gf64.polar_matrix_1),
+                                        rel(gf.polar_matrix_2, gf64.polar_matrix_2)),
+            "rel_l2_grad_phi_vs_fp64": rel(gp.scalar_matrix, gp64.scalar_matrix)}
+
+
 def copy_probe(torch):
     """A plain copy of known size: FETCH_SIZE / WRITE_SIZE of this kernel give
     the scale factors of the HBM counters, and its time the achievable copy
@@ -157,6 +291,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-wf", action="store_true")
+    ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -315,6 +451,12 @@ def main():
         result["serving"] = serving_run(torch, sims, dev, args.serve_requests, args.serve_rate)
         result["latency_ms"] = {"p50": result["serving"]["p50_ms"],
                                 "p99": result["serving"]["p99_ms"]}
+    if not args.no_grad:
+        print("[bench] gradients, 16 chains", file=sys.stderr, flush=True)
+        result["differentiation"] = grad_run(torch, pkg, sims, dev)
+    if not args.no_wf:
+        print("[bench] Wiener filter 512^2 QU, 8 RHS", file=sys.stderr, flush=True)
+        result["wiener_filter"] = wf_run(torch, pkg, sims, dev)
     if not args.no_cpu:
         fcpu = fq[0]
         cps, k, el = cpu_baseline(fcpu, phis[0], th, n, args.cpu_budget)

@@ -93,6 +93,14 @@ struct cmbl_plan {
   void* gF = nullptr;    // their rfft2 [max_maps][5][N][N/2+1]
   // serving executors' staging buffer [max_maps][N][N], allocated on first use
   void* stage = nullptr;
+  // phi preparation temporaries (double), allocated on the first prepare and
+  // kept, so a prepare neither allocates nor frees scratch: rfft2 of phi
+  // [N][N/2+1], its five derivative spectra, the packed FFT scratch
+  // [5][N][N/2] and the per-block minima of the magnification determinant
+  void* prep_F = nullptr;
+  void* prep_D = nullptr;
+  void* prep_Pk = nullptr;
+  double* prep_min = nullptr;
 };
 
 struct cmbl_phi {
@@ -175,7 +183,8 @@ cmbl_status cmbl_plan_destroy(cmbl_plan* p) {
   if (p->s1) (void)hipFree(p->s1);
   if (p->ptab_dev) (void)hipFree(p->ptab_dev);
   for (void* q : {p->wy1, p->wy2, p->wx1, p->wx2, (void*)p->red, p->gy, p->gacc, p->gzd, p->gu,
-                  p->gv, p->gsrc, p->gF, p->stage})
+                  p->gv, p->gsrc, p->gF, p->stage, p->prep_F, p->prep_D,
+                  p->prep_Pk, (void*)p->prep_min})
     if (q) (void)hipFree(q);
   if (p->ptab_host) (void)hipHostFree(p->ptab_host);
   delete p;
@@ -236,29 +245,23 @@ cmbl_status cmbl_phi_prepare(cmbl_plan* p, const void* phi, int phi_basis, int n
   h->nsteps = nsteps;
   h->dtype = p->dtype;
   h->plan = p;
-  cplx<double>* F = nullptr;
-  cplx<double>* D = nullptr;
-  cplx<double>* Pk = nullptr;
-  double* minpart = nullptr;
-  auto release = [&]() {
-    if (F) (void)hipFree(F);
-    if (D) (void)hipFree(D);
-    if (Pk) (void)hipFree(Pk);
-    if (minpart) (void)hipFree(minpart);
-  };
   auto bail = [&](cmbl_status st) {
-    release();
     cmbl_phi_release(h);
     return st;
   };
   hipError_t e;
   const size_t ntab = (size_t)(2 * nsteps + 1) * 2 * NN;
   const int nblk = (int)((NN + 255) / 256);
-  if ((e = hipMalloc(&F, NH1 * sizeof(cplx<double>))) != hipSuccess ||
-      (e = hipMalloc(&D, 5 * NH1 * sizeof(cplx<double>))) != hipSuccess ||
-      (e = hipMalloc(&Pk, 5 * NH * sizeof(cplx<double>))) != hipSuccess ||
-      (e = hipMalloc(&minpart, nblk * sizeof(double))) != hipSuccess ||
-      (e = hipMalloc(&h->deriv, 5 * NN * sizeof(double))) != hipSuccess ||
+  if ((!p->prep_F && (e = hipMalloc(&p->prep_F, NH1 * sizeof(cplx<double>))) != hipSuccess) ||
+      (!p->prep_D && (e = hipMalloc(&p->prep_D, 5 * NH1 * sizeof(cplx<double>))) != hipSuccess) ||
+      (!p->prep_Pk && (e = hipMalloc(&p->prep_Pk, 5 * NH * sizeof(cplx<double>))) != hipSuccess) ||
+      (!p->prep_min && (e = hipMalloc((void**)&p->prep_min, nblk * sizeof(double))) != hipSuccess))
+    return bail(fail(CMBL_ENOMEM, std::string("phi_prepare scratch: ") + hipGetErrorString(e)));
+  cplx<double>* F = (cplx<double>*)p->prep_F;
+  cplx<double>* D = (cplx<double>*)p->prep_D;
+  cplx<double>* Pk = (cplx<double>*)p->prep_Pk;
+  double* minpart = p->prep_min;
+  if ((e = hipMalloc(&h->deriv, 5 * NN * sizeof(double))) != hipSuccess ||
       (e = hipMalloc(&h->ptab, ntab * p->tsize)) != hipSuccess)
     return bail(fail(CMBL_ENOMEM, std::string("phi_prepare: ") + hipGetErrorString(e)));
   const cplx<double>* tw64 = (const cplx<double>*)p->tw64;
@@ -288,7 +291,6 @@ cmbl_status cmbl_phi_prepare(cmbl_plan* p, const void* phi, int phi_basis, int n
   for (double v : mp) md = std::min(md, v);
   h->min_det = md;
   if (min_det) *min_det = md;
-  release();
   *out = h;
   return CMBL_OK;
 }

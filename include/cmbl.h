@@ -58,7 +58,9 @@ cmbl_status cmbl_plan_destroy(cmbl_plan* plan);
  * j = 0..2n (lense_flow.py:372-381, 397-415), stored in the plan dtype.
  * phi is float64: MAP [N][N] or FOURIER [N][N/2+1] complex (lense_flow.py:299).
  * min_det (host out, may be NULL) receives min over pixels and t in [0,1] of
- * det(I + t Hess phi).  Synchronises `stream`. */
+ * det(I + t Hess phi).  Synchronises `stream`.  The temporaries live in the
+ * plan (allocated by its first prepare), so preparations on one plan must not
+ * be issued concurrently from several host threads. */
 cmbl_status cmbl_phi_prepare(cmbl_plan* plan, const void* phi, int phi_basis, int nsteps,
                              cmbl_phi** out, double* min_det, void* stream);
 cmbl_status cmbl_phi_release(cmbl_phi* phi);
