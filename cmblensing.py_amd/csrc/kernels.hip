@@ -458,14 +458,14 @@ __device__ __forceinline__ void add_sources(T* S, long NN, T cs, T tval, T fx, T
 // S0 holds the row spectra S of the stage input, S1 their x-derivative DX;
 // the line is Z = DX + i DY with DY = i ky S / N (DY's packed slot is zero:
 // ky = 0 at DC, and the real Nyquist value times i drops out of the C2R).
-template <typename T, int N, bool GRAD>
+template <typename T, int N, int C, bool GRAD>
 __global__ __launch_bounds__(256) void k_row_lf_std(cplx<T>* __restrict__ S0,
                                                     const cplx<T>* __restrict__ S1,
                                                     T* __restrict__ y, T* __restrict__ acc,
                                                     const T* const* __restrict__ ptabs,
-                                                    const cplx<T>* __restrict__ twg, int C,
-                                                    long npairs, int stage, T h, int tidx,
-                                                    int do_r2c, T cy, GradArgs<T> ga) {
+                                                    const cplx<T>* __restrict__ twg, long npairs,
+                                                    int stage, T h, int tidx, int do_r2c, T cy,
+                                                    GradArgs<T> ga) {
   constexpr int G = N / 8, PAIRS = 256 / G, H = N / 2;
   constexpr long NN = (long)N * N;
   __shared__ cplx<T> lbuf[PAIRS][2][lds_len<N>()];
@@ -477,8 +477,9 @@ __global__ __launch_bounds__(256) void k_row_lf_std(cplx<T>* __restrict__ S0,
   const RowPair rp = row_pair<N>((int)(active ? p : npairs - 1), C);
   cplx<T>* bufa = lbuf[grp][0];
   cplx<T>* bufb = lbuf[grp][1];
-  // loads in the order they are consumed (vmcnt retires in order): the row
-  // spectra for the transforms first, then the per-pixel operands
+  // the row spectra first; the per-pixel operands are loaded after the
+  // inverse transforms, so they are not live across the FFT (register
+  // pressure sets the occupancy of this kernel)
   cplx<T> dxa[4], sa[4], dxb[4], sb[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -487,21 +488,6 @@ __global__ __launch_bounds__(256) void k_row_lf_std(cplx<T>* __restrict__ S0,
     sa[q] = S0[rp.ra * H + k];
     dxb[q] = S1[rp.rb * H + k];
     sb[q] = S0[rp.rb * H + k];
-  }
-  const T* pt = ptabs[rp.b] + (long)tidx * 2 * NN;
-  T pxa[8], pya[8], pxb[8], pyb[8], ya[8], yb[8], aa[8], ab[8];
-#pragma unroll
-  for (int m = 0; m < 8; ++m) {
-    const int j = t + m * G;
-    const long ja = (long)rp.ia * N + j, jb = (long)rp.ib * N + j;
-    pxa[m] = pt[ja];
-    pya[m] = pt[NN + ja];
-    pxb[m] = pt[jb];
-    pyb[m] = pt[NN + jb];
-    ya[m] = stage == 4 ? T(0) : y[rp.ra * N + j];
-    yb[m] = stage == 4 ? T(0) : y[rp.rb * N + j];
-    aa[m] = stage == 1 ? T(0) : acc[rp.ra * N + j];
-    ab[m] = stage == 1 ? T(0) : acc[rp.rb * N + j];
   }
   cplx<T> v[2][8];
   __syncthreads();
@@ -520,27 +506,51 @@ __global__ __launch_bounds__(256) void k_row_lf_std(cplx<T>* __restrict__ S0,
   }
   cplx<T>* const bufs[2] = {bufa, bufb};
   fft_lines<N, true, 2>(v, bufs, tw, t);
-
+  // per-pixel work, one row at a time: the operands of row b are loaded after
+  // row a is done, so at most one row's operands are live
+  const T* pt = ptabs[rp.b] + (long)tidx * 2 * NN;
+  T px[8], py[8];
+  auto load_p = [&](long irow) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const long jj = irow * N + t + m * G;
+      px[m] = pt[jj];
+      py[m] = pt[NN + jj];
+    }
+  };
   if constexpr (GRAD) {
     // delta-phi sources at this stage: sum_c df_c grad f_c (lense_flow.py:201-213)
-    if (active) {
-      const T* gt = ga.gtabs[rp.b] + (long)tidx * 3 * NN;
-      T* Sb = ga.sacc + (long)rp.b * 5 * NN;
+    const T* gt = ga.gtabs[rp.b] + (long)tidx * 3 * NN;
+    T* Sb = ga.sacc + (long)rp.b * 5 * NN;
+    if constexpr (C == 2) {
+      load_p(rp.ia);
+      if (active) {
 #pragma unroll
-      for (int m = 0; m < 8; ++m) {
-        const int j = t + m * G;
-        const long ja = (long)rp.ia * N + j, jb = (long)rp.ib * N + j;
-        const T zda = ga.zd[rp.ra * N + j], zdb = ga.zd[rp.rb * N + j];
-        if (C == 2) {
+        for (int m = 0; m < 8; ++m) {
+          const int j = t + m * G;
+          const long ja = (long)rp.ia * N + j;
+          const T zda = ga.zd[rp.ra * N + j], zdb = ga.zd[rp.rb * N + j];
           const T fx = zda * v[0][m].x + zdb * v[1][m].x;
           const T fy = zda * v[0][m].y + zdb * v[1][m].y;
-          add_sources(Sb + ja, NN, ga.cs, ga.tval, fx, fy, pxa[m], pya[m], gt[ja], gt[NN + ja],
+          add_sources(Sb + ja, NN, ga.cs, ga.tval, fx, fy, px[m], py[m], gt[ja], gt[NN + ja],
                       gt[2 * NN + ja]);
-        } else {
-          add_sources(Sb + ja, NN, ga.cs, ga.tval, zda * v[0][m].x, zda * v[0][m].y, pxa[m], pya[m],
-                      gt[ja], gt[NN + ja], gt[2 * NN + ja]);
-          add_sources(Sb + jb, NN, ga.cs, ga.tval, zdb * v[1][m].x, zdb * v[1][m].y, pxb[m], pyb[m],
-                      gt[jb], gt[NN + jb], gt[2 * NN + jb]);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const long ir = r ? rp.ib : rp.ia;
+        const long rr = r ? rp.rb : rp.ra;
+        load_p(ir);
+        if (active) {
+#pragma unroll
+          for (int m = 0; m < 8; ++m) {
+            const int j = t + m * G;
+            const long jr = ir * N + j;
+            const T zd = ga.zd[rr * N + j];
+            add_sources(Sb + jr, NN, ga.cs, ga.tval, zd * v[r][m].x, zd * v[r][m].y, px[m],
+                        py[m], gt[jr], gt[NN + jr], gt[2 * NN + jr]);
+          }
         }
       }
     }
@@ -548,23 +558,33 @@ __global__ __launch_bounds__(256) void k_row_lf_std(cplx<T>* __restrict__ S0,
 
   cplx<T> z[1][8];
 #pragma unroll
-  for (int m = 0; m < 8; ++m) {
-    const int j = t + m * G;
-    const T ka = pxa[m] * v[0][m].x + pya[m] * v[0][m].y;
-    const T kb = pxb[m] * v[1][m].x + pyb[m] * v[1][m].y;
-    T za, zb;
-    rk4_stage(stage, h, ka, ya[m], aa[m], za);
-    rk4_stage(stage, h, kb, yb[m], ab[m], zb);
-    if (active) {
-      if (stage == 4) {
-        y[rp.ra * N + j] = za;
-        y[rp.rb * N + j] = zb;
-      } else {
-        acc[rp.ra * N + j] = aa[m];
-        acc[rp.rb * N + j] = ab[m];
-      }
+  for (int r = 0; r < 2; ++r) {
+    const long rr = r ? rp.rb : rp.ra;
+    if (r == 0 || C == 1) load_p(r ? rp.ib : rp.ia);
+    T yv[8], av[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int j = t + m * G;
+      yv[m] = stage == 4 ? T(0) : y[rr * N + j];
+      av[m] = stage == 1 ? T(0) : acc[rr * N + j];
     }
-    z[0][m] = mk(za, zb);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int j = t + m * G;
+      const T k = px[m] * v[r][m].x + py[m] * v[r][m].y;
+      T zz;
+      rk4_stage(stage, h, k, yv[m], av[m], zz);
+      if (active) {
+        if (stage == 4)
+          y[rr * N + j] = zz;
+        else
+          acc[rr * N + j] = av[m];
+      }
+      if (r == 0)
+        z[0][m].x = zz;
+      else
+        z[0][m].y = zz;
+    }
   }
   if (!do_r2c) return;  // uniform over the grid
   cplx<T>* const bufz[1] = {bufa};
@@ -578,12 +598,12 @@ __global__ __launch_bounds__(256) void k_row_lf_std(cplx<T>* __restrict__ S0,
 // grad_line); C2R of (W_a + i W_b) gives the two rates, then RK4 update.  Then,
 // unless this is the very last stage, u = p_x(t_next) z and v = p_y(t_next) z
 // are row-R2C'd per row (as u + i v) into S0 (U) and S1 (V).
-template <typename T, int N>
+template <typename T, int N, int C>
 __global__ __launch_bounds__(256) void k_row_lf_adj(cplx<T>* __restrict__ S0,
                                                     cplx<T>* __restrict__ S1,
                                                     T* __restrict__ y, T* __restrict__ acc,
                                                     const T* const* __restrict__ ptabs,
-                                                    const cplx<T>* __restrict__ twg, int C,
+                                                    const cplx<T>* __restrict__ twg,
                                                     long npairs, int stage, T h, int tidx_next,
                                                     int do_r2c, int init, T cy,
                                                     T* __restrict__ zstore) {
@@ -598,8 +618,9 @@ __global__ __launch_bounds__(256) void k_row_lf_adj(cplx<T>* __restrict__ S0,
   const RowPair rp = row_pair<N>((int)(active ? p : npairs - 1), C);
   cplx<T>* bufa = lbuf[grp][0];
   cplx<T>* bufb = lbuf[grp][1];
-  // loads in the order they are consumed (vmcnt retires in order): the row
-  // spectra first, then the per-pixel operands
+  // the row spectra first; the per-pixel operands are loaded after the
+  // inverse transform, so they are not live across it (register pressure
+  // sets the occupancy of this kernel)
   cplx<T> wa[4], wb[4], va[4], vb[4];
   if (!init) {
 #pragma unroll
@@ -611,31 +632,9 @@ __global__ __launch_bounds__(256) void k_row_lf_adj(cplx<T>* __restrict__ S0,
       vb[q] = S1[rp.rb * H + k];
     }
   }
-  const T* pt = ptabs[rp.b] + (long)(do_r2c ? tidx_next : 0) * 2 * NN;
-  T pxa[8], pya[8], pxb[8], pyb[8], ya[8], yb[8], aa[8], ab[8];
-#pragma unroll
-  for (int m = 0; m < 8; ++m) {
-    const int j = t + m * G;
-    const long ja = (long)rp.ia * N + j, jb = (long)rp.ib * N + j;
-    pxa[m] = do_r2c ? pt[ja] : T(0);
-    pya[m] = do_r2c ? pt[NN + ja] : T(0);
-    pxb[m] = do_r2c ? pt[jb] : T(0);
-    pyb[m] = do_r2c ? pt[NN + jb] : T(0);
-    ya[m] = (stage == 4 && !init) ? T(0) : y[rp.ra * N + j];
-    yb[m] = (stage == 4 && !init) ? T(0) : y[rp.rb * N + j];
-    aa[m] = (stage == 1 || init) ? T(0) : acc[rp.ra * N + j];
-    ab[m] = (stage == 1 || init) ? T(0) : acc[rp.rb * N + j];
-  }
-  T za[8], zb[8];
+  cplx<T> v[1][8];
   __syncthreads();
-  if (init) {
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      za[m] = ya[m];
-      zb[m] = yb[m];
-    }
-  } else {
-    cplx<T> v[1][8];
+  if (!init) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int k = t + q * G;
@@ -651,6 +650,35 @@ __global__ __launch_bounds__(256) void k_row_lf_adj(cplx<T>* __restrict__ S0,
     for (int q = 0; q < 8; ++q) v[0][q] = bufa[pad_idx(t + q * G)];
     cplx<T>* const bufs[1] = {bufa};
     fft_lines<N, true, 1>(v, bufs, tw, t);
+  }
+  const T* pt = ptabs[rp.b] + (long)(do_r2c ? tidx_next : 0) * 2 * NN;
+  T pxa[8], pya[8], pxb[8], pyb[8], ya[8], yb[8], aa[8], ab[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int j = t + m * G;
+    const long ja = (long)rp.ia * N + j, jb = (long)rp.ib * N + j;
+    pxa[m] = do_r2c ? pt[ja] : T(0);
+    pya[m] = do_r2c ? pt[NN + ja] : T(0);
+    if constexpr (C == 2) {  // Q and U rows share the pixel row
+      pxb[m] = pxa[m];
+      pyb[m] = pya[m];
+    } else {
+      pxb[m] = do_r2c ? pt[jb] : T(0);
+      pyb[m] = do_r2c ? pt[NN + jb] : T(0);
+    }
+    ya[m] = (stage == 4 && !init) ? T(0) : y[rp.ra * N + j];
+    yb[m] = (stage == 4 && !init) ? T(0) : y[rp.rb * N + j];
+    aa[m] = (stage == 1 || init) ? T(0) : acc[rp.ra * N + j];
+    ab[m] = (stage == 1 || init) ? T(0) : acc[rp.rb * N + j];
+  }
+  T za[8], zb[8];
+  if (init) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      za[m] = ya[m];
+      zb[m] = yb[m];
+    }
+  } else {
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
       const int j = t + m * G;
@@ -805,16 +833,28 @@ hipError_t launch_lf_row(int N, bool adjoint, cplx<T>* S0, cplx<T>* S1, T* y, T*
     const long npairs = nmaps * NN_ / 2;
     const unsigned nb = blocks_for(npairs, 256 / (NN_ / 8));
     const T cy = dk / T(NN_);
+    const GradArgs<T> g = ga ? *ga : GradArgs<T>{};
     if (adjoint) {
-      k_row_lf_adj<T, NN_><<<nb, 256, 0, s>>>(S0, S1, y, acc, ptabs, tw, C, npairs, stage, h, tidx,
-                                              do_r2c, init, cy, zstore);
+      if (C == 2)
+        k_row_lf_adj<T, NN_, 2><<<nb, 256, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage, h,
+                                                   tidx, do_r2c, init, cy, zstore);
+      else
+        k_row_lf_adj<T, NN_, 1><<<nb, 256, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage, h,
+                                                   tidx, do_r2c, init, cy, zstore);
     } else if (ga) {
-      k_row_lf_std<T, NN_, true><<<nb, 256, 0, s>>>(S0, S1, y, acc, ptabs, tw, C, npairs, stage, h,
-                                                    tidx, do_r2c, cy, *ga);
+      if (C == 2)
+        k_row_lf_std<T, NN_, 2, true><<<nb, 256, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage,
+                                                         h, tidx, do_r2c, cy, g);
+      else
+        k_row_lf_std<T, NN_, 1, true><<<nb, 256, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage,
+                                                         h, tidx, do_r2c, cy, g);
     } else {
-      GradArgs<T> none{};
-      k_row_lf_std<T, NN_, false><<<nb, 256, 0, s>>>(S0, S1, y, acc, ptabs, tw, C, npairs, stage,
-                                                     h, tidx, do_r2c, cy, none);
+      if (C == 2)
+        k_row_lf_std<T, NN_, 2, false><<<nb, 256, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage,
+                                                          h, tidx, do_r2c, cy, g);
+      else
+        k_row_lf_std<T, NN_, 1, false><<<nb, 256, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage,
+                                                          h, tidx, do_r2c, cy, g);
     }
   });
   return hipGetLastError();
