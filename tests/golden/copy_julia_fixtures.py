@@ -23,6 +23,10 @@ NAMES = [
     # test_wiener_filter.py)
     "data_t_field_I", "cn_I", "cf_I", "m_I", "b_I", "cphi_I", "gradf_I", "grad_phi_I",
     "f_wf_ffpp_I", "f_wf_f0pp_I", "f_wf_f0p0_I",
+    # gradients / logpdf / Wiener filter, polarisation (EB operators)
+    "data_e_field_P", "data_b_field_P", "cn_ee_P", "cn_bb_P", "cf_ee_P", "cf_bb_P",
+    "m_ee_P", "m_bb_P", "b_ee_P", "b_bb_P", "cphi_P", "gradf_e_P", "gradf_b_P",
+    "grad_phi_t_P", "f_wf_ffpp_e_P", "f_wf_ffpp_b_P",
 ]
 TEXT = ["logpdf_I.txt", "logpdf_P.txt", "n_side.txt", "theta_pix.txt"]
 

@@ -108,6 +108,26 @@ def test_grad_phi_julia_I(pkg, dev):
     assert rel(g, L("grad_phi_I")) < 2e-3
 
 
+def test_logpdf_and_grad_phi_julia_P(pkg, dev):
+    """logpdf and its phi gradient on CMBLensing.jl's polarisation problem
+    (EB operators, FOURIER phi): logpdf to 1e-12 relative, the gradient
+    against the FP64 oracle to 1e-9 and the Julia fixture to 2e-4."""
+    from cmblensing_amd import statistics
+    from test_gpu_wiener import _julia_P
+    _case("logpdf / grad_phi julia P fp64")
+    o, a, L = _julia_P(pkg, dev)
+    lp = statistics.logpdf(o["f"], o["phi"], o["d"], o["Cn"], o["Cphi"], o["Cf"], o["M"], o["B"],
+                           n=7)
+    ref = float((GOLDEN / "julia" / "logpdf_P.txt").read_text())
+    assert abs(float(lp) - ref) < 1e-12 * abs(ref)
+    g = statistics.grad_phi_logpdf(o["f"], o["phi"], o["d"], o["Cn"], o["Cphi"], o["Cf"], o["M"],
+                                   o["B"], n=7).scalar_matrix.cpu().numpy()
+    gref = owf.grad_phi_logpdf(a["f"], a["phi"], a["d"], a["Cn"], a["Cphi"], a["Cf"], a["M"],
+                               a["B"], 2.5, n=7)
+    assert rel(g, gref) < 1e-9
+    assert rel(g, L("grad_phi_t_P")) < 2e-4
+
+
 def test_differentiation_workload_batch16(pkg, dev, sims):
     """QU N=256, 2 arcmin, float32 gradients of logpdf w.r.t. f and phi for 16
     chains (square pixel mask, 1 arcmin beam, 5 uK-arcmin noise), checked

@@ -97,6 +97,55 @@ def test_gradf_julia_I_fp64(pkg, dev):
     assert rel(g[:, 1:-1], gj[:, 1:-1]) < 1e-9
 
 
+def _julia_P(pkg, dev, cdt=torch.complex128):
+    from cmblensing_amd import matrix_operators as mo
+    J = GOLDEN / "julia"
+    L = lambda n: np.load(J / f"{n}.npy")
+    T = lambda a, dt=cdt: torch.as_tensor(a, device=dev).to(dt)
+    F = lambda e, b: pkg.FlatS2(polar_matrix_1=T(L(e)), polar_matrix_2=T(L(b)), theta_pix=2.5)
+    D = lambda e, b: mo.DiagonalEB(matrix_EE=T(L(e), torch.float64),
+                                   matrix_BB=T(L(b), torch.float64), theta_pix=2.5)
+    objs = dict(f=F("e_field_P", "b_field_P"), d=F("data_e_field_P", "data_b_field_P"),
+                phi=pkg.FlatS0(scalar_matrix=T(L("phi_P"), torch.complex128), theta_pix=2.5),
+                Cf=D("cf_ee_P", "cf_bb_P"), Cn=D("cn_ee_P", "cn_bb_P"), M=D("m_ee_P", "m_bb_P"),
+                B=D("b_ee_P", "b_bb_P"),
+                Cphi=mo.DiagonalScalar(scalar_matrix=T(L("cphi_P"), torch.float64), theta_pix=2.5))
+    st = lambda a, b: np.stack([L(a), L(b)])
+    arrs = dict(f=st("e_field_P", "b_field_P"), d=st("data_e_field_P", "data_b_field_P"),
+                Cf=st("cf_ee_P", "cf_bb_P"), Cn=st("cn_ee_P", "cn_bb_P"), M=st("m_ee_P", "m_bb_P"),
+                B=st("b_ee_P", "b_bb_P"), phi=L("phi_P"), Cphi=L("cphi_P"))
+    return objs, arrs, L
+
+
+@pytest.mark.parametrize("cdt,tol", [(torch.complex128, 1e-10), (torch.complex64, 2e-5)])
+def test_gradf_julia_P(pkg, dev, cdt, tol):
+    """EB gradient on CMBLensing.jl's polarisation problem: against the FP64
+    oracle at the dtype's bar, and (FP64) against the Julia fixture on the
+    interior modes."""
+    from cmblensing_amd import gradients
+    _case(f"gradf julia P N=256 {cdt}")
+    o, a, L = _julia_P(pkg, dev, cdt)
+    g = stack(gradients.gradf_logpdf(o["f"], o["phi"], o["d"], o["Cf"], o["Cn"], o["M"], o["B"],
+                                     n=7))
+    ref = owf.gradf_logpdf(a["f"], a["phi"], a["d"], a["Cf"], a["Cn"], a["M"], a["B"], 2.5, n=7)
+    assert rel(g, ref) < tol
+    if cdt == torch.complex128:
+        gj = np.stack([L("gradf_e_P"), L("gradf_b_P")])
+        assert rel(g[:, :, 1:-1], gj[:, :, 1:-1]) < 1e-10
+
+
+def test_wiener_filter_julia_P(pkg, dev):
+    """The reference's stop rule (tol 0.1, maxiter 500) from the truth start on
+    the polarisation problem agrees with CMBLensing.jl's f_wf_ffpp_{e,b}_P."""
+    from cmblensing_amd import wiener_filter as wfm
+    _case("wf julia P N=256 fp64 tol=0.1")
+    o, a, L = _julia_P(pkg, dev)
+    x = stack(wfm.wiener_filter(o["f"], o["phi"], o["d"], o["Cf"], o["Cn"], o["M"], o["B"], n=7))
+    wj = np.stack([L("f_wf_ffpp_e_P"), L("f_wf_ffpp_b_P")])
+    assert rel(x, wj) < 2e-5
+    assert rel(x[:, :, 1:-1], wj[:, :, 1:-1]) < 1e-8
+
+
 @pytest.mark.parametrize("mask", ["fourier", "square"])
 def test_wiener_filter_matches_oracle_fp64(pkg, dev, sims, mask):
     from cmblensing_amd import wiener_filter as wfm

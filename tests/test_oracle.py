@@ -220,6 +220,38 @@ def test_julia_grad_phi_I():
     assert rel(g, jl("grad_phi_I")) < 2e-3
 
 
+def _ops_P():
+    st = lambda a, b: np.stack([jl(a), jl(b)])
+    return dict(f=st("e_field_P", "b_field_P"), d=st("data_e_field_P", "data_b_field_P"),
+                Cn=st("cn_ee_P", "cn_bb_P"), Cf=st("cf_ee_P", "cf_bb_P"),
+                M=st("m_ee_P", "m_bb_P"), B=st("b_ee_P", "b_bb_P"))
+
+
+def test_julia_logpdf_P():
+    o = _ops_P()
+    lp = owf.logpdf(o["f"], jl("phi_P"), o["d"], o["Cn"], jl("cphi_P"), o["Cf"], o["M"], o["B"],
+                    2.5, n=7)
+    ref = float((J / "logpdf_P.txt").read_text())
+    assert abs(lp - ref) < 1e-12 * abs(ref)
+
+
+def test_julia_gradf_P_interior_modes():
+    """EB gradient (QU rotation, LenseFlow of Q and U, adjoint) against
+    CMBLensing.jl on the interior modes; the E modes match to ~1e-13."""
+    o = _ops_P()
+    g = owf.gradf_logpdf(o["f"], jl("phi_P"), o["d"], o["Cf"], o["Cn"], o["M"], o["B"], 2.5, n=7)
+    gj = np.stack([jl("gradf_e_P"), jl("gradf_b_P")])
+    assert rel(g[:, :, 1:-1], gj[:, :, 1:-1]) < 1e-10
+    assert rel(g, gj) < 2e-5
+
+
+def test_julia_grad_phi_P():
+    o = _ops_P()
+    g = owf.grad_phi_logpdf(o["f"], jl("phi_P"), o["d"], o["Cn"], jl("cphi_P"), o["Cf"], o["M"],
+                            o["B"], 2.5, n=7)
+    assert rel(g, jl("grad_phi_t_P")) < 2e-4
+
+
 # ------------------------------------------------------- Wiener-filter pins
 def _sims():
     import importlib.util
@@ -273,6 +305,16 @@ def test_wiener_filter_julia_I():
     wj = jl("f_wf_ffpp_I")
     assert rel(x[0], wj) < 1e-5
     assert rel(x[0][:, 1:-1], wj[:, 1:-1]) < 1e-6
+
+
+def test_wiener_filter_julia_P():
+    """The same stop rule on the polarisation problem (EB operators): the
+    interior modes agree with CMBLensing.jl's f_wf_ffpp_{e,b}_P to ~1e-10."""
+    o = _ops_P()
+    x = owf.wiener_filter(o["f"], jl("phi_P"), o["d"], o["Cf"], o["Cn"], o["M"], o["B"], 2.5, n=7)
+    wj = np.stack([jl("f_wf_ffpp_e_P"), jl("f_wf_ffpp_b_P")])
+    assert rel(x, wj) < 2e-5
+    assert rel(x[:, :, 1:-1], wj[:, :, 1:-1]) < 1e-8
 
 
 def test_prior_round_trip_is_hermitian_projection():
