@@ -8,13 +8,14 @@ from .constants import (ARCMIN_PER_DEGREE, Basis, DEFAULT_NSTEPS, FORWARD_LENSE,
                         NSIDE_DEFAULT, PIX_WIDTH_DEFAULT, Parametrization, THETA_PIX_DEFAULT)
 from .fields import (Field, FlatS0, FlatS02, FlatS2, dot, eb2qu, fourier, fourier_weights_for,
                      map, qu2eb, undo_inner_product)
-from .lense_flow import lense_flow, lense_flow_matrices
+from .lense_flow import lense_flow, lense_flow_matrices, lense_flow_vjp, lense_flow_wrapper
 from ._lib import CmblError, clear_caches
 
 __all__ = [
     "ARCMIN_PER_DEGREE", "Basis", "DEFAULT_NSTEPS", "FORWARD_LENSE", "INVERSE_LENSE",
     "NSIDE_DEFAULT", "PIX_WIDTH_DEFAULT", "Parametrization", "THETA_PIX_DEFAULT",
     "Field", "FlatS0", "FlatS02", "FlatS2", "dot", "eb2qu", "fourier", "fourier_weights_for",
-    "map", "qu2eb", "undo_inner_product", "lense_flow", "lense_flow_matrices", "CmblError",
+    "map", "qu2eb", "undo_inner_product", "lense_flow", "lense_flow_matrices", "lense_flow_vjp",
+    "lense_flow_wrapper", "CmblError",
     "clear_caches",
 ]
