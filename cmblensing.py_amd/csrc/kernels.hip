@@ -458,15 +458,16 @@ __device__ __forceinline__ void add_sources(T* S, long NN, T cs, T tval, T fx, T
 // S0 holds the row spectra S of the stage input, S1 their x-derivative DX;
 // the line is Z = DX + i DY with DY = i ky S / N (DY's packed slot is zero:
 // ky = 0 at DC, and the real Nyquist value times i drops out of the C2R).
-template <typename T, int N, int C, bool GRAD>
-__global__ __launch_bounds__(256) void k_row_lf_std(cplx<T>* __restrict__ S0,
+template <typename T, int N, int C, bool GRAD, int BS>
+__global__ __launch_bounds__(BS) void k_row_lf_std(cplx<T>* __restrict__ S0,
                                                     const cplx<T>* __restrict__ S1,
                                                     T* __restrict__ y, T* __restrict__ acc,
                                                     const T* const* __restrict__ ptabs,
                                                     const cplx<T>* __restrict__ twg, long npairs,
                                                     int stage, T h, int tidx, int do_r2c, T cy,
                                                     GradArgs<T> ga) {
-  constexpr int G = N / 8, PAIRS = 256 / G, H = N / 2;
+  constexpr int G = N / 8, PAIRS = BS / G, H = N / 2;
+  static_assert(BS % G == 0 && BS >= G, "a block holds whole lines");
   constexpr long NN = (long)N * N;
   __shared__ cplx<T> lbuf[PAIRS][2][lds_len<N>()];
   const int grp = threadIdx.x / G, t = threadIdx.x % G;
@@ -489,7 +490,6 @@ __global__ __launch_bounds__(256) void k_row_lf_std(cplx<T>* __restrict__ S0,
     dxb[q] = S1[rp.rb * H + k];
     sb[q] = S0[rp.rb * H + k];
   }
-  cplx<T> v[2][8];
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -499,13 +499,6 @@ __global__ __launch_bounds__(256) void k_row_lf_std(cplx<T>* __restrict__ S0,
     put_herm<N>(bufb, k, dxb[q], k == 0 ? mk(T(0), T(0)) : imul(sb[q], c));
   }
   __syncthreads();
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    v[0][q] = bufa[pad_idx(t + q * G)];
-    v[1][q] = bufb[pad_idx(t + q * G)];
-  }
-  cplx<T>* const bufs[2] = {bufa, bufb};
-  fft_lines<N, true, 2>(v, bufs, tw, t);
   // per-pixel work, one row at a time: the operands of row b are loaded after
   // row a is done, so at most one row's operands are live
   const T* pt = ptabs[rp.b] + (long)tidx * 2 * NN;
@@ -518,7 +511,57 @@ __global__ __launch_bounds__(256) void k_row_lf_std(cplx<T>* __restrict__ S0,
       py[m] = pt[NN + jj];
     }
   };
-  if constexpr (GRAD) {
+  cplx<T> z[1][8];
+  // rate, RK4 update and next stage input of row r from its gradient g
+  auto row_update = [&](int r, const cplx<T>(&g)[8]) {
+    const long rr = r ? rp.rb : rp.ra;
+    if (r == 0 || C == 1) load_p(r ? rp.ib : rp.ia);
+    T yv[8], av[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int j = t + m * G;
+      yv[m] = stage == 4 ? T(0) : y[rr * N + j];
+      av[m] = stage == 1 ? T(0) : acc[rr * N + j];
+    }
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int j = t + m * G;
+      const T k = px[m] * g[m].x + py[m] * g[m].y;
+      T zz;
+      rk4_stage(stage, h, k, yv[m], av[m], zz);
+      if (active) {
+        if (stage == 4)
+          y[rr * N + j] = zz;
+        else
+          acc[rr * N + j] = av[m];
+      }
+      if (r == 0)
+        z[0][m].x = zz;
+      else
+        z[0][m].y = zz;
+    }
+  };
+  if constexpr (!GRAD) {
+    // one line at a time: row a's inverse transform and update, then row b's
+    // (only one line's gradient is live at a time)
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      cplx<T>* const buf1[1] = {r ? bufb : bufa};
+      cplx<T> w[1][8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) w[0][q] = buf1[0][pad_idx(t + q * G)];
+      fft_lines<N, true, 1>(w, buf1, tw, t);
+      row_update(r, w[0]);
+    }
+  } else {
+    cplx<T> v[2][8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      v[0][q] = bufa[pad_idx(t + q * G)];
+      v[1][q] = bufb[pad_idx(t + q * G)];
+    }
+    cplx<T>* const bufs[2] = {bufa, bufb};
+    fft_lines<N, true, 2>(v, bufs, tw, t);
     // delta-phi sources at this stage: sum_c df_c grad f_c (lense_flow.py:201-213)
     const T* gt = ga.gtabs[rp.b] + (long)tidx * 3 * NN;
     T* Sb = ga.sacc + (long)rp.b * 5 * NN;
@@ -554,37 +597,8 @@ __global__ __launch_bounds__(256) void k_row_lf_std(cplx<T>* __restrict__ S0,
         }
       }
     }
-  }
-
-  cplx<T> z[1][8];
-#pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    const long rr = r ? rp.rb : rp.ra;
-    if (r == 0 || C == 1) load_p(r ? rp.ib : rp.ia);
-    T yv[8], av[8];
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const int j = t + m * G;
-      yv[m] = stage == 4 ? T(0) : y[rr * N + j];
-      av[m] = stage == 1 ? T(0) : acc[rr * N + j];
-    }
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const int j = t + m * G;
-      const T k = px[m] * v[r][m].x + py[m] * v[r][m].y;
-      T zz;
-      rk4_stage(stage, h, k, yv[m], av[m], zz);
-      if (active) {
-        if (stage == 4)
-          y[rr * N + j] = zz;
-        else
-          acc[rr * N + j] = av[m];
-      }
-      if (r == 0)
-        z[0][m].x = zz;
-      else
-        z[0][m].y = zz;
-    }
+    row_update(0, v[0]);
+    row_update(1, v[1]);
   }
   if (!do_r2c) return;  // uniform over the grid
   cplx<T>* const bufz[1] = {bufa};
@@ -598,8 +612,8 @@ __global__ __launch_bounds__(256) void k_row_lf_std(cplx<T>* __restrict__ S0,
 // grad_line); C2R of (W_a + i W_b) gives the two rates, then RK4 update.  Then,
 // unless this is the very last stage, u = p_x(t_next) z and v = p_y(t_next) z
 // are row-R2C'd per row (as u + i v) into S0 (U) and S1 (V).
-template <typename T, int N, int C>
-__global__ __launch_bounds__(256) void k_row_lf_adj(cplx<T>* __restrict__ S0,
+template <typename T, int N, int C, int BS>
+__global__ __launch_bounds__(BS) void k_row_lf_adj(cplx<T>* __restrict__ S0,
                                                     cplx<T>* __restrict__ S1,
                                                     T* __restrict__ y, T* __restrict__ acc,
                                                     const T* const* __restrict__ ptabs,
@@ -607,7 +621,8 @@ __global__ __launch_bounds__(256) void k_row_lf_adj(cplx<T>* __restrict__ S0,
                                                     long npairs, int stage, T h, int tidx_next,
                                                     int do_r2c, int init, T cy,
                                                     T* __restrict__ zstore) {
-  constexpr int G = N / 8, PAIRS = 256 / G, H = N / 2;
+  constexpr int G = N / 8, PAIRS = BS / G, H = N / 2;
+  static_assert(BS % G == 0 && BS >= G, "a block holds whole lines");
   constexpr long NN = (long)N * N;
   __shared__ cplx<T> lbuf[PAIRS][2][lds_len<N>()];
   const int grp = threadIdx.x / G, t = threadIdx.x % G;
@@ -762,6 +777,15 @@ constexpr int col_kc(int N) {
 
 static inline unsigned blocks_for(long n, long per) { return (unsigned)((n + per - 1) / per); }
 
+// Launch shapes.  A grid of fewer than CMBL_SMALL_GRID default-shape blocks
+// (a few requests, chains or right-hand sides) uses 4x smaller blocks, so the
+// launch spreads over more CUs: 64-thread row blocks and a quarter of the
+// columns per column block.
+#ifndef CMBL_SMALL_GRID
+#define CMBL_SMALL_GRID 256
+#endif
+
+
 template <typename T>
 hipError_t launch_rfft2(int N, const T* x, cplx<T>* P, cplx<T>* X, long nmaps,
                         const cplx<T>* tw, hipStream_t s) {
@@ -818,10 +842,45 @@ hipError_t launch_lf_col(int N, const cplx<T>* src, cplx<T>* dst, long nmaps, co
                          T dk, hipStream_t s) {
   CMBL_DISPATCH_N(N, {
     constexpr int KC = col_kc<T>(NN_);
+    constexpr int KCS = KC / 4 > 2 ? KC / 4 : 2;
     const unsigned nb = (unsigned)(nmaps * (NN_ / 2 / KC));
-    k_col_lf_dx<T, NN_, KC><<<nb, KC * NN_ / 8, 0, s>>>(src, dst, tw, dk);
+    if (nb < CMBL_SMALL_GRID)
+      k_col_lf_dx<T, NN_, KCS><<<(unsigned)(nmaps * (NN_ / 2 / KCS)), KCS * NN_ / 8, 0, s>>>(
+          src, dst, tw, dk);
+    else
+      k_col_lf_dx<T, NN_, KC><<<nb, KC * NN_ / 8, 0, s>>>(src, dst, tw, dk);
   });
   return hipGetLastError();
+}
+
+template <typename T, int N, int BS>
+void row_lf_launch(bool adjoint, int C, const GradArgs<T>* ga, cplx<T>* S0, cplx<T>* S1, T* y,
+                   T* acc, const T* const* ptabs, const cplx<T>* tw, long npairs, int stage, T h,
+                   int tidx, int do_r2c, int init, T cy, T* zstore, hipStream_t s) {
+  const unsigned nb = blocks_for(npairs, BS / (N / 8));
+  const GradArgs<T> g = ga ? *ga : GradArgs<T>{};
+  if (adjoint) {
+    if (C == 2)
+      k_row_lf_adj<T, N, 2, BS><<<nb, BS, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage, h,
+                                                  tidx, do_r2c, init, cy, zstore);
+    else
+      k_row_lf_adj<T, N, 1, BS><<<nb, BS, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage, h,
+                                                  tidx, do_r2c, init, cy, zstore);
+  } else if (ga) {
+    if (C == 2)
+      k_row_lf_std<T, N, 2, true, BS><<<nb, BS, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage,
+                                                        h, tidx, do_r2c, cy, g);
+    else
+      k_row_lf_std<T, N, 1, true, BS><<<nb, BS, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage,
+                                                        h, tidx, do_r2c, cy, g);
+  } else {
+    if (C == 2)
+      k_row_lf_std<T, N, 2, false, BS><<<nb, BS, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage,
+                                                         h, tidx, do_r2c, cy, g);
+    else
+      k_row_lf_std<T, N, 1, false, BS><<<nb, BS, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage,
+                                                         h, tidx, do_r2c, cy, g);
+  }
 }
 
 template <typename T>
@@ -831,31 +890,14 @@ hipError_t launch_lf_row(int N, bool adjoint, cplx<T>* S0, cplx<T>* S1, T* y, T*
                          const GradArgs<T>* ga) {
   CMBL_DISPATCH_N(N, {
     const long npairs = nmaps * NN_ / 2;
-    const unsigned nb = blocks_for(npairs, 256 / (NN_ / 8));
     const T cy = dk / T(NN_);
-    const GradArgs<T> g = ga ? *ga : GradArgs<T>{};
-    if (adjoint) {
-      if (C == 2)
-        k_row_lf_adj<T, NN_, 2><<<nb, 256, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage, h,
-                                                   tidx, do_r2c, init, cy, zstore);
-      else
-        k_row_lf_adj<T, NN_, 1><<<nb, 256, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage, h,
-                                                   tidx, do_r2c, init, cy, zstore);
-    } else if (ga) {
-      if (C == 2)
-        k_row_lf_std<T, NN_, 2, true><<<nb, 256, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage,
-                                                         h, tidx, do_r2c, cy, g);
-      else
-        k_row_lf_std<T, NN_, 1, true><<<nb, 256, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage,
-                                                         h, tidx, do_r2c, cy, g);
-    } else {
-      if (C == 2)
-        k_row_lf_std<T, NN_, 2, false><<<nb, 256, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage,
-                                                          h, tidx, do_r2c, cy, g);
-      else
-        k_row_lf_std<T, NN_, 1, false><<<nb, 256, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage,
-                                                          h, tidx, do_r2c, cy, g);
-    }
+    constexpr int BS_SMALL = NN_ / 8 > 64 ? NN_ / 8 : 64;
+    if (blocks_for(npairs, 256 / (NN_ / 8)) < CMBL_SMALL_GRID)
+      row_lf_launch<T, NN_, BS_SMALL>(adjoint, C, ga, S0, S1, y, acc, ptabs, tw, npairs, stage, h,
+                                      tidx, do_r2c, init, cy, zstore, s);
+    else
+      row_lf_launch<T, NN_, 256>(adjoint, C, ga, S0, S1, y, acc, ptabs, tw, npairs, stage, h,
+                                 tidx, do_r2c, init, cy, zstore, s);
   });
   return hipGetLastError();
 }
