@@ -7,9 +7,10 @@ potentials whose operators are prepared once and stay resident in HBM (the
 serving form).  The plain path is one request per call with its phi prepared
 in the same call, as the reference's lense_flow does.
 
-The timed step replays the captured batched graph of the serving executor
+The timed step is the served form: the batch split over the serving
+backend's two streams, each half one replay of a captured batched graph
 (gather, 28 RK4 stages, scatter); the same batch through the direct batched
-entry point is timed beside it.
+entry point on one stream is timed beside it.
 
 Prints one JSON line (rank 0).  Extra keys: plain maps/s and the batched /
 plain ratio, p50/p99 latency of plain single requests, relative L2 error of a
@@ -286,6 +287,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=128)
     ap.add_argument("--plain-requests", type=int, default=40)
+    ap.add_argument("--streams", type=int, default=2,
+                    help="streams the served batch is split over (the serving backend's count)")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=10000)
@@ -328,15 +331,25 @@ def main():
     req_phi = [handles[r % NPHI] for r in range(B)]
     out = torch.empty_like(x)
 
-    # the served form: one replay of the captured batched graph per step
-    # (gather of the B request maps, the 28 RK4 stages, scatter of results)
-    ex = _lib.Executor(plan, B, C, n, 1, False)
-    in_ptrs = [x[b].data_ptr() for b in range(B)]
-    out_ptrs = [out[b].data_ptr() for b in range(B)]
-    stream = torch.cuda.current_stream(dev).cuda_stream
+    # the served form, as the serving backend runs it: the B requests of a
+    # step split over its streams (default 2, default priority), each half one
+    # replay of a captured batched graph (gather, 57 stage kernels, scatter)
+    # on a plan of its own, so the two batches overlap on the device
+    NS = args.streams
+    if B % NS:
+        sys.exit(f"--batch {B} must be a multiple of --streams {NS}")
+    Bs = B // NS
+    lanes = []
+    for k in range(NS):
+        pk = _lib.Plan(N, th, dtype, Bs * C, dev)
+        lanes.append((pk, _lib.Executor(pk, Bs, C, n, 1, False), torch.cuda.Stream(dev),
+                      req_phi[k * Bs:(k + 1) * Bs],
+                      [x[b].data_ptr() for b in range(k * Bs, (k + 1) * Bs)],
+                      [out[b].data_ptr() for b in range(k * Bs, (k + 1) * Bs)]))
 
     def step():
-        ex.run(req_phi, in_ptrs, out_ptrs, stream)
+        for _, ex, st, hs, ins, outs in lanes:
+            ex.run(hs, ins, outs, st.cuda_stream)
 
     def direct_step():
         _lib.lenseflow(plan, req_phi, x, out, B, C, n, 1, False)
@@ -425,7 +438,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded Gaussian random fields, 32 resident phi)",
         "config": {"workload": "lenseflow_fwd_QU_256_2arcmin_n7_fp32", "N": N, "theta_pix": th,
-                   "nsteps": n, "components": "QU", "batch": B, "phi_resident": NPHI},
+                   "nsteps": n, "components": "QU", "batch": B, "phi_resident": NPHI,
+                   "streams": NS},
         "direct_batched_maps_s": round(direct_maps_s, 2),
         "plain_maps_s": round(plain_maps_s, 2),
         "batched_over_plain": round(maps_s / plain_maps_s, 2),
@@ -446,7 +460,9 @@ def main():
                         "avg_launch_us": round(col_avg_ms * 1e3, 3),
                         "algorithmic_bytes_per_launch": col_bytes},
     }
-    ex.close()
+    for pk, ex, *_ in lanes:
+        ex.close()
+        pk.close()
     if not args.no_serve:
         result["serving"] = serving_run(torch, sims, dev, args.serve_requests, args.serve_rate)
         result["latency_ms"] = {"p50": result["serving"]["p50_ms"],

@@ -43,7 +43,7 @@ import torch
 from .constants import FORWARD_LENSE, INVERSE_LENSE
 
 GiB = 1 << 30
-DEFAULT_MAX_BATCH = {16: 256, 32: 256, 64: 256, 128: 128, 256: 128, 512: 32}
+DEFAULT_MAX_BATCH = {16: 256, 32: 256, 64: 256, 128: 128, 256: 64, 512: 32}
 
 
 @dataclass(eq=False)
