@@ -3,10 +3,12 @@ preconditioned-CG Wiener filter (oracle).
 
 TEST INFRASTRUCTURE ONLY -- imported by tests/ and bench.py's cpu_baseline leg.
 
-Fields are FOURIER arrays [C, N, N//2+1] complex128 with C=1 (T, FlatS0) or
-C=2 (E, B of a FlatS2 in the EB parametrisation).  Diagonal operators
-(DiagonalScalar / DiagonalEB, matrix_operators.py:45-90) are real arrays of
-the same shape, applied per mode.  ``theta_pix`` is in arcmin.
+Fields are FOURIER arrays [C, N, N//2+1] complex128 with C=1 (T, FlatS0),
+C=2 (E, B of a FlatS2 in the EB parametrisation) or C=3 (T, E, B of a
+FlatS02).  Diagonal operators (DiagonalScalar / DiagonalEB,
+matrix_operators.py:45-90) are real arrays of the same shape, applied per
+mode; BlockTEB operators (matrix_operators.py:92-148) are ``TEB`` objects.
+``theta_pix`` is in arcmin.
 """
 from __future__ import annotations
 
@@ -15,24 +17,80 @@ import numpy as np
 from . import lenseflow as lf
 
 
-def pinv(op):
-    """Nonzero-safe reciprocal (util.py:45-47; matrix_operators.py:208-219)."""
-    op = np.asarray(op)
-    out = np.zeros_like(op)
-    nz = op != 0
-    out[nz] = 1.0 / op[nz]
+def _recip(m):
+    """util.py:45-47: 1/m where m != 0, else 0."""
+    m = np.asarray(m)
+    out = np.zeros_like(m)
+    nz = m != 0
+    out[nz] = 1.0 / m[nz]
     return out
+
+
+def _log_det1(m, w):
+    """util.py:39-42: sign(prod) * sum log|m| w over the nonzero modes."""
+    val = np.sum(np.where(m != 0, np.log(np.abs(np.where(m != 0, m, 1))), 0) * w)
+    sign = np.prod(np.sign(np.where(m != 0, m, 1)))
+    return sign * val
+
+
+class TEB:
+    """BlockTEB operator (matrix_operators.py:92-148): per mode
+    [[TT, TE, 0], [ET, EE, 0], [0, 0, BB]] acting on (T, E, B)."""
+    __array_ufunc__ = None          # numpy defers array * TEB to __rmul__
+
+    def __init__(self, TT, TE, ET, EE, BB):
+        self.TT, self.TE, self.ET, self.EE, self.BB = (np.asarray(a) for a in (TT, TE, ET, EE, BB))
+
+    def __mul__(self, o):
+        if isinstance(o, TEB):                                      # :115-121
+            return TEB(self.TT * o.TT + self.TE * o.ET, self.TT * o.TE + self.TE * o.EE,
+                       self.ET * o.TT + self.EE * o.ET, self.ET * o.TE + self.EE * o.EE,
+                       self.BB * o.BB)
+        if isinstance(o, np.ndarray) and o.ndim >= 3 and o.shape[-3] == 3:   # :123-127
+            t, e, b = o[..., 0, :, :], o[..., 1, :, :], o[..., 2, :, :]
+            return np.stack([self.TT * t + self.TE * e, self.ET * t + self.EE * e, self.BB * b],
+                            axis=-3)
+        return TEB(*(m * o for m in (self.TT, self.TE, self.ET, self.EE, self.BB)))  # :129-133
+
+    def __rmul__(self, o):
+        return TEB(*(o * m for m in (self.TT, self.TE, self.ET, self.EE, self.BB)))
+
+    def __add__(self, o):
+        return TEB(self.TT + o.TT, self.TE + o.TE, self.ET + o.ET, self.EE + o.EE,
+                   self.BB + o.BB)
+
+    def pinv(self):
+        """Schur-complement inverse of the TE block (util.py:59-69), reciprocal BB."""
+        a, b, c, d = self.TT, self.TE, self.ET, self.EE
+        rd = _recip(d)
+        s = _recip(a - b * rd * c)
+        return TEB(s, -s * b * rd, -rd * c * s, rd + rd * c * s * b * rd, _recip(self.BB))
+
+    def log_det(self, N):
+        """util.py:51-53 for the TE block plus the BB block (matrix_operators.py:193-200)."""
+        w = lf.fourier_weights(N)
+        return (_log_det1(self.TT, w) + _log_det1(self.EE - self.ET * _recip(self.TT) * self.TE, w)
+                + _log_det1(self.BB, w))
+
+
+def pinv(op):
+    """Nonzero-safe reciprocal (util.py:45-47; matrix_operators.py:208-219), or
+    the block inverse of a TEB operator (matrix_operators.py:222-234)."""
+    if isinstance(op, TEB):
+        return op.pinv()
+    return _recip(op)
 
 
 def log_det(op, N):
     """Weighted log-determinant of a diagonal operator (util.py:39-42;
-    matrix_operators.py:179-190: DiagonalEB sums both blocks)."""
+    matrix_operators.py:179-190: DiagonalEB sums both blocks) or a TEB block
+    operator."""
+    if isinstance(op, TEB):
+        return op.log_det(N)
     w = lf.fourier_weights(N)
     total = 0.0
     for m in np.asarray(op).reshape(-1, *np.shape(op)[-2:]):
-        val = np.sum(np.where(m != 0, np.log(np.abs(np.where(m != 0, m, 1))), 0) * w)
-        sign = np.prod(np.sign(np.where(m != 0, m, 1)))
-        total = total + sign * val
+        total = total + _log_det1(m, w)
     return total
 
 
@@ -40,16 +98,16 @@ def eb2qu_field(f, N, theta_pix):
     """fields.py:182-199: T passes through, (E,B) -> (Q,U)."""
     if f.shape[0] == 1:
         return f
-    q, u = lf.eb2qu(f[0], f[1], N, theta_pix)
-    return np.stack([q, u])
+    q, u = lf.eb2qu(f[-2], f[-1], N, theta_pix)
+    return np.stack(([f[0]] if f.shape[0] == 3 else []) + [q, u])
 
 
 def qu2eb_field(f, N, theta_pix):
     """fields.py:209-226."""
     if f.shape[0] == 1:
         return f
-    e, b = lf.qu2eb(f[0], f[1], N, theta_pix)
-    return np.stack([e, b])
+    e, b = lf.qu2eb(f[-2], f[-1], N, theta_pix)
+    return np.stack(([f[0]] if f.shape[0] == 3 else []) + [e, b])
 
 
 def to_map(f):

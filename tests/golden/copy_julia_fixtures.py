@@ -27,8 +27,15 @@ NAMES = [
     "data_e_field_P", "data_b_field_P", "cn_ee_P", "cn_bb_P", "cf_ee_P", "cf_bb_P",
     "m_ee_P", "m_bb_P", "b_ee_P", "b_bb_P", "cphi_P", "gradf_e_P", "gradf_b_P",
     "grad_phi_t_P", "f_wf_ffpp_e_P", "f_wf_ffpp_b_P",
+    # intensity and polarisation (T+EB, BlockTEB operators)
+    "phi_IP", "t_field_IP", "e_field_IP", "b_field_IP", "t_lensed_field_IP",
+    "e_lensed_field_IP", "b_lensed_field_IP", "data_t_field_IP", "data_e_field_IP",
+    "data_b_field_IP", "cphi_IP",
+] + [f"{op}_{blk}_IP" for op in ("cn", "cf", "m", "b") for blk in ("tt", "te", "ee", "bb")] + [
+    "gradf_t_IP", "gradf_e_IP", "gradf_b_IP", "grad_phi_t_IP",
+    "f_wf_ffpp_t_IP", "f_wf_ffpp_e_IP", "f_wf_ffpp_b_IP",
 ]
-TEXT = ["logpdf_I.txt", "logpdf_P.txt", "n_side.txt", "theta_pix.txt"]
+TEXT = ["logpdf_I.txt", "logpdf_P.txt", "logpdf_IP.txt", "n_side.txt", "theta_pix.txt"]
 
 
 def main(ref_root="/root/reference"):

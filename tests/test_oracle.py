@@ -252,6 +252,68 @@ def test_julia_grad_phi_P():
     assert rel(g, jl("grad_phi_t_P")) < 2e-4
 
 
+def _ops_IP():
+    st = lambda *a: np.stack([jl(x) for x in a])
+    op = lambda p: owf.TEB(jl(f"{p}_tt_IP"), jl(f"{p}_te_IP"), jl(f"{p}_te_IP"), jl(f"{p}_ee_IP"),
+                           jl(f"{p}_bb_IP"))
+    return dict(f=st("t_field_IP", "e_field_IP", "b_field_IP"),
+                d=st("data_t_field_IP", "data_e_field_IP", "data_b_field_IP"),
+                Cn=op("cn"), Cf=op("cf"), M=op("m"), B=op("b"))
+
+
+def test_teb_block_operator_algebra():
+    """TEB product, application and Schur pinv against dense 2x2 algebra per
+    mode, and the block log-determinant against numpy's slogdet."""
+    rng = np.random.default_rng(5)
+    sh = (4, 3)
+    A = owf.TEB(*(rng.uniform(1, 2, sh) for _ in range(5)))
+    Bo = owf.TEB(*(rng.uniform(1, 2, sh) for _ in range(5)))
+    dense = lambda o: np.array([[o.TT, o.TE], [o.ET, o.EE]])      # [2, 2, *sh]
+    AB = A * Bo
+    assert np.allclose(dense(AB), np.einsum("ij...,jk...->ik...", dense(A), dense(Bo)))
+    assert np.allclose(AB.BB, A.BB * Bo.BB)
+    P = A.pinv()
+    eye = np.einsum("ij...,jk...->ik...", dense(P), dense(A))
+    assert np.allclose(eye, np.eye(2)[:, :, None, None] * np.ones(sh))
+    f = rng.standard_normal((3,) + sh) + 1j * rng.standard_normal((3,) + sh)
+    g = A * f
+    assert np.allclose(g[0], A.TT * f[0] + A.TE * f[1]) and np.allclose(g[2], A.BB * f[2])
+    N = 4
+    shp = (N, N // 2 + 1)       # positive-definite blocks: every sign is +1
+    Ad = owf.TEB(rng.uniform(1, 2, shp), rng.uniform(0, .5, shp), rng.uniform(0, .5, shp),
+                 rng.uniform(1, 2, shp), rng.uniform(1, 2, shp))
+    w = olf.fourier_weights(N)
+    blocks = np.moveaxis(np.array([[Ad.TT, Ad.TE], [Ad.ET, Ad.EE]]), (0, 1), (-2, -1))
+    ref = np.sum(np.linalg.slogdet(blocks)[1] * w) + np.sum(np.log(Ad.BB) * w)
+    assert abs(owf.log_det(Ad, N) - ref) < 1e-12 * abs(ref)
+
+
+def test_julia_forward_IP():
+    N = 256
+    st = lambda *a: np.stack([jl(x) for x in a])
+    pw = olf.pix_width_from_theta(2.5)
+    f = owf.to_map(owf.eb2qu_field(st("t_field_IP", "e_field_IP", "b_field_IP"), N, 2.5))
+    ref = owf.to_map(owf.eb2qu_field(st("t_lensed_field_IP", "e_lensed_field_IP",
+                                        "b_lensed_field_IP"), N, 2.5))
+    assert rel(olf.lense_flow(f, np.fft.irfft2(jl("phi_IP")), pw, 7), ref) < 1e-12
+
+
+def test_julia_logpdf_gradf_grad_phi_IP():
+    """T+EB with BlockTEB operators (the TE block of Cf couples T and E)."""
+    o = _ops_IP()
+    lp = owf.logpdf(o["f"], jl("phi_IP"), o["d"], o["Cn"], jl("cphi_IP"), o["Cf"], o["M"],
+                    o["B"], 2.5, n=7)
+    ref = float((J / "logpdf_IP.txt").read_text())
+    assert abs(lp - ref) < 1e-12 * abs(ref)
+    g = owf.gradf_logpdf(o["f"], jl("phi_IP"), o["d"], o["Cf"], o["Cn"], o["M"], o["B"], 2.5,
+                         n=7)
+    gj = np.stack([jl("gradf_t_IP"), jl("gradf_e_IP"), jl("gradf_b_IP")])
+    assert rel(g[:, :, 1:-1], gj[:, :, 1:-1]) < 1e-10
+    gp = owf.grad_phi_logpdf(o["f"], jl("phi_IP"), o["d"], o["Cn"], jl("cphi_IP"), o["Cf"],
+                             o["M"], o["B"], 2.5, n=7)
+    assert rel(gp, jl("grad_phi_t_IP")) < 2e-3
+
+
 # ------------------------------------------------------- Wiener-filter pins
 def _sims():
     import importlib.util
