@@ -516,6 +516,38 @@ cmbl_status run_gradf(cmbl_plan* p, const cmbl_phi* const* phis, const void* f, 
   const bool pix = ops->mask_pix != nullptr;
   const T* mask = (const T*)ops->mask_pix;
   void* st = (void*)s;
+  if (C == 3) {
+    // T + EB: the T maps (group of `batch`) and the Q,U pairs (group of
+    // 2 batch) are lensed separately and meet in the block-operator kernels
+    const long NH1 = (long)N * (N / 2 + 1);
+    cplx<T>* X1T = X1;
+    cplx<T>* X1P = X1 + batch * NH1;
+    cplx<T>* X2T = X2;
+    cplx<T>* X2P = X2 + batch * NH1;
+    T* Y1T = Y1;
+    T* Y1P = Y1 + batch * NN;
+    T* Y2T = Y2;
+    T* Y2P = Y2 + batch * NN;
+    if (f) {
+      HIP_TRY(cmbl::launch_teb_split<T>((const cplx<T>*)f, X1T, X1P, N, batch, s));
+      CMBL_TRY(cmbl_irfft2(p, X1, Y1, (int)nm, st));
+      CMBL_TRY(cmbl_lenseflow(p, phis, Y1T, Y2T, batch, 1, n, CMBL_FORWARD_LENSE, 0, st));
+      CMBL_TRY(cmbl_lenseflow(p, phis, Y1P, Y2P, batch, 2, n, CMBL_FORWARD_LENSE, 0, st));
+      CMBL_TRY(cmbl_rfft2(p, Y2, X2, (int)nm, st));
+    }
+    HIP_TRY(cmbl::launch_wf_data_teb<T>(f ? X2T : nullptr, f ? X2P : nullptr, (const cplx<T>*)d,
+                                        (const T*)ops->mb, (const T*)ops->bmcn, X2T, X2P, N,
+                                        batch, s));
+    CMBL_TRY(cmbl_irfft2(p, X2, Y2, (int)nm, st));
+    CMBL_TRY(cmbl_lenseflow(p, phis, Y2T, Y2T, batch, 1, n, CMBL_FORWARD_LENSE, 1, st));
+    CMBL_TRY(cmbl_lenseflow(p, phis, Y2P, Y2P, batch, 2, n, CMBL_FORWARD_LENSE, 1, st));
+    CMBL_TRY(cmbl_rfft2(p, Y2, X2, (int)nm, st));
+    const bool prior = f && ops->cfinv;
+    HIP_TRY(cmbl::launch_wf_final_teb<T>(X2T, X2P, prior ? (const cplx<T>*)f : nullptr,
+                                         prior ? (const T*)ops->cfinv : nullptr, (cplx<T>*)out, N,
+                                         batch, s));
+    return CMBL_OK;
+  }
   // 1. forward model on f (skipped when f = 0: L 0 = 0)
   if (f) {
     HIP_TRY(cmbl::launch_rotate<T>((const cplx<T>*)f, X1, nullptr, N, C, batch, 1, s));
@@ -568,9 +600,11 @@ extern "C" cmbl_status cmbl_gradf_logpdf(cmbl_plan* p, const cmbl_phi* const* ph
                                          const void* d, const cmbl_wf_ops* ops, void* out,
                                          int batch, int ncomp, int nsteps, void* stream) {
   if (!p || !phis || !ops || !out) return fail(CMBL_EINVAL, "NULL argument");
-  if (ncomp != 1 && ncomp != 2) return fail(CMBL_EINVAL, "ncomp must be 1 (T) or 2 (EB)");
+  if (ncomp < 1 || ncomp > 3) return fail(CMBL_EINVAL, "ncomp must be 1 (T), 2 (EB) or 3 (TEB)");
   if ((long)batch * ncomp > p->max_maps)
     return fail(CMBL_EINVAL, "batch*ncomp exceeds the plan's max_maps");
+  if (ncomp == 3 && ops->mask_pix)
+    return fail(CMBL_EINVAL, "T+EB fields take BlockTEB Fourier operators, not a pixel mask");
   if (ops->mask_pix) {
     if (!ops->beam || !ops->cninv) return fail(CMBL_EINVAL, "pixel-mask mode needs beam and cninv");
   } else if (!ops->mb || !ops->bmcn) {
@@ -604,7 +638,8 @@ extern "C" cmbl_status cmbl_cg_update_xrz(cmbl_plan* p, void* x, void* r, void* 
                                           int nrhs, int ncomp, double* rz, void* stream) {
   if (!p || !x || !r || !z || !pv || !Ap || !pinv || !alpha || !rz)
     return fail(CMBL_EINVAL, "NULL argument");
-  if (nrhs < 1 || ncomp < 1) return fail(CMBL_EINVAL, "nrhs and ncomp must be >= 1");
+  if (nrhs < 1 || ncomp < 1 || ncomp > 3)
+    return fail(CMBL_EINVAL, "nrhs must be >= 1 and ncomp 1, 2 or 3");
   CMBL_TRY(ensure_red(p, nrhs));
   const long len = (long)ncomp * p->N * (p->N / 2 + 1);
   hipStream_t s = (hipStream_t)stream;

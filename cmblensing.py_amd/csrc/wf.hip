@@ -168,6 +168,117 @@ __global__ void k_wf_final(const cplx<T>* XL, const cplx<T>* __restrict__ F,
   }
 }
 
+// ---------------------------------------------------------------------------
+// T + EB fields (C = 3, FlatS02) with BlockTEB operators (matrix_operators.py:
+// 92-148): per mode [[TT, TE, 0], [ET, EE, 0], [0, 0, BB]], stored as five
+// real planes [5][N][N/2+1] in that order.  Inside gradf the T maps and the
+// Q,U pairs are lensed as two groups (C = 1 and C = 2), so these kernels split
+// a [nfields][3] field into a T group [nfields] and a QU group [nfields][2]
+// and join the groups back.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void teb_apply(const T* __restrict__ op, long idx, long P, cplx<T> t,
+                                          cplx<T> e, cplx<T> b, cplx<T>& ot, cplx<T>& oe,
+                                          cplx<T>& ob) {
+  const T tt = op[idx], te = op[P + idx], et = op[2 * P + idx], ee = op[3 * P + idx];
+  ot = scale(t, tt) + scale(e, te);
+  oe = scale(t, et) + scale(e, ee);
+  ob = scale(b, op[4 * P + idx]);
+}
+
+// f [nfields][3] (T, E, B) -> XT [nfields] (T) and XP [nfields][2] (eb2qu)
+template <typename T>
+__global__ void k_teb_split(const cplx<T>* __restrict__ f, cplx<T>* __restrict__ XT,
+                            cplx<T>* __restrict__ XP, int N, long nfields) {
+  const long P = (long)N * (N / 2 + 1);
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < nfields * P;
+       g += (long)gridDim.x * blockDim.x) {
+    const long fi = g / P, idx = g % P;
+    const long base = fi * 3 * P + idx;
+    XT[fi * P + idx] = f[base];
+    T c, s;
+    rot2psi<T>(N, idx, c, s);
+    cplx<T> q, u;
+    eb2qu_mode(f[base + P], f[base + 2 * P], c, s, q, u);
+    XP[fi * 2 * P + idx] = q;
+    XP[fi * 2 * P + P + idx] = u;
+  }
+}
+
+// Data term for T + EB: from the rfft2 of the lensed T maps XT and Q,U maps XP
+// (NULL: f = 0), g = bmcn (d - mb (T, qu2eb(Q, U))), with d [nfields][3] TEB
+// (NULL: d = 0); written as T -> outT and eb2qu(E, B) -> outP.  outT / outP may
+// alias XT / XP.
+template <typename T>
+__global__ void k_wf_data_teb(const cplx<T>* XT, const cplx<T>* XP, const cplx<T>* __restrict__ d,
+                              const T* __restrict__ mb, const T* __restrict__ bmcn,
+                              cplx<T>* outT, cplx<T>* outP, int N, long nfields) {
+  const long P = (long)N * (N / 2 + 1);
+  const cplx<T> zero = mk(T(0), T(0));
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < nfields * P;
+       g += (long)gridDim.x * blockDim.x) {
+    const long fi = g / P, idx = g % P;
+    T c, s;
+    rot2psi<T>(N, idx, c, s);
+    cplx<T> t = zero, e = zero, b = zero;
+    if (XT) {
+      t = XT[fi * P + idx];
+      qu2eb_mode(XP[fi * 2 * P + idx], XP[fi * 2 * P + P + idx], c, s, e, b);
+    }
+    cplx<T> rt, re, rb;
+    teb_apply(mb, idx, P, t, e, b, rt, re, rb);
+    rt = mk(-rt.x, -rt.y);
+    re = mk(-re.x, -re.y);
+    rb = mk(-rb.x, -rb.y);
+    if (d) {
+      const long base = fi * 3 * P + idx;
+      rt = rt + d[base];
+      re = re + d[base + P];
+      rb = rb + d[base + 2 * P];
+    }
+    cplx<T> gt, ge, gb;
+    teb_apply(bmcn, idx, P, rt, re, rb, gt, ge, gb);
+    outT[fi * P + idx] = gt;
+    cplx<T> q, u;
+    eb2qu_mode(ge, gb, c, s, q, u);
+    outP[fi * 2 * P + idx] = q;
+    outP[fi * 2 * P + P + idx] = u;
+  }
+}
+
+// Last line of gradf for T + EB: out [nfields][3] = (XLT, qu2eb(XLP)) minus
+// cfinv (Herm F_T, Herm F_E, Herm F_B) (the prior term as in k_wf_final).
+// F and cfinv may be NULL.
+template <typename T>
+__global__ void k_wf_final_teb(const cplx<T>* XLT, const cplx<T>* XLP,
+                               const cplx<T>* __restrict__ F, const T* __restrict__ cfinv,
+                               cplx<T>* out, int N, long nfields) {
+  const long P = (long)N * (N / 2 + 1);
+  const int H1 = N / 2 + 1;
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < nfields * P;
+       g += (long)gridDim.x * blockDim.x) {
+    const long fi = g / P, idx = g % P;
+    const int i = (int)(idx / H1), j = (int)(idx % H1);
+    T c, s;
+    rot2psi<T>(N, idx, c, s);
+    cplx<T> t = XLT[fi * P + idx], e, b;
+    qu2eb_mode(XLP[fi * 2 * P + idx], XLP[fi * 2 * P + P + idx], c, s, e, b);
+    if (F && cfinv) {
+      const long bf = fi * 3 * P;
+      cplx<T> pt, pe, pb;
+      teb_apply(cfinv, idx, P, herm_col(F, bf, N, i, j), herm_col(F, bf + P, N, i, j),
+                herm_col(F, bf + 2 * P, N, i, j), pt, pe, pb);
+      t = t - pt;
+      e = e - pe;
+      b = b - pb;
+    }
+    const long base = fi * 3 * P + idx;
+    out[base] = t;
+    out[base + P] = e;
+    out[base + 2 * P] = b;
+  }
+}
+
 // Pixel-space multiply x[m][p] *= w[p] (a pixel mask shared by all maps).
 template <typename T>
 __global__ void k_pixel_mul(T* x, const T* __restrict__ w, long npix, long nmaps) {
@@ -282,6 +393,47 @@ __global__ __launch_bounds__(kRedBlock) void k_cg_xrz(cplx<double>* __restrict__
   if (threadIdx.x == 0) part[rhs * kRedParts + blockIdx.x] = s;
 }
 
+// The same update for T + EB (C = 3) with a BlockTEB pinv [5][N][N/2+1]:
+// z = pinv r mixes the T and E components of each mode.
+template <typename Tb>
+__global__ __launch_bounds__(kRedBlock) void k_cg_xrz_teb(cplx<double>* __restrict__ x,
+                                                          cplx<double>* __restrict__ r,
+                                                          cplx<double>* __restrict__ z,
+                                                          const cplx<double>* __restrict__ p,
+                                                          const cplx<Tb>* __restrict__ Ap,
+                                                          const double* __restrict__ pinv,
+                                                          const double* __restrict__ alpha, int N,
+                                                          double* __restrict__ part) {
+  const int rhs = blockIdx.y;
+  const long P = (long)N * (N / 2 + 1);
+  const double al = alpha[rhs];
+  const long off = rhs * 3 * P;
+  double acc = 0.0;
+  for (long i = blockIdx.x * (long)kRedBlock + threadIdx.x; i < P; i += (long)kRedParts * kRedBlock) {
+    cplx<double> rc[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long o = off + c * P + i;
+      const cplx<double> xi = x[o], pi = p[o];
+      const cplx<Tb> api = Ap[o];
+      const cplx<double> ri = r[o];
+      x[o] = mk(xi.x + al * pi.x, xi.y + al * pi.y);
+      rc[c] = mk(ri.x - al * (double)api.x, ri.y - al * (double)api.y);
+      r[o] = rc[c];
+    }
+    cplx<double> zc[3];
+    teb_apply(pinv, i, P, rc[0], rc[1], rc[2], zc[0], zc[1], zc[2]);
+    const double w = wgt(i, N);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      z[off + c * P + i] = zc[c];
+      acc += (rc[c].x * zc[c].x + rc[c].y * zc[c].y) * w;
+    }
+  }
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) part[rhs * kRedParts + blockIdx.x] = s;
+}
+
 // p = z + beta p, and p rounded to the operator precision (pT)
 template <typename T>
 __global__ void k_cg_p(cplx<double>* __restrict__ p, const cplx<double>* __restrict__ z,
@@ -363,8 +515,12 @@ hipError_t launch_cg_xrz(cplx<double>* x, cplx<double>* r, cplx<double>* z,
                          const double* alpha, int N, int C, long len, int nrhs, double* part,
                          double* rz, hipStream_t s) {
   const double sc = 1.0 / ((double)N * (double)N);
-  k_cg_xrz<Tb><<<dim3(kRedParts, nrhs), kRedBlock, 0, s>>>(x, r, z, p, Ap, pinv, alpha, N, C, len,
-                                                          part);
+  if (C == 3)
+    k_cg_xrz_teb<Tb><<<dim3(kRedParts, nrhs), kRedBlock, 0, s>>>(x, r, z, p, Ap, pinv, alpha, N,
+                                                                part);
+  else
+    k_cg_xrz<Tb><<<dim3(kRedParts, nrhs), kRedBlock, 0, s>>>(x, r, z, p, Ap, pinv, alpha, N, C,
+                                                            len, part);
   k_dot_final<<<(nrhs + 63) / 64, 64, 0, s>>>(part, sc, rz, nrhs);
   return hipGetLastError();
 }
@@ -373,6 +529,30 @@ template <typename T>
 hipError_t launch_cg_p(cplx<double>* p, const cplx<double>* z, const double* beta, cplx<T>* pT,
                        long len, int nrhs, hipStream_t s) {
   k_cg_p<T><<<grid_for(len * nrhs), 256, 0, s>>>(p, z, beta, pT, len, nrhs);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_teb_split(const cplx<T>* f, cplx<T>* XT, cplx<T>* XP, int N, long nfields,
+                            hipStream_t s) {
+  k_teb_split<T><<<grid_for(nfields * (long)N * (N / 2 + 1)), 256, 0, s>>>(f, XT, XP, N, nfields);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_wf_data_teb(const cplx<T>* XT, const cplx<T>* XP, const cplx<T>* d, const T* mb,
+                              const T* bmcn, cplx<T>* outT, cplx<T>* outP, int N, long nfields,
+                              hipStream_t s) {
+  k_wf_data_teb<T><<<grid_for(nfields * (long)N * (N / 2 + 1)), 256, 0, s>>>(
+      XT, XP, d, mb, bmcn, outT, outP, N, nfields);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_wf_final_teb(const cplx<T>* XLT, const cplx<T>* XLP, const cplx<T>* F,
+                               const T* cfinv, cplx<T>* out, int N, long nfields, hipStream_t s) {
+  k_wf_final_teb<T><<<grid_for(nfields * (long)N * (N / 2 + 1)), 256, 0, s>>>(
+      XLT, XLP, F, cfinv, out, N, nfields);
   return hipGetLastError();
 }
 
@@ -395,7 +575,14 @@ int red_parts() { return kRedParts; }
                                        const double*, int, int, long, int, double*, double*,    \
                                        hipStream_t);                                            \
   template hipError_t launch_cg_p<T>(cplx<double>*, const cplx<double>*, const double*,         \
-                                     cplx<T>*, long, int, hipStream_t);
+                                     cplx<T>*, long, int, hipStream_t);                         \
+  template hipError_t launch_teb_split<T>(const cplx<T>*, cplx<T>*, cplx<T>*, int, long,        \
+                                          hipStream_t);                                         \
+  template hipError_t launch_wf_data_teb<T>(const cplx<T>*, const cplx<T>*, const cplx<T>*,     \
+                                            const T*, const T*, cplx<T>*, cplx<T>*, int, long,  \
+                                            hipStream_t);                                       \
+  template hipError_t launch_wf_final_teb<T>(const cplx<T>*, const cplx<T>*, const cplx<T>*,    \
+                                             const T*, cplx<T>*, int, long, hipStream_t);
 CMBL_WF_INST(float)
 CMBL_WF_INST(double)
 

@@ -32,6 +32,16 @@ hipError_t launch_cg_xrz(cplx<double>* x, cplx<double>* r, cplx<double>* z,
 template <typename T>
 hipError_t launch_cg_p(cplx<double>* p, const cplx<double>* z, const double* beta, cplx<T>* pT,
                        long len, int nrhs, hipStream_t s);
+template <typename T>
+hipError_t launch_teb_split(const cplx<T>* f, cplx<T>* XT, cplx<T>* XP, int N, long nfields,
+                            hipStream_t s);
+template <typename T>
+hipError_t launch_wf_data_teb(const cplx<T>* XT, const cplx<T>* XP, const cplx<T>* d, const T* mb,
+                              const T* bmcn, cplx<T>* outT, cplx<T>* outP, int N, long nfields,
+                              hipStream_t s);
+template <typename T>
+hipError_t launch_wf_final_teb(const cplx<T>* XLT, const cplx<T>* XLP, const cplx<T>* F,
+                               const T* cfinv, cplx<T>* out, int N, long nfields, hipStream_t s);
 int red_parts();
 
 }  // namespace cmbl

@@ -22,7 +22,7 @@ from .matrix_operators import (BlockTEB, DiagonalEB, DiagonalScalar, PixelMask, 
 
 def _ncomp(field: Field) -> int:
     if isinstance(field, FlatS02):
-        raise NotImplementedError("T+EB (FlatS02) fields are not supported by the GPU gradf yet")
+        return 3
     if isinstance(field, FlatS2):
         return 2
     if isinstance(field, FlatS0):
@@ -43,23 +43,33 @@ def unstack_matrices(field: Field, X: torch.Tensor, lead) -> Field:
 
 
 def per_mode(op, dtype, device) -> torch.Tensor:
-    if isinstance(op, (DiagonalScalar, DiagonalEB)):
+    """[C, N, N//2+1] planes of a diagonal operator, or the five BlockTEB planes
+    [TT, TE, ET, EE, BB]."""
+    if isinstance(op, (DiagonalScalar, DiagonalEB, BlockTEB)):
         return op.per_mode().to(device=device, dtype=dtype).contiguous()
-    if isinstance(op, BlockTEB):
-        raise NotImplementedError("BlockTEB operators are not supported by the GPU gradf yet")
     raise TypeError(f"Unsupported operator: {type(op).__name__}")
 
 
 class WfOperators:
     """Per-mode operator arrays in the plan dtype, built once per set of
     operators: mb = M B, bmcn = B M Cn^-1, cfinv = Cf^-1 (Fourier-diagonal
-    mask), or beam, cninv and a pixel mask."""
+    mask), or beam, cninv and a pixel mask.  BlockTEB products are formed in
+    the reference's order (gradients.py:20: (B M) Cn^-1 and M B) before they are
+    reduced to planes."""
 
     def __init__(self, field_covariance, noise_covariance, mask, beam, dtype, device):
         self.dtype, self.device = dtype, device
         self.pixel = isinstance(mask, PixelMask)
         pm = lambda op: per_mode(op, dtype, device)
         self.cfinv = pm(pinv(field_covariance)) if field_covariance is not None else None
+        if any(isinstance(o, BlockTEB) for o in (noise_covariance, mask, beam)):
+            if self.pixel:
+                raise NotImplementedError("T+EB fields take BlockTEB Fourier operators, not a "
+                                          "PixelMask")
+            self.mb = pm(mask * beam)
+            self.bmcn = pm(beam * mask * pinv(noise_covariance))
+            self.beam = self.cninv = self.mask = None
+            return
         if self.pixel:
             self.beam = pm(beam)
             self.cninv = pm(pinv(noise_covariance))

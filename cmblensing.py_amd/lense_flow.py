@@ -46,11 +46,17 @@ def phi_handles_for(plan, phi_m: torch.Tensor, batch: int, n: int):
 
 def lense_flow_matrices(x: torch.Tensor, phi_m: torch.Tensor, theta_pix: float, n: int,
                         direction: int, adjoint: bool) -> torch.Tensor:
-    """LenseFlow of stacked MAP matrices x [B, C, N, N] (C = 1 or 2)."""
+    """LenseFlow of stacked MAP matrices x [B, C, N, N] (C = 1: T, 2: Q,U, 3:
+    T,Q,U, lensed as a T group and a Q,U group)."""
     if direction not in (FORWARD_LENSE, INVERSE_LENSE):
         raise ValueError("direction must be FORWARD_LENSE (+1) or INVERSE_LENSE (-1)")
-    if x.dim() != 4 or x.shape[1] not in (1, 2) or x.shape[-1] != x.shape[-2]:
-        raise ValueError(f"expected MAP matrices [B, C(1|2), N, N], got {tuple(x.shape)}")
+    if x.dim() != 4 or x.shape[1] not in (1, 2, 3) or x.shape[-1] != x.shape[-2]:
+        raise ValueError(f"expected MAP matrices [B, C(1|2|3), N, N], got {tuple(x.shape)}")
+    if x.shape[1] == 3:
+        return torch.cat([lense_flow_matrices(x[:, :1].contiguous(), phi_m, theta_pix, n,
+                                              direction, adjoint),
+                          lense_flow_matrices(x[:, 1:].contiguous(), phi_m, theta_pix, n,
+                                              direction, adjoint)], dim=1)
     if not x.is_cuda:
         raise _lib.CmblError("LenseFlow runs on the GPU: move the field to a HIP device")
     B, C, N, _ = x.shape
@@ -106,6 +112,13 @@ def lense_flow_vjp_matrices(lensed: torch.Tensor, cot: torch.Tensor, phi_m: torc
     <cot, L f> with respect to a MAP phi."""
     if direction not in (FORWARD_LENSE, INVERSE_LENSE):
         raise ValueError("direction must be FORWARD_LENSE (+1) or INVERSE_LENSE (-1)")
+    if lensed.shape[1] == 3:
+        # T and Q,U groups; the delta-phi rate sums over all components
+        dT, pT = lense_flow_vjp_matrices(lensed[:, :1], cot[:, :1], phi_m, theta_pix, n,
+                                         direction)
+        dP, pP = lense_flow_vjp_matrices(lensed[:, 1:], cot[:, 1:], phi_m, theta_pix, n,
+                                         direction)
+        return torch.cat([dT, dP], dim=1), pT + pP
     B, C, N, _ = lensed.shape
     lensed = lensed.contiguous()
     cot = cot.to(lensed.dtype).contiguous()

@@ -158,6 +158,12 @@ class BlockTEB(FieldOperator):
             return o.__mul__(self)
         return self.replace(**{n: getattr(self, n) * o for n in self._matrix_names()})
 
+    def per_mode(self) -> torch.Tensor:
+        """[5, N, N//2+1] planes TT, TE, ET, EE, BB used by the native Wiener
+        filter."""
+        return torch.stack([self.matrix_TT, self.matrix_TE, self.matrix_ET, self.matrix_EE,
+                            self.matrix_BB])
+
 
 @dataclass
 class PixelMask:

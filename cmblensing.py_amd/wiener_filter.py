@@ -30,7 +30,7 @@ from .constants import DEFAULT_NSTEPS
 from .fields import Field
 from .gradients import (WfOperators, _ncomp, gradf_apply, phi_handles, stack_matrices,
                         unstack_matrices)
-from .matrix_operators import DiagonalEB, DiagonalScalar, PixelMask, pinv
+from .matrix_operators import BlockTEB, DiagonalEB, DiagonalScalar, PixelMask, pinv
 
 
 @dataclass
@@ -139,10 +139,10 @@ def conjugate_gradient(x0, data, phi, mask, beam, noise_covariance, field_covari
         bvec = bvec.expand(B, *bvec.shape[1:])
     bvec = bvec.contiguous()
     pinv_P = pinv(preconditioner)
-    if isinstance(pinv_P, (DiagonalScalar, DiagonalEB)):
+    if isinstance(pinv_P, (DiagonalScalar, DiagonalEB, BlockTEB)):
         pinv_arr = pinv_P.per_mode().to(device=dev, dtype=torch.float64).contiguous()
     else:
-        raise NotImplementedError("preconditioner must be DiagonalScalar or DiagonalEB")
+        raise NotImplementedError("preconditioner must be DiagonalScalar, DiagonalEB or BlockTEB")
 
     r = torch.empty_like(x)
     z = torch.empty_like(x)

@@ -98,7 +98,10 @@ cmbl_status cmbl_irfft2(cmbl_plan* plan, const void* X, void* x, int nmaps, void
 /* ---- lensed Wiener filter -------------------------------------------------
  * Diagonal operators are real arrays [ncomp][N][N/2+1] in the plan dtype,
  * shared by all batch elements (DiagonalScalar / DiagonalEB,
- * matrix_operators.py:45-90).  Fields are EB (or T) FOURIER arrays. */
+ * matrix_operators.py:45-90).  Fields are EB (or T) FOURIER arrays.  With
+ * ncomp = 3 (T, E, B: FlatS02, fields.py:107-116) the operators are BlockTEB
+ * (matrix_operators.py:92-148): real [5][N][N/2+1] planes TT, TE, ET, EE, BB,
+ * in the Fourier-mask mode only. */
 typedef struct {
   const void* mb;       /* M B, per mode (Fourier-mask mode) */
   const void* bmcn;     /* B M Cn^-1, per mode (Fourier-mask mode) */
@@ -110,7 +113,8 @@ typedef struct {
 
 /* gradf_logpdf (gradients.py:12-26):
  *   out = L^T B M^T Cn^-1 (d - M B L f) - Cf^-1 f       (EB FOURIER in and out)
- * with L = LenseFlow(phi, nsteps) forward and L^T its adjoint, and the EB<->QU /
+ * with L = LenseFlow(phi, nsteps) forward and L^T its adjoint (the T maps and
+ * the Q,U pairs of a T+EB field are lensed as two groups), and the EB<->QU /
  * map / fourier conversions of the reference.  f == NULL means f = 0 (the
  * forward LenseFlow of zeros is skipped); d == NULL means d = 0 (the CG
  * operator A, wiener_filter.py:45-49).  d is EB FOURIER in Fourier-mask mode,
@@ -129,7 +133,8 @@ cmbl_status cmbl_dot_f64(cmbl_plan* plan, const void* a, const void* b, int b_f6
 /* One PCG update (wiener_filter.py:84-92) for nrhs systems, FP64 vectors:
  *   x += alpha p ; r -= alpha Ap ; z = pinv .* r ; rz = dot(r, z)
  * Ap is complex in the plan dtype (the operator's output); pinv is a double
- * [ncomp][N][N/2+1] diagonal; alpha and rz are device arrays of nrhs doubles. */
+ * [ncomp][N][N/2+1] diagonal, or for ncomp = 3 a BlockTEB [5][N][N/2+1] (TT,
+ * TE, ET, EE, BB); alpha and rz are device arrays of nrhs doubles. */
 cmbl_status cmbl_cg_update_xrz(cmbl_plan* plan, void* x, void* r, void* z, const void* p,
                                const void* Ap, const double* pinv, const double* alpha, int nrhs,
                                int ncomp, double* rz, void* stream);

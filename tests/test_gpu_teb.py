@@ -1,0 +1,134 @@
+"""GPU parity of T+EB (FlatS02) fields with BlockTEB operators: LenseFlow,
+gradf_logpdf, logpdf, grad_phi_logpdf and the Wiener filter against the FP64
+oracle and CMBLensing.jl's IP fixtures.  Tolerances: FP64 1e-10 relative L2
+against the oracle (gradients) and 1e-12 relative (logpdf), FP32 2e-5; Julia
+fixtures as pinned for the oracle (interior modes, see test_oracle)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import lenseflow as olf
+from oracle import wiener as owf
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+J = GOLDEN / "julia"
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def _case(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def L(n):
+    return np.load(J / f"{n}.npy", allow_pickle=False)
+
+
+def stack(field):
+    return np.stack([getattr(field, n).cpu().numpy() for n in field._matrix_names()])
+
+
+def _problem(pkg, dev, cdt=torch.complex128):
+    from cmblensing_amd import matrix_operators as mo
+    T = lambda a, dt=cdt: torch.as_tensor(a, device=dev).to(dt)
+    F = lambda t, e, b: pkg.FlatS02(scalar_matrix=T(L(t)), polar_matrix_1=T(L(e)),
+                                    polar_matrix_2=T(L(b)), theta_pix=2.5)
+    R = lambda n: T(L(n), torch.float64)
+    O = lambda p: mo.BlockTEB(matrix_TT=R(f"{p}_tt_IP"), matrix_TE=R(f"{p}_te_IP"),
+                              matrix_ET=R(f"{p}_te_IP"), matrix_EE=R(f"{p}_ee_IP"),
+                              matrix_BB=R(f"{p}_bb_IP"), theta_pix=2.5)
+    g = dict(f=F("t_field_IP", "e_field_IP", "b_field_IP"),
+             d=F("data_t_field_IP", "data_e_field_IP", "data_b_field_IP"),
+             phi=pkg.FlatS0(scalar_matrix=T(L("phi_IP"), torch.complex128), theta_pix=2.5),
+             Cf=O("cf"), Cn=O("cn"), M=O("m"), B=O("b"),
+             Cphi=mo.DiagonalScalar(scalar_matrix=R("cphi_IP"), theta_pix=2.5))
+    st = lambda *a: np.stack([L(x) for x in a])
+    op = lambda p: owf.TEB(L(f"{p}_tt_IP"), L(f"{p}_te_IP"), L(f"{p}_te_IP"), L(f"{p}_ee_IP"),
+                           L(f"{p}_bb_IP"))
+    a = dict(f=st("t_field_IP", "e_field_IP", "b_field_IP"),
+             d=st("data_t_field_IP", "data_e_field_IP", "data_b_field_IP"),
+             Cf=op("cf"), Cn=op("cn"), M=op("m"), B=op("b"), phi=L("phi_IP"), Cphi=L("cphi_IP"))
+    return g, a
+
+
+def test_lense_flow_IP_julia(pkg, dev):
+    _case("lense_flow FlatS02 julia IP fp64")
+    g, a = _problem(pkg, dev)
+    f = pkg.map(pkg.eb2qu(g["f"]))
+    y = pkg.lense_flow(f, pkg.map(g["phi"]), n=7)
+    st = lambda *n: np.stack([L(x) for x in n])
+    ref = owf.to_map(owf.eb2qu_field(st("t_lensed_field_IP", "e_lensed_field_IP",
+                                        "b_lensed_field_IP"), 256, 2.5))
+    assert rel(stack(y), ref) < 1e-11
+
+
+@pytest.mark.parametrize("cdt,tol", [(torch.complex128, 1e-10), (torch.complex64, 2e-5)])
+def test_gradf_IP(pkg, dev, cdt, tol):
+    from cmblensing_amd import gradients
+    _case(f"gradf julia IP {cdt}")
+    g, a = _problem(pkg, dev, cdt)
+    out = stack(gradients.gradf_logpdf(g["f"], g["phi"], g["d"], g["Cf"], g["Cn"], g["M"], g["B"],
+                                       n=7))
+    ref = owf.gradf_logpdf(a["f"], a["phi"], a["d"], a["Cf"], a["Cn"], a["M"], a["B"], 2.5, n=7)
+    assert rel(out, ref) < tol
+    if cdt == torch.complex128:
+        gj = np.stack([L("gradf_t_IP"), L("gradf_e_IP"), L("gradf_b_IP")])
+        assert rel(out[:, :, 1:-1], gj[:, :, 1:-1]) < 1e-10
+
+
+def test_logpdf_grad_phi_IP(pkg, dev):
+    from cmblensing_amd import statistics
+    _case("logpdf / grad_phi julia IP fp64")
+    g, a = _problem(pkg, dev)
+    lp = statistics.logpdf(g["f"], g["phi"], g["d"], g["Cn"], g["Cphi"], g["Cf"], g["M"], g["B"],
+                           n=7)
+    ref = float((J / "logpdf_IP.txt").read_text())
+    assert abs(float(lp) - ref) < 1e-12 * abs(ref)
+    gp = statistics.grad_phi_logpdf(g["f"], g["phi"], g["d"], g["Cn"], g["Cphi"], g["Cf"], g["M"],
+                                    g["B"], n=7).scalar_matrix.cpu().numpy()
+    gref = owf.grad_phi_logpdf(a["f"], a["phi"], a["d"], a["Cn"], a["Cphi"], a["Cf"], a["M"],
+                               a["B"], 2.5, n=7)
+    assert rel(gp, gref) < 1e-9
+    assert rel(gp, L("grad_phi_t_IP")) < 2e-3
+
+
+def test_wiener_filter_IP_julia(pkg, dev):
+    """The reference's stop rule from the truth start with BlockTEB operators
+    (block preconditioner in the FP64 CG update) against f_wf_ffpp_{t,e,b}_IP."""
+    from cmblensing_amd import wiener_filter as wfm
+    _case("wf julia IP fp64 tol=0.1")
+    g, a = _problem(pkg, dev)
+    x, info = wfm.wiener_filter(g["f"], g["phi"], g["d"], g["Cf"], g["Cn"], g["M"], g["B"], n=7,
+                                return_info=True)
+    x = stack(x)
+    wj = np.stack([L("f_wf_ffpp_t_IP"), L("f_wf_ffpp_e_IP"), L("f_wf_ffpp_b_IP")])
+    print("iterations", info.iterations, flush=True)
+    assert 40 <= info.iterations[0] <= 56       # the oracle takes 48
+    assert rel(x, wj) < 2e-5
+    assert rel(x[:, :, 1:-1], wj[:, :, 1:-1]) < 1e-7
+
+
+def test_wiener_filter_IP_mixed_precision(pkg, dev):
+    """FP32 operator, FP64 CG and residual refinement with the block
+    preconditioner: converges to a 1e-6 true residual and matches the FP64
+    solve."""
+    from cmblensing_amd import wiener_filter as wfm
+    _case("wf IP mixed precision rtol 1e-6")
+    g32, _ = _problem(pkg, dev, torch.complex64)
+    x0 = g32["f"] * 0
+    x, info = wfm.wiener_filter(x0, g32["phi"], g32["d"], g32["Cf"], g32["Cn"], g32["M"], g32["B"],
+                                n=7, tol=0.0, rtol=1e-6, recompute_every=10, maxiter=500,
+                                return_info=True)
+    print("iterations", info.iterations, "true rel residual", info.final_true_rel_residual,
+          flush=True)
+    assert all(info.converged) and max(info.final_true_rel_residual) < 1e-6
+    g64, _ = _problem(pkg, dev)
+    x64 = wfm.wiener_filter(g64["f"] * 0, g64["phi"], g64["d"], g64["Cf"], g64["Cn"], g64["M"],
+                            g64["B"], n=7, tol=0.0, rtol=1e-10, maxiter=1500)
+    err = rel(stack(x).astype(np.complex128), stack(x64))
+    print("rel err vs FP64 solve", err, flush=True)
+    assert err < 1e-4
