@@ -23,7 +23,14 @@ a set of shared lensing potentials -- and runs many requests together:
   batch class), with the request maps gathered into and scattered out of a
   staging buffer by the graph's own kernels;
 * latency is arrival -> completion, the completion time read from a HIP
-  event recorded after the batch on its stream.
+  event recorded after the batch on its stream;
+* Wiener-filter requests (``WFRequest``: data, phi and operators of one
+  lensed Wiener filter, wiener_filter.py:10-26) pass the same admission and
+  run one at a time in a lane of their own stream.  The solve is a process
+  (``wiener_filter_process``) that yields an event wherever the PCG needs a
+  host scalar; the scheduler advances it when that event has completed, so
+  the host never blocks on a solve and LenseFlow batches keep being formed
+  and overlap with it on the device.
 
 The scheduler (``Server``) is independent of the device: ``GpuBackend`` runs
 the batches through libcmbl.so, and any object with the same methods (tests
@@ -83,6 +90,55 @@ class Request:
 
 
 @dataclass(eq=False)
+class WFRequest:
+    """One lensed Wiener-filter solve (wiener_filter.py:10-26): ``data`` is an
+    EB (or T) FOURIER field on the serving device (maps with a PixelMask),
+    ``ops`` = (field_covariance, noise_covariance, mask, beam); the field dtype
+    sets the operator precision (complex64: FP32 operator, FP64 CG).  The
+    result is ``out`` (the filtered field) and ``info`` (CGInfo)."""
+    data: object
+    phi_id: int
+    ops: tuple
+    n: int = 7
+    rtol: float = 1e-6
+    maxiter: int = 500
+    arrival: Optional[float] = None
+    rid: int = -1
+    out: object = None
+    info: object = None
+    done: Optional[float] = None
+    state: str = "new"
+    error: Optional[str] = None
+    nbytes: int = 0
+
+    @property
+    def N(self) -> int:
+        m = getattr(self.data, self.data._matrix_names()[0])
+        return int(m.shape[-2])
+
+    @property
+    def C(self) -> int:
+        return len(self.data._matrix_names())
+
+    @property
+    def latency(self) -> Optional[float]:
+        if self.done is None or self.arrival is None:
+            return None
+        return self.done - self.arrival
+
+
+@dataclass(eq=False)
+class _WFToken:
+    gen: object
+    event: object = None
+    started: bool = False
+    finished: bool = False
+    result: object = None
+    error: Optional[str] = None
+    end: object = None
+
+
+@dataclass(eq=False)
 class _Batch:
     token: object
     requests: List[Request]
@@ -118,22 +174,32 @@ class Server:
         self.phis: "OrderedDict[tuple, _PhiEntry]" = OrderedDict()
         self.hits = self.misses = self.evictions = 0
         self.completed: List[Request] = []
+        self.completed_wf: List[WFRequest] = []
+        self.wf_queue: deque = deque()
+        self.wf_running = 0
+        self.wf_slots = getattr(backend, "wf_slots", 1)
         self.batch_sizes: List[int] = []
         self._next_rid = 0
         self._by_rid: Dict[int, Request] = {}
 
     # -- requests ----------------------------------------------------------
-    def submit(self, req: Request) -> Request:
+    def submit(self, req) -> object:
+        if isinstance(req, WFRequest):
+            return self._submit_common(req)
         if req.N not in self.backend.sizes or req.C not in (1, 2):
             raise ValueError(f"unsupported request shape {tuple(req.x.shape)}")
         if req.direction not in (FORWARD_LENSE, INVERSE_LENSE):
             raise ValueError("direction must be FORWARD_LENSE (+1) or INVERSE_LENSE (-1)")
+        return self._submit_common(req)
+
+    def _submit_common(self, req):
         if req.arrival is None:
             req.arrival = time.perf_counter()
         if req.rid < 0:
             req.rid = self._next_rid
             self._next_rid += 1
-        req.nbytes = self.backend.request_bytes(req)
+        req.nbytes = (self.backend.wf_bytes(req) if isinstance(req, WFRequest)
+                      else self.backend.request_bytes(req))
         req.state = "waiting"
         self._by_rid[req.rid] = req
         self.waiting.append(req)
@@ -150,7 +216,10 @@ class Server:
             self.waiting.remove(req)
             req.state = "cancelled"
         elif req.state == "queued":
-            self.buckets[req.key].remove(req)
+            if isinstance(req, WFRequest):
+                self.wf_queue.remove(req)
+            else:
+                self.buckets[req.key].remove(req)
             self._release(req)
             req.state = "cancelled"
         else:  # running
@@ -158,7 +227,9 @@ class Server:
         return True
 
     # -- admission -----------------------------------------------------------
-    def _phi_cost(self, req: Request) -> int:
+    def _phi_cost(self, req) -> int:
+        if isinstance(req, WFRequest):
+            return 0  # prepared on the Wiener-filter lane's own plans (counted in wf_bytes)
         k = (req.phi_id, req.N)
         return 0 if k in self.phis else self.backend.phi_bytes(req.N)
 
@@ -186,6 +257,12 @@ class Server:
             if self.used + need > self.budget and not self._evict_for(need, (req.phi_id, req.N)):
                 return  # wait for running batches to free memory
             self.waiting.popleft()
+            if isinstance(req, WFRequest):
+                self.used += req.nbytes
+                self.peak_used = max(self.peak_used, self.used)
+                req.state = "queued"
+                self.wf_queue.append(req)
+                continue
             k = (req.phi_id, req.N)
             ent = self.phis.get(k)
             if ent is None:
@@ -205,9 +282,10 @@ class Server:
             req.state = "queued"
             self.buckets.setdefault(req.key, deque()).append(req)
 
-    def _release(self, req: Request):
+    def _release(self, req):
         self.used -= req.nbytes
-        self.phis[(req.phi_id, req.N)].refs -= 1
+        if not isinstance(req, WFRequest):
+            self.phis[(req.phi_id, req.N)].refs -= 1
 
     # -- batching ------------------------------------------------------------
     def _free_stream(self) -> Optional[int]:
@@ -219,6 +297,13 @@ class Server:
 
     def _launch(self) -> int:
         launched = 0
+        while self.wf_queue and self.wf_running < self.wf_slots:
+            req = self.wf_queue.popleft()
+            req.state = "running"
+            token = self.backend.launch_wf(req, self.phi_source(req.phi_id, req.N))
+            self.inflight.append(_Batch(token, [req], -1, ("wf",)))
+            self.wf_running += 1
+            launched += 1
         while True:
             s = self._free_stream()
             if s is None:
@@ -249,6 +334,20 @@ class Server:
                 still.append(b)
                 continue
             t = self.backend.finish_time(b.token)
+            if b.stream < 0:  # the Wiener-filter lane
+                self.wf_running -= 1
+                r = b.requests[0]
+                self._release(r)
+                err = self.backend.wf_result(b.token, r)
+                if r.state == "cancelling":
+                    r.state, r.out, r.info = "cancelled", None, None
+                elif err is not None:
+                    r.state, r.error = "failed", err
+                else:
+                    r.done, r.state = t, "done"
+                    self.completed_wf.append(r)
+                    finished += 1
+                continue
             for r in b.requests:
                 self._release(r)
                 if r.state == "cancelling":
@@ -269,7 +368,7 @@ class Server:
         return n
 
     def busy(self) -> bool:
-        return bool(self.waiting or self.inflight or any(self.buckets.values()))
+        return bool(self.waiting or self.inflight or self.wf_queue or any(self.buckets.values()))
 
     def drain(self):
         while self.busy():
@@ -309,6 +408,12 @@ class Server:
                         "p99_ms": float(np.percentile(lat, 99) * 1e3),
                         "max_ms": float(lat.max() * 1e3),
                         "maps_per_s": len(lat) / max(last - first, 1e-12)})
+        if self.completed_wf:
+            wl = np.array([r.latency for r in self.completed_wf], dtype=np.float64)
+            out.update({"wf_completed": len(wl), "wf_p50_ms": float(np.percentile(wl, 50) * 1e3),
+                        "wf_p99_ms": float(np.percentile(wl, 99) * 1e3),
+                        "wf_iterations_mean": float(np.mean([max(r.info.iterations)
+                                                             for r in self.completed_wf]))})
         return out
 
 
@@ -344,6 +449,8 @@ class GpuBackend:
         self._fixed = 0
         self._anchor_ev = None
         self._anchor_t = 0.0
+        self._wf_stream = None
+        self.wf_slots = 1
 
     # memory model ---------------------------------------------------------------
     def free_bytes(self) -> Optional[int]:
@@ -436,10 +543,67 @@ class GpuBackend:
         ev.record(stream)
         return ev
 
+    # Wiener-filter lane ---------------------------------------------------------------
+    def wf_bytes(self, req: WFRequest) -> int:
+        """CG vectors (x, r, z, p, b: complex double), the operator's copy of p
+        and the two plans' scratch (FP32 operator + FP64 residual), per
+        right-hand side."""
+        m = getattr(req.data, req.data._matrix_names()[0])
+        nrhs = m.shape[0] if m.dim() == 3 else 1
+        N, C = req.N, req.C
+        P, H = N * N, N * (N // 2 + 1)
+        return nrhs * C * (6 * 16 * H + 10 * 8 * P)
+
+    def launch_wf(self, req: WFRequest, phi: torch.Tensor):
+        """Start the solve as a process (wiener_filter_process) on the lane's
+        own stream; ready() advances it whenever the event it waits on has
+        completed, so the host never blocks on it."""
+        from .fields import FlatS0
+        from .wiener_filter import wiener_filter_process
+        if self._anchor_ev is None:
+            self.start_clock()
+        if self._wf_stream is None:
+            self._wf_stream = torch.cuda.Stream(self.device)
+        self._wf_stream.wait_stream(torch.cuda.current_stream(self.device))
+        Cf, Cn, M, B = req.ops
+        phif = FlatS0(scalar_matrix=phi, theta_pix=self.theta_pix)
+        tok = _WFToken(wiener_filter_process(req.data * 0, phif, req.data, Cf, Cn, M, B, n=req.n,
+                                             tol=0.0, rtol=req.rtol, recompute_every=10,
+                                             maxiter=req.maxiter, return_info=True))
+        self._advance(tok)
+        return tok
+
+    def _advance(self, tok: "_WFToken"):
+        """Run the process until it waits on work that has not finished."""
+        with torch.cuda.device(self.device), torch.cuda.stream(self._wf_stream):
+            while not tok.finished and (tok.event is None or tok.event.query()):
+                try:
+                    tok.event = tok.gen.send(None) if tok.started else next(tok.gen)
+                    tok.started = True
+                except StopIteration as stop:
+                    tok.result = stop.value
+                    tok.finished = True
+                except Exception as e:  # reported on the request; the server keeps running
+                    tok.error = f"{type(e).__name__}: {e}"
+                    tok.finished = True
+                if tok.finished:
+                    tok.end = torch.cuda.Event(enable_timing=True)
+                    tok.end.record(self._wf_stream)
+
+    def wf_result(self, token, req: WFRequest) -> Optional[str]:
+        if token.error is None:
+            req.out, req.info = token.result
+        return token.error
+
     def ready(self, token) -> bool:
+        if isinstance(token, _WFToken):
+            self._advance(token)
+            return token.finished and token.end.query()
         return token.query()
 
     def finish_time(self, token) -> float:
+        if isinstance(token, _WFToken):
+            token = token.end
         return self._anchor_t + self._anchor_ev.elapsed_time(token) * 1e-3
 
     def synchronize(self):

@@ -122,12 +122,36 @@ def A_matrix_operator(field, phi, data, field_covariance, noise_covariance, mask
     return unstack_matrices(field, prob.A(X.to(prob.cdt)), lead)
 
 
-def conjugate_gradient(x0, data, phi, mask, beam, noise_covariance, field_covariance, b_vector,
-                       preconditioner, maxiter: int = 500, tol: float = 1e-1, *,
-                       n: int = DEFAULT_NSTEPS, rtol: Optional[float] = None,
-                       recompute_every: Optional[int] = None, return_info: bool = False):
+def _host(t: torch.Tensor):
+    """Wait point of a solve process: yields an event recorded after the work
+    that produces t, then returns t on the host as float64."""
+    ev = torch.cuda.Event()
+    ev.record(torch.cuda.current_stream(t.device))
+    yield ev
+    return t.cpu().double()
+
+
+def run_process(gen):
+    """Run a solve process to completion on the calling thread."""
+    try:
+        ev = next(gen)
+        while True:
+            ev.synchronize()
+            ev = gen.send(None)
+    except StopIteration as stop:
+        return stop.value
+
+
+def conjugate_gradient_process(x0, data, phi, mask, beam, noise_covariance, field_covariance,
+                               b_vector, preconditioner, maxiter: int = 500, tol: float = 1e-1, *,
+                               n: int = DEFAULT_NSTEPS, rtol: Optional[float] = None,
+                               recompute_every: Optional[int] = None, return_info: bool = False):
     """PCG of wiener_filter.py:51-105, batched over right-hand sides, float64
-    vectors and scalars around a float32 (or float64) operator."""
+    vectors and scalars around a float32 (or float64) operator, written as a
+    process: a generator that enqueues device work and yields a HIP event at
+    every point where the host needs a scalar (alpha, beta, the stop test), so
+    a scheduler can interleave it with other work; ``run_process`` runs it to
+    completion.  Returns what ``conjugate_gradient`` returns."""
     X0, lead = stack_matrices(x0)
     B = X0.shape[0]
     prob = _Problem(x0, phi, field_covariance, noise_covariance, mask, beam, n, B)
@@ -167,15 +191,16 @@ def conjugate_gradient(x0, data, phi, mask, beam, noise_covariance, field_covari
     res = residual(x)                                         # :61-69
     p = z.clone()
     p_op = p.to(prob.cdt).contiguous()
-    bb = prob.dot(bvec, bvec).cpu().double()
-    res_h = res.cpu().double()
+    bb = yield from _host(prob.dot(bvec, bvec))
+    res_h = yield from _host(res)
     res_curr = res_h.clone()
     k = torch.zeros(B, dtype=torch.long)
 
     def rel_now():
-        return torch.sqrt(prob.dot(r, r).cpu().double() / torch.where(bb > 0, bb, torch.ones_like(bb)))
+        rr = yield from _host(prob.dot(r, r))
+        return torch.sqrt(rr / torch.where(bb > 0, bb, torch.ones_like(bb)))
 
-    rel = rel_now() if rtol is not None else None
+    rel = (yield from rel_now()) if rtol is not None else None
 
     def active_mask():
         act = (res_curr >= tol) & (k < maxiter)               # :75-77
@@ -190,14 +215,14 @@ def conjugate_gradient(x0, data, phi, mask, beam, noise_covariance, field_covari
     it = 0
     while bool(act.any()):
         Ap = prob.A(p_op)                                     # :82
-        pAp = prob.dot(p, Ap).cpu().double()
+        pAp = yield from _host(prob.dot(p, Ap))
         alpha = torch.where(act, res_h / torch.where(pAp != 0, pAp, torch.ones_like(pAp)),
                             torch.zeros_like(res_h))          # :84
         rz = torch.empty(B, dtype=torch.float64, device=dev)
         _lib.cg_update_xrz(prob.plan, torch.view_as_real(x), torch.view_as_real(r),
                            torch.view_as_real(z), torch.view_as_real(p), torch.view_as_real(Ap),
                            pinv_arr, alpha.to(dev), B, prob.C, rz)       # :86-92
-        rz_h = rz.cpu().double()
+        rz_h = yield from _host(rz)
         beta = torch.where(act, rz_h / torch.where(res_h != 0, res_h, torch.ones_like(res_h)),
                            torch.zeros_like(res_h))       # :94
         _lib.cg_update_p(prob.plan, torch.view_as_real(p), torch.view_as_real(z), beta.to(dev),
@@ -207,19 +232,19 @@ def conjugate_gradient(x0, data, phi, mask, beam, noise_covariance, field_covari
         k += act.long()
         it += 1
         if recompute_every and it % recompute_every == 0:
-            res_new = residual(x).cpu().double()
+            res_new = yield from _host(residual(x))
             res_curr = torch.where(act, res_new, res_curr)
             res_h = torch.where(act, res_new, res_h)
         if rtol is not None:
-            rel = rel_now()
+            rel = yield from rel_now()
             # before a right-hand side stops on the relative test, replace its
             # recursive residual by the recomputed b - A x (FP64 arithmetic);
             # it stops only if that one passes too
             if bool((act & (rel < rtol)).any()):
-                res_new = residual(x).cpu().double()
+                res_new = yield from _host(residual(x))
                 res_curr = torch.where(act, res_new, res_curr)
                 res_h = torch.where(act, res_new, res_h)
-                rel = rel_now()
+                rel = yield from rel_now()
         for i in range(B):
             if bool(act[i]):
                 info.res_history[i].append(float(res_curr[i]))
@@ -234,9 +259,20 @@ def conjugate_gradient(x0, data, phi, mask, beam, noise_covariance, field_covari
     info.converged = [bool(v) for v in conv]
     if return_info:
         residual(x)
-        info.final_true_rel_residual = [float(v) for v in rel_now()]
+        rel_final = yield from rel_now()
+        info.final_true_rel_residual = [float(v) for v in rel_final]
     out = unstack_matrices(x0, x.to(X0.dtype if X0.is_complex() else torch.complex128), lead)
     return (out, info) if return_info else out
+
+
+def conjugate_gradient(x0, data, phi, mask, beam, noise_covariance, field_covariance, b_vector,
+                       preconditioner, maxiter: int = 500, tol: float = 1e-1, *,
+                       n: int = DEFAULT_NSTEPS, rtol: Optional[float] = None,
+                       recompute_every: Optional[int] = None, return_info: bool = False):
+    """PCG of wiener_filter.py:51-105 (see conjugate_gradient_process)."""
+    return run_process(conjugate_gradient_process(
+        x0, data, phi, mask, beam, noise_covariance, field_covariance, b_vector, preconditioner,
+        maxiter, tol, n=n, rtol=rtol, recompute_every=recompute_every, return_info=return_info))
 
 
 def wiener_filter(field, phi, data, field_covariance, noise_covariance, mask, beam, *,
@@ -245,8 +281,18 @@ def wiener_filter(field, phi, data, field_covariance, noise_covariance, mask, be
                   return_info: bool = False):
     """wiener_filter.py:10-26: b = -gradf(0, d), P = Cf^-1 + B M Cn^-1 M B,
     PCG from `field`."""
+    return run_process(wiener_filter_process(
+        field, phi, data, field_covariance, noise_covariance, mask, beam, n=n, maxiter=maxiter,
+        tol=tol, rtol=rtol, recompute_every=recompute_every, return_info=return_info))
+
+
+def wiener_filter_process(field, phi, data, field_covariance, noise_covariance, mask, beam, *,
+                          n: int = DEFAULT_NSTEPS, maxiter: int = 500, tol: float = 1e-1,
+                          rtol: Optional[float] = None, recompute_every: Optional[int] = None,
+                          return_info: bool = False):
+    """wiener_filter as a process (see conjugate_gradient_process)."""
     b = get_b_vector(phi, data, field_covariance, noise_covariance, mask, beam, n=n, like=field)
     P = hessian_logpdf_preconditioner(field_covariance, beam, mask, noise_covariance)
-    return conjugate_gradient(field, data, phi, mask, beam, noise_covariance, field_covariance, b, P,
-                              maxiter, tol, n=n, rtol=rtol, recompute_every=recompute_every,
-                              return_info=return_info)
+    return (yield from conjugate_gradient_process(
+        field, data, phi, mask, beam, noise_covariance, field_covariance, b, P, maxiter, tol, n=n,
+        rtol=rtol, recompute_every=recompute_every, return_info=return_info))

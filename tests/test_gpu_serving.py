@@ -163,3 +163,50 @@ def test_server_small_budget(pkg, srv, sims, dev):
                                         r.direction, False)
         assert torch.equal(r.out, alone[0])
     be.close()
+
+
+def test_server_wiener_filter_requests(pkg, srv, sims, dev):
+    """Wiener-filter requests admitted beside a LenseFlow stream: each solve
+    runs in the Wiener-filter lane (worker thread, own stream), converges to
+    the 1e-6 true residual and equals the same solve run directly; the
+    LenseFlow requests still equal their plain results bit for bit."""
+    from cmblensing_amd import matrix_operators as mo, wiener_filter as wfm
+    _case("server with Wiener-filter requests N=64 QU")
+    N, th = 64, TH
+    sizes = (64, 128)
+    reqs, phi_maps = _mixed_requests(srv, sims, dev, 80, sizes, (0.5, 0.5), 2, seed=11)
+    wf_reqs, direct = [], []
+    for k in range(2):
+        prob = sims.wf_problem(N, th, pol=True, seed=30 + k, mask="fourier")
+        fq = np.fft.irfft2(np.stack(olf.eb2qu(prob["f"][0], prob["f"][1], N, th)))
+        lensed = olf.lense_flow(fq, phi_maps[(k, N)].cpu().numpy(), olf.pix_width_from_theta(th),
+                                NSTEPS)
+        prob = sims.wf_data(prob, lensed)
+        T = lambda a, dt=torch.complex64: torch.as_tensor(a, device=dev).to(dt)
+        op = lambda A: mo.DiagonalEB(matrix_EE=T(A[0], torch.float64),
+                                     matrix_BB=T(A[1], torch.float64), theta_pix=th)
+        data = pkg.FlatS2(polar_matrix_1=T(prob["d"][0]), polar_matrix_2=T(prob["d"][1]),
+                          theta_pix=th)
+        ops = (op(prob["Cf"]), op(prob["Cn"]), op(prob["M"]), op(prob["B"]))
+        wf_reqs.append(srv.WFRequest(data=data, phi_id=k, ops=ops, n=NSTEPS, rtol=1e-6))
+        phif = pkg.FlatS0(scalar_matrix=phi_maps[(k, N)], theta_pix=th)
+        direct.append(wfm.wiener_filter(data * 0, phif, data, *ops, n=NSTEPS, tol=0.0, rtol=1e-6,
+                                        recompute_every=10, maxiter=500))
+    be = srv.GpuBackend(dev, sizes=sizes, nstreams=2, max_batch={64: 16, 128: 16})
+    server = srv.Server(be, lambda k, N_: phi_maps[(k, N_)], hbm_budget=8 * srv.GiB)
+    allreq = reqs[:40] + wf_reqs + reqs[40:]
+    offs = srv.poisson_schedule(len(allreq), rate=5000.0, seed=12)
+    st = server.serve(allreq, list(offs))
+    torch.cuda.synchronize()
+    print(f"[stats] {st}", flush=True)
+    assert st["wf_completed"] == 2 and st["completed"] == len(reqs)
+    for w, x in zip(wf_reqs, direct):
+        assert w.state == "done" and all(w.info.converged)
+        assert max(w.info.final_true_rel_residual) < 1e-6
+        for nm in ("polar_matrix_1", "polar_matrix_2"):
+            assert torch.equal(getattr(w.out, nm), getattr(x, nm))
+    for r in reqs:
+        alone = pkg.lense_flow_matrices(r.x[None], phi_maps[(r.phi_id, r.N)], TH, NSTEPS,
+                                        r.direction, False)
+        assert torch.equal(r.out, alone[0])
+    be.close()

@@ -85,6 +85,24 @@ class HostBackend:
         import time
         return time.perf_counter()
 
+    # Wiener-filter lane: "solves" by scaling the data by 3
+    wf_slots = 1
+
+    def wf_bytes(self, req):
+        return 4096
+
+    def launch_wf(self, req, phi):
+        self.running += 1
+        self.concurrent_max = max(self.concurrent_max, self.running)
+        tok = {"polls": self.latency_polls, "reqs": [req], "stream": -1, "wf": True}
+        self.launches.append(tok)
+        return tok
+
+    def wf_result(self, tok, req):
+        req.out = req.data * 3
+        req.info = type("Info", (), {"iterations": [7]})()
+        return None
+
 
 def _req(srv, N, C, phi, d=FWD, seed=0):
     g = torch.Generator().manual_seed(seed)
@@ -243,3 +261,51 @@ def test_mixed_workload_distribution(srv):
 def test_batch_class(srv):
     assert [srv._batch_class(c, 128) for c in (1, 2, 3, 5, 64, 65, 128)] == [1, 2, 4, 8, 64, 128, 128]
     assert srv._batch_class(20, 16) == 16
+
+
+class _Field:
+    """Minimal stand-in for a Fourier field (matrix names, scaling)."""
+
+    def __init__(self, t):
+        self.t = t
+
+    def _matrix_names(self):
+        return ["t"]
+
+    def __mul__(self, k):
+        return _Field(self.t * k)
+
+
+def test_wiener_filter_requests_share_admission_and_overlap(srv):
+    """WF requests pass the same budget, run one at a time in their own lane,
+    and LenseFlow batches keep launching while one runs."""
+    be = HostBackend(nstreams=1, latency_polls=4)
+    s = srv.Server(be, _phi_source, hbm_budget=10 ** 9, max_inflight=1)
+    F = _Field
+    w1 = s.submit(srv.WFRequest(data=F(torch.ones(2, 16, 9)), phi_id=0, ops=(None,) * 4))
+    w2 = s.submit(srv.WFRequest(data=F(torch.ones(2, 16, 9) * 2), phi_id=1, ops=(None,) * 4))
+    lf = [s.submit(_req(srv, 16, 1, 0, FWD, i)) for i in range(3)]
+    s.step()
+    assert w1.state == "running" and w2.state == "queued"      # one WF at a time
+    assert any(r.state == "running" for r in lf)               # LenseFlow batch alongside
+    assert s.used == be.fixed + 2 * 4096 + 5000 + 3 * 2048
+    s.drain()
+    assert w1.state == w2.state == "done"
+    assert torch.equal(w1.out.t, torch.ones(2, 16, 9) * 3)
+    assert all(r.state == "done" for r in lf)
+    st = s.stats()
+    assert st["wf_completed"] == 2 and st["completed"] == 3
+    assert s.used == be.fixed + 5000
+
+
+def test_cancel_queued_wiener_filter(srv):
+    be = HostBackend(nstreams=1, latency_polls=3)
+    s = srv.Server(be, _phi_source, hbm_budget=10 ** 9)
+    F = _Field
+    a = s.submit(srv.WFRequest(data=F(torch.ones(1, 16, 9)), phi_id=0, ops=(None,) * 4))
+    b = s.submit(srv.WFRequest(data=F(torch.ones(1, 16, 9)), phi_id=0, ops=(None,) * 4))
+    s.step()
+    assert s.cancel(b.rid) and b.state == "cancelled"
+    s.drain()
+    assert a.state == "done" and b.out is None
+    assert s.used == be.fixed
