@@ -70,12 +70,16 @@ def cpu_baseline(f, phi, th, n, budget_s=10.0):
     return k / el, k, el
 
 
-def serving_run(torch, sims, dev, n_requests, rate, seed=2, nphi=32, budget_gib=64.0):
+def serving_run(torch, sims, dev, n_requests, rate, seed=2, nphi=32, budget_gib=64.0,
+                wf_every=0, pkg=None):
     """The mixed serving stream: Poisson arrivals at `rate`, N in {64, 128,
     256, 512} with weights {.2, .4, .3, .1}, T/QU and lense/unlense 50/50,
     phi ~ U{0..31} of 32 seeded potentials per N, float32, 2 arcmin pixels,
     7 RK4 steps, a 64 GiB HBM admission budget.  Inputs are resident in HBM
-    (a pool of 8 seeded maps per (N, C)); phi operators start cold."""
+    (a pool of 8 seeded maps per (N, C)); phi operators start cold.  With
+    wf_every = k, every k-th arrival is instead a lensed Wiener-filter
+    request (Q/U at N = 64 or 128, Fourier-diagonal mask, FP32 operator with
+    FP64 CG to a 1e-6 relative residual) on one of 4 seeded problems."""
     import importlib
     srv = importlib.import_module("cmblensing_amd.serving")
     th, n = 2.0, 7
@@ -95,6 +99,31 @@ def serving_run(torch, sims, dev, n_requests, rate, seed=2, nphi=32, budget_gib=
                                                device=dev)
     reqs = [srv.Request(x=pool[(N, C)][i % 8], phi_id=phi, direction=d)
             for i, (N, C, d, phi) in enumerate(desc)]
+    if wf_every:
+        mo = importlib.import_module("cmblensing_amd.matrix_operators")
+        wf_pool = []
+        for k, N in enumerate((64, 64, 128, 128)):
+            pr = sims.wf_problem(N, th, pol=True, seed=60 + k, mask="fourier")
+            pr["phi"] = phi_maps[(k, N)].cpu().numpy()      # the served phi lenses the truth
+            pr = sims.wf_data(pr, _lensed_truth(torch, pkg, dev, pr, n))
+            T = lambda a, dt=torch.complex64: torch.as_tensor(a, device=dev).to(dt)
+            op = lambda A: mo.DiagonalEB(matrix_EE=T(A[0], torch.float64),
+                                         matrix_BB=T(A[1], torch.float64), theta_pix=th)
+            data = pkg.FlatS2(polar_matrix_1=T(pr["d"][0]), polar_matrix_2=T(pr["d"][1]),
+                              theta_pix=th)
+            wf_pool.append((k, data, (op(pr["Cf"]), op(pr["Cn"]), op(pr["M"]), op(pr["B"]))))
+        for i in range(wf_every - 1, n_requests, wf_every):
+            k, data, ops = wf_pool[(i // wf_every) % 4]
+            reqs[i] = srv.WFRequest(data=data, phi_id=k, ops=ops, n=n, rtol=1e-6)
+        # warm the Wiener-filter lane before traffic (plans, operator arrays,
+        # phi tables and torch's kernels loaded on first use), as the graphs are
+        wfm = importlib.import_module("cmblensing_amd.wiener_filter")
+        fields = importlib.import_module("cmblensing_amd.fields")
+        for k, data, ops in wf_pool:
+            N = data.polar_matrix_1.shape[-2]
+            wfm.wiener_filter(data * 0, fields.FlatS0(scalar_matrix=phi_maps[(k, N)], theta_pix=th),
+                              data, *ops, n=n, tol=0.0, rtol=1e-6, recompute_every=10, maxiter=500)
+        torch.cuda.synchronize()
     be = srv.GpuBackend(dev, theta_pix=th, nsteps=n, nstreams=2, sizes=sizes)
     t0 = time.perf_counter()
     be.warmup()
@@ -106,7 +135,8 @@ def serving_run(torch, sims, dev, n_requests, rate, seed=2, nphi=32, budget_gib=
     torch.cuda.synchronize()
     per_n = {}
     for N in sizes:
-        lat = np.array([r.latency for r in reqs if r.N == N and r.state == "done"])
+        lat = np.array([r.latency for r in reqs
+                        if isinstance(r, srv.Request) and r.N == N and r.state == "done"])
         if len(lat):
             per_n[str(N)] = {"p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
                              "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
@@ -120,6 +150,9 @@ def serving_run(torch, sims, dev, n_requests, rate, seed=2, nphi=32, budget_gib=
            "hbm_budget_gib": round(st["hbm_budget_bytes"] / srv.GiB, 2),
            "hbm_peak_gib": round(st["hbm_peak_bytes"] / srv.GiB, 3),
            "graph_warmup_s": round(warm_s, 2), "per_nside": per_n}
+    if wf_every:
+        out.update({k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()
+                     if k.startswith("wf_")})
     be.close()
     return out
 
@@ -294,6 +327,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-wf-every", type=int, default=200,
+                    help="second serving run (2000 requests): every k-th request a Wiener "
+                         "filter (0: skip)")
     ap.add_argument("--no-wf", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -467,6 +503,11 @@ def main():
         result["serving"] = serving_run(torch, sims, dev, args.serve_requests, args.serve_rate)
         result["latency_ms"] = {"p50": result["serving"]["p50_ms"],
                                 "p99": result["serving"]["p99_ms"]}
+        if args.serve_wf_every:
+            print("[bench] serving with Wiener-filter requests", file=sys.stderr, flush=True)
+            result["serving_with_wf"] = serving_run(torch, sims, dev, 2000, args.serve_rate,
+                                                    seed=3, wf_every=args.serve_wf_every,
+                                                    pkg=pkg)
     if not args.no_grad:
         print("[bench] gradients, 16 chains", file=sys.stderr, flush=True)
         result["differentiation"] = grad_run(torch, pkg, sims, dev)
