@@ -19,6 +19,10 @@ HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 ops are disabled for all device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # multiply-adds fused only within one source expression, so every
+    # instantiation of a kernel (launch shapes, batch sizes) rounds alike and
+    # a request's result does not depend on the batch it runs in
+    "-ffp-contract=on",
 ]
 
 

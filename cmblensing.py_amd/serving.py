@@ -159,8 +159,13 @@ class Server:
     or with serve() for a timed arrival schedule."""
 
     def __init__(self, backend, phi_source: Callable[[int, int], torch.Tensor],
-                 hbm_budget: int = 64 * GiB, max_inflight: int = 2):
+                 hbm_budget: int = 64 * GiB, max_inflight: int = 2, min_det: float = 0.0):
         self.backend = backend
+        # phi whose magnification determinant det(I + t Hess phi) falls to
+        # min_det or below somewhere in t in [0, 1] (a caustic: M(t) is
+        # singular and the flow is not defined) are refused
+        self.min_det = min_det
+        self.bad_phi: Dict[tuple, float] = {}
         self.phi_source = phi_source
         free = backend.free_bytes()
         self.budget = int(min(hbm_budget, free)) if free is not None else int(hbm_budget)
@@ -257,6 +262,11 @@ class Server:
             if self.used + need > self.budget and not self._evict_for(need, (req.phi_id, req.N)):
                 return  # wait for running batches to free memory
             self.waiting.popleft()
+            kb = (req.phi_id, req.N)
+            if kb in self.bad_phi:
+                req.state = "rejected"
+                req.error = f"phi {req.phi_id} has a caustic (min det {self.bad_phi[kb]:.3g})"
+                continue
             if isinstance(req, WFRequest):
                 self.used += req.nbytes
                 self.peak_used = max(self.peak_used, self.used)
@@ -268,8 +278,15 @@ class Server:
             if ent is None:
                 self.misses += 1
                 nb = self.backend.phi_bytes(req.N)
-                ent = _PhiEntry(self.backend.prepare_phi(req.N, self.phi_source(req.phi_id, req.N)),
-                                nb)
+                handle = self.backend.prepare_phi(req.N, self.phi_source(req.phi_id, req.N))
+                md = getattr(handle, "min_det", None)
+                if md is not None and md <= self.min_det:
+                    self.backend.release_phi(handle)
+                    self.bad_phi[k] = md
+                    req.state = "rejected"
+                    req.error = f"phi {req.phi_id} has a caustic (min det {md:.3g})"
+                    continue
+                ent = _PhiEntry(handle, nb)
                 self.phis[k] = ent
                 self.used += nb
             else:

@@ -121,3 +121,49 @@ def test_julia_fixture_T_fp64(pkg, dev):
     adj = pkg.lense_flow(field, phif, n=7, adjoint=True).scalar_matrix.cpu().numpy()
     assert rel(out, tl) < 1e-11
     assert rel(adj, ta) < 1e-11
+
+
+@pytest.mark.parametrize("N", [16, 32, 64, 512])
+@pytest.mark.parametrize("C", [1, 2])
+@pytest.mark.parametrize("dtype,tol", [(torch.float32, 1e-5), (torch.float64, 1e-11)])
+def test_sizes_components_steps(pkg, dev, sims, N, C, dtype, tol):
+    """Every supported size beyond the two configurations (the smallest and
+    the largest N), T and Q/U, forward with 7 steps and the adjoint with a
+    single step, against the FP64 oracle."""
+    th = 2.0
+    _case(f"sweep N={N} C={C} {dtype}")
+    f = (sims.t_map if C == 1 else sims.qu_map)(N, th, 3)
+    phi = sims.phi_map(N, th, 4)
+    x = torch.as_tensor(f, dtype=dtype, device=dev)[None]
+    P = torch.as_tensor(phi, device=dev)
+    for n, adjoint in ((7, False), (1, True)):
+        y = pkg.lense_flow_matrices(x, P, th, n, 1, adjoint)[0].double().cpu().numpy()
+        ref = olf.lense_flow(f, phi, olf.pix_width_from_theta(th), n, 1, adjoint)
+        assert rel(y, ref) < tol, (n, adjoint)
+
+
+def test_empty_batch_is_a_no_op(pkg, dev):
+    from cmblensing_amd import _lib
+    _case("empty batch")
+    plan = _lib.get_plan(64, 2.0, torch.float32, 4, dev)
+    x = torch.zeros((1, 2, 64, 64), device=dev)
+    out = torch.full_like(x, 3.0)
+    _lib.lenseflow(plan, [], x, out, 0, 2, 7, 1, False)
+    torch.cuda.synchronize()
+    assert torch.all(out == 3.0)
+
+
+@pytest.mark.parametrize("adjoint", [False, True])
+def test_small_and_large_launch_shapes_agree(pkg, dev, sims, adjoint):
+    """A lone request runs with the small-grid launch shapes (64-thread row
+    blocks, 4 columns per column block), the same request inside a batch of 64
+    with the default shapes: the results are identical."""
+    N, th, n = 256, 2.0, 7
+    _case(f"launch shapes N=256 QU adjoint={adjoint}")
+    fs = np.stack([sims.qu_map(N, th, s % 8) for s in range(64)])
+    x = torch.as_tensor(fs, dtype=torch.float32, device=dev)
+    P = torch.as_tensor(sims.phi_map(N, th, 9), device=dev)
+    yb = pkg.lense_flow_matrices(x, P, th, n, 1, adjoint)
+    for s in (0, 37):
+        ys = pkg.lense_flow_matrices(x[s:s + 1].contiguous(), P, th, n, 1, adjoint)
+        assert torch.equal(ys[0], yb[s])

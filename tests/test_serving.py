@@ -309,3 +309,27 @@ def test_cancel_queued_wiener_filter(srv):
     s.drain()
     assert a.state == "done" and b.out is None
     assert s.used == be.fixed
+
+
+def test_caustic_phi_rejected(srv):
+    """A phi whose magnification determinant reaches 0 is refused: its
+    requests are rejected (prepared once, not kept), others proceed."""
+    class Handle(tuple):
+        pass
+
+    class CausticBackend(HostBackend):
+        def prepare_phi(self, N, phi):
+            h = Handle(("phi", float(phi), N))
+            h.min_det = -0.3 if float(phi) == 1.0 else 0.8
+            self.prepared.append(h)
+            return h
+
+    be = CausticBackend()
+    s = srv.Server(be, _phi_source, hbm_budget=10 ** 9)
+    bad = [s.submit(_req(srv, 16, 1, 1, FWD, i)) for i in range(3)]
+    good = s.submit(_req(srv, 16, 1, 0, FWD, 9))
+    s.drain()
+    assert all(r.state == "rejected" and "caustic" in r.error for r in bad)
+    assert good.state == "done"
+    assert len([h for h in be.prepared if h[1] == 1.0]) == 1 and len(be.released) == 1
+    assert s.used == be.fixed + 5000
