@@ -34,6 +34,9 @@ NAMES = [
 ] + [f"{op}_{blk}_IP" for op in ("cn", "cf", "m", "b") for blk in ("tt", "te", "ee", "bb")] + [
     "gradf_t_IP", "gradf_e_IP", "gradf_b_IP", "grad_phi_t_IP",
     "f_wf_ffpp_t_IP", "f_wf_ffpp_e_IP", "f_wf_ffpp_b_IP",
+    # mixed parametrisation (mixing.py, gradients.py:40-85)
+    "d_I", "g_I", "d_ee_P", "d_bb_P", "d_tt_IP", "d_te_IP", "d_ee_IP", "d_bb_IP",
+    "mixed_grad_phi_I", "mixed_grad_phi_P", "mixed_grad_phi_IP",
 ]
 TEXT = ["logpdf_I.txt", "logpdf_P.txt", "logpdf_IP.txt", "n_side.txt", "theta_pix.txt"]
 
