@@ -64,3 +64,27 @@ def mixed_grad_phi_logpdf(fm, phim, d, Cn, Cphi, Cf, M, B, D, G, theta_pix, n=10
     gphi = w.grad_phi_logpdf(f, phi, d, Cn, Cphi, Cf, M, B, theta_pix, n)
     gf = w.gradf_logpdf(f, phi, d, Cf, Cn, M, B, theta_pix, n)
     return w.pinv(G) * (gphi + mixing_jacobian_phi(fm, phi, gf, D, theta_pix, n))
+
+
+def symplectic_integrate(x0, p0, fm, d, Cn, Cphi, Cf, M, B, D, G, mass, theta_pix, n=10,
+                         num_steps=30, step_size=0.01):
+    """sampling_ar.py:105-140 (leapfrog on the mixed phi; mass is a diagonal
+    array).  Returns (delta_h, x, p)."""
+    N = fm.shape[-2]
+    minv = w.pinv(mass)
+
+    def grad(x):
+        return mixed_grad_phi_logpdf(fm, x, d, Cn, Cphi, Cf, M, B, D, G, theta_pix, n)
+
+    def hamiltonian(x, p):
+        lp = mixed_logpdf(fm, x, d, Cn, Cphi, Cf, M, B, G, D, theta_pix, n)
+        return lp - lf.primal_dot(p, minv * p, N) / 2
+
+    g = grad(x0)
+    x, p = x0, p0
+    for _ in range(num_steps):
+        g_prev = g
+        x = x - step_size * minv * (p - 0.5 * step_size * g)
+        g = grad(x)
+        p = p - 0.5 * step_size * (g + g_prev)
+    return hamiltonian(x, p) - hamiltonian(x0, p0), x, p

@@ -1,0 +1,146 @@
+"""GPU parity of the mixed parametrisation and the HMC phi sampler against the
+FP64 oracle (oracle/mixing.py) and CMBLensing.jl's mixed gradient fixtures.
+Tolerances: FP64 GPU vs oracle 1e-9 relative L2 (gradients, leapfrog
+trajectory) and 1e-9 relative (delta H); Julia fixtures as pinned for the
+oracle (1e-3)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import lenseflow as olf
+from oracle import mixing as omx
+from oracle import wiener as owf
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+J = GOLDEN / "julia"
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def _case(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def L(n):
+    return np.load(J / f"{n}.npy", allow_pickle=False)
+
+
+@pytest.mark.parametrize("kind", ["I", "P"])
+def test_mixed_grad_phi_julia(pkg, dev, kind):
+    from cmblensing_amd import matrix_operators as mo, mixing
+    _case(f"mixed grad phi julia {kind} fp64")
+    T = lambda a: torch.as_tensor(a, device=dev)
+    S = lambda a: pkg.FlatS0(scalar_matrix=T(a), theta_pix=2.5)
+    DS = lambda a: mo.DiagonalScalar(scalar_matrix=T(a), theta_pix=2.5)
+    G = L("g_I")
+    if kind == "I":
+        f, d = S(L("t_field_I")), S(L("data_t_field_I"))
+        ops = {p: L(f"{p}_I") for p in ("cn", "cf", "m", "b", "d")}
+        gop = {p: DS(v) for p, v in ops.items()}
+        arr = {p: v[None] for p, v in ops.items()}
+        fa, da = L("t_field_I")[None], L("data_t_field_I")[None]
+    else:
+        P2 = lambda e, b: pkg.FlatS2(polar_matrix_1=T(L(e)), polar_matrix_2=T(L(b)), theta_pix=2.5)
+        f, d = P2("e_field_P", "b_field_P"), P2("data_e_field_P", "data_b_field_P")
+        gop = {p: mo.DiagonalEB(matrix_EE=T(L(f"{p}_ee_P")), matrix_BB=T(L(f"{p}_bb_P")),
+                                theta_pix=2.5) for p in ("cn", "cf", "m", "b", "d")}
+        arr = {p: np.stack([L(f"{p}_ee_P"), L(f"{p}_bb_P")]) for p in ("cn", "cf", "m", "b", "d")}
+        fa = np.stack([L("e_field_P"), L("b_field_P")])
+        da = np.stack([L("data_e_field_P"), L("data_b_field_P")])
+    phi, cphi = S(L(f"phi_{kind}")), DS(L(f"cphi_{kind}"))
+    fm, pm = mixing.mix(f, phi, gop["d"], DS(G), n=7)
+    g = mixing.mixed_grad_phi_logpdf(fm, pm, d, gop["cn"], cphi, gop["cf"], gop["m"], gop["b"],
+                                     gop["d"], DS(G), n=7).scalar_matrix.cpu().numpy()
+    fma, pma = omx.mix(fa, L(f"phi_{kind}"), arr["d"], G, 2.5, 7)
+    ref = omx.mixed_grad_phi_logpdf(fma, pma, da, arr["cn"], L(f"cphi_{kind}"), arr["cf"],
+                                    arr["m"], arr["b"], arr["d"], G, 2.5, 7)
+    assert rel(g, ref) < 1e-9
+    assert rel(g, L(f"mixed_grad_phi_{kind}")) < 1e-3
+
+
+def _small_problem(pkg, dev, sims, N=64, th=2.0, chains=3):
+    from cmblensing_amd import matrix_operators as mo
+    prob = sims.wf_problem(N, th, pol=False, seed=5, mask="fourier")
+    phim, Cphi = sims.phi_map_and_cov(N, th, 6)
+    phi = np.fft.rfft2(phim)
+    prob = sims.wf_data(prob, olf.lense_flow(owf.to_map(prob["f"]), phim,
+                                             olf.pix_width_from_theta(th), 7))
+    D = np.sqrt(prob["Cf"] / (prob["Cf"] + prob["Cn"]))
+    G = np.ones_like(Cphi)
+    T = lambda a: torch.as_tensor(a, device=dev)
+    DS = lambda a: mo.DiagonalScalar(scalar_matrix=T(a), theta_pix=th)
+    phis = np.stack([phi * (1 + 0.02 * k) for k in range(chains)])
+    gpu = dict(f=pkg.FlatS0(scalar_matrix=T(np.stack([prob["f"][0]] * chains)), theta_pix=th),
+               d=pkg.FlatS0(scalar_matrix=T(prob["d"][0]), theta_pix=th),
+               phi=pkg.FlatS0(scalar_matrix=T(phis), theta_pix=th),
+               Cn=DS(prob["Cn"][0]), Cf=DS(prob["Cf"][0]), M=DS(prob["M"][0]),
+               B=DS(prob["B"][0]), D=DS(D[0]), G=DS(G), Cphi=DS(Cphi),
+               Nphi=DS(Cphi * 1e-2))
+    ora = dict(f=prob["f"], d=prob["d"], phis=phis, Cn=prob["Cn"], Cf=prob["Cf"], M=prob["M"],
+               B=prob["B"], D=D, G=G, Cphi=Cphi)
+    return gpu, ora, th
+
+
+def test_hmc_leapfrog_matches_oracle(pkg, dev, sims):
+    """Three chains of the leapfrog integrator from the same momenta: each
+    chain's end point, momentum and delta H equal the oracle's.  The gradient
+    is sensitive to phi (a 1e-15 change of x moves it by ~1e-9), so the
+    trajectories drift apart by ~100x per step at the rounding level: one step
+    is compared at 1e-12, two steps at 1e-7."""
+    from cmblensing_amd import mixing, sampling
+    _case("hmc leapfrog 3 chains N=64 fp64")
+    g, o, th = _small_problem(pkg, dev, sims)
+    n, eps = 7, 0.01
+    fm, pm = mixing.mix(g["f"], g["phi"], g["D"], g["G"], n=n)
+    # the reference's mass matrix uses the quadratic-estimator noise Nphi; a
+    # posterior-like precision (Nphi = Cphi / 100) keeps the leapfrog stable
+    mass = sampling.get_mass_matrix(g["Cphi"], g["Nphi"], g["G"])
+    mass_a = (1 / o["G"]) ** 2 * (1 / o["Cphi"] + 1 / (o["Cphi"] * 1e-2))
+    gen = torch.Generator(device=dev).manual_seed(11)
+    p0 = sampling.draw_momentum(mass, pm, gen)
+    # delta H is a difference of Hamiltonians of size |H| ~ |logpdf|: compared
+    # at the precision the logpdf itself is reproduced (tol_dh of |logpdf|)
+    for K, tol_x, tol_dh in ((1, 1e-12, 1e-11), (2, 1e-7, 1e-9)):
+        dh, x, p = sampling.symplectic_integrate(pm, p0, fm, g["d"], g["Cn"], g["Cphi"], g["Cf"],
+                                                 g["M"], g["B"], g["D"], g["G"], mass,
+                                                 num_steps=K, step_size=eps, n=n)
+        dh = np.asarray(torch.as_tensor(dh).cpu())
+        for c in range(3):
+            fma, pma = omx.mix(o["f"], o["phis"][c], o["D"], o["G"], th, n)
+            dh_ref, x_ref, p_ref = omx.symplectic_integrate(
+                pma, p0.scalar_matrix[c].cpu().numpy(), fma, o["d"], o["Cn"], o["Cphi"], o["Cf"],
+                o["M"], o["B"], o["D"], o["G"], mass_a, th, n, num_steps=K, step_size=eps)
+            assert np.isfinite(x_ref).all()
+            assert rel(x.scalar_matrix[c].cpu().numpy(), x_ref) < tol_x, (K, c)
+            assert rel(p.scalar_matrix[c].cpu().numpy(), p_ref) < 10 * tol_x, (K, c)
+            lp0 = omx.mixed_logpdf(fma, pma, o["d"], o["Cn"], o["Cphi"], o["Cf"], o["M"], o["B"],
+                                   o["G"], o["D"], th, n)
+            assert abs(dh[c] - dh_ref) < tol_dh * abs(lp0), (K, c, dh[c], dh_ref, lp0)
+
+
+def test_hmc_step_accept_and_chains_independent(pkg, dev, sims):
+    """always_accept takes every proposal; a chain's proposal is the same run
+    alone or among the other chains (bit for bit)."""
+    from cmblensing_amd import mixing, sampling
+    _case("hmc step accept / chain independence")
+    g, o, th = _small_problem(pkg, dev, sims)
+    n = 7
+    fm, pm = mixing.mix(g["f"], g["phi"], g["D"], g["G"], n=n)
+    mass = sampling.get_mass_matrix(g["Cphi"], g["Nphi"], g["G"])
+    args = (fm, g["d"], g["Cn"], g["Cphi"], g["Cf"], g["M"], g["B"], g["D"], g["G"])
+    gen = torch.Generator(device=dev).manual_seed(3)
+    p0 = sampling.draw_momentum(mass, pm, gen)
+    _, xb, _ = sampling.symplectic_integrate(pm, p0, *args, mass, num_steps=3, n=n)
+    c = 1
+    one = lambda F: F.replace(**{nm: getattr(F, nm)[c:c + 1] for nm in F._matrix_names()})
+    args1 = (one(fm),) + args[1:]
+    _, x1, _ = sampling.symplectic_integrate(one(pm), one(p0), *args1, mass, num_steps=3, n=n)
+    assert torch.equal(x1.scalar_matrix[0], xb.scalar_matrix[c])
+    gen = torch.Generator(device=dev).manual_seed(4)
+    x_new, dh, acc = sampling.hmc_step(pm, True, mass, *args, gen, num_steps=2, n=n)
+    assert bool(acc.all()) and dh.shape == (3,)
+    assert not torch.equal(x_new.scalar_matrix, pm.scalar_matrix)
