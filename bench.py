@@ -291,6 +291,53 @@ def grad_run(torch, pkg, sims, dev, N=256, th=2.0, chains=16, n=7, reps=5):
             "rel_l2_grad_phi_vs_fp64": rel(gp.scalar_matrix, gp64.scalar_matrix)}
 
 
+def hmc_run(torch, pkg, sims, dev, N=256, th=2.0, chains=16, n=7, num_steps=30,
+            step_size=1e-3):
+    """One HMC proposal (30 leapfrog steps) for 16 chains of the mixed-
+    parametrisation phi posterior, T at N = 256, in FP64 like the reference
+    (the accept test compares Hamiltonians of size |logpdf|): 31 mixed phi
+    gradients and two Hamiltonians per chain.  Mass matrix G^-2 (Cphi^-1 +
+    Nphi^-1) with Nphi = Cphi / 100 standing in for the quadratic-estimator
+    noise; the step size is tuned to this posterior (0.01, the reference's
+    default, diverges here)."""
+    import importlib
+    mixing = importlib.import_module("cmblensing_amd.mixing")
+    sampling = importlib.import_module("cmblensing_amd.sampling")
+    mo = importlib.import_module("cmblensing_amd.matrix_operators")
+    pr = sims.wf_problem(N, th, pol=False, seed=41, mask="fourier")
+    phim, Cphi = sims.phi_map_and_cov(N, th, 42)
+    f0 = pkg.FlatS0(scalar_matrix=torch.as_tensor(pr["f"][0], device=dev), theta_pix=th)
+    y = pkg.lense_flow_matrices(pkg.map(f0).scalar_matrix[None, None],
+                                torch.as_tensor(phim, device=dev), th, n, 1, False)
+    pr = sims.wf_data(pr, y[0].cpu().numpy())
+    T = lambda a: torch.as_tensor(a, device=dev).to(torch.complex128)
+    R = lambda a: mo.DiagonalScalar(scalar_matrix=torch.as_tensor(a, device=dev), theta_pix=th)
+    D = np.sqrt(pr["Cf"] / (pr["Cf"] + pr["Cn"]))
+    f = pkg.FlatS0(scalar_matrix=T(np.stack([pr["f"][0]] * chains)), theta_pix=th)
+    d = pkg.FlatS0(scalar_matrix=T(pr["d"][0]), theta_pix=th)
+    phis = np.stack([np.fft.rfft2(phim) * (1 + 0.01 * k) for k in range(chains)])
+    phi = pkg.FlatS0(scalar_matrix=T(phis), theta_pix=th)
+    ops = dict(Cn=R(pr["Cn"][0]), Cf=R(pr["Cf"][0]), M=R(pr["M"][0]), B=R(pr["B"][0]),
+               D=R(D[0]), G=R(np.ones_like(Cphi)), Cphi=R(Cphi))
+    mass = sampling.get_mass_matrix(ops["Cphi"], R(Cphi * 1e-2), ops["G"])
+    fm, pm = mixing.mix(f, phi, ops["D"], ops["G"], n=n)
+    args = (fm, d, ops["Cn"], ops["Cphi"], ops["Cf"], ops["M"], ops["B"], ops["D"], ops["G"])
+    gen = torch.Generator(device=dev).manual_seed(5)
+    sampling.hmc_step(pm, False, mass, *args, gen, num_steps=1, step_size=step_size, n=n)  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x, dh, acc = sampling.hmc_step(pm, False, mass, *args, gen, num_steps=num_steps,
+                                   step_size=step_size, n=n)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ngrad = num_steps + 1
+    return {"workload": "hmc_mixed_phi_T_256_2arcmin_16chains_30steps_fp64", "chains": chains,
+            "step_size": step_size, "s_per_proposal": round(dt, 3),
+            "mixed_gradients_per_s": round(chains * ngrad / dt, 1),
+            "accept_rate": float(acc.double().mean()),
+            "delta_h_median": float(dh.real.median())}
+
+
 def copy_probe(torch):
     """A plain copy of known size: FETCH_SIZE / WRITE_SIZE of this kernel give
     the scale factors of the HBM counters, and its time the achievable copy
@@ -332,6 +379,7 @@ def main():
                          "filter (0: skip)")
     ap.add_argument("--no-wf", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--no-hmc", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -511,6 +559,9 @@ def main():
     if not args.no_grad:
         print("[bench] gradients, 16 chains", file=sys.stderr, flush=True)
         result["differentiation"] = grad_run(torch, pkg, sims, dev)
+    if not args.no_hmc:
+        print("[bench] HMC, 16 chains", file=sys.stderr, flush=True)
+        result["hmc"] = hmc_run(torch, pkg, sims, dev)
     if not args.no_wf:
         print("[bench] Wiener filter 512^2 QU, 8 RHS", file=sys.stderr, flush=True)
         result["wiener_filter"] = wf_run(torch, pkg, sims, dev)
