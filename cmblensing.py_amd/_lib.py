@@ -37,6 +37,8 @@ _SIGS = {
                                c_int, c_int, c_int, c_int, c_void_p]),
     "cmbl_rfft2": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "cmbl_irfft2": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "cmbl_grf_fourier": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
+                                 ctypes.c_ulonglong, ctypes.c_ulonglong, c_void_p]),
     "cmbl_gradf_logpdf": (c_int, [c_void_p, ctypes.POINTER(c_void_p), c_void_p, c_void_p, c_void_p,
                                   c_void_p, c_int, c_int, c_int, c_void_p]),
     "cmbl_dot_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p,
@@ -262,6 +264,13 @@ def get_plan(N: int, theta_pix: float, dtype: torch.dtype, nmaps: int, device) -
     return plan
 
 
+def reserve_phi_cache(count: int):
+    """Grow the phi cache so that `count` handles fetched back to back all stay
+    alive (the oldest entry is released when the cache overflows)."""
+    global PHI_CACHE_SIZE
+    PHI_CACHE_SIZE = max(PHI_CACHE_SIZE, int(count))
+
+
 def get_phi(plan: Plan, phi: torch.Tensor, nsteps: int, index=None) -> PhiHandle:
     """Prepared phi for (plan geometry, the tensor object and its version,
     nsteps) -- or for ``phi[index]`` of a batched phi tensor.  Entries are
@@ -314,6 +323,15 @@ def irfft2(plan: Plan, X: torch.Tensor, x: torch.Tensor, nmaps: int):
     with torch.cuda.device(plan.device):
         check(plan.lib.cmbl_irfft2(plan.handle, X.data_ptr(), x.data_ptr(), nmaps,
                                    stream_ptr(plan.device)), "cmbl_irfft2")
+
+
+def grf_fourier(plan: Plan, cov: torch.Tensor, mul, add, out: torch.Tensor, ntrials: int,
+                ncomp: int, seed: int, stream0: int):
+    with torch.cuda.device(plan.device):
+        check(plan.lib.cmbl_grf_fourier(plan.handle, cov.data_ptr(), _ptr(mul), _ptr(add),
+                                        out.data_ptr(), ntrials, ncomp, int(seed) & (2**64 - 1),
+                                        int(stream0) & (2**64 - 1), stream_ptr(plan.device)),
+              "cmbl_grf_fourier")
 
 
 class WfOps(ctypes.Structure):

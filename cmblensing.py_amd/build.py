@@ -14,7 +14,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 INCLUDE = PKG.parent / "include"
 LIB = PKG / "libcmbl.so"
-SOURCES = ["kernels.hip", "wf.hip", "api.hip"]
+SOURCES = ["kernels.hip", "wf.hip", "sim.hip", "api.hip"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 ops are disabled for all device code

@@ -10,6 +10,7 @@
 
 #include "../../include/cmbl.h"
 #include "kernels.hpp"
+#include "sim.hpp"
 #include "wf.hpp"
 
 using cmbl::cplx;
@@ -227,6 +228,23 @@ cmbl_status cmbl_irfft2(cmbl_plan* p, const void* X, void* x, int nmaps, void* s
                                           (cplx<double>*)p->s0, (double*)x + m0 * NN, nm,
                                           (const cplx<double>*)p->tw, s));
   }
+  return CMBL_OK;
+}
+
+cmbl_status cmbl_grf_fourier(cmbl_plan* p, const double* cov, const double* mul,
+                             const void* add, void* out, int ntrials, int ncomp,
+                             unsigned long long seed, unsigned long long stream0, void* stream) {
+  if (!p || !cov || !out) return fail(CMBL_EINVAL, "NULL argument");
+  if (ntrials < 0) return fail(CMBL_EINVAL, "ntrials < 0");
+  if (ncomp < 1 || ncomp > 3) return fail(CMBL_EINVAL, "ncomp must be 1, 2 or 3");
+  if (mul && !add) return fail(CMBL_EINVAL, "mul given without add");
+  hipStream_t s = (hipStream_t)stream;
+  if (p->dtype == CMBL_F32)
+    HIP_TRY(cmbl::launch_grf<float>(cov, mul, (const cplx<float>*)add, (cplx<float>*)out, p->N,
+                                    ncomp, ntrials, seed, stream0, s));
+  else
+    HIP_TRY(cmbl::launch_grf<double>(cov, mul, (const cplx<double>*)add, (cplx<double>*)out,
+                                     p->N, ncomp, ntrials, seed, stream0, s));
   return CMBL_OK;
 }
 

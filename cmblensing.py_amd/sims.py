@@ -307,3 +307,37 @@ def phi_map_and_cov(N, theta_pix_arcmin, seed, rms_arcmin=2.5, damping_ell=500.0
     # dominated by that rounding
     C = np.maximum(C, 1e-10 * C.max())
     return phi * s, C
+
+
+# ---------------------------------------------------------------------------
+# CAMB-layout spectra for the simulation code (no CAMB here)
+# ---------------------------------------------------------------------------
+def camb_like_spectra(lmax_prime=5000, phi_rms_arcmin=2.5, phi_damping=1000.0, r=0.2):
+    """Analytic stand-ins for CAMB's output in its layout (rows l = 0..lmax_prime-1,
+    D_l = l(l+1) C_l / 2pi in muK^2; columns TT, EE, BB, TE; lens potential
+    [l(l+1)]^2 C_l^phiphi / 2pi): (unlensed_scalar, tensor, total, lens_potential).
+
+    Scalar TT/EE follow cl_tt / cl_ee, TE = 0.5 sqrt(TT EE) cos(l / 300)
+    (changes sign: the reference then interpolates linearly), scalar BB = 0;
+    tensors carry r times a low-l bump; 'total' adds a lensing-like B-mode and
+    10 % extra TT/EE power.  C_l^phiphi ~ (l+20)^-4 exp(-(l/phi_damping)^2),
+    scaled so that sum_l l^3 C_l / 2pi = (phi_rms_arcmin)^2."""
+    ell = np.arange(lmax_prime, dtype=np.float64)
+    l2 = np.where(ell >= 2, ell, 2.0)
+    to_d = l2 * (l2 + 1) / (2 * np.pi)
+    tt, ee = cl_tt(l2), cl_ee(l2)
+    te = 0.5 * np.sqrt(tt * ee) * np.cos(l2 / 300.0)
+    zero = np.zeros_like(l2)
+    scal = np.stack([tt, ee, zero, te], axis=1) * to_d[:, None]
+    bump = r * 1e-2 * (l2 + 10.0) ** -2 * np.exp(-(l2 / 150.0) ** 2)
+    ten = np.stack([bump, 0.1 * bump, 0.1 * bump, 0.05 * bump], axis=1) * to_d[:, None]
+    lensb = 2e-6 * np.ones_like(l2) * np.exp(-(l2 / 3000.0) ** 2)
+    tot = np.stack([1.1 * tt, 1.1 * ee, lensb, te], axis=1) * to_d[:, None] + ten
+    cpp = (l2 + 20.0) ** -4 * np.exp(-(l2 / phi_damping) ** 2)
+    msk = ell >= 2
+    amp = (phi_rms_arcmin * ARCMIN) ** 2 / np.sum((l2 ** 3 * cpp)[msk] / (2 * np.pi))
+    lens = amp * cpp * (l2 * (l2 + 1)) ** 2 / (2 * np.pi)
+    for a in (scal, ten, tot):
+        a[~msk] = 0.0
+    lens[~msk] = 0.0
+    return scal, ten, tot, lens

@@ -95,6 +95,23 @@ cmbl_status cmbl_lenseflow_vjp(cmbl_plan* plan, const cmbl_phi* const* phis, con
 cmbl_status cmbl_rfft2(cmbl_plan* plan, const void* x, void* X, int nmaps, void* stream);
 cmbl_status cmbl_irfft2(cmbl_plan* plan, const void* X, void* x, int nmaps, void* stream);
 
+/* Gaussian random fields in the FOURIER basis (simulate.py:100-113:
+ * field_from_covar followed by rfft2) with the data model of load_sim fused
+ * (simulate.py:699-702: mask * beam * lensed + rfft2(noise)).  For trial t and
+ * component c (map m = t * ncomp + c, [ntrials][ncomp][N][N/2+1] in the plan
+ * dtype):
+ *   out[m] = (add ? mul[c] * add[m] : 0) + rfft2(irfft2(F_m, norm="ortho")),
+ *   F_m = sqrt(cov[c] / 2) (a + i b),
+ * with a, b standard normal per mode from Philox4x64-10 (key {seed, 0},
+ * counter {kx (N/2+1) + ky, stream0 + (c << 32) + t, 0, 0}) and Box-Muller on
+ * its first two outputs.  cov and mul are float64 [ncomp][N][N/2+1] device
+ * arrays; mul may be NULL (1) when add is given and must be NULL without it.
+ * The reference draws with jax.random; the streams differ, the distribution
+ * and the per-mode construction are the same.  out may equal add. */
+cmbl_status cmbl_grf_fourier(cmbl_plan* plan, const double* cov, const double* mul,
+                             const void* add, void* out, int ntrials, int ncomp,
+                             unsigned long long seed, unsigned long long stream0, void* stream);
+
 /* ---- lensed Wiener filter -------------------------------------------------
  * Diagonal operators are real arrays [ncomp][N][N/2+1] in the plan dtype,
  * shared by all batch elements (DiagonalScalar / DiagonalEB,

@@ -1,0 +1,383 @@
+"""Pins of the simulation oracle (oracle/simulate.py) and of the package's
+seed-independent simulation set-up (torch, float64, run here on CPU tensors)
+against it.  The random generator is pinned to numpy's own Philox bit
+generator; the quadratic-estimate normalisations to an independent mode-sum
+(convolution-theorem) evaluation; the block algebra and mixing matrices to
+the identities they must satisfy; spectra and interpolation to closed forms."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import simulate as osim
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def psim(pkg):
+    from cmblensing_amd import simulate
+    return simulate
+
+
+@pytest.fixture(scope="module")
+def spectra(sims):
+    return sims.camb_like_spectra(5000)
+
+
+# ------------------------------------------------------------ generator
+@pytest.mark.parametrize("key,ctr", [(0, 0), (12345, 7), ((9 << 64) | 3, (5 << 64) | 11),
+                                     (2 ** 64 - 1, 2 ** 64 - 1)])
+def test_philox_matches_numpy_philox(key, ctr):
+    """numpy's Philox (4x64, 10 rounds) increments its 256-bit counter before
+    each block: its first block for counter c is philox(c + 1)."""
+    g = np.random.Philox(counter=ctr, key=key)
+    ref = [int(v) for v in g.random_raw(8)]
+    out = []
+    for blk in (1, 2):
+        c = ctr + blk
+        words = [(c >> (64 * i)) & (2 ** 64 - 1) for i in range(4)]
+        x = osim.philox4x64_10(*[np.uint64(v) for v in words], key & (2 ** 64 - 1), key >> 64)
+        out += [int(v) for v in x]
+    assert out == ref
+
+
+def test_philox_vectorised_matches_scalar():
+    modes = np.arange(50, dtype=np.uint64)
+    x = osim.philox4x64_10(modes, np.uint64(3 << 32), 0, 0, 99, 0)
+    for i in (0, 17, 49):
+        g = np.random.Philox(counter=int(i) + (3 << 96) - 1, key=99)
+        assert [int(v[i]) for v in x] == [int(v) for v in g.random_raw(4)]
+
+
+def test_normals_moments():
+    a, b = osim.normals(256, 7, 1 << 32)
+    z = np.concatenate([a.ravel(), b.ravel()])
+    n = z.size
+    assert abs(z.mean()) < 5 / math.sqrt(n)
+    assert abs(z.var() - 1) < 5 * math.sqrt(2 / n)
+    assert abs(np.mean(a * b)) < 5 / math.sqrt(a.size)
+    # fourth moment of a normal is 3
+    assert abs(np.mean(z ** 4) - 3) < 5 * math.sqrt(96 / n)
+
+
+def test_grf_round_trip_is_scaled_hermitian_projection():
+    """rfft2(irfft2(F, ortho)) = N (F + conj F(-kx)) / 2 on the ky = 0 and N/2
+    columns (real where kx is its own partner) and N F elsewhere: the formula
+    the device kernel evaluates in Fourier space."""
+    N = 32
+    rng = np.random.default_rng(3)
+    cov = rng.uniform(0.5, 2.0, (2, N, N // 2 + 1))
+    G = osim.grf_fourier(cov, 2, 11, 5 << 32)
+    for t in range(2):
+        for c in range(2):
+            a, b = osim.normals(N, 11, (5 << 32) + (c << 32) + t)
+            F = np.sqrt(cov[c] / 2) * (a + 1j * b)
+            exp = N * F.copy()
+            for j in (0, N // 2):
+                for i in range(N):
+                    ip = (N - i) % N
+                    exp[i, j] = N * ((F[i, j] + np.conj(F[ip, j])) / 2 if ip != i else F[i, j].real)
+            assert np.allclose(G[t, c], exp, rtol=1e-12, atol=1e-12 * np.abs(exp).max())
+
+
+def test_grf_power_matches_covariance():
+    """E|G|^2 = N^2 C off the self-conjugate columns."""
+    N, T = 32, 40
+    cov = np.ones((1, N, N // 2 + 1))
+    G = osim.grf_fourier(cov, T, 5, 0)
+    p = np.abs(G[:, 0, :, 1:N // 2]) ** 2 / N ** 2
+    m = p.mean()
+    assert abs(m - 1) < 5 / math.sqrt(p.size)
+
+
+# ------------------------------------------------------ interpolation
+def test_interp_matches_numpy_inside_and_extrapolates_linearly():
+    xp = np.array([1.0, 2.0, 4.0, 7.0])
+    fp = np.array([3.0, -1.0, 2.0, 5.0])
+    x = np.linspace(1.0, 7.0, 31)
+    assert np.allclose(osim.interp(x, xp, fp), np.interp(x, xp, fp), atol=1e-14)
+    lo = osim.interp(np.array([0.0]), xp, fp, "extrapolate", "extrapolate")
+    hi = osim.interp(np.array([9.0]), xp, fp, "extrapolate", "extrapolate")
+    assert np.allclose(lo, 3.0 + (0.0 - 1.0) * (-4.0), atol=1e-14)
+    assert np.allclose(hi, 5.0 + (9.0 - 7.0) * 1.0, atol=1e-14)
+    assert osim.interp(np.array([0.0]), xp, fp, 0.0, 0.0)[0] == 0.0
+
+
+def test_covar_power_law_is_exact_in_log_log():
+    N, th = 32, 2.0
+    lg, pw = osim.gen_ell_grid(N, th)
+    ells = np.arange(2, 17000).astype(float)
+    cl = 7.0 * ells ** -2.5
+    C = osim.covar_matrix_from_cls(N, pw, lg, ells, cl, origin_value=0)
+    ref = 7.0 * np.where(lg > 0, lg, 1.0) ** -2.5 / pw ** 2
+    ref[0, 0] = 0.0
+    assert rel(C, ref) < 1e-12
+
+
+def test_mask_and_beam_closed_forms():
+    N, th = 128, 1.0
+    lg, pw = osim.gen_ell_grid(N, th)
+    m = osim.get_mask(3000, N, pw, lg)
+    assert m[0, 0] == 1.0
+    assert np.all(m[(lg > 0) & (lg <= 2950)] == 1.0)
+    assert np.all(m[lg >= 3000] == 0.0)
+    band = (lg > 2950) & (lg < 3000)
+    assert band.any()
+    # the taper is the cosine ramp sampled at integer l, linearly interpolated
+    x = (lg[band] - 2951) / 49.0
+    assert np.allclose(m[band], 1 - (np.cos(np.pi * (1 - x)) + 1) / 2, atol=2e-3)
+    b = osim.get_beam(N, pw, lg, 5000)
+    assert b[0, 0] == 0.0 and np.all(b.ravel()[1:] == 1.0)
+    fw = 3.0
+    b = osim.get_beam(N, pw, lg, 5000, beam_fwhm=fw)
+    sig = math.radians(fw / 60) / math.sqrt(8 * math.log(2))
+    inside = (lg > 0) & (lg < 4999)
+    assert np.allclose(b[inside], np.exp(-0.5 * (lg[inside] * sig) ** 2), rtol=1e-5)
+    assert np.all(b[lg >= 4999.0 + 1] == 1.0)
+
+
+def test_noise_cls_closed_form():
+    tt, te, ee, bb = osim.noise_cls(500, 3.0)
+    ell = np.arange(2, 500)
+    ref = math.radians(3.0 / 60) ** 2 * (1 + (100 / ell) ** 3)
+    assert np.allclose(tt, ref, rtol=1e-14)
+    assert np.all(te == 0) and np.allclose(ee, 2 * ref) and np.allclose(bb, 2 * ref)
+
+
+def test_dl2cl_power_law_and_linear_branch():
+    lmax, lmp = 3000, 1000
+    ell = np.arange(lmp, dtype=float)
+    cl = 4.0 * np.maximum(ell, 2.0) ** -3
+    dl = cl * ell * (ell + 1) / (2 * np.pi)
+    out = osim.dl2cl(dl, lmax, lmp)
+    assert rel(out, 4.0 * np.arange(2, lmax, dtype=float) ** -3) < 1e-12
+    dl[500] = -1.0          # a non-positive entry switches to linear interpolation, 0 beyond
+    out = osim.dl2cl(dl, lmax, lmp)
+    assert np.all(out[lmp - 2:] == 0.0)
+
+
+# ------------------------------------------------------------ block algebra
+def _pd_blocks(rng, shape):
+    a = rng.uniform(1, 3, shape)
+    d = rng.uniform(1, 3, shape)
+    b = rng.uniform(-0.5, 0.5, shape)
+    return a, b, b.copy(), d
+
+
+def test_block_sqrt_squares_back_and_inverse_inverts():
+    rng = np.random.default_rng(1)
+    a, b, c, d = _pd_blocks(rng, (8, 5))
+    r11, r12, r21, r22 = osim.block_matrix_sqrt(a, b, c, d)
+    assert np.allclose(r11 * r11 + r12 * r21, a) and np.allclose(r11 * r12 + r12 * r22, b)
+    assert np.allclose(r21 * r11 + r22 * r21, c) and np.allclose(r21 * r12 + r22 * r22, d)
+    i11, i12, i21, i22 = osim.invert_block_matrix(a, b, c, d)
+    assert np.allclose(a * i11 + b * i21, 1) and np.allclose(a * i12 + b * i22, 0)
+    assert np.allclose(c * i11 + d * i21, 0) and np.allclose(c * i12 + d * i22, 1)
+
+
+def test_d_matrix_identities():
+    rng = np.random.default_rng(2)
+    shp = (6, 4)
+    cf_tt, cf_ee, cf_bb = (rng.uniform(1e-6, 1e-5, shp) for _ in range(3))
+    cn_tt, cn_ee, cn_bb = (rng.uniform(1e-7, 1e-6, shp) for _ in range(3))
+    pre = math.radians(5 / 60) ** 2
+    zero = np.zeros(shp)
+    d_tt, d_te, d_ee, d_bb = osim.get_d_matrix(cf_tt, zero, cf_ee, cf_bb, cn_tt, zero, cn_ee,
+                                               cn_bb)
+    # diagonal blocks: D = sqrt((Cf + pre + 2 Cn) / Cf) per component
+    assert np.allclose(d_tt, np.sqrt((cf_tt + pre + 2 * cn_tt) / cf_tt), rtol=1e-13)
+    assert np.allclose(d_ee, np.sqrt((cf_ee + pre + 2 * cn_ee) / cf_ee), rtol=1e-13)
+    assert np.allclose(d_bb, np.sqrt((cf_bb + pre + 2 * cn_bb) / cf_bb), rtol=1e-13)
+    assert np.all(d_te == 0)
+    # with TE: D^2 is the product's [[TT, ET], [ET, EE]] (the reference's choice)
+    cf_te = 0.3 * np.sqrt(cf_tt * cf_ee)
+    d_tt, d_te, d_ee, _ = osim.get_d_matrix(cf_tt, cf_te, cf_ee, cf_bb, cn_tt, zero, cn_ee, cn_bb)
+    inv = osim.invert_block_matrix(cf_tt, cf_te, cf_te, cf_ee)
+    prod = osim.teb_matrix_mult(cf_tt + pre + 2 * cn_tt, cf_te, cf_te, cf_ee + pre + 2 * cn_ee,
+                                cf_bb, *inv, cf_bb)
+    assert np.allclose(d_tt * d_tt + d_te * d_te, prod[0], rtol=1e-10)
+    assert np.allclose(d_tt * d_te + d_te * d_ee, prod[2], rtol=1e-10)
+    assert np.allclose(d_te * d_te + d_ee * d_ee, prod[3], rtol=1e-10)
+
+
+def test_g_matrix_closed_form():
+    rng = np.random.default_rng(4)
+    cphi = rng.uniform(1, 2, (5, 3))
+    nphi = rng.uniform(0.1, 1, (5, 3))
+    g = osim.get_g_matrix(cphi, nphi, 1.0, a_phi=1.0)
+    assert np.allclose(g, 1.0, rtol=1e-15)
+    g = osim.get_g_matrix(cphi, nphi, 1.0, a_phi=2.0)
+    assert np.allclose(g, np.sqrt(1 + nphi / cphi) / np.sqrt(1 + 2 * nphi / cphi), rtol=1e-14)
+
+
+# ------------------------------------------- quadratic estimate by mode sums
+def test_get_indices_order():
+    assert osim.get_indices(2).tolist() == [[1, 1], [0, 1], [1, 0], [0, 0]]
+    six = osim.get_indices(6)
+    assert six.shape == (64, 6) and len({tuple(r) for r in six}) == 64
+    assert six[0].tolist() == [1] * 6 and six[-1].tolist() == [0] * 6
+    assert six[1].tolist() == [0, 1, 1, 1, 1, 1]
+
+
+def _full_k(N, pw):
+    k = 2 * np.pi * np.fft.fftfreq(N, pw)
+    KX, KY = np.meshgrid(k, k, indexing="ij")
+    return KX, KY
+
+
+def _leg_full(fn, N, pw, enc, iso):
+    """A leg on the full index plane: prod_s (i K_s) f(|K|) / |K|^len(iso)."""
+    KX, KY = _full_k(N, pw)
+    lk = np.sqrt(KX ** 2 + KY ** 2)
+    out = fn(lk).astype(complex)
+    for s in list(enc) + list(iso):
+        out = out * 1j * (KX if s == 0 else KY)
+    if iso:
+        with np.errstate(divide="ignore", invalid="ignore"):
+            out = np.where(lk > 0, out / lk ** len(iso), 0.0)
+    out[0, 0] = 0.0 if (enc or iso) else out[0, 0]
+    return out
+
+
+def _conv_half(P, Q, N):
+    """(1/N^2) sum_k P(k) Q(L - k) on the half plane L (index arithmetic mod N):
+    the rfft2 of the product of the two maps whose irfft2 spectra are P, Q."""
+    H1 = N // 2 + 1
+    out = np.zeros((N, H1), dtype=complex)
+    for L0 in range(N):
+        for L1 in range(H1):
+            sh = np.roll(np.roll(Q[::-1, ::-1], L0 + 1, axis=0), L1 + 1, axis=1)
+            out[L0, L1] = np.sum(P * sh) / N ** 2
+    return out
+
+
+def _outer(N, pw, i, j):
+    kx = 2 * np.pi * np.fft.fftfreq(N, pw)
+    ky = 2 * np.pi * np.fft.rfftfreq(N, pw)
+    KX, KY = np.meshgrid(kx, ky, indexing="ij")
+    K = (KX, KY)
+    return np.abs(K[i] * K[j])
+
+
+def _radial_inputs(N, pw):
+    dk = 2 * np.pi / (N * pw)
+    cut = (N / 2 - 0.5) * dk         # every Nyquist mode lies above the cut
+    tf = lambda l: np.where((l > 0) & (l < cut), np.exp(-(l / (0.3 * N * dk)) ** 2), 0.0)
+    cf = lambda l: 1.0 / (1.0 + (l / (3 * dk)) ** 2)
+    cfl = lambda l: 1.2 * cf(l)
+    cn = lambda l: 0.05 + 0.0 * l
+    return tf, cf, cfl, cn
+
+
+def test_scalar_qe_matches_mode_sum():
+    N, th = 16, 2.0
+    lg, pw = osim.gen_ell_grid(N, th)
+    tf, cf, cfl, cn = _radial_inputs(N, pw)
+    qe = osim.scalar_quadratic_estimate(cn(lg), cf(lg), cfl(lg), tf(lg), np.ones_like(lg), pw)
+    sig = lambda l: tf(l) ** 2 * cfl(l) + cn(l)
+    a = lambda l: tf(l) ** 2 * cf(l) ** 2 / sig(l)
+    b = lambda l: tf(l) ** 2 / sig(l)
+    c = lambda l: tf(l) ** 2 * cf(l) / sig(l)
+    tot = np.zeros(lg.shape)
+    for i, j in [(1, 1), (0, 1), (1, 0), (0, 0)]:
+        internal = (_conv_half(_leg_full(a, N, pw, [i, j], []), _leg_full(b, N, pw, [], []), N)
+                    + _conv_half(_leg_full(c, N, pw, [i], []), _leg_full(c, N, pw, [j], []), N))
+        tot += _outer(N, pw, i, j) * np.abs(internal)
+    exp = np.where(tot > 0, 1 / np.where(tot > 0, tot, 1), 0.0)
+    assert rel(qe, exp) < 1e-10
+
+
+def test_polar_qe_matches_mode_sum():
+    N, th = 8, 2.0
+    lg, pw = osim.gen_ell_grid(N, th)
+    tf, cf, cfl, cn = _radial_inputs(N, pw)
+    cfb = lambda l: 0.3 * cf(l)
+    one = np.ones_like(lg)
+    qe = osim.polar_quadratic_estimate(cf(lg), cfb(lg), cfl(lg), 1.1 * cfb(lg), cn(lg), cn(lg),
+                                       tf(lg), tf(lg), one, one, pw)
+    se = lambda l: tf(l) ** 2 * cfl(l) + cn(l)
+    sb = lambda l: tf(l) ** 2 * 1.1 * cfb(l) + cn(l)
+    e = [lambda l, k=k: tf(l) ** 2 * cf(l) ** k / se(l) for k in range(3)]
+    bb = [lambda l, k=k: tf(l) ** 2 * cfb(l) ** k / sb(l) for k in range(3)]
+    eps = osim.levi_civita_3d()
+    tot = np.zeros(lg.shape)
+    for i, j in [(1, 1), (0, 1), (1, 0), (0, 0)]:
+        internal = np.zeros(lg.shape, dtype=complex)
+        for k, l, m, n, p, q in osim.get_indices(6):
+            wgt = 4 * eps[m, p, 2] * eps[n, q, 2]
+            if wgt == 0:
+                continue
+            A, B = [k, l, m, n], [k, l, p, q]
+            lf = lambda fn, enc, iso: _leg_full(fn, N, pw, enc, iso)
+            internal += wgt * (_conv_half(lf(e[2], [i, j], A), lf(bb[0], [], B), N)
+                               - 2 * _conv_half(lf(e[1], [i], A), lf(bb[1], [j], B), N)
+                               + _conv_half(lf(e[0], [], A), lf(bb[2], [i, j], B), N))
+        tot += _outer(N, pw, i, j) * np.abs(internal)
+    exp = np.where(tot > 0, 1 / np.where(tot > 0, tot, 1), 0.0)
+    assert rel(np.real(qe), exp) < 1e-10
+    assert np.all(np.imag(qe) == 0)
+
+
+# ------------------------------------ package set-up (torch) vs the oracle
+def test_package_spectra_and_covariances_match_oracle(psim, spectra):
+    s, t, to, lp = spectra
+    ocl = osim.extract_all_cls(s, t, to, lp, 17000, 5000)
+    pcl = psim.extract_all_cls(s, t, to, lp, 17000, 5000, device="cpu")
+    for k in ocl:
+        assert np.allclose(pcl[k].numpy(), ocl[k], rtol=1e-13, atol=0), k
+    N, th = 64, 2.0
+    lg, pw = osim.gen_ell_grid(N, th)
+    plg, ppw = psim.gen_ell_grid(N, th, "cpu")
+    assert ppw == pw and np.array_equal(plg.numpy(), lg)
+    ells = np.arange(2, 17000).astype(float)
+    for k in ("phi", "scalar_TT", "scalar_TE", "tensor_BB", "total_EE"):
+        o = osim.covar_matrix_from_cls(N, pw, lg, ells, ocl[k], origin_value=0)
+        p = psim.covar_matrix_from_cls(N, pw, plg, torch.as_tensor(ells), pcl[k], origin_value=0)
+        assert np.allclose(p.numpy(), o, rtol=1e-12, atol=0), k
+    # torch's and numpy's linspace/cos differ in the last bit on the taper
+    assert np.allclose(psim.get_mask(3000, N, pw, plg).numpy(), osim.get_mask(3000, N, pw, lg),
+                       rtol=0, atol=1e-15)
+    assert np.allclose(psim.get_beam(N, pw, plg, 5000, 2.0).numpy(),
+                       osim.get_beam(N, pw, lg, 5000, 2.0), rtol=1e-14)
+    for a, b in zip(psim.noise_cls(5000, 3.0, device="cpu"), osim.noise_cls(5000, 3.0)):
+        assert np.allclose(a.numpy(), b, rtol=1e-14, atol=0)
+
+
+def test_package_mixing_matrices_match_oracle(psim):
+    rng = np.random.default_rng(8)
+    shp = (16, 9)
+    arrs = [rng.uniform(1e-6, 1e-5, shp) for _ in range(3)] + \
+           [rng.uniform(1e-7, 1e-6, shp) for _ in range(3)]
+    cf_tt, cf_ee, cf_bb, cn_tt, cn_ee, cn_bb = arrs
+    cf_te = 0.4 * np.sqrt(cf_tt * cf_ee) * np.sign(rng.standard_normal(shp))
+    cf_tt[0, 0] = 0.0               # a zero mode exercises the nonzero-safe inverses
+    zero = np.zeros(shp)
+    args = (cf_tt, cf_te, cf_ee, cf_bb, cn_tt, zero, cn_ee, cn_bb)
+    o = osim.get_d_matrix(*args)
+    p = psim.get_d_matrix(*[torch.as_tensor(a) for a in args])
+    for a, b in zip(p, o):
+        assert np.allclose(a.numpy(), b, rtol=1e-13, atol=0, equal_nan=True)
+    cphi, nphi = rng.uniform(1, 2, shp), rng.uniform(0.1, 1, shp)
+    assert np.allclose(psim.get_g_matrix(torch.as_tensor(cphi), torch.as_tensor(nphi), 1.0,
+                                         2.0).numpy(),
+                       osim.get_g_matrix(cphi, nphi, 1.0, 2.0), rtol=1e-14)
+    assert np.allclose(psim.get_d_tt_matrix(*[torch.as_tensor(a) for a in (cf_tt, cf_ee, cn_tt)],
+                                            0.1, 0.2).numpy(),
+                       osim.get_d_tt_matrix(cf_tt, cf_ee, cn_tt, 0.1, 0.2), rtol=1e-14)
+
+
+def test_package_qe_legs_match_oracle(psim):
+    N, th = 16, 2.0
+    lg, pw = osim.gen_ell_grid(N, th)
+    f = np.exp(-lg / lg.max())
+    f[0, 0] = np.nan
+    for enc, iso in [([], []), ([0, 1], []), ([1], [0, 1, 1, 0]), ([], [1, 1, 1, 1])]:
+        o = osim.qe_leg(f, pw, enc, iso)
+        p = psim.qe_leg(torch.as_tensor(f), pw, enc, iso).numpy()
+        assert np.allclose(p, o, rtol=1e-14, atol=1e-300), (enc, iso)
+    assert psim.get_indices(6) == [tuple(r) for r in osim.get_indices(6).tolist()]
