@@ -142,6 +142,16 @@ class FlatS02(Field):
         return ["scalar_matrix", "polar_matrix_1", "polar_matrix_2"]
 
 
+def zero_scalar_field_like(field: Field) -> FlatS0:
+    """fields.py:123-133: a FOURIER FlatS0 of zeros (complex128) on the field's
+    grid and device."""
+    N = field.nside
+    dev = field._first().device
+    return FlatS0(fourier_weights=fourier_weights_for(N, dev), nside=N, theta_pix=field.theta_pix,
+                  pix_width=field.pix_width, basis=Basis.FOURIER,
+                  scalar_matrix=torch.zeros((N, N // 2 + 1), dtype=torch.complex128, device=dev))
+
+
 # --------------------------------------------------------------------------
 # basis changes (fields.py:156-172) through the library's HIP FFT passes
 # --------------------------------------------------------------------------

@@ -210,3 +210,14 @@ def log_det(op):
 def get_identity_like(op):
     """matrix_operators.py:151-155"""
     return op.replace(**{n: torch.ones_like(getattr(op, n)) for n in op._matrix_names()})
+
+
+def get_scalar_identity(op) -> DiagonalScalar:
+    """matrix_operators.py:158-170: a DiagonalScalar of ones on op's grid.
+    The reference's ones are complex128; a real identity multiplies alike."""
+    first = getattr(op, op._matrix_names()[0])
+    N = op.nside
+    return DiagonalScalar(fourier_weights=fourier_weights_for(N, first.device), nside=N,
+                          theta_pix=op.theta_pix, pix_width=op.pix_width,
+                          scalar_matrix=torch.ones((N, N // 2 + 1), dtype=torch.float64,
+                                                   device=first.device))

@@ -319,7 +319,7 @@ def camb_like_spectra(lmax_prime=5000, phi_rms_arcmin=2.5, phi_damping=1000.0, r
 
     Scalar TT/EE follow cl_tt / cl_ee, TE = 0.5 sqrt(TT EE) cos(l / 300)
     (changes sign: the reference then interpolates linearly), scalar BB = 0;
-    tensors carry r times a low-l bump; 'total' adds a lensing-like B-mode and
+    tensors carry r times a low-l bump with a power-law tail; 'total' adds a lensing-like B-mode and
     10 % extra TT/EE power.  C_l^phiphi ~ (l+20)^-4 exp(-(l/phi_damping)^2),
     scaled so that sum_l l^3 C_l / 2pi = (phi_rms_arcmin)^2."""
     ell = np.arange(lmax_prime, dtype=np.float64)
@@ -329,7 +329,9 @@ def camb_like_spectra(lmax_prime=5000, phi_rms_arcmin=2.5, phi_damping=1000.0, r
     te = 0.5 * np.sqrt(tt * ee) * np.cos(l2 / 300.0)
     zero = np.zeros_like(l2)
     scal = np.stack([tt, ee, zero, te], axis=1) * to_d[:, None]
-    bump = r * 1e-2 * (l2 + 10.0) ** -2 * np.exp(-(l2 / 150.0) ** 2)
+    # a power-law tail (as CAMB's tensor spectra have): a Gaussian cut-off would
+    # leave C_l^BB ~ 1e-77 and pinv(Cf) overflowing in the B-mode prior
+    bump = r * 1e-2 * (l2 + 10.0) ** -2 / (1.0 + (l2 / 150.0) ** 4)
     ten = np.stack([bump, 0.1 * bump, 0.1 * bump, 0.05 * bump], axis=1) * to_d[:, None]
     lensb = 2e-6 * np.ones_like(l2) * np.exp(-(l2 / 3000.0) ** 2)
     tot = np.stack([1.1 * tt, 1.1 * ee, lensb, te], axis=1) * to_d[:, None] + ten

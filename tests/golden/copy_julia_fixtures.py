@@ -37,6 +37,12 @@ NAMES = [
     # mixed parametrisation (mixing.py, gradients.py:40-85)
     "d_I", "g_I", "d_ee_P", "d_bb_P", "d_tt_IP", "d_te_IP", "d_ee_IP", "d_bb_IP",
     "mixed_grad_phi_I", "mixed_grad_phi_P", "mixed_grad_phi_IP",
+    # quadratic-estimate noise and the binned spectra of simulated maps
+    # (test_covariance_matrices.py, test_simulated_cls.py)
+    "nphi_I", "nphi_P", "nphi_IP", "ell_I", "ell_P", "ell_IP", "cl_ll_avg_I", "cl_tt_avg_I",
+] + [f"cl_{k}_{c}_avg_{p}" for k in ("ll", "tt") for c in ("e", "b") for p in ("P", "IP")] + [
+    # MAP-joint outputs after 30 steps (test_map_joint.py)
+    "fJ_I", "phiJ_I", "fJ_e_P", "fJ_b_P", "phiJ_P", "fJ_t_IP", "fJ_e_IP", "fJ_b_IP", "phiJ_IP",
 ]
 TEXT = ["logpdf_I.txt", "logpdf_P.txt", "logpdf_IP.txt", "n_side.txt", "theta_pix.txt"]
 

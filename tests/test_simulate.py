@@ -381,3 +381,80 @@ def test_package_qe_legs_match_oracle(psim):
         p = psim.qe_leg(torch.as_tensor(f), pw, enc, iso).numpy()
         assert np.allclose(p, o, rtol=1e-14, atol=1e-300), (enc, iso)
     assert psim.get_indices(6) == [tuple(r) for r in osim.get_indices(6).tolist()]
+
+
+# ----------------------------------------- CMBLensing.jl fixtures (256^2, 2.5')
+from conftest import GOLDEN  # noqa: E402
+
+J = GOLDEN / "julia"
+
+
+def jl(name):
+    return np.load(J / f"{name}.npy", allow_pickle=False)
+
+
+def _julia_qe_inputs(pol):
+    """Cn, Cf, mask, beam of a Julia data set and its lensed Cf: the fixtures
+    hold no lensed covariance, so Cf is scaled by the ratio of the binned
+    spectra of the lensed and unlensed simulated maps (cl_ll / cl_tt)."""
+    th, N = 2.5, 256
+    lg, pw = osim.gen_ell_grid(N, th)
+    if pol == "I":
+        cf = jl("cf_I")
+        cfl = cf * np.interp(lg, jl("ell_I"), jl("cl_ll_avg_I") / jl("cl_tt_avg_I"))
+        return pw, dict(cn=jl("cn_I"), cf=cf, cfl=cfl, m=jl("m_I"), b=jl("b_I"))
+    s = f"_{pol}"
+    out = {}
+    for c in ("ee", "bb"):
+        cf = jl(f"cf_{c}{s}")
+        r = jl(f"cl_ll_{c[0]}_avg{s}") / jl(f"cl_tt_{c[0]}_avg{s}")
+        out.update({f"cf_{c}": cf, f"cfl_{c}": cf * np.interp(lg, jl(f"ell{s}"), r),
+                    f"cn_{c}": jl(f"cn_{c}{s}"), f"m_{c}": jl(f"m_{c}{s}"), f"b_{c}": jl(f"b_{c}{s}")})
+    return pw, out
+
+
+def _qe_agreement(qe, nphi):
+    """relative L2 of the normalisation sums 1/qe over the modes where the
+    Julia sum is above 1e-6 of its maximum (elsewhere 1/qe is FFT round-off,
+    implementation noise in every code) and the median ratio there."""
+    inv = lambda q: np.where(q != 0, 1 / np.where(q != 0, q, 1), 0.0)
+    a, b = inv(np.real(qe)), inv(np.real(nphi))
+    good = b > 1e-6 * b.max()
+    assert good.mean() > 0.99
+    return rel(a[good], b[good]), float(np.median(np.real(qe)[good] / np.real(nphi)[good]))
+
+
+@pytest.mark.parametrize("pol", ["I", "P", "IP"])
+def test_quadratic_estimate_matches_julia_nphi(pol):
+    """nphi of CMBLensing.jl's data sets: agreement to the accuracy of the
+    lensed-Cf reconstruction from binned spectra (~0.3 %); a wrong term,
+    derivative or weight would be off by O(1)."""
+    pw, a = _julia_qe_inputs(pol)
+    if pol == "I":
+        qe = osim.scalar_quadratic_estimate(a["cn"], a["cf"], a["cfl"], a["m"], a["b"], pw) / 2
+    else:
+        qe = osim.polar_quadratic_estimate(a["cf_ee"], a["cf_bb"], a["cfl_ee"], a["cfl_bb"],
+                                           a["cn_ee"], a["cn_bb"], a["m_ee"], a["m_bb"],
+                                           a["b_ee"], a["b_bb"], pw) / 2
+    r, med = _qe_agreement(qe, jl(f"nphi_{pol}"))
+    assert r < 1e-2 and abs(med - 1) < 1e-2, (r, med)
+
+
+def test_d_and_g_matrices_match_julia(psim):
+    z = np.zeros_like(jl("cf_I"))
+    cases = {
+        "I": ((jl("cf_I"), z, jl("cf_I"), jl("cf_I"), jl("cn_I"), z, jl("cn_I"), jl("cn_I")),
+              {0: "d_I"}),
+        "P": ((jl("cf_ee_P"), z, jl("cf_ee_P"), jl("cf_bb_P"), jl("cn_ee_P"), z, jl("cn_ee_P"),
+               jl("cn_bb_P")), {2: "d_ee_P", 3: "d_bb_P"}),
+        "IP": (tuple(jl(f"{o}_{b}_IP") for o in ("cf", "cn") for b in ("tt", "te", "ee", "bb")),
+               {0: "d_tt_IP", 1: "d_te_IP", 2: "d_ee_IP", 3: "d_bb_IP"}),
+    }
+    for pol, (args, want) in cases.items():
+        o = osim.get_d_matrix(*args)
+        p = psim.get_d_matrix(*[torch.as_tensor(x) for x in args])
+        for i, name in want.items():
+            assert rel(np.nan_to_num(o[i]), jl(name)) < 1e-14, (pol, name)
+            assert rel(np.nan_to_num(p[i].numpy()), jl(name)) < 1e-14, (pol, name)
+    g = osim.get_g_matrix(jl("cphi_I"), np.real(jl("nphi_I")), 1.0, 1.0)
+    assert np.allclose(g, jl("g_I"), rtol=1e-15, atol=0)
