@@ -17,7 +17,10 @@ plain ratio, p50/p99 latency of plain single requests, relative L2 error of a
 batched result against the FP64 oracle, the live roofline of the dominant
 kernel, the CPU baseline (the FP64 oracle on this host's cores), and the
 mixed serving stream (10k Poisson requests at 2k req/s of mixed N, T/QU,
-lense/unlense: p50/p99 latency arrival -> done, maps/s, phi hit rate).
+lense/unlense: p50/p99 latency arrival -> done, maps/s, phi hit rate), and the
+consumers of the path: gradients for 16 chains, one HMC proposal, the 512^2
+Wiener filter with 8 right-hand sides, 64 simulated data sets and MAP-joint
+(one BFGS run, 16 batched grid runs).
 """
 from __future__ import annotations
 
@@ -338,6 +341,84 @@ def hmc_run(torch, pkg, sims, dev, N=256, th=2.0, chains=16, n=7, num_steps=30,
             "delta_h_median": float(dh.real.median())}
 
 
+def sim_run(torch, sims, dev, N=256, th=2.0, pol="P", trials=64, n=10, reps=3):
+    """Simulated data sets on the device (simulate.py's load_sim without CAMB,
+    vmapped over trials like batch_simulated_trials): per trial one phi, the
+    unlensed E/B fields, their lensing (10 RK4 steps, FP32) and the data
+    M B L f + n.  The seed-independent set-up (covariances, D, the polar
+    quadratic estimate) is timed once, separately."""
+    import importlib
+    simulate = importlib.import_module("cmblensing_amd.simulate")
+    cls = simulate.extract_all_cls(*sims.camb_like_spectra(5000), 17000, 5000, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cfg = simulate.sim_config(N, th, pol, cls, device=dev)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    simulate.simulate(cfg, 0, trials, torch.float32, n)          # warm (plans, phi cache)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in range(reps):
+        out = simulate.simulate(cfg, 1 + r, trials, torch.float32, n)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    # the draw kernel alone: E, B fields of every trial (one launch)
+    covs = torch.stack([cfg.cf["EE"], cfg.cf["BB"]])
+    simulate.grf_fourier(covs, trials, 9, 2 << 32, th, torch.float32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in range(20):
+        simulate.grf_fourier(covs, trials, 9, 2 << 32, th, torch.float32)
+    torch.cuda.synchronize()
+    gdt = (time.perf_counter() - t0) / 20
+    nmaps = trials * 2
+    gbytes = nmaps * N * (N // 2 + 1) * (8 + 8) / 2 + nmaps * N * (N // 2 + 1) * 8
+    return {"workload": f"simulated_datasets_{pol}_{N}_{th:g}arcmin_{trials}trials_n{n}_fp32",
+            "trials": trials, "s_per_batch": round(dt, 4),
+            "datasets_per_s": round(trials / dt, 1), "config_setup_s": round(setup_s, 3),
+            "grf_draw_us": round(gdt * 1e6, 1),
+            "grf_draw_maps_per_s": round(nmaps / gdt, 1),
+            "finite": bool(torch.isfinite(torch.view_as_real(out[3])).all())}
+
+
+def map_joint_run(torch, sims, dev, N=256, th=2.0, num_steps=30, trials=16, n=10):
+    """MAP-joint (map_joint.py) on simulated temperature data sets, FP64 like
+    the reference: one run with its BFGS line search (30 steps, one Wiener
+    filter, a mix, a mixed phi gradient and the line search per step), and
+    `trials` independent runs advanced together with the batched grid line
+    search (every alpha of every trial in one LenseFlow batch)."""
+    import importlib
+    simulate = importlib.import_module("cmblensing_amd.simulate")
+    mj = importlib.import_module("cmblensing_amd.map_joint")
+    cls = simulate.extract_all_cls(*sims.camb_like_spectra(5000), 17000, 5000, device=dev)
+    cfg = simulate.sim_config(N, th, "I", cls, uk_arcmin_t=1.0, device=dev)
+    arrs = simulate.simulate(cfg, 3, trials, torch.float64, n)
+    one = simulate.build_dataset(cfg, *[a[:1] for a in arrs])
+    many = simulate.build_dataset(cfg, *arrs)
+    mj.map_joint(one, num_steps=1, n=n)                          # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    f, phi = mj.map_joint(one, num_steps=num_steps, n=n)
+    torch.cuda.synchronize()
+    dt1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    fb, phib = mj.map_joint(many, num_steps=num_steps, n=n, step="grid")
+    torch.cuda.synchronize()
+    dtb = time.perf_counter() - t0
+
+    def corr(a, b):
+        a, b = a.flatten(), b.flatten()
+        return float(torch.real(torch.vdot(a, b)) / torch.sqrt(torch.vdot(a, a).real *
+                                                             torch.vdot(b, b).real))
+    return {"workload": f"map_joint_T_{N}_{th:g}arcmin_{num_steps}steps_fp64",
+            "s_per_run_bfgs": round(dt1, 3),
+            "batched_grid_trials": trials, "s_batched_grid": round(dtb, 3),
+            "runs_per_s_batched_grid": round(trials / dtb, 2),
+            "corr_phi_truth_bfgs": round(corr(phi.scalar_matrix, one.phi.scalar_matrix), 3),
+            "corr_phi_truth_grid_mean": round(float(np.mean(
+                [corr(phib.scalar_matrix[t], many.phi.scalar_matrix[t]) for t in range(trials)])), 3)}
+
+
 def copy_probe(torch):
     """A plain copy of known size: FETCH_SIZE / WRITE_SIZE of this kernel give
     the scale factors of the HBM counters, and its time the achievable copy
@@ -380,6 +461,8 @@ def main():
     ap.add_argument("--no-wf", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--no-hmc", action="store_true")
+    ap.add_argument("--no-sim", action="store_true")
+    ap.add_argument("--no-map-joint", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -454,12 +537,9 @@ def main():
     ms_per_step = dt / args.steps * 1e3
     maps_s = B * args.steps / dt
 
-    # error of one batched result against the FP64 oracle
-    from oracle import lenseflow as olf
+    # one batched result, checked against the FP64 oracle in the cpu_baseline leg
     r_chk = 5
-    ref = olf.lense_flow(fq[r_chk % 8], phis[r_chk % NPHI], olf.pix_width_from_theta(th), n)
     got = out[r_chk].double().cpu().numpy()
-    err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
 
     # live per-kernel timing (HIP events on the launch stream) over a few steps
     plan.set_timing(True)
@@ -531,7 +611,6 @@ def main():
                              "p99": round(float(np.percentile(lat, 99)), 4)},
         "plain_resident_phi_latency_ms": {"p50": round(float(np.percentile(lat_res, 50)), 4),
                                           "p99": round(float(np.percentile(lat_res, 99)), 4)},
-        "rel_l2_vs_fp64_oracle": err,
         "model_gbs": round(model_b * maps_s / 1e9, 1),
         "model_frac_hbm": round(model_b * maps_s / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "kernel": "k_row_lf_std<float,256>",
@@ -562,10 +641,20 @@ def main():
     if not args.no_hmc:
         print("[bench] HMC, 16 chains", file=sys.stderr, flush=True)
         result["hmc"] = hmc_run(torch, pkg, sims, dev)
+    if not args.no_sim:
+        print("[bench] simulated data sets, 64 trials 256^2 P", file=sys.stderr, flush=True)
+        result["simulation"] = sim_run(torch, sims, dev)
+    if not args.no_map_joint:
+        print("[bench] MAP-joint 256^2 T, 30 steps", file=sys.stderr, flush=True)
+        result["map_joint"] = map_joint_run(torch, sims, dev)
     if not args.no_wf:
         print("[bench] Wiener filter 512^2 QU, 8 RHS", file=sys.stderr, flush=True)
         result["wiener_filter"] = wf_run(torch, pkg, sims, dev)
     if not args.no_cpu:
+        # the CPU leg: the FP64 oracle checks one batched result, then is timed
+        from oracle import lenseflow as olf
+        ref = olf.lense_flow(fq[r_chk % 8], phis[r_chk % NPHI], olf.pix_width_from_theta(th), n)
+        result["rel_l2_vs_fp64_oracle"] = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
         fcpu = fq[0]
         cps, k, el = cpu_baseline(fcpu, phis[0], th, n, args.cpu_budget)
         result["cpu_baseline"] = {"value": round(cps, 4), "unit": "maps/s", "cores": 1,
