@@ -32,6 +32,9 @@ _SIGS = {
     "cmbl_phi_prepare": (c_int, [c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(c_void_p),
                                  ctypes.POINTER(c_double), c_void_p]),
     "cmbl_phi_release": (c_int, [c_void_p]),
+    "cmbl_phi_prepare_batch": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int,
+                                       ctypes.POINTER(c_void_p), ctypes.POINTER(c_double),
+                                       c_void_p]),
     "cmbl_phi_derivatives": (c_void_p, [c_void_p]),
     "cmbl_lenseflow": (c_int, [c_void_p, ctypes.POINTER(c_void_p), c_void_p, c_void_p, c_int,
                                c_int, c_int, c_int, c_int, c_void_p]),
@@ -139,8 +142,23 @@ class Plan:
             pass
 
 
+def _phi_basis(phi: torch.Tensor, N: int) -> int:
+    if tuple(phi.shape[-2:]) == (N, N) and not phi.is_complex():
+        return CMBL_PHI_MAP
+    if tuple(phi.shape[-2:]) == (N, N // 2 + 1):
+        return CMBL_PHI_FOURIER
+    raise CmblError(f"phi shape {tuple(phi.shape)} does not match N={N}")
+
+
 class PhiHandle:
     """A prepared phi: float64 derivatives and the p(t) table for nsteps."""
+
+    @classmethod
+    def _wrap(cls, plan: Plan, handle: c_void_p, nsteps: int, min_det: float) -> "PhiHandle":
+        self = cls.__new__(cls)
+        self.lib, self.plan, self.nsteps = plan.lib, plan, nsteps
+        self.handle, self.min_det = handle, min_det
+        return self
 
     def __init__(self, plan: Plan, phi: torch.Tensor, nsteps: int):
         self.lib = plan.lib
@@ -262,6 +280,55 @@ def get_plan(N: int, theta_pix: float, dtype: torch.dtype, nmaps: int, device) -
             if k[0] == key:
                 v[2].plan = plan
     return plan
+
+
+def prepare_batch(plan: Plan, phis: torch.Tensor, nsteps: int):
+    """PhiHandles for every phi of a [count, N, N] (MAP) or [count, N, N//2+1]
+    (FOURIER) tensor, prepared in one cmbl_phi_prepare_batch call."""
+    N = plan.N
+    count = int(phis.shape[0])
+    basis = _phi_basis(phis, N)
+    phis = phis.to(device=plan.device,
+                   dtype=torch.complex128 if basis == CMBL_PHI_FOURIER else torch.float64)
+    phis = phis.contiguous()
+    hs = (c_void_p * count)()
+    md = (c_double * count)()
+    with torch.cuda.device(plan.device):
+        check(plan.lib.cmbl_phi_prepare_batch(plan.handle, phis.data_ptr(), basis, nsteps, count,
+                                              hs, md, stream_ptr(plan.device)),
+              "cmbl_phi_prepare_batch")
+    return [PhiHandle._wrap(plan, c_void_p(hs[i]), nsteps, md[i]) for i in range(count)]
+
+
+def get_phi_batch(plan: Plan, phi: torch.Tensor, nsteps: int):
+    """Prepared phis for every element of a batched phi [B, ...] (cached like
+    get_phi per element; any miss prepares the whole batch in one call)."""
+    import weakref
+    B = int(phi.shape[0])
+    reserve_phi_cache(B)
+    pkey = (plan.device.index, plan.N, round(plan.theta_pix, 12), plan.dtype)
+    keys = [(pkey, id(phi), i, nsteps) for i in range(B)]
+    hit = []
+    for k in keys:
+        ent = _phis.get(k)
+        if ent is None or ent[0]() is not phi or ent[1] != phi._version:
+            break
+        hit.append(ent[2])
+    if len(hit) == B:
+        for k in keys:
+            _phis.move_to_end(k)
+        return hit
+    handles = prepare_batch(plan, phi, nsteps)
+    ref = weakref.ref(phi)
+    for k, h in zip(keys, handles):
+        old = _phis.pop(k, None)
+        if old is not None:
+            old[2].close()
+        _phis[k] = (ref, phi._version, h)
+    while len(_phis) > PHI_CACHE_SIZE:
+        _, (_, _, old) = _phis.popitem(last=False)
+        old.close()
+    return handles
 
 
 def reserve_phi_cache(count: int):

@@ -2,6 +2,7 @@
 // RK4 stage sequencing of LenseFlow on top of the kernels in kernels.hip.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -29,6 +30,12 @@ cmbl_status fail(cmbl_status code, const std::string& msg) {
     hipError_t e_ = (expr);                                                             \
     if (e_ != hipSuccess)                                                               \
       return fail(CMBL_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+#define CMBL_TRY(expr)                  \
+  do {                                  \
+    cmbl_status st_ = (expr);           \
+    if (st_ != CMBL_OK) return st_;     \
   } while (0)
 
 constexpr int kRing = 8;
@@ -94,14 +101,24 @@ struct cmbl_plan {
   void* gF = nullptr;    // their rfft2 [max_maps][5][N][N/2+1]
   // serving executors' staging buffer [max_maps][N][N], allocated on first use
   void* stage = nullptr;
-  // phi preparation temporaries (double), allocated on the first prepare and
-  // kept, so a prepare neither allocates nor frees scratch: rfft2 of phi
-  // [N][N/2+1], its five derivative spectra, the packed FFT scratch
-  // [5][N][N/2] and the per-block minima of the magnification determinant
+  // phi preparation temporaries (double) for up to prep_cap phis at once,
+  // allocated on the first prepare and kept (grown only for a larger batch):
+  // rfft2 of phi [N][N/2+1], its five derivative spectra, the packed FFT
+  // scratch [5][N][N/2] and the per-block minima of the magnification
+  // determinant, each times prep_cap
   void* prep_F = nullptr;
   void* prep_D = nullptr;
   void* prep_Pk = nullptr;
   double* prep_min = nullptr;
+  int prep_cap = 0;
+};
+
+// Device memory of the phis prepared in one call: their derivative maps
+// [count][5][N][N] double followed by their tables [count][2n+1][2][N][N] T.
+// Every handle of the call holds a reference; the last release frees it.
+struct PhiBlock {
+  void* mem = nullptr;
+  std::atomic<int> refs{0};
 };
 
 struct cmbl_phi {
@@ -109,11 +126,121 @@ struct cmbl_phi {
   int nsteps = 0;
   int dtype = CMBL_F32;
   const cmbl_plan* plan = nullptr;
-  double* deriv = nullptr;  // [5][N][N] double
-  void* ptab = nullptr;     // [(2n+1)][2][N][N] T
-  void* gtab = nullptr;     // [(2n+1)][3][N][N] T, M^-1 for the gradient ODE (lazy)
+  double* deriv = nullptr;  // [5][N][N] double (inside block)
+  void* ptab = nullptr;     // [(2n+1)][2][N][N] T (inside block)
+  void* gtab = nullptr;     // [(2n+1)][3][N][N] T, M^-1 for the gradient ODE (lazy, own)
+  PhiBlock* block = nullptr;
   double min_det = 0;
 };
+
+namespace {
+
+// phis prepared per pass of the batched preparation (bounds its scratch)
+constexpr int kPrepChunk = 64;
+
+cmbl_status ensure_prep(cmbl_plan* p, int cap) {
+  if (p->prep_cap >= cap) return CMBL_OK;
+  for (void* q : {p->prep_F, p->prep_D, p->prep_Pk, (void*)p->prep_min})
+    if (q) (void)hipFree(q);
+  p->prep_F = p->prep_D = p->prep_Pk = nullptr;
+  p->prep_min = nullptr;
+  p->prep_cap = 0;
+  const size_t N = p->N, NN = N * N, NH1 = N * (N / 2 + 1), NH = N * (N / 2);
+  const size_t nblk = (NN + 255) / 256;
+  hipError_t e;
+  if ((e = hipMalloc(&p->prep_F, cap * NH1 * sizeof(cplx<double>))) != hipSuccess ||
+      (e = hipMalloc(&p->prep_D, cap * 5 * NH1 * sizeof(cplx<double>))) != hipSuccess ||
+      (e = hipMalloc(&p->prep_Pk, cap * 5 * NH * sizeof(cplx<double>))) != hipSuccess ||
+      (e = hipMalloc((void**)&p->prep_min, cap * nblk * sizeof(double))) != hipSuccess)
+    return fail(CMBL_ENOMEM, std::string("phi_prepare scratch: ") + hipGetErrorString(e));
+  p->prep_cap = cap;
+  return CMBL_OK;
+}
+
+// Prepare `count` phis (contiguous [count][N][N] MAP or [count][N][N/2+1]
+// FOURIER, float64) into handles sharing one allocation: one launch per
+// stage per chunk of kPrepChunk phis, one host synchronisation for min_det.
+cmbl_status prepare_phis(cmbl_plan* p, const void* phi, int phi_basis, int nsteps, int count,
+                         cmbl_phi** out, double* min_det, hipStream_t s) {
+  if (!p || !phi) return fail(CMBL_EINVAL, "NULL argument");
+  if (phi_basis != CMBL_PHI_MAP && phi_basis != CMBL_PHI_FOURIER)
+    return fail(CMBL_EINVAL, "phi_basis must be MAP or FOURIER");
+  if (nsteps < 1) return fail(CMBL_EINVAL, "nsteps must be >= 1");
+  const int N = p->N;
+  const size_t NN = (size_t)N * N, NH1 = (size_t)N * (N / 2 + 1);
+  const size_t ntab = (size_t)(2 * nsteps + 1) * 2 * NN;
+  const size_t dbytes = (size_t)count * 5 * NN * sizeof(double);
+  const size_t toff = (dbytes + 255) / 256 * 256;
+  CMBL_TRY(ensure_prep(p, std::min(count, kPrepChunk)));
+  PhiBlock* blk = new PhiBlock();
+  hipError_t e;
+  if ((e = hipMalloc(&blk->mem, toff + (size_t)count * ntab * p->tsize)) != hipSuccess) {
+    delete blk;
+    return fail(CMBL_ENOMEM, std::string("phi_prepare: ") + hipGetErrorString(e));
+  }
+  blk->refs = count;
+  char* base = (char*)blk->mem;
+  for (int i = 0; i < count; ++i) {
+    cmbl_phi* h = new cmbl_phi();
+    h->N = N;
+    h->nsteps = nsteps;
+    h->dtype = p->dtype;
+    h->plan = p;
+    h->deriv = (double*)(base + (size_t)i * 5 * NN * sizeof(double));
+    h->ptab = base + toff + (size_t)i * ntab * p->tsize;
+    h->block = blk;
+    out[i] = h;
+  }
+  auto bail = [&](cmbl_status st) {
+    for (int i = 0; i < count; ++i) {
+      cmbl_phi_release(out[i]);
+      out[i] = nullptr;
+    }
+    return st;
+  };
+  cplx<double>* F = (cplx<double>*)p->prep_F;
+  cplx<double>* D = (cplx<double>*)p->prep_D;
+  cplx<double>* Pk = (cplx<double>*)p->prep_Pk;
+  const cplx<double>* tw64 = (const cplx<double>*)p->tw64;
+  const int nblk = (int)((NN + 255) / 256);
+  std::vector<double> mp((size_t)count * nblk);
+  for (int c0 = 0; c0 < count; c0 += kPrepChunk) {
+    const int cc = std::min(kPrepChunk, count - c0);
+    if (phi_basis == CMBL_PHI_MAP) {
+      e = cmbl::launch_rfft2<double>(N, (const double*)phi + c0 * NN, Pk, F, cc, tw64, s);
+    } else {
+      e = hipMemcpyAsync(F, (const cplx<double>*)phi + c0 * NH1, cc * NH1 * sizeof(cplx<double>),
+                         hipMemcpyDeviceToDevice, s);
+    }
+    if (e != hipSuccess) return bail(fail(CMBL_EHIP, std::string("phi input: ") + hipGetErrorString(e)));
+    double* deriv = out[c0]->deriv;
+    if ((e = cmbl::launch_phi_mult(N, F, D, p->dk, s, cc)) != hipSuccess ||
+        (e = cmbl::launch_irfft2<double>(N, D, Pk, deriv, 5L * cc, tw64, s)) != hipSuccess)
+      return bail(fail(CMBL_EHIP, std::string("phi derivatives: ") + hipGetErrorString(e)));
+    int nb = 0;
+    if (p->dtype == CMBL_F32)
+      e = cmbl::launch_ptab<float>(N, deriv, (float*)out[c0]->ptab, nsteps, p->prep_min, &nb, s,
+                                   cc);
+    else
+      e = cmbl::launch_ptab<double>(N, deriv, (double*)out[c0]->ptab, nsteps, p->prep_min, &nb,
+                                    s, cc);
+    if (e != hipSuccess) return bail(fail(CMBL_EHIP, std::string("phi table: ") + hipGetErrorString(e)));
+    if ((e = hipMemcpyAsync(mp.data() + (size_t)c0 * nblk, p->prep_min, (size_t)cc * nb * sizeof(double),
+                            hipMemcpyDeviceToHost, s)) != hipSuccess)
+      return bail(fail(CMBL_EHIP, std::string("phi min det: ") + hipGetErrorString(e)));
+  }
+  if ((e = hipStreamSynchronize(s)) != hipSuccess)
+    return bail(fail(CMBL_EHIP, std::string("phi min det: ") + hipGetErrorString(e)));
+  for (int i = 0; i < count; ++i) {
+    double md = 1.0;
+    for (int b = 0; b < nblk; ++b) md = std::min(md, mp[(size_t)i * nblk + b]);
+    out[i]->min_det = md;
+    if (min_det) min_det[i] = md;
+  }
+  return CMBL_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -250,74 +377,26 @@ cmbl_status cmbl_grf_fourier(cmbl_plan* p, const double* cov, const double* mul,
 
 cmbl_status cmbl_phi_prepare(cmbl_plan* p, const void* phi, int phi_basis, int nsteps,
                              cmbl_phi** out, double* min_det, void* stream) {
-  if (!p || !phi || !out) return fail(CMBL_EINVAL, "NULL argument");
+  if (!out) return fail(CMBL_EINVAL, "NULL argument");
   *out = nullptr;
-  if (phi_basis != CMBL_PHI_MAP && phi_basis != CMBL_PHI_FOURIER)
-    return fail(CMBL_EINVAL, "phi_basis must be MAP or FOURIER");
-  if (nsteps < 1) return fail(CMBL_EINVAL, "nsteps must be >= 1");
-  hipStream_t s = (hipStream_t)stream;
-  const int N = p->N;
-  const size_t NN = (size_t)N * N, NH1 = (size_t)N * (N / 2 + 1), NH = (size_t)N * (N / 2);
-  cmbl_phi* h = new cmbl_phi();
-  h->N = N;
-  h->nsteps = nsteps;
-  h->dtype = p->dtype;
-  h->plan = p;
-  auto bail = [&](cmbl_status st) {
-    cmbl_phi_release(h);
-    return st;
-  };
-  hipError_t e;
-  const size_t ntab = (size_t)(2 * nsteps + 1) * 2 * NN;
-  const int nblk = (int)((NN + 255) / 256);
-  if ((!p->prep_F && (e = hipMalloc(&p->prep_F, NH1 * sizeof(cplx<double>))) != hipSuccess) ||
-      (!p->prep_D && (e = hipMalloc(&p->prep_D, 5 * NH1 * sizeof(cplx<double>))) != hipSuccess) ||
-      (!p->prep_Pk && (e = hipMalloc(&p->prep_Pk, 5 * NH * sizeof(cplx<double>))) != hipSuccess) ||
-      (!p->prep_min && (e = hipMalloc((void**)&p->prep_min, nblk * sizeof(double))) != hipSuccess))
-    return bail(fail(CMBL_ENOMEM, std::string("phi_prepare scratch: ") + hipGetErrorString(e)));
-  cplx<double>* F = (cplx<double>*)p->prep_F;
-  cplx<double>* D = (cplx<double>*)p->prep_D;
-  cplx<double>* Pk = (cplx<double>*)p->prep_Pk;
-  double* minpart = p->prep_min;
-  if ((e = hipMalloc(&h->deriv, 5 * NN * sizeof(double))) != hipSuccess ||
-      (e = hipMalloc(&h->ptab, ntab * p->tsize)) != hipSuccess)
-    return bail(fail(CMBL_ENOMEM, std::string("phi_prepare: ") + hipGetErrorString(e)));
-  const cplx<double>* tw64 = (const cplx<double>*)p->tw64;
-  if (phi_basis == CMBL_PHI_MAP) {
-    if ((e = cmbl::launch_rfft2<double>(N, (const double*)phi, Pk, F, 1, tw64, s)) != hipSuccess)
-      return bail(fail(CMBL_EHIP, std::string("phi rfft2: ") + hipGetErrorString(e)));
-  } else {
-    if ((e = hipMemcpyAsync(F, phi, NH1 * sizeof(cplx<double>), hipMemcpyDeviceToDevice, s)) !=
-        hipSuccess)
-      return bail(fail(CMBL_EHIP, std::string("phi copy: ") + hipGetErrorString(e)));
-  }
-  if ((e = cmbl::launch_phi_mult(N, F, D, p->dk, s)) != hipSuccess ||
-      (e = cmbl::launch_irfft2<double>(N, D, Pk, h->deriv, 5, tw64, s)) != hipSuccess)
-    return bail(fail(CMBL_EHIP, std::string("phi derivatives: ") + hipGetErrorString(e)));
-  int nb = 0;
-  if (p->dtype == CMBL_F32)
-    e = cmbl::launch_ptab<float>(N, h->deriv, (float*)h->ptab, nsteps, minpart, &nb, s);
-  else
-    e = cmbl::launch_ptab<double>(N, h->deriv, (double*)h->ptab, nsteps, minpart, &nb, s);
-  if (e != hipSuccess) return bail(fail(CMBL_EHIP, std::string("phi table: ") + hipGetErrorString(e)));
-  std::vector<double> mp(nb);
-  if ((e = hipMemcpyAsync(mp.data(), minpart, nb * sizeof(double), hipMemcpyDeviceToHost, s)) !=
-          hipSuccess ||
-      (e = hipStreamSynchronize(s)) != hipSuccess)
-    return bail(fail(CMBL_EHIP, std::string("phi min det: ") + hipGetErrorString(e)));
-  double md = 1.0;
-  for (double v : mp) md = std::min(md, v);
-  h->min_det = md;
-  if (min_det) *min_det = md;
-  *out = h;
-  return CMBL_OK;
+  return prepare_phis(p, phi, phi_basis, nsteps, 1, out, min_det, (hipStream_t)stream);
+}
+
+cmbl_status cmbl_phi_prepare_batch(cmbl_plan* p, const void* phis, int phi_basis, int nsteps,
+                                   int count, cmbl_phi** out, double* min_det, void* stream) {
+  if (!out) return fail(CMBL_EINVAL, "NULL argument");
+  if (count < 1) return fail(CMBL_EINVAL, "count must be >= 1");
+  for (int i = 0; i < count; ++i) out[i] = nullptr;
+  return prepare_phis(p, phis, phi_basis, nsteps, count, out, min_det, (hipStream_t)stream);
 }
 
 cmbl_status cmbl_phi_release(cmbl_phi* h) {
   if (!h) return CMBL_OK;
-  if (h->deriv) (void)hipFree(h->deriv);
-  if (h->ptab) (void)hipFree(h->ptab);
   if (h->gtab) (void)hipFree(h->gtab);
+  if (h->block && h->block->refs.fetch_sub(1) == 1) {
+    if (h->block->mem) (void)hipFree(h->block->mem);
+    delete h->block;
+  }
   delete h;
   return CMBL_OK;
 }
@@ -515,11 +594,6 @@ cmbl_status ensure_red(cmbl_plan* p, int nrhs) {
   return CMBL_OK;
 }
 
-#define CMBL_TRY(expr)                  \
-  do {                                  \
-    cmbl_status st_ = (expr);           \
-    if (st_ != CMBL_OK) return st_;     \
-  } while (0)
 
 template <typename T>
 cmbl_status run_gradf(cmbl_plan* p, const cmbl_phi* const* phis, const void* f, const void* d,

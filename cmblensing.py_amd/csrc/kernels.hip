@@ -275,6 +275,8 @@ __global__ void k_phi_mult(const cplx<double>* __restrict__ F, cplx<double>* __r
                            double dk) {
   const int H1 = N / 2 + 1;
   const long n = (long)N * H1;
+  F += blockIdx.y * n;          // phi number blockIdx.y of a batch
+  D += blockIdx.y * 5 * n;
   for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < n;
        idx += (long)gridDim.x * blockDim.x) {
     const int kx = (int)(idx / H1), ky = (int)(idx % H1);
@@ -298,6 +300,10 @@ __global__ __launch_bounds__(256) void k_ptab(const double* __restrict__ Dm, T* 
                                               int N, int nsteps, double* __restrict__ minpart) {
   const long P = (long)N * N;
   const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  // phi number blockIdx.y of a batch: its derivatives, table and partials
+  Dm += blockIdx.y * 5 * P;
+  ptab += blockIdx.y * (2L * nsteps + 1) * 2 * P;
+  minpart += blockIdx.y * gridDim.x;
   double mdet = 1.0;
   if (idx < P) {
     const double px = Dm[idx], py = Dm[P + idx], hxx = Dm[2 * P + idx], hxy = Dm[3 * P + idx],
@@ -812,18 +818,20 @@ hipError_t launch_irfft2(int N, const cplx<T>* X, cplx<T>* P, T* x, long nmaps,
 }
 
 hipError_t launch_phi_mult(int N, const cplx<double>* F, cplx<double>* D, double dk,
-                           hipStream_t s) {
+                           hipStream_t s, int count) {
+  if (count < 1) return hipSuccess;
   const long n = (long)N * (N / 2 + 1);
-  k_phi_mult<<<blocks_for(n, 256), 256, 0, s>>>(F, D, N, dk);
+  k_phi_mult<<<dim3(blocks_for(n, 256), count), 256, 0, s>>>(F, D, N, dk);
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_ptab(int N, const double* D, T* ptab, int nsteps, double* minpart,
-                       int* nblocks, hipStream_t s) {
+                       int* nblocks, hipStream_t s, int count) {
   const long P = (long)N * N;
-  *nblocks = (int)blocks_for(P, 256);
-  k_ptab<T><<<*nblocks, 256, 0, s>>>(D, ptab, N, nsteps, minpart);
+  *nblocks = (int)((P + 255) / 256);
+  if (count < 1) return hipSuccess;
+  k_ptab<T><<<dim3(*nblocks, count), 256, 0, s>>>(D, ptab, N, nsteps, minpart);
   return hipGetLastError();
 }
 
@@ -939,7 +947,7 @@ hipError_t launch_scatter(const T* src, T* const* dst, long len, int count, hipS
                                       hipStream_t);                                             \
   template hipError_t launch_irfft2<T>(int, const cplx<T>*, cplx<T>*, T*, long, const cplx<T>*, \
                                        hipStream_t);                                            \
-  template hipError_t launch_ptab<T>(int, const double*, T*, int, double*, int*, hipStream_t);  \
+  template hipError_t launch_ptab<T>(int, const double*, T*, int, double*, int*, hipStream_t, int);  \
   template hipError_t launch_lf_row_r2c<T>(int, const T*, cplx<T>*, long, const cplx<T>*,       \
                                            hipStream_t);                                        \
   template hipError_t launch_lf_col<T>(int, const cplx<T>*, cplx<T>*, long, const cplx<T>*, T, \

@@ -16,13 +16,16 @@ hipError_t launch_rfft2(int N, const T* x, cplx<T>* P, cplx<T>* X, long nmaps,
 template <typename T>
 hipError_t launch_irfft2(int N, const cplx<T>* X, cplx<T>* P, T* x, long nmaps,
                          const cplx<T>* tw, hipStream_t s);
-// Five derivative spectra of F (double precision, natural layout).
+// Five derivative spectra of F (double precision, natural layout), for
+// `count` phis: F [count][N][N/2+1] -> D [count][5][N][N/2+1].
 hipError_t launch_phi_mult(int N, const cplx<double>* F, cplx<double>* D, double dk,
-                           hipStream_t s);
-// Deflection-operator table and per-block minimum of det(I + t Hess phi).
+                           hipStream_t s, int count = 1);
+// Deflection-operator table and per-block minimum of det(I + t Hess phi), for
+// `count` phis: D [count][5][N][N] -> ptab [count][2n+1][2][N][N], minpart
+// [count][*nblocks].
 template <typename T>
 hipError_t launch_ptab(int N, const double* D, T* ptab, int nsteps, double* minpart,
-                       int* nblocks, hipStream_t s);
+                       int* nblocks, hipStream_t s, int count = 1);
 // LenseFlow stage passes.
 template <typename T>
 hipError_t launch_lf_row_r2c(int N, const T* y, cplx<T>* S0, long nmaps, const cplx<T>* tw,

@@ -41,9 +41,9 @@ def phi_handles_for(plan, phi_m: torch.Tensor, batch: int, n: int):
         return [_lib.get_phi(plan, phi_m, n, index=(0,) * len(lead))] * batch
     if count != batch or len(lead) != 1:
         raise ValueError(f"phi batch {tuple(lead)} does not match field batch {batch}")
-    # the cache must hold every handle of this call: an evicted handle is released
-    _lib.reserve_phi_cache(batch)
-    return [_lib.get_phi(plan, phi_m, n, index=i) for i in range(batch)]
+    # one batched preparation for the whole batch (the cache holds every handle
+    # of the call: an evicted handle is released)
+    return _lib.get_phi_batch(plan, phi_m, n)
 
 
 def lense_flow_matrices(x: torch.Tensor, phi_m: torch.Tensor, theta_pix: float, n: int,

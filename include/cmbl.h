@@ -63,6 +63,17 @@ cmbl_status cmbl_plan_destroy(cmbl_plan* plan);
  * be issued concurrently from several host threads. */
 cmbl_status cmbl_phi_prepare(cmbl_plan* plan, const void* phi, int phi_basis, int nsteps,
                              cmbl_phi** out, double* min_det, void* stream);
+/* Batched phi preparation: `count` phis (contiguous [count][N][N] float64
+ * MAP or [count][N][N/2+1] complex float64 FOURIER) prepared in one call:
+ * their tables share one allocation, each preparation stage is one launch
+ * over all of them, and the host waits once (for the min_det values, an array
+ * of `count`, may be NULL).  out receives `count` handles, each released on
+ * its own with cmbl_phi_release (the shared memory goes with the last).
+ * Replaces the per-element preparation of a field batch with a phi per
+ * element (lense_flow.py:297-302 applied to each; the reference recomputes
+ * the derivatives on every call). */
+cmbl_status cmbl_phi_prepare_batch(cmbl_plan* plan, const void* phis, int phi_basis, int nsteps,
+                                   int count, cmbl_phi** out, double* min_det, void* stream);
 cmbl_status cmbl_phi_release(cmbl_phi* phi);
 /* Device pointer of the phi derivative maps, float64 [5][N][N]
  * (phi_x, phi_y, phi_xx, phi_xy, phi_yy). */

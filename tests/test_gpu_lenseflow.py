@@ -167,3 +167,42 @@ def test_small_and_large_launch_shapes_agree(pkg, dev, sims, adjoint):
     for s in (0, 37):
         ys = pkg.lense_flow_matrices(x[s:s + 1].contiguous(), P, th, n, 1, adjoint)
         assert torch.equal(ys[0], yb[s])
+
+
+@pytest.mark.parametrize("basis", ["map", "fourier"])
+def test_phi_prepare_batch_equals_single(pkg, dev, basis):
+    """cmbl_phi_prepare_batch (70 phis: two preparation chunks) gives the
+    derivatives, min_det and LenseFlow results of one-at-a-time preparation,
+    bit for bit; handles are released in any order."""
+    print(f"[case] phi prepare batch {basis}", flush=True)
+    from cmblensing_amd import _lib
+    N, th, n, B = 32, 2.0, 5, 70
+    rng = np.random.default_rng(11)
+    maps = np.stack([sims_phi(N, th, rng) for _ in range(B)])
+    phis = torch.as_tensor(maps if basis == "map" else np.fft.rfft2(maps), device=dev)
+    plan = _lib.Plan(N, th, torch.float64, B, dev)
+    batch = _lib.prepare_batch(plan, phis, n)
+    single = [_lib.PhiHandle(plan, phis[i], n) for i in range(B)]
+    x = torch.as_tensor(rng.standard_normal((B, 1, N, N)), device=dev)
+    ya, yb = torch.empty_like(x), torch.empty_like(x)
+    _lib.lenseflow(plan, batch, x, ya, B, 1, n, 1, False)
+    _lib.lenseflow(plan, single, x, yb, B, 1, n, 1, False)
+    assert torch.equal(ya, yb)
+    for i in (0, 63, 64, 69):
+        assert torch.equal(batch[i].derivatives(), single[i].derivatives())
+        assert batch[i].min_det == single[i].min_det
+    for i in list(range(1, B, 2)) + list(range(0, B, 2)):   # odd first, then even
+        batch[i].close()
+    for h in single:
+        h.close()
+    plan.close()
+
+
+def sims_phi(N, th, rng):
+    """A smooth random potential with a small deflection (no caustics)."""
+    F = np.fft.rfft2(rng.standard_normal((N, N)))
+    kx = np.fft.fftfreq(N)[:, None]
+    ky = np.fft.rfftfreq(N)[None, :]
+    F *= np.exp(-40 * (kx ** 2 + ky ** 2))
+    m = np.fft.irfft2(F)
+    return m * (1e-6 / np.abs(m).max())
