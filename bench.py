@@ -419,6 +419,47 @@ def map_joint_run(torch, sims, dev, N=256, th=2.0, num_steps=30, trials=16, n=10
                 [corr(phib.scalar_matrix[t], many.phi.scalar_matrix[t]) for t in range(trials)])), 3)}
 
 
+def config0_run(torch, sims, dev, B=256, nphi=32, steps=20, plain=20):
+    """The reference's CPU-runnable case: T, N = 128, 3 arcmin, float64, n = 7,
+    forward LenseFlow.  Batched: B requests over `nphi` resident phi through
+    cmbl_lenseflow; plain: one request per call with its phi prepared in the
+    call.  Returns the leg and one batched result for the oracle check."""
+    from cmblensing_amd import _lib
+    f0, phi0, th, n = sims.lenseflow_inputs(0, seed=0)
+    N = f0.shape[-1]
+    phis = [torch.as_tensor(sims.phi_map(N, th, 200 + i), device=dev) for i in range(nphi)]
+    phis[0] = torch.as_tensor(phi0, device=dev)
+    x = torch.as_tensor(np.stack([f0] * B), device=dev)          # [B, 1, N, N] float64
+    plan = _lib.get_plan(N, th, torch.float64, B, dev)
+    hs = [_lib.get_phi(plan, p, n) for p in phis]
+    req = [hs[r % nphi] for r in range(B)]
+    out = torch.empty_like(x)
+    for _ in range(3):
+        _lib.lenseflow(plan, req, x, out, B, 1, n, 1, False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _lib.lenseflow(plan, req, x, out, B, 1, n, 1, False)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    lat = []
+    x1, o1 = x[:1].contiguous(), torch.empty_like(x[:1])
+    for r in range(plain + 3):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        h = _lib.PhiHandle(plan, phis[r % nphi], n)
+        _lib.lenseflow(plan, [h], x1, o1, 1, 1, n, 1, False)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t1)
+        h.close()
+    lat = np.array(lat[3:]) * 1e3
+    leg = {"workload": "lenseflow_fwd_T_128_3arcmin_n7_fp64", "batch": B,
+           "batched_maps_s": round(B * steps / dt, 1), "plain_maps_s": round(1e3 / lat.mean(), 1),
+           "plain_latency_ms_p50": round(float(np.percentile(lat, 50)), 4),
+           "model_gbs": round(model_bytes_per_map(N, 1, n, 8) * B * steps / dt / 1e9, 1)}
+    return leg, out[0].cpu().numpy(), f0, phi0, th, n
+
+
 def copy_probe(torch):
     """A plain copy of known size: FETCH_SIZE / WRITE_SIZE of this kernel give
     the scale factors of the HBM counters, and its time the achievable copy
@@ -462,6 +503,7 @@ def main():
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--no-hmc", action="store_true")
     ap.add_argument("--no-sim", action="store_true")
+    ap.add_argument("--no-config0", action="store_true")
     ap.add_argument("--no-map-joint", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device-to-device copy (HBM counter calibration)")
@@ -641,6 +683,11 @@ def main():
     if not args.no_hmc:
         print("[bench] HMC, 16 chains", file=sys.stderr, flush=True)
         result["hmc"] = hmc_run(torch, pkg, sims, dev)
+    c0_leg = c0_chk = None
+    if not args.no_config0:
+        print("[bench] T 128^2 fp64 (CPU-runnable case)", file=sys.stderr, flush=True)
+        c0_leg, *c0_chk = config0_run(torch, sims, dev)
+        result["config0"] = c0_leg
     if not args.no_sim:
         print("[bench] simulated data sets, 64 trials 256^2 P", file=sys.stderr, flush=True)
         result["simulation"] = sim_run(torch, sims, dev)
@@ -661,6 +708,15 @@ def main():
                                   "kind": "port",
                                   "sample": f"{k} forward LenseFlows of one 256^2 QU pair, float64 "
                                             f"numpy oracle, {el:.1f} s"}
+        if c0_leg is not None:
+            # the T 128^2 FP64 case on the same core: its oracle check and CPU rate
+            got0, f0, phi0, th0, n0 = c0_chk
+            ref0 = olf.lense_flow(f0, phi0, olf.pix_width_from_theta(th0), n0)
+            c0_leg["rel_l2_vs_fp64_oracle"] = float(np.linalg.norm(got0 - ref0) /
+                                                    np.linalg.norm(ref0))
+            cps0, k0, el0 = cpu_baseline(f0, phi0, th0, n0, min(args.cpu_budget, 5.0))
+            c0_leg["cpu_maps_s"] = round(cps0, 3)
+            c0_leg["cpu_sample"] = f"{k0} forward LenseFlows, float64 numpy oracle, 1 core, {el0:.1f} s"
     print(json.dumps(result), flush=True)
 
 
