@@ -213,6 +213,56 @@ def conjugate_gradient_process(x0, data, phi, mask, beam, noise_covariance, fiel
                   rel_history=[[float(v)] for v in rel] if rel is not None else [])
     act = active_mask()
     it = 0
+    if rtol is None and not recompute_every and bool(act.any()):
+        # The reference's stop rule only (absolute tol on dot(r, z), maxiter):
+        # alpha, beta and the per-RHS active mask stay on the device (the same
+        # float64 divisions, so the same iterates), and the host reads whether
+        # any RHS is still active one iteration late, so the next iteration is
+        # always queued while the device runs the current one.  The iteration
+        # queued after the last active one has alpha = beta = 0 for every RHS:
+        # it leaves x, the residuals and the counts unchanged.
+        res_d = res_h.to(dev)
+        cur_d = res_curr.to(dev)
+        k_d = torch.zeros(B, dtype=torch.long, device=dev)
+        act_d = (cur_d >= tol) & (k_d < maxiter)
+        hist = []
+        pending = None
+        while True:
+            Ap = prob.A(p_op)                                 # :82
+            pAp = prob.dot(p, Ap)
+            alpha = torch.where(act_d, res_d / torch.where(pAp != 0, pAp, torch.ones_like(pAp)),
+                                torch.zeros_like(res_d))      # :84
+            rz = torch.empty(B, dtype=torch.float64, device=dev)
+            _lib.cg_update_xrz(prob.plan, torch.view_as_real(x), torch.view_as_real(r),
+                               torch.view_as_real(z), torch.view_as_real(p),
+                               torch.view_as_real(Ap), pinv_arr, alpha, B, prob.C, rz)  # :86-92
+            beta = torch.where(act_d, rz / torch.where(res_d != 0, res_d, torch.ones_like(res_d)),
+                               torch.zeros_like(res_d))       # :94
+            _lib.cg_update_p(prob.plan, torch.view_as_real(p), torch.view_as_real(z), beta,
+                             torch.view_as_real(p_op), B, prob.C)              # :95
+            cur_d = torch.where(act_d, rz, cur_d)
+            res_d = torch.where(act_d, rz, res_d)
+            k_d = k_d + act_d.long()
+            hist.append((act_d, cur_d))
+            act_d = (cur_d >= tol) & (k_d < maxiter)          # :75-77
+            flag = act_d.any()
+            ev = torch.cuda.Event()
+            ev.record(torch.cuda.current_stream(dev))
+            if pending is not None:
+                yield pending[0]
+                if not bool(pending[1].cpu()):
+                    break
+            pending = (ev, flag)
+        res_curr = cur_d.cpu()
+        k = k_d.cpu()
+        if hist:
+            acts = torch.stack([h[0] for h in hist]).cpu()
+            curs = torch.stack([h[1] for h in hist]).cpu()
+            for a_row, c_row in zip(acts, curs):
+                for i in range(B):
+                    if bool(a_row[i]):
+                        info.res_history[i].append(float(c_row[i]))
+        act = torch.zeros(B, dtype=torch.bool)
     while bool(act.any()):
         Ap = prob.A(p_op)                                     # :82
         pAp = yield from _host(prob.dot(p, Ap))
