@@ -410,7 +410,19 @@ def map_joint_run(torch, sims, dev, N=256, th=2.0, num_steps=30, trials=16, n=10
         a, b = a.flatten(), b.flatten()
         return float(torch.real(torch.vdot(a, b)) / torch.sqrt(torch.vdot(a, a).real *
                                                              torch.vdot(b, b).real))
+    # one Gibbs f-step (sampling_ar.py:14-46) for the 16 trials as chains:
+    # device draws, one batched LenseFlow, a 16-RHS Wiener filter
+    sampling = importlib.import_module("cmblensing_amd.sampling")
+    gargs = dict(field_covariance=many.field_covariance, noise_covariance=many.noise_covariance,
+                 mask=many.mask, beam=many.beam)
+    sampling.gibbs_sample_f(fb, many.data, phib, gargs, seed=1, n=n)      # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sampling.gibbs_sample_f(fb, many.data, phib, gargs, seed=2, n=n)
+    torch.cuda.synchronize()
+    dtg = time.perf_counter() - t0
     return {"workload": f"map_joint_T_{N}_{th:g}arcmin_{num_steps}steps_fp64",
+            "gibbs_f_step_16_chains_s": round(dtg, 3),
             "s_per_run_bfgs": round(dt1, 3),
             "batched_grid_trials": trials, "s_batched_grid": round(dtb, 3),
             "runs_per_s_batched_grid": round(trials / dtb, 2),

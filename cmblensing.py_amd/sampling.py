@@ -13,11 +13,16 @@ of them.  The integrator, Hamiltonian and accept rule are the reference's:
 with 30 steps of 0.01 by default and the mass matrix G^-2 (Cphi^-1 + Nphi^-1).
 Momenta are drawn like simulate.py:106-113: sqrt(M/2)(a + i b), a, b standard
 normal, taken to a map by an orthonormal irfft2 and back by rfft2.
+
+The Gibbs steps of sampling_ar.py:14-62 complete the sampler: the f-step is a
+constrained realisation (device draws, one batched LenseFlow, a Wiener
+filter), the phi-step the HMC step above.
 """
 from __future__ import annotations
 
 from typing import Optional
 
+import numpy as np
 import torch
 
 from .constants import DEFAULT_NSTEPS
@@ -93,3 +98,74 @@ def hmc_step(x, always_accept: bool, mass_matrix, mixed_field, data, noise_covar
     else:
         new = tm if bool(accept[0]) else xm
     return x.replace(scalar_matrix=new), delta_h, accept
+
+
+# --------------------------------------------------------------------------
+# Gibbs steps (sampling_ar.py:14-62)
+# --------------------------------------------------------------------------
+def _diag_planes(op, what: str) -> torch.Tensor:
+    from .matrix_operators import DiagonalEB, DiagonalScalar
+    if not isinstance(op, (DiagonalScalar, DiagonalEB)):
+        raise TypeError(f"{what}: the f-step draws from DiagonalScalar or DiagonalEB "
+                        f"operators, got {type(op).__name__}")
+    return op.per_mode().to(torch.float64)
+
+
+def gibbs_sample_f(field_start, data_field, phi, args: dict, seed: int, trial0: int = 0,
+                   n: int = 10, maxiter: int = 500, tol: float = 1e-1):
+    """sampling_ar.py:14-46: a constrained realisation of f given phi and the
+    data.  Draws f' ~ N(0, Cf) and n' ~ N(0, Cn) on the device, forms their
+    data d' = M B L(phi) f' + n' (one batched LenseFlow with 10 RK4 steps, the
+    noise drawn into the data in place), and returns f' + WF(d - d') with the
+    Wiener filter started from field_start.  field_start (and phi) may carry a
+    leading chain axis: chain c then draws from the streams of trial trial0 + c
+    (kinds 1..3 fields, 4..6 noise, like simulate.simulate).  args holds
+    field_covariance, noise_covariance, mask and beam (DiagonalScalar for T,
+    DiagonalEB for EB fields).  Returns a field like field_start."""
+    from .fields import FlatS0, FlatS2
+    from .lense_flow import _phi_matrix
+    from .simulate import KIND_E, KIND_T, grf_fourier, lens_fields
+    from .wiener_filter import wiener_filter
+    Cf, Cn = args["field_covariance"], args["noise_covariance"]
+    M, B = args["mask"], args["beam"]
+    if isinstance(field_start, FlatS0):
+        names, k0 = ["scalar_matrix"], KIND_T
+    elif isinstance(field_start, FlatS2):
+        names, k0 = ["polar_matrix_1", "polar_matrix_2"], KIND_E
+    else:
+        raise TypeError("gibbs_sample_f takes FlatS0 (T) or FlatS2 (EB) fields")
+    m0 = getattr(field_start, names[0])
+    lead = tuple(m0.shape[:-2])
+    ntr = int(np.prod(lead)) if lead else 1
+    th = field_start.theta_pix
+    N = m0.shape[-2]
+    rdt = torch.float32 if m0.dtype in (torch.complex64, torch.float32) else torch.float64
+    new = grf_fourier(_diag_planes(Cf, "field_covariance"), ntr, seed, (k0 << 32) + trial0, th,
+                      rdt)
+    phi_m = _phi_matrix(phi)
+    lensed = lens_fields(new, phi_m.reshape(ntr, *phi_m.shape[-2:]) if phi_m.dim() > 2 else phi_m,
+                         N, th, n)
+    mb = _diag_planes(M, "mask") * _diag_planes(B, "beam")
+    new_data = grf_fourier(_diag_planes(Cn, "noise_covariance"), ntr, seed,
+                           ((k0 + 3) << 32) + trial0, th, rdt, mul=mb, add=lensed)
+
+    def as_field(X):
+        return field_start.replace(**{nm: X[:, c].reshape(*lead, *X.shape[-2:]) if lead
+                                      else X[0, c] for c, nm in enumerate(names)})
+    data_diff = data_field - as_field(new_data)
+    delta = wiener_filter(field_start, phi, data_diff, Cf, Cn, M, B, n=n, maxiter=maxiter,
+                          tol=tol)
+    return as_field(new) + delta
+
+
+def gibbs_sample_phi(mixed_phi, mixed_temp, data_field, generator: torch.Generator, args: dict,
+                     iteration: int, num_burn_in_always_accept: int, num_steps: int = 30,
+                     step_size: float = 0.01, n: int = DEFAULT_NSTEPS):
+    """sampling_ar.py:49-62: one HMC step of the mixed phi (always accepted
+    during burn-in) with the mass matrix G^-2 (Cphi^-1 + QE^-1)."""
+    always_accept = iteration < num_burn_in_always_accept
+    mass = get_mass_matrix(args["phi_covariance"], args["quadratic_estimate"], args["mixing_g"])
+    return hmc_step(mixed_phi, always_accept, mass, mixed_temp, data_field,
+                    args["noise_covariance"], args["phi_covariance"], args["field_covariance"],
+                    args["mask"], args["beam"], args["mixing_d"], args["mixing_g"], generator,
+                    num_steps=num_steps, step_size=step_size, n=n)

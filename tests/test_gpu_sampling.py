@@ -144,3 +144,73 @@ def test_hmc_step_accept_and_chains_independent(pkg, dev, sims):
     x_new, dh, acc = sampling.hmc_step(pm, True, mass, *args, gen, num_steps=2, n=n)
     assert bool(acc.all()) and dh.shape == (3,)
     assert not torch.equal(x_new.scalar_matrix, pm.scalar_matrix)
+
+
+# ------------------------------------------------------------- Gibbs f-step
+def _gibbs_setup(pkg, dev, pol, N=32, th=2.0):
+    from cmblensing_amd import simulate
+    from oracle import simulate as osim
+    from conftest import ROOT
+    import __graft_entry__ as ge
+    sims = ge._load_module("cmbl_sims_g", ROOT / "cmblensing.py_amd" / "sims.py")
+    cls = osim.extract_all_cls(*sims.camb_like_spectra(5000), 17000, 5000)
+    cfg = simulate.sim_config(N, th, pol, cls, uk_arcmin_t=1.0, with_qe=False, device=dev)
+    ds = simulate.build_dataset(cfg, *simulate.simulate(cfg, 21, 1, torch.float64, n=10))
+    args = dict(field_covariance=ds.field_covariance, noise_covariance=ds.noise_covariance,
+                mask=ds.mask, beam=ds.beam)
+    return ds, args
+
+
+def _planes(op):
+    return op.per_mode().cpu().numpy()
+
+
+def _fstack(f):
+    return np.stack([getattr(f, nm).cpu().numpy() for nm in f._matrix_names()], axis=-3)
+
+
+@pytest.mark.parametrize("pol", ["I", "P"])
+def test_gibbs_sample_f_matches_oracle(pkg, dev, pol):
+    """Two chains, each with its own phi and start, FP64: the device draws,
+    data and Wiener filter against oracle/gibbs.py (1e-9 relative L2)."""
+    from cmblensing_amd import sampling
+    from oracle import gibbs as ogb
+    _case(f"gibbs f-step vs oracle {pol}")
+    ds, args = _gibbs_setup(pkg, dev, pol)
+    phi2 = torch.stack([ds.phi.scalar_matrix, 1.1 * ds.phi.scalar_matrix])
+    phi = ds.phi.replace(scalar_matrix=phi2)
+    f0 = ds.data.replace(**{nm: torch.stack([0.5 * getattr(ds.data, nm), 0 * getattr(ds.data, nm)])
+                            for nm in ds.data._matrix_names()})
+    out = sampling.gibbs_sample_f(f0, ds.data, phi, args, seed=77, trial0=3, n=10, maxiter=15,
+                                  tol=1e-6)
+    got = _fstack(out)
+    d = _fstack(ds.data)
+    Cf, Cn, M, B = (_planes(args[k]) for k in ("field_covariance", "noise_covariance", "mask",
+                                               "beam"))
+    f0n = _fstack(f0)
+    for t in range(2):
+        ref = ogb.gibbs_sample_f(f0n[t], d, phi2[t].cpu().numpy(), Cf, Cn, M, B, 2.0, 77,
+                                 trial=3 + t, n=10, maxiter=15, tol=1e-6)
+        assert rel(got[t], ref) < 1e-9
+
+
+def test_gibbs_sample_f_mean_is_the_wiener_filter(pkg, dev):
+    """E[f' + WF(d - d')] = WF(d): the mean of 64 constrained realisations
+    lies within the sampling error (~1/8 of one draw's distance) of WF(d)."""
+    from cmblensing_amd import sampling, wiener_filter as wfm
+    _case("gibbs f-step mean = WF(d)")
+    ds, args = _gibbs_setup(pkg, dev, "I")
+    K = 64
+    f0 = ds.data.replace(scalar_matrix=torch.zeros((K, *ds.data.scalar_matrix.shape),
+                                                   dtype=ds.data.scalar_matrix.dtype, device=dev))
+    out = sampling.gibbs_sample_f(f0, ds.data, ds.phi, args, seed=5, n=10, maxiter=300,
+                                  tol=1e-10)
+    wf = wfm.wiener_filter(0 * ds.data, ds.phi, ds.data, args["field_covariance"],
+                           args["noise_covariance"], args["mask"], args["beam"], n=10,
+                           maxiter=300, tol=1e-10)
+    s = out.scalar_matrix
+    w = wf.scalar_matrix
+    spread = float(torch.sqrt(torch.mean(torch.sum(torch.abs(s - w) ** 2, dim=(-2, -1)))))
+    err = float(torch.sqrt(torch.sum(torch.abs(s.mean(0) - w) ** 2)))
+    print(f"mean error {err:.4g}, one-draw spread {spread:.4g}, ratio {err / spread:.3f}")
+    assert err < 0.3 * spread
