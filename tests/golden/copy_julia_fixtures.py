@@ -41,6 +41,11 @@ NAMES = [
     # (test_covariance_matrices.py, test_simulated_cls.py)
     "nphi_I", "nphi_P", "nphi_IP", "ell_I", "ell_P", "ell_IP", "cl_ll_avg_I", "cl_tt_avg_I",
 ] + [f"cl_{k}_{c}_avg_{p}" for k in ("ll", "tt") for c in ("e", "b") for p in ("P", "IP")] + [
+    # the Wiener filter from every start (field or 0) and phi (phi or 0)
+    # (test_wiener_filter.py): the rest of the matrix
+    "f_wf_ffp0_I",
+] + [f"f_wf_{s}_{c}_P" for s in ("ffp0", "f0pp", "f0p0") for c in ("e", "b")] + [
+    f"f_wf_{s}_{c}_IP" for s in ("ffp0", "f0pp", "f0p0") for c in ("t", "e", "b")] + [
     # MAP-joint outputs after 30 steps (test_map_joint.py)
     "fJ_I", "phiJ_I", "fJ_e_P", "fJ_b_P", "phiJ_P", "fJ_t_IP", "fJ_e_IP", "fJ_b_IP", "phiJ_IP",
 ]

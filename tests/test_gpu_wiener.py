@@ -214,3 +214,56 @@ def test_wiener_filter_mixed_precision_batch(pkg, dev, sims):
         err = rel(xs[i], ref)
         print("rhs", i, "rel err vs FP64 oracle WF", err)
         assert err < 1e-4
+
+
+def _julia_wf_problem(pkg, dev, pol):
+    """The reference's test_wiener_filter.py problem for I, P or IP (CMBLensing.jl
+    fixtures, 256^2 at 2.5 arcmin, FP64): truth field, phi, data, operators."""
+    from cmblensing_amd import matrix_operators as mo
+    J = GOLDEN / "julia"
+    L = lambda n: torch.as_tensor(np.load(J / f"{n}.npy", allow_pickle=False), device=dev)
+    kw = dict(theta_pix=2.5)
+    if pol == "I":
+        f = pkg.FlatS0(scalar_matrix=L("t_field_I"), **kw)
+        d = pkg.FlatS0(scalar_matrix=L("data_t_field_I"), **kw)
+        op = lambda o: mo.DiagonalScalar(scalar_matrix=L(f"{o}_I"), **kw)
+    elif pol == "P":
+        f = pkg.FlatS2(polar_matrix_1=L("e_field_P"), polar_matrix_2=L("b_field_P"), **kw)
+        d = pkg.FlatS2(polar_matrix_1=L("data_e_field_P"), polar_matrix_2=L("data_b_field_P"), **kw)
+        op = lambda o: mo.DiagonalEB(matrix_EE=L(f"{o}_ee_P"), matrix_BB=L(f"{o}_bb_P"), **kw)
+    else:
+        f = pkg.FlatS02(scalar_matrix=L("t_field_IP"), polar_matrix_1=L("e_field_IP"),
+                        polar_matrix_2=L("b_field_IP"), **kw)
+        d = pkg.FlatS02(scalar_matrix=L("data_t_field_IP"), polar_matrix_1=L("data_e_field_IP"),
+                        polar_matrix_2=L("data_b_field_IP"), **kw)
+        op = lambda o: mo.BlockTEB(matrix_TT=L(f"{o}_tt_IP"), matrix_TE=L(f"{o}_te_IP"),
+                                   matrix_ET=L(f"{o}_te_IP"), matrix_EE=L(f"{o}_ee_IP"),
+                                   matrix_BB=L(f"{o}_bb_IP"), **kw)
+    phi = pkg.FlatS0(scalar_matrix=L(f"phi_{pol}"), **kw)
+    return f, phi, d, op("cf"), op("cn"), op("m"), op("b"), L
+
+
+_WF_COMPONENTS = {"I": [""], "P": ["_e", "_b"], "IP": ["_t", "_e", "_b"]}
+
+
+@pytest.mark.parametrize("pol", ["I", "P", "IP"])
+@pytest.mark.parametrize("start", ["ffpp", "ffp0", "f0pp", "f0p0"])
+def test_wiener_filter_julia_start_matrix(pkg, dev, pol, start):
+    """test_wiener_filter.py's matrix: the Wiener filter from the truth field
+    or 0 (ff / f0) with phi or 0 * phi (pp / p0), the reference's stop rule
+    (absolute tol 0.1, maxiter 500), against CMBLensing.jl's f_wf_<start>_*.
+    With phi = 0 the preconditioner is A itself and one iteration solves the
+    system (measured 1e-15); with phi the solutions agree to 4e-6 - 7e-6."""
+    from cmblensing_amd import wiener_filter as wfm
+    _case(f"wf julia {pol} start {start} fp64 tol=0.1")
+    f, phi, d, Cf, Cn, M, B, L = _julia_wf_problem(pkg, dev, pol)
+    x0 = f if start[1] == "f" else 0 * f
+    ph = phi if start[2:] == "pp" else 0 * phi
+    x, info = wfm.wiener_filter(x0, ph, d, Cf, Cn, M, B, n=7, return_info=True)
+    got = stack(x)
+    want = np.stack([L(f"f_wf_{start}{c}_{pol}").cpu().numpy() for c in _WF_COMPONENTS[pol]])
+    got = got.reshape(want.shape)
+    r_all, r_int = rel(got, want), rel(got[..., 1:-1], want[..., 1:-1])
+    print(f"{pol} {start}: iterations {info.iterations}, rel L2 {r_all:.3g}, interior {r_int:.3g}",
+          flush=True)
+    assert r_all < (1e-13 if start.endswith("p0") else 2e-5)
