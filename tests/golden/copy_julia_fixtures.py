@@ -41,6 +41,8 @@ NAMES = [
     # (test_covariance_matrices.py, test_simulated_cls.py)
     "nphi_I", "nphi_P", "nphi_IP", "ell_I", "ell_P", "ell_IP", "cl_ll_avg_I", "cl_tt_avg_I",
 ] + [f"cl_{k}_{c}_avg_{p}" for k in ("ll", "tt") for c in ("e", "b") for p in ("P", "IP")] + [
+    # adjoint lensing of the T+EB fields (test_lensing.py)
+    "t_adjoint_lensed_field_IP", "e_adjoint_lensed_field_IP", "b_adjoint_lensed_field_IP",
     # the Wiener filter from every start (field or 0) and phi (phi or 0)
     # (test_wiener_filter.py): the rest of the matrix
     "f_wf_ffp0_I",
