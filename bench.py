@@ -73,6 +73,88 @@ def cpu_baseline(f, phi, th, n, budget_s=10.0):
     return k / el, k, el
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads():
+    """Host threads for the multi-core CPU baseline: the cores this process
+    may run on, at most 16 (one GPU's share of the host)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def cpu_baseline_threads(fs, phi, th, n, threads, budget_s=10.0):
+    """The FP64 oracle over many requests at once: `threads` threads, each
+    lensing one request at a time (numpy releases the interpreter lock in its
+    FFTs and array arithmetic), until `budget_s` has passed.  Returns maps/s,
+    the requests done and the elapsed time."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import lenseflow as olf
+    pw = olf.pix_width_from_theta(th)
+    t0 = time.perf_counter()
+    done = [0] * threads
+
+    def worker(w):
+        i = w
+        while time.perf_counter() - t0 < budget_s:
+            olf.lense_flow(fs[i % len(fs)], phi, pw, n)
+            done[w] += 1
+            i += threads
+
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(worker, range(threads)))
+    el = time.perf_counter() - t0
+    return sum(done) / el, sum(done), el
+
+
+LF_MODES = {"forward": (1, False), "inverse": (-1, False), "adjoint": (1, True),
+            "inverse_adjoint": (-1, True)}
+
+
+def config1_run(torch, _lib, plan, phis, x, B, C, n, steps, warmup, timed, sims, N, th, r_chk):
+    """Q/U 256^2 FP32, 7 RK4 steps: batched maps/s of forward, inverse,
+    adjoint and inverse adjoint LenseFlow (B requests, resident phi, direct
+    batched entry point); the round trip ||L^-1 L f - f|| / ||f||; the adjoint
+    dot-product test |<L y, x> - <y, L^T x>| / |<L y, x>| with x a second set
+    of seeded Q/U fields (sums in float64).  Returns the leg and request
+    r_chk's four outputs for the oracle check of the CPU leg."""
+    out = torch.empty_like(x)
+    leg, outs = {}, {}
+    for name, (d, a) in LF_MODES.items():
+        dt = timed(lambda: _lib.lenseflow(plan, phis, x, out, B, C, n, d, a), steps, warmup)
+        leg[name] = {"batched_maps_s": round(B * steps / dt, 1),
+                     "ms_per_batch": round(dt / steps * 1e3, 4)}
+        outs[name] = out[r_chk].double().cpu().numpy()
+    fwd, back = torch.empty_like(x), torch.empty_like(x)
+    _lib.lenseflow(plan, phis, x, fwd, B, C, n, 1, False)
+    _lib.lenseflow(plan, phis, fwd, back, B, C, n, -1, False)
+    x64 = x.double().flatten(1)
+    rt = ((back.double().flatten(1) - x64).norm(dim=1) / x64.norm(dim=1))
+    xs = torch.as_tensor(np.stack([sims.qu_map(N, th, 300 + r % 8) for r in range(B)]),
+                         dtype=x.dtype, device=x.device)
+    ltx = torch.empty_like(x)
+    _lib.lenseflow(plan, phis, xs, ltx, B, C, n, 1, True)
+    lhs = (fwd.double().flatten(1) * xs.double().flatten(1)).sum(dim=1)
+    rhs = (x64 * ltx.double().flatten(1)).sum(dim=1)
+    dot_rel = ((lhs - rhs).abs() / lhs.abs())
+    torch.cuda.synchronize()
+    leg.update({
+        "workload": "lenseflow_QU_256_2arcmin_n7_fp32 (forward, inverse, adjoint)",
+        "batch": B,
+        "round_trip_rel_l2_max": float(rt.max()),
+        "round_trip_rel_l2_median": float(rt.median()),
+        "adjoint_dot_test_rel_max": float(dot_rel.max()),
+        "adjoint_dot_test_rel_median": float(dot_rel.median()),
+    })
+    return leg, outs
+
+
 def serving_run(torch, sims, dev, n_requests, rate, seed=2, nphi=32, budget_gib=64.0,
                 wf_every=0, pkg=None):
     """The mixed serving stream: Poisson arrivals at `rate`, N in {64, 128,
@@ -634,6 +716,11 @@ def main():
         lat_res.append(time.perf_counter() - t1)
     lat_res = np.array(lat_res) * 1e3
 
+    # the 256^2 QU configuration in all four modes, its round trip and its
+    # adjoint dot-product test (the oracle errors are added in the CPU leg)
+    c1_leg, c1_chk = config1_run(torch, _lib, plan, req_phi, x, B, C, n, args.steps,
+                                 args.warmup, timed, sims, N, th, r_chk)
+
     # HBM traffic per launch of the row pass, from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated on a copy)
     traffic = None
@@ -676,6 +763,7 @@ def main():
                         "achieved_gbs": round(col_gbs, 1),
                         "avg_launch_us": round(col_avg_ms * 1e3, 3),
                         "algorithmic_bytes_per_launch": col_bytes},
+        "config1": c1_leg,
     }
     for pk, ex, *_ in lanes:
         ex.close()
@@ -714,12 +802,31 @@ def main():
         from oracle import lenseflow as olf
         ref = olf.lense_flow(fq[r_chk % 8], phis[r_chk % NPHI], olf.pix_width_from_theta(th), n)
         result["rel_l2_vs_fp64_oracle"] = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
-        fcpu = fq[0]
-        cps, k, el = cpu_baseline(fcpu, phis[0], th, n, args.cpu_budget)
-        result["cpu_baseline"] = {"value": round(cps, 4), "unit": "maps/s", "cores": 1,
+        # the four modes of the 256^2 QU configuration against the oracle, and
+        # the oracle's own round trip (the RK4 discretisation at n = 7)
+        pw = olf.pix_width_from_theta(th)
+        for name, (d, a) in LF_MODES.items():
+            ref_m = olf.lense_flow(fq[r_chk % 8], phis[r_chk % NPHI], pw, n, d, a)
+            c1_leg[name]["rel_l2_vs_fp64_oracle"] = float(
+                np.linalg.norm(c1_chk[name] - ref_m) / np.linalg.norm(ref_m))
+        back = olf.lense_flow(ref, phis[r_chk % NPHI], pw, n, -1, False)
+        c1_leg["oracle_fp64_round_trip_rel_l2"] = float(
+            np.linalg.norm(back - fq[r_chk % 8]) / np.linalg.norm(fq[r_chk % 8]))
+        # the same workload on the host: Q/U 256^2 requests (8 seeded pairs,
+        # one resident phi), forward, n = 7, FP64 oracle, threads over requests
+        nth = cpu_threads()
+        cps, k, el = cpu_baseline_threads(fq, phis[0], th, n, nth, args.cpu_budget)
+        cps1, k1, el1 = cpu_baseline(fq[0], phis[0], th, n, min(args.cpu_budget, 5.0))
+        result["cpu_baseline"] = {"value": round(cps, 4), "unit": "maps/s", "cores": nth,
                                   "kind": "port",
-                                  "sample": f"{k} forward LenseFlows of one 256^2 QU pair, float64 "
-                                            f"numpy oracle, {el:.1f} s"}
+                                  "sample": f"{k} forward LenseFlows of 256^2 QU pairs (8 seeded "
+                                            f"pairs, one phi), float64 numpy oracle, {nth} "
+                                            f"threads over requests, {el:.1f} s",
+                                  "one_core_maps_s": round(cps1, 4),
+                                  "one_core_sample": f"{k1} LenseFlows on 1 thread, {el1:.1f} s",
+                                  "host_cpu": cpu_model(),
+                                  "os_cpu_count": os.cpu_count(),
+                                  "affinity_cpus": len(os.sched_getaffinity(0))}
         if c0_leg is not None:
             # the T 128^2 FP64 case on the same core: its oracle check and CPU rate
             got0, f0, phi0, th0, n0 = c0_chk
