@@ -299,7 +299,9 @@ def wf_run(torch, pkg, sims, dev, N=512, th=1.5, nrhs=8, n=7, rtol=1e-6):
         return x0, phi, data, Cf, Cn, M, Bm
 
     args32 = batched(torch.complex64)
-    wfm.wiener_filter(*args32, n=n, tol=0.0, rtol=rtol, recompute_every=10, maxiter=2)  # warm
+    # warm-up: one whole untimed solve (loads the kernels of every launch
+    # shape the solve uses, including the shrinking active batches)
+    wfm.wiener_filter(*args32, n=n, tol=0.0, rtol=rtol, recompute_every=10, maxiter=1000)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     x32, info = wfm.wiener_filter(*args32, n=n, tol=0.0, rtol=rtol, recompute_every=10,

@@ -33,6 +33,12 @@ from .gradients import (WfOperators, _ncomp, gradf_apply, phi_handles, stack_mat
 from .matrix_operators import BlockTEB, DiagonalEB, DiagonalScalar, PixelMask, pinv
 
 
+# Apply the operator and the recomputed residuals to the right-hand sides still
+# active only (False: to every right-hand side, whose inactive ones then get
+# alpha = beta = 0; the iterates are the same either way).
+COMPACT_ACTIVE = True
+
+
 @dataclass
 class CGInfo:
     iterations: List[int]
@@ -64,19 +70,35 @@ class _Problem:
         self.plan64 = None
         self.mixed = self.rdt == torch.float32
 
-    def A64(self, X64: torch.Tensor) -> torch.Tensor:
+    def A64(self, X64: torch.Tensor, idx: Optional[List[int]] = None) -> torch.Tensor:
         """A x with the float64 operator (residual recomputation of the mixed
-        mode): a second plan, operator arrays and phi handles in float64."""
+        mode): a second plan, operator arrays and phi handles in float64.
+        With idx, X64 holds only those right-hand sides (len(idx) of them)."""
         if self.plan64 is None:
             th, phi, Cf, Cn, M, B = self._args
             self.plan64 = _lib.get_plan(self.N, th, torch.float64, self.B * self.C, self.device)
             self.ops64 = WfOperators(Cf, Cn, M, B, torch.float64, self.device)
             self.phis64 = phi_handles(self.plan64, phi, self.B, self.n)
-        return gradf_apply(self.plan64, self.phis64, X64, None, self.ops64, self.B, self.C, self.n)
+        phis = self.phis64 if idx is None else [self.phis64[i] for i in idx]
+        return gradf_apply(self.plan64, phis, X64, None, self.ops64, len(phis), self.C, self.n)
 
-    def A(self, X_op: torch.Tensor) -> torch.Tensor:
-        """A x = gradf(x, d = 0) (wiener_filter.py:45-49), operator precision."""
-        return gradf_apply(self.plan, self.phis, X_op, None, self.ops, self.B, self.C, self.n)
+    def A(self, X_op: torch.Tensor, idx: Optional[List[int]] = None) -> torch.Tensor:
+        """A x = gradf(x, d = 0) (wiener_filter.py:45-49), operator precision.
+        With idx, X_op holds only those right-hand sides (len(idx) of them)."""
+        phis = self.phis if idx is None else [self.phis[i] for i in idx]
+        return gradf_apply(self.plan, phis, X_op, None, self.ops, len(phis), self.C, self.n)
+
+    def A_active(self, X_op: torch.Tensor, idx: Optional[List[int]]) -> torch.Tensor:
+        """A x for the right-hand sides in idx (all when idx is None), zero for
+        the others: a request's maps go through the same kernels with the same
+        arithmetic whatever batch they are in, so each applied one is the same
+        bit for bit as in the full batch."""
+        if idx is None:
+            return self.A(X_op)
+        sub = torch.tensor(idx, device=self.device)
+        out = torch.zeros_like(X_op)
+        out.index_copy_(0, sub, self.A(X_op.index_select(0, sub), idx))
+        return out
 
     def dot(self, a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
         out = torch.empty(self.B, dtype=torch.float64, device=self.device)
@@ -170,25 +192,46 @@ def conjugate_gradient_process(x0, data, phi, mask, beam, noise_covariance, fiel
 
     r = torch.empty_like(x)
     z = torch.empty_like(x)
-    scratch = torch.empty_like(x)
     ones = torch.ones(B, dtype=torch.float64, device=dev)
+    true_r = [False] * B        # r[i] is a recomputed b - A x at the current x[i]
 
-    def residual(xv):
-        """r = b - A x, z = P^-1 r, returns dot(r, z), all in float64 (with the
-        float64 operator in the mixed mode: iterative refinement)."""
-        if prob.mixed:
-            Ax, plan_r = prob.A64(xv), prob.plan64
+    def residual(xv, idx: Optional[List[int]] = None, x_zero: bool = False):
+        """r = b - A x, z = P^-1 r for the right-hand sides in idx (all when
+        None; the others keep r and z), all in float64 (with the float64
+        operator in the mixed mode: iterative refinement).  Returns dot(r, z)
+        per right-hand side (zero outside idx).  x_zero: x is all zeros, so
+        A x = 0 is not applied."""
+        sub = None if idx is None else torch.tensor(idx, device=dev)
+        xs = xv if sub is None else xv.index_select(0, sub)
+        k_ = xs.shape[0]
+        if x_zero:
+            Ax, plan_r = torch.zeros(xs.shape, dtype=prob.cdt, device=dev), prob.plan
+        elif prob.mixed:
+            Ax, plan_r = prob.A64(xs, idx), prob.plan64
         else:
-            Ax, plan_r = prob.A(xv.to(prob.cdt)), prob.plan
-        r.copy_(bvec)
-        scratch.zero_()
-        rz = torch.empty(B, dtype=torch.float64, device=dev)
-        _lib.cg_update_xrz(plan_r, torch.view_as_real(scratch), torch.view_as_real(r),
-                           torch.view_as_real(z), torch.view_as_real(xv), torch.view_as_real(Ax),
-                           pinv_arr, ones, B, prob.C, rz)
-        return rz
+            Ax, plan_r = prob.A(xs.to(prob.cdt), idx), prob.plan
+        rs = (bvec if sub is None else bvec.index_select(0, sub)).clone()
+        zs = torch.empty_like(rs)
+        rzs = torch.empty(k_, dtype=torch.float64, device=dev)
+        _lib.cg_update_xrz(plan_r, torch.view_as_real(torch.zeros_like(rs)),
+                           torch.view_as_real(rs), torch.view_as_real(zs),
+                           torch.view_as_real(xs.contiguous()), torch.view_as_real(Ax), pinv_arr,
+                           ones[:k_], k_, prob.C, rzs)
+        if sub is None:
+            r.copy_(rs)
+            z.copy_(zs)
+            for i in range(B):
+                true_r[i] = True
+            return rzs
+        r.index_copy_(0, sub, rs)
+        z.index_copy_(0, sub, zs)
+        for i in idx:
+            true_r[i] = True
+        return torch.zeros(B, dtype=torch.float64, device=dev).index_copy_(0, sub, rzs)
 
-    res = residual(x)                                         # :61-69
+    # A x0 is skipped for x0 = 0 (every term of A is linear: exactly zero)
+    x_nonzero = yield from _host(torch.any(x))
+    res = residual(x, x_zero=not bool(x_nonzero))             # :61-69
     p = z.clone()
     p_op = p.to(prob.cdt).contiguous()
     bb = yield from _host(prob.dot(bvec, bvec))
@@ -255,6 +298,7 @@ def conjugate_gradient_process(x0, data, phi, mask, beam, noise_covariance, fiel
             pending = (ev, flag)
         res_curr = cur_d.cpu()
         k = k_d.cpu()
+        true_r[:] = [False] * B
         if hist:
             acts = torch.stack([h[0] for h in hist]).cpu()
             curs = torch.stack([h[1] for h in hist]).cpu()
@@ -263,38 +307,51 @@ def conjugate_gradient_process(x0, data, phi, mask, beam, noise_covariance, fiel
                     if bool(a_row[i]):
                         info.res_history[i].append(float(c_row[i]))
         act = torch.zeros(B, dtype=torch.bool)
+    # Relative test and/or periodic refinement: alpha and beta stay on the
+    # device (the same float64 divisions as on the host, so the same
+    # iterates); the host reads dot(r, z) and the relative residual once per
+    # iteration.  The operator and the recomputed residuals are applied to the
+    # right-hand sides still active only.
+    bb_d = torch.where(bb > 0, bb, torch.ones_like(bb)).to(dev)
+
+    def rel_dev():
+        return torch.sqrt(prob.dot(r, r) / bb_d)
+
     while bool(act.any()):
-        Ap = prob.A(p_op)                                     # :82
-        pAp = yield from _host(prob.dot(p, Ap))
-        alpha = torch.where(act, res_h / torch.where(pAp != 0, pAp, torch.ones_like(pAp)),
-                            torch.zeros_like(res_h))          # :84
+        idx = None if (bool(act.all()) or not COMPACT_ACTIVE) else \
+            [i for i in range(B) if bool(act[i])]
+        act_d = act.to(dev)
+        res_d = res_h.to(dev)
+        Ap = prob.A_active(p_op, idx)                         # :82
+        pAp = prob.dot(p, Ap)
+        alpha = torch.where(act_d, res_d / torch.where(pAp != 0, pAp, torch.ones_like(pAp)),
+                            torch.zeros_like(res_d))          # :84
         rz = torch.empty(B, dtype=torch.float64, device=dev)
         _lib.cg_update_xrz(prob.plan, torch.view_as_real(x), torch.view_as_real(r),
                            torch.view_as_real(z), torch.view_as_real(p), torch.view_as_real(Ap),
-                           pinv_arr, alpha.to(dev), B, prob.C, rz)       # :86-92
-        rz_h = yield from _host(rz)
-        beta = torch.where(act, rz_h / torch.where(res_h != 0, res_h, torch.ones_like(res_h)),
-                           torch.zeros_like(res_h))       # :94
-        _lib.cg_update_p(prob.plan, torch.view_as_real(p), torch.view_as_real(z), beta.to(dev),
+                           pinv_arr, alpha, B, prob.C, rz)   # :86-92
+        beta = torch.where(act_d, rz / torch.where(res_d != 0, res_d, torch.ones_like(res_d)),
+                           torch.zeros_like(res_d))           # :94
+        _lib.cg_update_p(prob.plan, torch.view_as_real(p), torch.view_as_real(z), beta,
                          torch.view_as_real(p_op), B, prob.C)            # :95
-        res_curr = torch.where(act, rz_h, res_curr)
-        res_h = torch.where(act, rz_h, res_h)
+        for i in (range(B) if idx is None else idx):
+            true_r[i] = False
+        res_d = torch.where(act_d, rz, res_d)
         k += act.long()
         it += 1
         if recompute_every and it % recompute_every == 0:
-            res_new = yield from _host(residual(x))
-            res_curr = torch.where(act, res_new, res_curr)
-            res_h = torch.where(act, res_new, res_h)
+            res_d = torch.where(act_d, residual(x, idx), res_d)
         if rtol is not None:
-            rel = yield from rel_now()
+            res_h, rel = (yield from _host(torch.stack([res_d, rel_dev()])))
             # before a right-hand side stops on the relative test, replace its
             # recursive residual by the recomputed b - A x (FP64 arithmetic);
             # it stops only if that one passes too
             if bool((act & (rel < rtol)).any()):
-                res_new = yield from _host(residual(x))
-                res_curr = torch.where(act, res_new, res_curr)
-                res_h = torch.where(act, res_new, res_h)
-                rel = yield from rel_now()
+                res_d = torch.where(act_d, residual(x, idx), res_d)
+                res_h, rel = (yield from _host(torch.stack([res_d, rel_dev()])))
+        else:
+            res_h = yield from _host(res_d)
+        res_curr = torch.where(act, res_h, res_curr)
         for i in range(B):
             if bool(act[i]):
                 info.res_history[i].append(float(res_curr[i]))
@@ -308,7 +365,11 @@ def conjugate_gradient_process(x0, data, phi, mask, beam, noise_covariance, fiel
         conv = conv | (rel < rtol)
     info.converged = [bool(v) for v in conv]
     if return_info:
-        residual(x)
+        # the true residual of every right-hand side whose r is not already
+        # a recomputed b - A x at its final x
+        stale = [i for i in range(B) if not true_r[i]]
+        if stale:
+            residual(x, None if len(stale) == B else stale)
         rel_final = yield from rel_now()
         info.final_true_rel_residual = [float(v) for v in rel_final]
     out = unstack_matrices(x0, x.to(X0.dtype if X0.is_complex() else torch.complex128), lead)

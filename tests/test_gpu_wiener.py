@@ -216,6 +216,48 @@ def test_wiener_filter_mixed_precision_batch(pkg, dev, sims):
         assert err < 1e-4
 
 
+def test_wiener_filter_active_compaction_is_exact(pkg, dev, sims):
+    """The relative-test solve applies the operator and the recomputed FP64
+    residuals to the right-hand sides still active only.  With 3 RHS -- two
+    problems, and zero data, which is solved before the first iteration -- the
+    solution, the iteration counts, the dot(r, z) histories and the final true
+    residuals equal, bit for bit, those of the same solve applying everything
+    to the whole batch."""
+    from cmblensing_amd import wiener_filter as wfm
+    N, th = 64, 1.5
+    _case(f"wf active compaction N={N} 3 RHS")
+    probs = [make_problem(sims, N, th, "holes", seed=20 + i, ell_knee=100.0, noise_uk=1.0)
+             for i in range(3)]
+    objs = [gpu_objects(pkg, p, dev, torch.complex64) for p in probs]
+    f0 = objs[0][0]
+    z = lambda: torch.zeros((3, N, N // 2 + 1), dtype=torch.complex64, device=dev)
+    x0 = f0.replace(polar_matrix_1=z(), polar_matrix_2=z())
+    dq = torch.stack([o[2].polar_matrix_1 for o in objs])
+    du = torch.stack([o[2].polar_matrix_2 for o in objs])
+    dq[2] = 0
+    du[2] = 0
+    data = objs[0][2].replace(polar_matrix_1=dq, polar_matrix_2=du)
+    phi = objs[0][1].replace(scalar_matrix=torch.stack([o[1].scalar_matrix for o in objs]))
+    _, _, _, Cf, Cn, M, B = objs[0]
+    runs = []
+    try:
+        for compact in (True, False):
+            wfm.COMPACT_ACTIVE = compact
+            runs.append(wfm.wiener_filter(x0, phi, data, Cf, Cn, M, B, n=7, tol=0.0, rtol=1e-6,
+                                          recompute_every=5, maxiter=500, return_info=True))
+    finally:
+        wfm.COMPACT_ACTIVE = True
+    (xa, ia), (xb, ib) = runs
+    print("iterations", ia.iterations, ib.iterations)
+    assert ia.iterations[2] == 0 and min(ia.iterations[:2]) > 0
+    assert ia.iterations == ib.iterations
+    assert ia.res_history == ib.res_history
+    assert ia.final_true_rel_residual == ib.final_true_rel_residual
+    assert max(ia.final_true_rel_residual) < 1e-6
+    assert torch.equal(xa.polar_matrix_1, xb.polar_matrix_1)
+    assert torch.equal(xa.polar_matrix_2, xb.polar_matrix_2)
+
+
 def _julia_wf_problem(pkg, dev, pol):
     """The reference's test_wiener_filter.py problem for I, P or IP (CMBLensing.jl
     fixtures, 256^2 at 2.5 arcmin, FP64): truth field, phi, data, operators."""
