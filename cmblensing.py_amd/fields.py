@@ -155,6 +155,11 @@ def zero_scalar_field_like(field: Field) -> FlatS0:
 # --------------------------------------------------------------------------
 # basis changes (fields.py:156-172) through the library's HIP FFT passes
 # --------------------------------------------------------------------------
+# The basis changes transform up to this many maps per launch (the shared
+# plan of the geometry grows to hold them; larger stacks go in chunks).
+FFT_BATCH_MAPS = 64
+
+
 def _rfft2(x: torch.Tensor, theta_pix: float) -> torch.Tensor:
     N = x.shape[-1]
     if x.shape[-2] != N:
@@ -162,7 +167,7 @@ def _rfft2(x: torch.Tensor, theta_pix: float) -> torch.Tensor:
     xc = x.contiguous()
     nmaps = xc.numel() // (N * N)
     out = torch.empty(x.shape[:-1] + (N // 2 + 1,), dtype=_complex_dtype(xc), device=x.device)
-    plan = _lib.get_plan(N, theta_pix, xc.dtype, 1, x.device)
+    plan = _lib.get_plan(N, theta_pix, xc.dtype, min(nmaps, FFT_BATCH_MAPS), x.device)
     _lib.rfft2(plan, xc, torch.view_as_real(out), nmaps)
     return out
 
@@ -175,7 +180,7 @@ def _irfft2(X: torch.Tensor, theta_pix: float) -> torch.Tensor:
     nmaps = Xc.numel() // (N * (N // 2 + 1))
     rdt = _real_dtype(Xc)
     out = torch.empty(X.shape[:-1] + (N,), dtype=rdt, device=X.device)
-    plan = _lib.get_plan(N, theta_pix, rdt, 1, X.device)
+    plan = _lib.get_plan(N, theta_pix, rdt, min(nmaps, FFT_BATCH_MAPS), X.device)
     _lib.irfft2(plan, torch.view_as_real(Xc), out, nmaps)
     return out
 

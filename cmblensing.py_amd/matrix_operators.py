@@ -24,6 +24,14 @@ from .constants import ARCMIN_PER_DEGREE, THETA_PIX_DEFAULT
 from .fields import FlatS0, FlatS02, FlatS2, fourier_weights_for
 
 
+def _like(m: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
+    """Operator plane m in the precision of the field matrix x it multiplies
+    (float32 for float32 / complex64 fields): an operator applied to a field
+    keeps the field's dtype, so a float32 field stays float32."""
+    rdt = {torch.complex64: torch.float32, torch.complex128: torch.float64}.get(x.dtype, x.dtype)
+    return m if m.dtype == rdt or not m.is_floating_point() else m.to(rdt)
+
+
 def reciprocal_matrix(m: torch.Tensor) -> torch.Tensor:
     """util.py:45-47"""
     return torch.where(m != 0, 1.0 / torch.where(m != 0, m, torch.ones_like(m)),
@@ -91,7 +99,8 @@ class DiagonalScalar(FieldOperator):
         if isinstance(other, DiagonalScalar):
             return self.replace(scalar_matrix=self.scalar_matrix * other.scalar_matrix)
         if isinstance(other, FlatS0):
-            return other.replace(scalar_matrix=self.scalar_matrix * other.scalar_matrix)
+            x = other.scalar_matrix
+            return other.replace(scalar_matrix=_like(self.scalar_matrix, x) * x)
         return self.replace(scalar_matrix=self.scalar_matrix * other)
 
     def __rmul__(self, other):
@@ -116,8 +125,9 @@ class DiagonalEB(FieldOperator):
             return self.replace(matrix_EE=self.matrix_EE * other.matrix_EE,
                                 matrix_BB=self.matrix_BB * other.matrix_BB)
         if isinstance(other, FlatS2):
-            return other.replace(polar_matrix_1=self.matrix_EE * other.polar_matrix_1,
-                                 polar_matrix_2=self.matrix_BB * other.polar_matrix_2)
+            q, u = other.polar_matrix_1, other.polar_matrix_2
+            return other.replace(polar_matrix_1=_like(self.matrix_EE, q) * q,
+                                 polar_matrix_2=_like(self.matrix_BB, u) * u)
         return self.replace(matrix_EE=self.matrix_EE * other, matrix_BB=self.matrix_BB * other)
 
     def __rmul__(self, other):
@@ -147,10 +157,12 @@ class BlockTEB(FieldOperator):
                                 matrix_EE=self.matrix_ET * o.matrix_TE + self.matrix_EE * o.matrix_EE,
                                 matrix_BB=self.matrix_BB * o.matrix_BB)
         if isinstance(o, FlatS02):
+            t, e, b = o.scalar_matrix, o.polar_matrix_1, o.polar_matrix_2
+            L = lambda m: _like(m, t)
             return o.replace(
-                scalar_matrix=self.matrix_TT * o.scalar_matrix + self.matrix_TE * o.polar_matrix_1,
-                polar_matrix_1=self.matrix_ET * o.scalar_matrix + self.matrix_EE * o.polar_matrix_1,
-                polar_matrix_2=self.matrix_BB * o.polar_matrix_2)
+                scalar_matrix=L(self.matrix_TT) * t + L(self.matrix_TE) * e,
+                polar_matrix_1=L(self.matrix_ET) * t + L(self.matrix_EE) * e,
+                polar_matrix_2=L(self.matrix_BB) * b)
         return self.replace(**{n: getattr(self, n) * o for n in self._matrix_names()})
 
     def __rmul__(self, o):
