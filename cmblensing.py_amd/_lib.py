@@ -32,6 +32,8 @@ _SIGS = {
     "cmbl_phi_prepare": (c_int, [c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(c_void_p),
                                  ctypes.POINTER(c_double), c_void_p]),
     "cmbl_phi_release": (c_int, [c_void_p]),
+    "cmbl_phi_release_async": (c_int, [c_void_p, c_void_p]),
+    "cmbl_phi_min_det": (c_int, [c_void_p, ctypes.POINTER(c_double)]),
     "cmbl_phi_prepare_batch": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int,
                                        ctypes.POINTER(c_void_p), ctypes.POINTER(c_double),
                                        c_void_p]),
@@ -151,16 +153,34 @@ def _phi_basis(phi: torch.Tensor, N: int) -> int:
 
 
 class PhiHandle:
-    """A prepared phi: float64 derivatives and the p(t) table for nsteps."""
+    """A prepared phi: float64 derivatives and the p(t) table for nsteps.
+
+    ``streams`` records the streams its tables were used on (preparation and
+    every library call that read them); close() releases the handle without
+    waiting when that is only the current stream, and after waiting for the
+    device otherwise.  ``min_det`` (the caustic guard) is read from the device
+    the first time it is asked for."""
 
     @classmethod
-    def _wrap(cls, plan: Plan, handle: c_void_p, nsteps: int, min_det: float) -> "PhiHandle":
+    def _wrap(cls, plan: Plan, handle: c_void_p, nsteps: int, stream: int) -> "PhiHandle":
         self = cls.__new__(cls)
         self.lib, self.plan, self.nsteps = plan.lib, plan, nsteps
-        self.handle, self.min_det = handle, min_det
+        self.handle, self._min_det, self.streams = handle, None, {stream}
         return self
 
-    def __init__(self, plan: Plan, phi: torch.Tensor, nsteps: int):
+    @property
+    def min_det(self) -> float:
+        if self._min_det is None:
+            md = c_double()
+            with torch.cuda.device(self.plan.device):
+                check(self.lib.cmbl_phi_min_det(self.handle, ctypes.byref(md)), "cmbl_phi_min_det")
+            self._min_det = md.value
+        return self._min_det
+
+    def __init__(self, plan: Plan, phi: torch.Tensor, nsteps: int, with_min_det: bool = False):
+        """Prepare phi on the current stream.  with_min_det: also read the
+        caustic guard now (waits for this stream; the handle is then complete
+        on return), otherwise nothing waits."""
         self.lib = plan.lib
         self.plan = plan
         self.nsteps = nsteps
@@ -175,12 +195,15 @@ class PhiHandle:
             raise CmblError(f"phi shape {tuple(phi.shape)} does not match N={N}")
         h = c_void_p()
         md = c_double()
+        st = stream_ptr(plan.device)
         with torch.cuda.device(plan.device):
             check(self.lib.cmbl_phi_prepare(plan.handle, phi.data_ptr(), basis, nsteps,
-                                            ctypes.byref(h), ctypes.byref(md),
-                                            stream_ptr(plan.device)), "cmbl_phi_prepare")
+                                            ctypes.byref(h),
+                                            ctypes.byref(md) if with_min_det else None, st),
+                  "cmbl_phi_prepare")
         self.handle = h
-        self.min_det = md.value
+        self._min_det = md.value if with_min_det else None
+        self.streams = {st}
 
     def derivatives(self) -> torch.Tensor:
         """The five float64 phi derivative maps as a [5, N, N] tensor (a copy)."""
@@ -193,8 +216,13 @@ class PhiHandle:
 
     def close(self):
         if self.handle:
-            torch.cuda.synchronize(self.plan.device)
-            self.lib.cmbl_phi_release(self.handle)
+            cur = stream_ptr(self.plan.device)
+            with torch.cuda.device(self.plan.device):
+                if self.streams <= {cur}:
+                    self.lib.cmbl_phi_release_async(self.handle, cur)
+                else:
+                    torch.cuda.synchronize(self.plan.device)
+                    self.lib.cmbl_phi_release(self.handle)
             self.handle = None
 
     def __del__(self):
@@ -202,6 +230,12 @@ class PhiHandle:
             self.close()
         except Exception:
             pass
+
+
+def note_streams(phis, stream: int):
+    """Record that the tables of these handles are read on `stream`."""
+    for h in phis:
+        h.streams.add(stream)
 
 
 class Executor:
@@ -227,6 +261,7 @@ class Executor:
         addresses of [ncomp, N, N] maps; enqueued on `stream` (a hipStream_t)."""
         count = len(phis)
         a, i, o = self._arrs
+        note_streams(phis, stream)
         for b in range(count):
             a[b] = phis[b].handle.value
         i[:count] = in_ptrs
@@ -292,12 +327,11 @@ def prepare_batch(plan: Plan, phis: torch.Tensor, nsteps: int):
                    dtype=torch.complex128 if basis == CMBL_PHI_FOURIER else torch.float64)
     phis = phis.contiguous()
     hs = (c_void_p * count)()
-    md = (c_double * count)()
+    st = stream_ptr(plan.device)
     with torch.cuda.device(plan.device):
         check(plan.lib.cmbl_phi_prepare_batch(plan.handle, phis.data_ptr(), basis, nsteps, count,
-                                              hs, md, stream_ptr(plan.device)),
-              "cmbl_phi_prepare_batch")
-    return [PhiHandle._wrap(plan, c_void_p(hs[i]), nsteps, md[i]) for i in range(count)]
+                                              hs, None, st), "cmbl_phi_prepare_batch")
+    return [PhiHandle._wrap(plan, c_void_p(hs[i]), nsteps, st) for i in range(count)]
 
 
 def get_phi_batch(plan: Plan, phi: torch.Tensor, nsteps: int):
@@ -374,10 +408,12 @@ def clear_caches():
 def lenseflow(plan: Plan, phis, x: torch.Tensor, out: torch.Tensor, batch: int, ncomp: int,
               nsteps: int, direction: int, adjoint: bool):
     arr = (c_void_p * batch)(*[p.handle for p in phis])
+    st = stream_ptr(plan.device)
+    note_streams(phis, st)
     with torch.cuda.device(plan.device):
         check(plan.lib.cmbl_lenseflow(plan.handle, arr, x.data_ptr(), out.data_ptr(), batch, ncomp,
-                                      nsteps, int(direction), int(bool(adjoint)),
-                                      stream_ptr(plan.device)), "cmbl_lenseflow")
+                                      nsteps, int(direction), int(bool(adjoint)), st),
+              "cmbl_lenseflow")
 
 
 def rfft2(plan: Plan, x: torch.Tensor, X: torch.Tensor, nmaps: int):
@@ -413,10 +449,12 @@ def _ptr(t):
 
 def gradf_logpdf(plan: Plan, phis, f, d, ops: "WfOps", out, batch: int, ncomp: int, nsteps: int):
     arr = (c_void_p * batch)(*[p.handle for p in phis])
+    st = stream_ptr(plan.device)
+    note_streams(phis, st)
     with torch.cuda.device(plan.device):
         check(plan.lib.cmbl_gradf_logpdf(plan.handle, arr, _ptr(f), _ptr(d), ctypes.byref(ops),
-                                         out.data_ptr(), batch, ncomp, nsteps,
-                                         stream_ptr(plan.device)), "cmbl_gradf_logpdf")
+                                         out.data_ptr(), batch, ncomp, nsteps, st),
+              "cmbl_gradf_logpdf")
 
 
 def dot_f64(plan: Plan, a, b, nrhs: int, ncomp: int, out):
@@ -444,8 +482,9 @@ def cg_update_p(plan: Plan, p, z, beta, p_op, nrhs: int, ncomp: int):
 def lenseflow_vjp(plan: Plan, phis, lensed, cot, d_field, d_phi, batch: int, ncomp: int,
                   nsteps: int, direction: int):
     arr = (c_void_p * batch)(*[p.handle for p in phis])
+    st = stream_ptr(plan.device)
+    note_streams(phis, st)
     with torch.cuda.device(plan.device):
         check(plan.lib.cmbl_lenseflow_vjp(plan.handle, arr, lensed.data_ptr(), cot.data_ptr(),
                                           d_field.data_ptr(), d_phi.data_ptr(), batch, ncomp,
-                                          nsteps, int(direction), stream_ptr(plan.device)),
-              "cmbl_lenseflow_vjp")
+                                          nsteps, int(direction), st), "cmbl_lenseflow_vjp")

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -103,22 +104,27 @@ struct cmbl_plan {
   void* stage = nullptr;
   // phi preparation temporaries (double) for up to prep_cap phis at once,
   // allocated on the first prepare and kept (grown only for a larger batch):
-  // rfft2 of phi [N][N/2+1], its five derivative spectra, the packed FFT
-  // scratch [5][N][N/2] and the per-block minima of the magnification
-  // determinant, each times prep_cap
+  // rfft2 of phi [N][N/2+1], its five derivative spectra and the packed FFT
+  // scratch [5][N][N/2], each times prep_cap
   void* prep_F = nullptr;
   void* prep_D = nullptr;
   void* prep_Pk = nullptr;
-  double* prep_min = nullptr;
   int prep_cap = 0;
 };
 
 // Device memory of the phis prepared in one call: their derivative maps
-// [count][5][N][N] double followed by their tables [count][2n+1][2][N][N] T.
-// Every handle of the call holds a reference; the last release frees it.
+// [count][5][N][N] double, their tables [count][2n+1][2][N][N] T and the
+// per-block minima of the magnification determinant [count][nblk] double.
+// Every handle of the call holds a reference; the last release returns the
+// memory to the block pool.  `evs` are the events of the asynchronous
+// releases of its handles (the memory is reusable once all have completed).
 struct PhiBlock {
   void* mem = nullptr;
+  size_t bytes = 0;
+  int device = 0;
   std::atomic<int> refs{0};
+  std::mutex mu;
+  std::vector<hipEvent_t> evs;
 };
 
 struct cmbl_phi {
@@ -129,29 +135,102 @@ struct cmbl_phi {
   double* deriv = nullptr;  // [5][N][N] double (inside block)
   void* ptab = nullptr;     // [(2n+1)][2][N][N] T (inside block)
   void* gtab = nullptr;     // [(2n+1)][3][N][N] T, M^-1 for the gradient ODE (lazy, own)
+  size_t gtab_bytes = 0;
   PhiBlock* block = nullptr;
+  const double* minpart = nullptr;  // [nblk] per-block minima (inside block)
+  int nblk = 0;
+  bool min_ready = false;
   double min_det = 0;
 };
 
 namespace {
+
+// Block pool.  Released phi memory (tables, gradient tables) is kept with the
+// events recorded on the streams of its last uses and handed out again for an
+// allocation of the same size on the same device once they have completed, so
+// a sampler whose phi changes every step neither allocates nor frees (hipFree
+// waits for the device).  Bounded: beyond kPoolMaxBlocks blocks or
+// kPoolMaxBytes the oldest are freed.
+struct PoolBlock {
+  void* mem;
+  size_t bytes;
+  int device;
+  std::vector<hipEvent_t> evs;
+};
+std::mutex g_pool_mu;
+std::vector<PoolBlock> g_pool;
+size_t g_pool_bytes = 0;
+constexpr size_t kPoolMaxBlocks = 64;
+constexpr size_t kPoolMaxBytes = (size_t)16 << 30;
+
+bool events_done(const std::vector<hipEvent_t>& evs) {
+  for (hipEvent_t e : evs)
+    if (hipEventQuery(e) != hipSuccess) return false;
+  return true;
+}
+
+void destroy_events(std::vector<hipEvent_t>& evs) {
+  for (hipEvent_t e : evs) (void)hipEventDestroy(e);
+  evs.clear();
+}
+
+// A pooled block of exactly `bytes` on the current device whose last uses have
+// completed, or NULL.
+void* pool_take(size_t bytes) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  for (size_t i = 0; i < g_pool.size(); ++i) {
+    PoolBlock& b = g_pool[i];
+    if (b.device != dev || b.bytes != bytes || !events_done(b.evs)) continue;
+    void* mem = b.mem;
+    destroy_events(b.evs);
+    g_pool_bytes -= b.bytes;
+    g_pool.erase(g_pool.begin() + i);
+    return mem;
+  }
+  return nullptr;
+}
+
+// Return a block (with the events of its last uses) to the pool.
+void pool_give(void* mem, size_t bytes, int device, std::vector<hipEvent_t> evs) {
+  std::vector<PoolBlock> drop;
+  {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    g_pool.push_back(PoolBlock{mem, bytes, device, std::move(evs)});
+    g_pool_bytes += bytes;
+    while (g_pool.size() > kPoolMaxBlocks || g_pool_bytes > kPoolMaxBytes) {
+      drop.push_back(std::move(g_pool.front()));
+      g_pool_bytes -= drop.back().bytes;
+      g_pool.erase(g_pool.begin());
+    }
+  }
+  for (PoolBlock& b : drop) {
+    for (hipEvent_t e : b.evs) (void)hipEventSynchronize(e);
+    destroy_events(b.evs);
+    (void)hipFree(b.mem);
+  }
+}
+
+hipError_t pool_alloc(void** mem, size_t bytes) {
+  *mem = pool_take(bytes);
+  return *mem ? hipSuccess : hipMalloc(mem, bytes);
+}
 
 // phis prepared per pass of the batched preparation (bounds its scratch)
 constexpr int kPrepChunk = 64;
 
 cmbl_status ensure_prep(cmbl_plan* p, int cap) {
   if (p->prep_cap >= cap) return CMBL_OK;
-  for (void* q : {p->prep_F, p->prep_D, p->prep_Pk, (void*)p->prep_min})
+  for (void* q : {p->prep_F, p->prep_D, p->prep_Pk})
     if (q) (void)hipFree(q);
   p->prep_F = p->prep_D = p->prep_Pk = nullptr;
-  p->prep_min = nullptr;
   p->prep_cap = 0;
-  const size_t N = p->N, NN = N * N, NH1 = N * (N / 2 + 1), NH = N * (N / 2);
-  const size_t nblk = (NN + 255) / 256;
+  const size_t N = p->N, NH1 = N * (N / 2 + 1), NH = N * (N / 2);
   hipError_t e;
   if ((e = hipMalloc(&p->prep_F, cap * NH1 * sizeof(cplx<double>))) != hipSuccess ||
       (e = hipMalloc(&p->prep_D, cap * 5 * NH1 * sizeof(cplx<double>))) != hipSuccess ||
-      (e = hipMalloc(&p->prep_Pk, cap * 5 * NH * sizeof(cplx<double>))) != hipSuccess ||
-      (e = hipMalloc((void**)&p->prep_min, cap * nblk * sizeof(double))) != hipSuccess)
+      (e = hipMalloc(&p->prep_Pk, cap * 5 * NH * sizeof(cplx<double>))) != hipSuccess)
     return fail(CMBL_ENOMEM, std::string("phi_prepare scratch: ") + hipGetErrorString(e));
   p->prep_cap = cap;
   return CMBL_OK;
@@ -171,15 +250,20 @@ cmbl_status prepare_phis(cmbl_plan* p, const void* phi, int phi_basis, int nstep
   const size_t ntab = (size_t)(2 * nsteps + 1) * 2 * NN;
   const size_t dbytes = (size_t)count * 5 * NN * sizeof(double);
   const size_t toff = (dbytes + 255) / 256 * 256;
+  const int nblk = (int)((NN + 255) / 256);
+  const size_t moff = (toff + (size_t)count * ntab * p->tsize + 255) / 256 * 256;
+  const size_t bytes = moff + (size_t)count * nblk * sizeof(double);
   CMBL_TRY(ensure_prep(p, std::min(count, kPrepChunk)));
   PhiBlock* blk = new PhiBlock();
   hipError_t e;
-  if ((e = hipMalloc(&blk->mem, toff + (size_t)count * ntab * p->tsize)) != hipSuccess) {
+  if ((e = hipGetDevice(&blk->device)) != hipSuccess || (e = pool_alloc(&blk->mem, bytes)) != hipSuccess) {
     delete blk;
     return fail(CMBL_ENOMEM, std::string("phi_prepare: ") + hipGetErrorString(e));
   }
+  blk->bytes = bytes;
   blk->refs = count;
   char* base = (char*)blk->mem;
+  double* minparts = (double*)(base + moff);
   for (int i = 0; i < count; ++i) {
     cmbl_phi* h = new cmbl_phi();
     h->N = N;
@@ -189,6 +273,8 @@ cmbl_status prepare_phis(cmbl_plan* p, const void* phi, int phi_basis, int nstep
     h->deriv = (double*)(base + (size_t)i * 5 * NN * sizeof(double));
     h->ptab = base + toff + (size_t)i * ntab * p->tsize;
     h->block = blk;
+    h->minpart = minparts + (size_t)i * nblk;
+    h->nblk = nblk;
     out[i] = h;
   }
   auto bail = [&](cmbl_status st) {
@@ -202,8 +288,6 @@ cmbl_status prepare_phis(cmbl_plan* p, const void* phi, int phi_basis, int nstep
   cplx<double>* D = (cplx<double>*)p->prep_D;
   cplx<double>* Pk = (cplx<double>*)p->prep_Pk;
   const cplx<double>* tw64 = (const cplx<double>*)p->tw64;
-  const int nblk = (int)((NN + 255) / 256);
-  std::vector<double> mp((size_t)count * nblk);
   for (int c0 = 0; c0 < count; c0 += kPrepChunk) {
     const int cc = std::min(kPrepChunk, count - c0);
     if (phi_basis == CMBL_PHI_MAP) {
@@ -218,26 +302,67 @@ cmbl_status prepare_phis(cmbl_plan* p, const void* phi, int phi_basis, int nstep
         (e = cmbl::launch_irfft2<double>(N, D, Pk, deriv, 5L * cc, tw64, s)) != hipSuccess)
       return bail(fail(CMBL_EHIP, std::string("phi derivatives: ") + hipGetErrorString(e)));
     int nb = 0;
+    double* mpart = minparts + (size_t)c0 * nblk;
     if (p->dtype == CMBL_F32)
-      e = cmbl::launch_ptab<float>(N, deriv, (float*)out[c0]->ptab, nsteps, p->prep_min, &nb, s,
-                                   cc);
+      e = cmbl::launch_ptab<float>(N, deriv, (float*)out[c0]->ptab, nsteps, mpart, &nb, s, cc);
     else
-      e = cmbl::launch_ptab<double>(N, deriv, (double*)out[c0]->ptab, nsteps, p->prep_min, &nb,
-                                    s, cc);
+      e = cmbl::launch_ptab<double>(N, deriv, (double*)out[c0]->ptab, nsteps, mpart, &nb, s, cc);
     if (e != hipSuccess) return bail(fail(CMBL_EHIP, std::string("phi table: ") + hipGetErrorString(e)));
-    if ((e = hipMemcpyAsync(mp.data() + (size_t)c0 * nblk, p->prep_min, (size_t)cc * nb * sizeof(double),
-                            hipMemcpyDeviceToHost, s)) != hipSuccess)
-      return bail(fail(CMBL_EHIP, std::string("phi min det: ") + hipGetErrorString(e)));
   }
-  if ((e = hipStreamSynchronize(s)) != hipSuccess)
+  if (!min_det) return CMBL_OK;  // computed on demand (cmbl_phi_min_det)
+  std::vector<double> mp((size_t)count * nblk);
+  if ((e = hipMemcpyAsync(mp.data(), minparts, mp.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          s)) != hipSuccess ||
+      (e = hipStreamSynchronize(s)) != hipSuccess)
     return bail(fail(CMBL_EHIP, std::string("phi min det: ") + hipGetErrorString(e)));
   for (int i = 0; i < count; ++i) {
     double md = 1.0;
     for (int b = 0; b < nblk; ++b) md = std::min(md, mp[(size_t)i * nblk + b]);
     out[i]->min_det = md;
-    if (min_det) min_det[i] = md;
+    out[i]->min_ready = true;
+    min_det[i] = md;
   }
   return CMBL_OK;
+}
+
+// Drop one handle: its gradient table and (with the last handle of its
+// block) the block go to the pool.  async: the caller enqueued every use of
+// the handle on stream s (or on streams ordered before its current point), and
+// the memory is reused after an event recorded there; otherwise the device is
+// synchronised first.
+void release_handle(cmbl_phi* h, bool async, hipStream_t s) {
+  auto mark = [&](std::vector<hipEvent_t>& v) {
+    if (!async) return;
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess &&
+        hipEventRecord(e, s) == hipSuccess) {
+      v.push_back(e);
+      return;
+    }
+    if (e) (void)hipEventDestroy(e);
+    (void)hipDeviceSynchronize();  // no event: fall back to waiting for the device
+  };
+  if (!async) (void)hipDeviceSynchronize();
+  const int dev = h->block ? h->block->device : 0;
+  if (h->gtab) {
+    std::vector<hipEvent_t> g;
+    mark(g);
+    pool_give(h->gtab, h->gtab_bytes, dev, std::move(g));
+  }
+  PhiBlock* b = h->block;
+  if (b) {
+    {
+      std::vector<hipEvent_t> v;
+      mark(v);
+      std::lock_guard<std::mutex> lk(b->mu);
+      for (hipEvent_t e : v) b->evs.push_back(e);
+    }
+    if (b->refs.fetch_sub(1) == 1) {
+      if (b->mem) pool_give(b->mem, b->bytes, b->device, std::move(b->evs));
+      delete b;
+    }
+  }
+  delete h;
 }
 
 }  // namespace
@@ -312,7 +437,7 @@ cmbl_status cmbl_plan_destroy(cmbl_plan* p) {
   if (p->ptab_dev) (void)hipFree(p->ptab_dev);
   for (void* q : {p->wy1, p->wy2, p->wx1, p->wx2, (void*)p->red, p->gy, p->gacc, p->gzd, p->gu,
                   p->gv, p->gsrc, p->gF, p->stage, p->prep_F, p->prep_D,
-                  p->prep_Pk, (void*)p->prep_min})
+                  p->prep_Pk})
     if (q) (void)hipFree(q);
   if (p->ptab_host) (void)hipHostFree(p->ptab_host);
   delete p;
@@ -392,12 +517,28 @@ cmbl_status cmbl_phi_prepare_batch(cmbl_plan* p, const void* phis, int phi_basis
 
 cmbl_status cmbl_phi_release(cmbl_phi* h) {
   if (!h) return CMBL_OK;
-  if (h->gtab) (void)hipFree(h->gtab);
-  if (h->block && h->block->refs.fetch_sub(1) == 1) {
-    if (h->block->mem) (void)hipFree(h->block->mem);
-    delete h->block;
+  release_handle(h, false, nullptr);
+  return CMBL_OK;
+}
+
+cmbl_status cmbl_phi_release_async(cmbl_phi* h, void* stream) {
+  if (!h) return CMBL_OK;
+  release_handle(h, true, (hipStream_t)stream);
+  return CMBL_OK;
+}
+
+cmbl_status cmbl_phi_min_det(cmbl_phi* h, double* out) {
+  if (!h || !out) return fail(CMBL_EINVAL, "NULL argument");
+  if (!h->min_ready) {
+    std::vector<double> mp(h->nblk);
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(mp.data(), h->minpart, mp.size() * sizeof(double), hipMemcpyDeviceToHost));
+    double md = 1.0;
+    for (double v : mp) md = std::min(md, v);
+    h->min_det = md;
+    h->min_ready = true;
   }
-  delete h;
+  *out = h->min_det;
   return CMBL_OK;
 }
 
@@ -785,7 +926,9 @@ cmbl_status ensure_grad_scratch(cmbl_plan* p) {
 cmbl_status ensure_gtab(cmbl_plan* p, cmbl_phi* h, hipStream_t s) {
   if (h->gtab) return CMBL_OK;
   const size_t NN = (size_t)p->N * p->N;
-  HIP_TRY(hipMalloc(&h->gtab, (size_t)(2 * h->nsteps + 1) * 3 * NN * p->tsize));
+  const size_t bytes = (size_t)(2 * h->nsteps + 1) * 3 * NN * p->tsize;
+  HIP_TRY(pool_alloc(&h->gtab, bytes));
+  h->gtab_bytes = bytes;
   if (p->dtype == CMBL_F32)
     HIP_TRY(cmbl::launch_gtab<float>(p->N, h->deriv, (float*)h->gtab, h->nsteps, s));
   else

@@ -168,6 +168,25 @@ struct Twiddles {
   }
 };
 
+// Synchronisation of an LDS exchange.  Block scope: __syncthreads().  Wave
+// scope (WAVE): for kernels in which the threads of every line lie inside one
+// wavefront (N/8 threads per line, groups aligned to the 64 lanes).  LDS
+// operations of a wavefront execute in issue order, so only the compiler has
+// to be kept from moving LDS accesses across the exchange -- the fences and
+// wave barrier do that (the idiom of rocPRIM's wave_barrier()), and the
+// other wavefronts of the block run on unhindered.  Every lane of the
+// wavefront must reach it.
+template <bool WAVE>
+__device__ __forceinline__ void lds_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 template <bool INV, typename T>
 __device__ __forceinline__ cplx<T> cmul_tw(cplx<T> a, cplx<T> w) {
   return INV ? cmulc(a, w) : cmul(a, w);
@@ -176,9 +195,9 @@ __device__ __forceinline__ cplx<T> cmul_tw(cplx<T> a, cplx<T> w) {
 // NL independent lines, each handled by the same group of N/8 threads
 // (thread index t within the group).  bufs[l] points to that line's LDS
 // buffer (lds_len<N>() entries), tw to the thread's twiddles.  Every thread
-// of the workgroup must call this the same number of times: the passes
-// synchronise with __syncthreads().
-template <int N, bool INV, int NL, typename T>
+// of the workgroup (WAVE: of the wavefront) must call this the same number
+// of times: the passes synchronise with lds_sync<WAVE>().
+template <int N, bool INV, int NL, bool WAVE = false, typename T>
 __device__ __forceinline__ void fft_lines(cplx<T> (&v)[NL][8], cplx<T>* const (&bufs)[NL],
                                           const Twiddles<N, T>& tw, int t) {
   using P = Passes<N>;
@@ -204,12 +223,12 @@ __device__ __forceinline__ void fft_lines(cplx<T> (&v)[NL][8], cplx<T>* const (&
     const bool last = (p == P8 - 1) && (REM == 0);
     if (!last) {
       const int base = (t / Ns) * Ns * 8 + k;
-      __syncthreads();
+      lds_sync<WAVE>();
 #pragma unroll
       for (int l = 0; l < NL; ++l)
 #pragma unroll
         for (int q = 0; q < 8; ++q) bufs[l][pad_idx(base + q * Ns)] = v[l][q];
-      __syncthreads();
+      lds_sync<WAVE>();
       Ns *= 8;
       if (p < P8 - 1) {
 #pragma unroll

@@ -37,6 +37,13 @@ __device__ __forceinline__ cplx<T> imul(cplx<T> F, T s) {
   return mk(-s * F.y, s * F.x);
 }
 
+// The row kernels give each line N/8 <= 64 threads in groups aligned to the
+// wavefront, so their LDS exchanges synchronise at wave scope (lds_sync).
+#ifndef CMBL_WAVE_SYNC
+#define CMBL_WAVE_SYNC 1
+#endif
+constexpr bool kRowWave = CMBL_WAVE_SYNC != 0;
+
 // Row pairing used by the fused stage kernels.  Pair index p covers two
 // map-rows: for C == 2 the Q and U rows of the same pixel row (they share the
 // deflection operator rows), for C == 1 the pixel rows 2r and 2r+1.
@@ -82,7 +89,7 @@ __device__ __forceinline__ void put_herm(cplx<T>* buf, int k, cplx<T> a, cplx<T>
 
 // Z = A + i B of two packed half rows into LDS, then the 8 register points of
 // thread t.
-template <int N, typename T>
+template <int N, bool WAVE = false, typename T>
 __device__ __forceinline__ void herm_pair_to_line(const cplx<T>* A, const cplx<T>* B, cplx<T>* buf,
                                                   cplx<T> (&v)[8], int t) {
   constexpr int G = N / 8;
@@ -91,21 +98,21 @@ __device__ __forceinline__ void herm_pair_to_line(const cplx<T>* A, const cplx<T
     const int k = t + q * G;
     put_herm<N>(buf, k, A[k], B[k]);
   }
-  __syncthreads();
+  lds_sync<WAVE>();
 #pragma unroll
   for (int q = 0; q < 8; ++q) v[q] = buf[pad_idx(t + q * G)];
 }
 
 // After a forward FFT of z = a + i b (v[m] = Z[t + m*G]) split into the two
 // packed half spectra and store them to A and B.
-template <int N, typename T>
+template <int N, bool WAVE = false, typename T>
 __device__ __forceinline__ void split_store(const cplx<T> (&v)[8], cplx<T>* buf, cplx<T>* A,
                                             cplx<T>* B, int t, bool active) {
   constexpr int G = N / 8;
-  __syncthreads();
+  lds_sync<WAVE>();
 #pragma unroll
   for (int m = 0; m < 8; ++m) buf[pad_idx(t + m * G)] = v[m];
-  __syncthreads();
+  lds_sync<WAVE>();
   if (!active) return;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -149,10 +156,10 @@ __global__ __launch_bounds__(256) void k_row_r2c(const T* __restrict__ x, cplx<T
   cplx<T> v[1][8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) v[0][q] = mk(xa[t + q * G], xb[t + q * G]);
-  __syncthreads();
+  lds_sync<kRowWave>();
   cplx<T>* const bufs[1] = {lbuf[grp]};
-  fft_lines<N, false, 1>(v, bufs, tw, t);
-  split_store<N>(v[0], lbuf[grp], P + (2 * pp) * H, P + (2 * pp + 1) * H, t, active);
+  fft_lines<N, false, 1, kRowWave>(v, bufs, tw, t);
+  split_store<N, kRowWave>(v[0], lbuf[grp], P + (2 * pp) * H, P + (2 * pp + 1) * H, t, active);
 }
 
 // Row C2R of row pairs: packed P[nrows][N/2] -> x[nrows][N] (no scaling).
@@ -168,10 +175,10 @@ __global__ __launch_bounds__(256) void k_row_c2r(const cplx<T>* __restrict__ P, 
   const bool active = p < npairs;
   const long pp = active ? p : npairs - 1;
   cplx<T> v[1][8];
-  __syncthreads();
-  herm_pair_to_line<N>(P + (2 * pp) * H, P + (2 * pp + 1) * H, lbuf[grp], v[0], t);
+  lds_sync<kRowWave>();
+  herm_pair_to_line<N, kRowWave>(P + (2 * pp) * H, P + (2 * pp + 1) * H, lbuf[grp], v[0], t);
   cplx<T>* const bufs[1] = {lbuf[grp]};
-  fft_lines<N, true, 1>(v, bufs, tw, t);
+  fft_lines<N, true, 1, kRowWave>(v, bufs, tw, t);
   if (!active) return;
   T* xa = x + (2 * pp) * N;
   T* xb = xa + N;
@@ -496,7 +503,7 @@ __global__ __launch_bounds__(BS) void k_row_lf_std(cplx<T>* __restrict__ S0,
     dxb[q] = S1[rp.rb * H + k];
     sb[q] = S0[rp.rb * H + k];
   }
-  __syncthreads();
+  lds_sync<kRowWave>();
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int k = t + q * G;
@@ -504,7 +511,7 @@ __global__ __launch_bounds__(BS) void k_row_lf_std(cplx<T>* __restrict__ S0,
     put_herm<N>(bufa, k, dxa[q], k == 0 ? mk(T(0), T(0)) : imul(sa[q], c));
     put_herm<N>(bufb, k, dxb[q], k == 0 ? mk(T(0), T(0)) : imul(sb[q], c));
   }
-  __syncthreads();
+  lds_sync<kRowWave>();
   // per-pixel work, one row at a time: the operands of row b are loaded after
   // row a is done, so at most one row's operands are live
   const T* pt = ptabs[rp.b] + (long)tidx * 2 * NN;
@@ -556,7 +563,7 @@ __global__ __launch_bounds__(BS) void k_row_lf_std(cplx<T>* __restrict__ S0,
       cplx<T> w[1][8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) w[0][q] = buf1[0][pad_idx(t + q * G)];
-      fft_lines<N, true, 1>(w, buf1, tw, t);
+      fft_lines<N, true, 1, kRowWave>(w, buf1, tw, t);
       row_update(r, w[0]);
     }
   } else {
@@ -567,7 +574,7 @@ __global__ __launch_bounds__(BS) void k_row_lf_std(cplx<T>* __restrict__ S0,
       v[1][q] = bufb[pad_idx(t + q * G)];
     }
     cplx<T>* const bufs[2] = {bufa, bufb};
-    fft_lines<N, true, 2>(v, bufs, tw, t);
+    fft_lines<N, true, 2, kRowWave>(v, bufs, tw, t);
     // delta-phi sources at this stage: sum_c df_c grad f_c (lense_flow.py:201-213)
     const T* gt = ga.gtabs[rp.b] + (long)tidx * 3 * NN;
     T* Sb = ga.sacc + (long)rp.b * 5 * NN;
@@ -608,8 +615,8 @@ __global__ __launch_bounds__(BS) void k_row_lf_std(cplx<T>* __restrict__ S0,
   }
   if (!do_r2c) return;  // uniform over the grid
   cplx<T>* const bufz[1] = {bufa};
-  fft_lines<N, false, 1>(z, bufz, tw, t);
-  split_store<N>(z[0], bufa, S0 + rp.ra * H, S0 + rp.rb * H, t, active);
+  fft_lines<N, false, 1, kRowWave>(z, bufz, tw, t);
+  split_store<N, kRowWave>(z[0], bufa, S0 + rp.ra * H, S0 + rp.rb * H, t, active);
 }
 
 // Row pass, adjoint rate.  init != 0: no rate, z = y (first stage input).
@@ -654,7 +661,7 @@ __global__ __launch_bounds__(BS) void k_row_lf_adj(cplx<T>* __restrict__ S0,
     }
   }
   cplx<T> v[1][8];
-  __syncthreads();
+  lds_sync<kRowWave>();
   if (!init) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -666,11 +673,11 @@ __global__ __launch_bounds__(BS) void k_row_lf_adj(cplx<T>* __restrict__ S0,
       }
       put_herm<N>(bufa, k, a, b);
     }
-    __syncthreads();
+    lds_sync<kRowWave>();
 #pragma unroll
     for (int q = 0; q < 8; ++q) v[0][q] = bufa[pad_idx(t + q * G)];
     cplx<T>* const bufs[1] = {bufa};
-    fft_lines<N, true, 1>(v, bufs, tw, t);
+    fft_lines<N, true, 1, kRowWave>(v, bufs, tw, t);
   }
   const T* pt = ptabs[rp.b] + (long)(do_r2c ? tidx_next : 0) * 2 * NN;
   T pxa[8], pya[8], pxb[8], pyb[8], ya[8], yb[8], aa[8], ab[8];
@@ -731,9 +738,9 @@ __global__ __launch_bounds__(BS) void k_row_lf_adj(cplx<T>* __restrict__ S0,
     uv[1][m] = mk(pxb[m] * zb[m], pyb[m] * zb[m]);
   }
   cplx<T>* const bufs2[2] = {bufa, bufb};
-  fft_lines<N, false, 2>(uv, bufs2, tw, t);
-  split_store<N>(uv[0], bufa, S0 + rp.ra * H, S1 + rp.ra * H, t, active);
-  split_store<N>(uv[1], bufb, S0 + rp.rb * H, S1 + rp.rb * H, t, active);
+  fft_lines<N, false, 2, kRowWave>(uv, bufs2, tw, t);
+  split_store<N, kRowWave>(uv[0], bufa, S0 + rp.ra * H, S1 + rp.ra * H, t, active);
+  split_store<N, kRowWave>(uv[1], bufb, S0 + rp.rb * H, S1 + rp.rb * H, t, active);
 }
 
 // Gather / scatter of whole maps through device pointer arrays (serving

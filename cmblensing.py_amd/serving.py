@@ -521,9 +521,11 @@ class GpuBackend:
 
     def prepare_phi(self, N: int, phi: torch.Tensor):
         # on a stream of its own (preparation waits for its own work only); the
-        # handle is complete on return and serves every plan of this N
+        # handle is complete on return (its caustic guard is read) and serves
+        # every plan of this N
         with torch.cuda.stream(self.prep_stream):
-            return self._lib.PhiHandle(self._plan(0, N), phi.to(self.device), self.nsteps)
+            return self._lib.PhiHandle(self._plan(0, N), phi.to(self.device), self.nsteps,
+                                       with_min_det=True)
 
     def release_phi(self, handle):
         handle.close()

@@ -58,23 +58,38 @@ cmbl_status cmbl_plan_destroy(cmbl_plan* plan);
  * j = 0..2n (lense_flow.py:372-381, 397-415), stored in the plan dtype.
  * phi is float64: MAP [N][N] or FOURIER [N][N/2+1] complex (lense_flow.py:299).
  * min_det (host out, may be NULL) receives min over pixels and t in [0,1] of
- * det(I + t Hess phi).  Synchronises `stream`.  The temporaries live in the
- * plan (allocated by its first prepare), so preparations on one plan must not
- * be issued concurrently from several host threads. */
+ * det(I + t Hess phi); with it the call synchronises `stream`, without it the
+ * call only enqueues and the value is available later (cmbl_phi_min_det).
+ * The temporaries live in the plan (allocated by its first prepare), so
+ * preparations on one plan must not be issued concurrently from several host
+ * threads.  Handle memory comes from a pool of released handles' memory when
+ * a block of the same size is free (see cmbl_phi_release_async). */
 cmbl_status cmbl_phi_prepare(cmbl_plan* plan, const void* phi, int phi_basis, int nsteps,
                              cmbl_phi** out, double* min_det, void* stream);
 /* Batched phi preparation: `count` phis (contiguous [count][N][N] float64
  * MAP or [count][N][N/2+1] complex float64 FOURIER) prepared in one call:
  * their tables share one allocation, each preparation stage is one launch
- * over all of them, and the host waits once (for the min_det values, an array
- * of `count`, may be NULL).  out receives `count` handles, each released on
- * its own with cmbl_phi_release (the shared memory goes with the last).
+ * over all of them, and the host waits once if min_det (an array of `count`)
+ * is given, not at all if it is NULL.  out receives `count` handles, each
+ * released on its own (the shared memory goes with the last).
  * Replaces the per-element preparation of a field batch with a phi per
  * element (lense_flow.py:297-302 applied to each; the reference recomputes
  * the derivatives on every call). */
 cmbl_status cmbl_phi_prepare_batch(cmbl_plan* plan, const void* phis, int phi_basis, int nsteps,
                                    int count, cmbl_phi** out, double* min_det, void* stream);
+/* Release a handle after waiting for the device (safe whatever streams used
+ * it).  Its memory returns to the library's pool of handle memory. */
 cmbl_status cmbl_phi_release(cmbl_phi* phi);
+/* Release a handle without waiting: the caller has enqueued every use of the
+ * handle on `stream` (or on streams already ordered before its current
+ * point); the memory is reused by a later preparation only after an event
+ * recorded here has completed.  A sampler whose phi changes every step thus
+ * neither frees nor allocates device memory.  No reference counterpart: the
+ * reference's arrays are garbage collected. */
+cmbl_status cmbl_phi_release_async(cmbl_phi* phi, void* stream);
+/* min over pixels and t in [0,1] of det(I + t Hess phi) of a prepared phi
+ * (the caustic guard); waits for the device the first time it is asked. */
+cmbl_status cmbl_phi_min_det(cmbl_phi* phi, double* out);
 /* Device pointer of the phi derivative maps, float64 [5][N][N]
  * (phi_x, phi_y, phi_xx, phi_xy, phi_yy). */
 const void* cmbl_phi_derivatives(const cmbl_phi* phi);
