@@ -142,6 +142,60 @@ def test_sizes_components_steps(pkg, dev, sims, N, C, dtype, tol):
         assert rel(y, ref) < tol, (n, adjoint)
 
 
+def test_released_handle_memory_is_reused_correctly(pkg, dev, sims):
+    """Handles released without waiting hand their memory to later
+    preparations of the same size: lensing with phis prepared into reused
+    memory equals lensing with the same phis prepared while nothing was
+    released (bit for bit), the caustic guard read on demand equals the one
+    read at preparation, and a handle used on a second stream still releases
+    safely (after waiting for the device)."""
+    from cmblensing_amd import _lib
+    N, th, n, B = 64, 2.0, 7, 4
+    _case("phi handle pool N=64 fp32 QU")
+    plan = _lib.Plan(N, th, torch.float32, 2 * B, dev)
+    x = torch.as_tensor(np.stack([sims.qu_map(N, th, s) for s in range(B)]),
+                        dtype=torch.float32, device=dev)
+    P = [torch.as_tensor(np.stack([sims.phi_map(N, th, 100 * r + s) for s in range(B)]),
+                         device=dev) for r in range(3)]
+    ref = []
+    keep = []
+    for r in range(3):                      # reference: nothing released in between
+        hs = _lib.prepare_batch(plan, P[r], n)
+        keep.append(hs)
+        out = torch.empty_like(x)
+        _lib.lenseflow(plan, hs, x, out, B, 2, n, 1, False)
+        ref.append(out)
+    guards = [[h.min_det for h in hs] for hs in keep]
+    torch.cuda.synchronize()
+    for hs in keep:
+        for h in hs:
+            h.close()                       # back to the pool, no wait
+    for rep in range(2):
+        for r in range(3):                  # prepared into reused memory
+            hs = _lib.prepare_batch(plan, P[r], n)
+            out = torch.empty_like(x)
+            _lib.lenseflow(plan, hs, x, out, B, 2, n, 1, False)
+            assert torch.equal(out, ref[r]), (rep, r)
+            assert [h.min_det for h in hs] == guards[r]
+            for h in hs:
+                h.close()
+    # eager guard == guard read on demand
+    h1 = _lib.PhiHandle(plan, P[0][0], n, with_min_det=True)
+    h2 = _lib.PhiHandle(plan, P[0][0], n)
+    assert h1.min_det == h2.min_det == guards[0][0]
+    # used on another stream: close() waits for the device instead
+    side = torch.cuda.Stream(dev)
+    with torch.cuda.stream(side):
+        out = torch.empty_like(x[:1])
+        _lib.lenseflow(plan, [h2], x[:1], out, 1, 2, n, 1, False)
+    assert len(h2.streams) == 2
+    h2.close()
+    h1.close()
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref[0][:1])
+    plan.close()
+
+
 def test_empty_batch_is_a_no_op(pkg, dev):
     from cmblensing_amd import _lib
     _case("empty batch")
