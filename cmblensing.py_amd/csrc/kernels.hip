@@ -471,7 +471,17 @@ __device__ __forceinline__ void add_sources(T* S, long NN, T cs, T tval, T fx, T
 // S0 holds the row spectra S of the stage input, S1 their x-derivative DX;
 // the line is Z = DX + i DY with DY = i ky S / N (DY's packed slot is zero:
 // ky = 0 at DC, and the real Nyquist value times i drops out of the C2R).
-template <typename T, int N, int C, bool GRAD, int BS>
+//
+// RK selects the RK4 bookkeeping.  0: the classic accumulator (acc written at
+// stage 1, updated at 2 and 3, consumed at 4; the gradient ODE uses it).
+// 1 and 2: the same RK4 with the accumulator rebuilt from the stage inputs
+// z_s (z_1 = y): since z_2 = y + h/2 k_1, z_3 = y + h/2 k_2, z_4 = y + h k_3,
+//   acc = (2 y + z_2)/3 + h/3 k_2   (stage 2; = (z_2 + 2 z_3)/3)
+//   y'  = acc + (z_4 - y)/3 + h/6 k_4   (stage 4)
+// and stages 1 and 3 touch no accumulator: 7 map transfers per component and
+// step instead of 10.  RK = 2 (stages 2 and 4) recovers z_s from the row
+// spectra S it reads anyway (one more inverse row FFT of S_a + i S_b).
+template <typename T, int N, int C, bool GRAD, int BS, int RK = 0>
 __global__ __launch_bounds__(BS) void k_row_lf_std(cplx<T>* __restrict__ S0,
                                                     const cplx<T>* __restrict__ S1,
                                                     T* __restrict__ y, T* __restrict__ acc,
@@ -504,6 +514,19 @@ __global__ __launch_bounds__(BS) void k_row_lf_std(cplx<T>* __restrict__ S0,
     sb[q] = S0[rp.rb * H + k];
   }
   lds_sync<kRowWave>();
+  // RK == 2: the stage input of both rows, N (z_a + i z_b), from C2R of their
+  // row spectra
+  cplx<T> zr[1][RK == 2 ? 8 : 1];
+  if constexpr (RK == 2) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) put_herm<N>(bufa, t + q * G, sa[q], sb[q]);
+    lds_sync<kRowWave>();
+#pragma unroll
+    for (int q = 0; q < 8; ++q) zr[0][q] = bufa[pad_idx(t + q * G)];
+    cplx<T>* const bz[1] = {bufa};
+    fft_lines<N, true, 1, kRowWave>(zr, bz, tw, t);
+    lds_sync<kRowWave>();
+  }
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int k = t + q * G;
@@ -530,28 +553,62 @@ __global__ __launch_bounds__(BS) void k_row_lf_std(cplx<T>* __restrict__ S0,
     const long rr = r ? rp.rb : rp.ra;
     if (r == 0 || C == 1) load_p(r ? rp.ib : rp.ia);
     T yv[8], av[8];
+    if constexpr (RK == 0) {
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const int j = t + m * G;
-      yv[m] = stage == 4 ? T(0) : y[rr * N + j];
-      av[m] = stage == 1 ? T(0) : acc[rr * N + j];
-    }
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const int j = t + m * G;
-      const T k = px[m] * g[m].x + py[m] * g[m].y;
-      T zz;
-      rk4_stage(stage, h, k, yv[m], av[m], zz);
-      if (active) {
-        if (stage == 4)
-          y[rr * N + j] = zz;
-        else
-          acc[rr * N + j] = av[m];
+      for (int m = 0; m < 8; ++m) {
+        const int j = t + m * G;
+        yv[m] = stage == 4 ? T(0) : y[rr * N + j];
+        av[m] = stage == 1 ? T(0) : acc[rr * N + j];
       }
-      if (r == 0)
-        z[0][m].x = zz;
-      else
-        z[0][m].y = zz;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const int j = t + m * G;
+        const T k = px[m] * g[m].x + py[m] * g[m].y;
+        T zz;
+        rk4_stage(stage, h, k, yv[m], av[m], zz);
+        if (active) {
+          if (stage == 4)
+            y[rr * N + j] = zz;
+          else
+            acc[rr * N + j] = av[m];
+        }
+        if (r == 0)
+          z[0][m].x = zz;
+        else
+          z[0][m].y = zz;
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const int j = t + m * G;
+        yv[m] = y[rr * N + j];
+        av[m] = stage == 4 ? acc[rr * N + j] : T(0);
+      }
+      const T inv_n = T(1) / T(N);  // exact: N is a power of two
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const int j = t + m * G;
+        const T k = px[m] * g[m].x + py[m] * g[m].y;
+        T zin = T(0);
+        if constexpr (RK == 2) zin = (r ? zr[0][m].y : zr[0][m].x) * inv_n;
+        T zz;
+        if (stage == 1) {
+          zz = yv[m] + (h / T(2)) * k;
+        } else if (stage == 2) {
+          const T a = (T(2) * yv[m] + zin) / T(3) + (h / T(3)) * k;
+          if (active) acc[rr * N + j] = a;
+          zz = yv[m] + (h / T(2)) * k;
+        } else if (stage == 3) {
+          zz = yv[m] + h * k;
+        } else {
+          zz = av[m] + (zin - yv[m]) / T(3) + (h / T(6)) * k;
+          if (active) y[rr * N + j] = zz;
+        }
+        if (r == 0)
+          z[0][m].x = zz;
+        else
+          z[0][m].y = zz;
+      }
     }
   };
   if constexpr (!GRAD) {
@@ -868,6 +925,13 @@ hipError_t launch_lf_col(int N, const cplx<T>* src, cplx<T>* dst, long nmaps, co
   return hipGetLastError();
 }
 
+// The forward/inverse LenseFlow uses the low-storage RK4 bookkeeping of
+// k_row_lf_std (RK = 1, 2); the gradient ODE keeps the classic one.
+#ifndef CMBL_RK4_LOWSTORE
+#define CMBL_RK4_LOWSTORE 1
+#endif
+constexpr bool kLowStorageRK4 = CMBL_RK4_LOWSTORE != 0;
+
 template <typename T, int N, int BS>
 void row_lf_launch(bool adjoint, int C, const GradArgs<T>* ga, cplx<T>* S0, cplx<T>* S1, T* y,
                    T* acc, const T* const* ptabs, const cplx<T>* tw, long npairs, int stage, T h,
@@ -888,13 +952,27 @@ void row_lf_launch(bool adjoint, int C, const GradArgs<T>* ga, cplx<T>* S0, cplx
     else
       k_row_lf_std<T, N, 1, true, BS><<<nb, BS, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage,
                                                         h, tidx, do_r2c, cy, g);
-  } else {
+  } else if (!kLowStorageRK4) {
     if (C == 2)
       k_row_lf_std<T, N, 2, false, BS><<<nb, BS, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage,
                                                          h, tidx, do_r2c, cy, g);
     else
       k_row_lf_std<T, N, 1, false, BS><<<nb, BS, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage,
                                                          h, tidx, do_r2c, cy, g);
+  } else if (stage == 2 || stage == 4) {
+    if (C == 2)
+      k_row_lf_std<T, N, 2, false, BS, 2><<<nb, BS, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs,
+                                                            stage, h, tidx, do_r2c, cy, g);
+    else
+      k_row_lf_std<T, N, 1, false, BS, 2><<<nb, BS, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs,
+                                                            stage, h, tidx, do_r2c, cy, g);
+  } else {
+    if (C == 2)
+      k_row_lf_std<T, N, 2, false, BS, 1><<<nb, BS, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs,
+                                                            stage, h, tidx, do_r2c, cy, g);
+    else
+      k_row_lf_std<T, N, 1, false, BS, 1><<<nb, BS, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs,
+                                                            stage, h, tidx, do_r2c, cy, g);
   }
 }
 
