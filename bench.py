@@ -47,10 +47,13 @@ def model_bytes_per_map(N, C, n, s):
 
 
 def row_bytes_per_launch(B, N, C, s):
-    """Algorithmic bytes of one fused row-pass launch over B requests: per
-    request 2C packed half planes read (N^2 s each), C written, the 2 maps of
-    p(t), and on average 2.5 map transfers of y/acc per component."""
-    return B * (5.5 * C + 2) * N * N * s
+    """Algorithmic bytes of one fused row-pass launch over B requests,
+    averaged over the four RK4 stages: per request 2C packed half planes read
+    (N^2 s each) and C written, the 2 maps of p(t), and per component 7 map
+    transfers of the RK4 state per step (y read at every stage and written at
+    stage 4, the accumulator written at stage 2 and read at stage 4), 1.75
+    per stage."""
+    return B * (4.75 * C + 2) * N * N * s
 
 
 def col_bytes_per_launch(B, N, C, s):
