@@ -8,6 +8,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/cmbl.h"
@@ -595,9 +596,14 @@ struct Timer {
     if (ts_ != CMBL_OK) return ts_;       \
   } while (0)
 
+// src / outp (serving executor, standard rate only): the input maps are read
+// through the request pointers by the first launch (and copied into y, the
+// staging state), and the last stage writes the result through the output
+// pointers -- no separate gather and scatter passes.
 template <typename T>
 cmbl_status run_lenseflow(cmbl_plan* p, const void* const* ptabs, T* y, long nmaps, int C,
-                          int n, int direction, int adjoint, hipStream_t s) {
+                          int n, int direction, int adjoint, hipStream_t s,
+                          const T* const* src = nullptr, T* const* outp = nullptr) {
   const int N = p->N;
   const cplx<T>* tw = (const cplx<T>*)p->tw;
   cplx<T>* S0 = (cplx<T>*)p->s0;
@@ -616,7 +622,9 @@ cmbl_status run_lenseflow(cmbl_plan* p, const void* const* ptabs, T* y, long nma
   static const int stage_half[5] = {0, 0, 1, 1, 2};
   const bool adj = adjoint != 0;
   Timer tm{p, s};
-  if (!adj) {
+  if (!adj && src) {
+    TIMED(2, cmbl::launch_lf_row_r2c_gather<T>(N, src, C, y, S0, nmaps, tw, s));
+  } else if (!adj) {
     TIMED(2, cmbl::launch_lf_row_r2c<T>(N, y, S0, nmaps, tw, s));
   } else {
     TIMED(2, cmbl::launch_lf_row<T>(N, true, S0, S1, y, acc, pt, tw, C, nmaps, 1, h,
@@ -638,7 +646,8 @@ cmbl_status run_lenseflow(cmbl_plan* p, const void* const* ptabs, T* y, long nma
         if (last) tidx = 0;
       }
       TIMED(1, cmbl::launch_lf_row<T>(N, adj, S0, S1, y, acc, pt, tw, C, nmaps, st, h, tidx,
-                                      last ? 0 : 1, 0, (T)p->dk, s));
+                                      last ? 0 : 1, 0, (T)p->dk, s, nullptr, nullptr,
+                                      (last && !adj) ? outp : nullptr));
     }
   }
   return tm.flush();
@@ -1109,23 +1118,27 @@ cmbl_status capture_exec(cmbl_exec* ex, int slot, hipStream_t cs) {
   auto step = [&](hipError_t r) {
     if (st == CMBL_OK && r != hipSuccess) st = fail(CMBL_EHIP, hipGetErrorString(r));
   };
-  if (p->dtype == CMBL_F32) {
-    float* stage = (float*)p->stage;
-    step(cmbl::launch_gather<float>((const float* const*)(ptrs + B), stage, len, B, cs));
+  // standard rate: the first and last stage kernels read and write the
+  // requests' maps themselves; adjoint: gather and scatter passes
+  auto seq = [&](auto* stage) {
+    using T = std::remove_pointer_t<decltype(stage)>;
+    const T* const* in = (const T* const*)(ptrs + B);
+    T* const* outs = (T* const*)(ptrs + 2 * B);
+    if (!ex->adjoint) {
+      st = run_lenseflow<T>(p, (const void* const*)ptrs, stage, nm, ex->ncomp, ex->nsteps,
+                            ex->direction, ex->adjoint, cs, in, outs);
+      return;
+    }
+    step(cmbl::launch_gather<T>(in, stage, len, B, cs));
     if (st == CMBL_OK)
-      st = run_lenseflow<float>(p, (const void* const*)ptrs, stage, nm, ex->ncomp, ex->nsteps,
-                                ex->direction, ex->adjoint, cs);
-    if (st == CMBL_OK)
-      step(cmbl::launch_scatter<float>(stage, (float* const*)(ptrs + 2 * B), len, B, cs));
-  } else {
-    double* stage = (double*)p->stage;
-    step(cmbl::launch_gather<double>((const double* const*)(ptrs + B), stage, len, B, cs));
-    if (st == CMBL_OK)
-      st = run_lenseflow<double>(p, (const void* const*)ptrs, stage, nm, ex->ncomp, ex->nsteps,
-                                 ex->direction, ex->adjoint, cs);
-    if (st == CMBL_OK)
-      step(cmbl::launch_scatter<double>(stage, (double* const*)(ptrs + 2 * B), len, B, cs));
-  }
+      st = run_lenseflow<T>(p, (const void* const*)ptrs, stage, nm, ex->ncomp, ex->nsteps,
+                            ex->direction, ex->adjoint, cs);
+    if (st == CMBL_OK) step(cmbl::launch_scatter<T>(stage, outs, len, B, cs));
+  };
+  if (p->dtype == CMBL_F32)
+    seq((float*)p->stage);
+  else
+    seq((double*)p->stage);
   p->timing = timing;
   e = hipStreamEndCapture(cs, &ex->graph[slot]);
   if (st != CMBL_OK) return st;

@@ -162,6 +162,46 @@ __global__ __launch_bounds__(256) void k_row_r2c(const T* __restrict__ x, cplx<T
   split_store<N, kRowWave>(v[0], lbuf[grp], P + (2 * pp) * H, P + (2 * pp + 1) * H, t, active);
 }
 
+// Row R2C of a LenseFlow input taken from per-request maps (the serving
+// executor's first launch): row pair p covers rows 2p, 2p+1 of map
+// m = 2p / N, i.e. of request m / ncomp, component m % ncomp, found at
+// src[request] + component N^2 (NULL: zeros, a padding slot).  The rows are
+// also stored to the staging state ystage [nmaps][N][N] (the RK4 state y).
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_row_r2c_gather(const T* const* __restrict__ src, int ncomp,
+                                                        T* __restrict__ ystage,
+                                                        cplx<T>* __restrict__ P,
+                                                        const cplx<T>* __restrict__ twg,
+                                                        long npairs) {
+  constexpr int G = N / 8, PAIRS = 256 / G, H = N / 2;
+  __shared__ cplx<T> lbuf[PAIRS][lds_len<N>()];
+  const int grp = threadIdx.x / G, t = threadIdx.x % G;
+  Twiddles<N, T> tw;
+  tw.load(twg, t);
+  const long p = (long)blockIdx.x * PAIRS + grp;
+  const bool active = p < npairs;
+  const long pp = active ? p : npairs - 1;
+  const long row0 = 2 * pp;
+  const long m = row0 / N;
+  const T* base = src[m / ncomp];
+  const T* xa = base ? base + ((m % ncomp) * (long)N + row0 % N) * N : nullptr;
+  cplx<T> v[1][8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    v[0][q] = xa ? mk(xa[t + q * G], xa[N + t + q * G]) : mk(T(0), T(0));
+  if (active) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      ystage[row0 * N + t + q * G] = v[0][q].x;
+      ystage[(row0 + 1) * N + t + q * G] = v[0][q].y;
+    }
+  }
+  lds_sync<kRowWave>();
+  cplx<T>* const bufs[1] = {lbuf[grp]};
+  fft_lines<N, false, 1, kRowWave>(v, bufs, tw, t);
+  split_store<N, kRowWave>(v[0], lbuf[grp], P + row0 * H, P + (row0 + 1) * H, t, active);
+}
+
 // Row C2R of row pairs: packed P[nrows][N/2] -> x[nrows][N] (no scaling).
 template <typename T, int N>
 __global__ __launch_bounds__(256) void k_row_c2r(const cplx<T>* __restrict__ P, T* __restrict__ x,
@@ -481,14 +521,18 @@ __device__ __forceinline__ void add_sources(T* S, long NN, T cs, T tval, T fx, T
 // and stages 1 and 3 touch no accumulator: 7 map transfers per component and
 // step instead of 10.  RK = 2 (stages 2 and 4) recovers z_s from the row
 // spectra S it reads anyway (one more inverse row FFT of S_a + i S_b).
-template <typename T, int N, int C, bool GRAD, int BS, int RK = 0>
+//
+// OUTP: the last stage of the serving executor; the new state y goes to the
+// per-request outputs outp[b] + (component N + row) N (NULL: a padding slot)
+// instead of the staging state.
+template <typename T, int N, int C, bool GRAD, int BS, int RK = 0, bool OUTP = false>
 __global__ __launch_bounds__(BS) void k_row_lf_std(cplx<T>* __restrict__ S0,
                                                     const cplx<T>* __restrict__ S1,
                                                     T* __restrict__ y, T* __restrict__ acc,
                                                     const T* const* __restrict__ ptabs,
                                                     const cplx<T>* __restrict__ twg, long npairs,
                                                     int stage, T h, int tidx, int do_r2c, T cy,
-                                                    GradArgs<T> ga) {
+                                                    GradArgs<T> ga, T* const* outp) {
   constexpr int G = N / 8, PAIRS = BS / G, H = N / 2;
   static_assert(BS % G == 0 && BS >= G, "a block holds whole lines");
   constexpr long NN = (long)N * N;
@@ -548,6 +592,16 @@ __global__ __launch_bounds__(BS) void k_row_lf_std(cplx<T>* __restrict__ S0,
     }
   };
   cplx<T> z[1][8];
+  // the new state of pixel j of row r (row rr of the flat state)
+  auto store_y = [&](int r, long rr, int j, T v) {
+    if constexpr (OUTP) {
+      T* o = outp[rp.b];
+      const long row = C == 2 ? (long)r * N + rp.ia : (r ? rp.ib : rp.ia);
+      if (o) o[row * N + j] = v;
+    } else {
+      y[rr * N + j] = v;
+    }
+  };
   // rate, RK4 update and next stage input of row r from its gradient g
   auto row_update = [&](int r, const cplx<T>(&g)[8]) {
     const long rr = r ? rp.rb : rp.ra;
@@ -568,7 +622,7 @@ __global__ __launch_bounds__(BS) void k_row_lf_std(cplx<T>* __restrict__ S0,
         rk4_stage(stage, h, k, yv[m], av[m], zz);
         if (active) {
           if (stage == 4)
-            y[rr * N + j] = zz;
+            store_y(r, rr, j, zz);
           else
             acc[rr * N + j] = av[m];
         }
@@ -602,7 +656,7 @@ __global__ __launch_bounds__(BS) void k_row_lf_std(cplx<T>* __restrict__ S0,
           zz = yv[m] + h * k;
         } else {
           zz = av[m] + (zin - yv[m]) / T(3) + (h / T(6)) * k;
-          if (active) y[rr * N + j] = zz;
+          if (active) store_y(r, rr, j, zz);
         }
         if (r == 0)
           z[0][m].x = zz;
@@ -910,6 +964,17 @@ hipError_t launch_lf_row_r2c(int N, const T* y, cplx<T>* S0, long nmaps, const c
 }
 
 template <typename T>
+hipError_t launch_lf_row_r2c_gather(int N, const T* const* src, int ncomp, T* ystage, cplx<T>* S0,
+                                    long nmaps, const cplx<T>* tw, hipStream_t s) {
+  CMBL_DISPATCH_N(N, {
+    const long npairs = nmaps * NN_ / 2;
+    k_row_r2c_gather<T, NN_><<<blocks_for(npairs, 256 / (NN_ / 8)), 256, 0, s>>>(
+        src, ncomp, ystage, S0, tw, npairs);
+  });
+  return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_lf_col(int N, const cplx<T>* src, cplx<T>* dst, long nmaps, const cplx<T>* tw,
                          T dk, hipStream_t s) {
   CMBL_DISPATCH_N(N, {
@@ -935,9 +1000,20 @@ constexpr bool kLowStorageRK4 = CMBL_RK4_LOWSTORE != 0;
 template <typename T, int N, int BS>
 void row_lf_launch(bool adjoint, int C, const GradArgs<T>* ga, cplx<T>* S0, cplx<T>* S1, T* y,
                    T* acc, const T* const* ptabs, const cplx<T>* tw, long npairs, int stage, T h,
-                   int tidx, int do_r2c, int init, T cy, T* zstore, hipStream_t s) {
+                   int tidx, int do_r2c, int init, T cy, T* zstore, hipStream_t s,
+                   T* const* outp) {
   const unsigned nb = blocks_for(npairs, BS / (N / 8));
   const GradArgs<T> g = ga ? *ga : GradArgs<T>{};
+  if (outp) {  // last stage of the executor (standard rate only)
+    constexpr int RKL = kLowStorageRK4 ? 2 : 0;
+    if (C == 2)
+      k_row_lf_std<T, N, 2, false, BS, RKL, true><<<nb, BS, 0, s>>>(
+          S0, S1, y, acc, ptabs, tw, npairs, stage, h, tidx, do_r2c, cy, g, outp);
+    else
+      k_row_lf_std<T, N, 1, false, BS, RKL, true><<<nb, BS, 0, s>>>(
+          S0, S1, y, acc, ptabs, tw, npairs, stage, h, tidx, do_r2c, cy, g, outp);
+    return;
+  }
   if (adjoint) {
     if (C == 2)
       k_row_lf_adj<T, N, 2, BS><<<nb, BS, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage, h,
@@ -948,31 +1024,31 @@ void row_lf_launch(bool adjoint, int C, const GradArgs<T>* ga, cplx<T>* S0, cplx
   } else if (ga) {
     if (C == 2)
       k_row_lf_std<T, N, 2, true, BS><<<nb, BS, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage,
-                                                        h, tidx, do_r2c, cy, g);
+                                                        h, tidx, do_r2c, cy, g, nullptr);
     else
       k_row_lf_std<T, N, 1, true, BS><<<nb, BS, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage,
-                                                        h, tidx, do_r2c, cy, g);
+                                                        h, tidx, do_r2c, cy, g, nullptr);
   } else if (!kLowStorageRK4) {
     if (C == 2)
       k_row_lf_std<T, N, 2, false, BS><<<nb, BS, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage,
-                                                         h, tidx, do_r2c, cy, g);
+                                                         h, tidx, do_r2c, cy, g, nullptr);
     else
       k_row_lf_std<T, N, 1, false, BS><<<nb, BS, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage,
-                                                         h, tidx, do_r2c, cy, g);
+                                                         h, tidx, do_r2c, cy, g, nullptr);
   } else if (stage == 2 || stage == 4) {
     if (C == 2)
       k_row_lf_std<T, N, 2, false, BS, 2><<<nb, BS, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs,
-                                                            stage, h, tidx, do_r2c, cy, g);
+                                                            stage, h, tidx, do_r2c, cy, g, nullptr);
     else
       k_row_lf_std<T, N, 1, false, BS, 2><<<nb, BS, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs,
-                                                            stage, h, tidx, do_r2c, cy, g);
+                                                            stage, h, tidx, do_r2c, cy, g, nullptr);
   } else {
     if (C == 2)
       k_row_lf_std<T, N, 2, false, BS, 1><<<nb, BS, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs,
-                                                            stage, h, tidx, do_r2c, cy, g);
+                                                            stage, h, tidx, do_r2c, cy, g, nullptr);
     else
       k_row_lf_std<T, N, 1, false, BS, 1><<<nb, BS, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs,
-                                                            stage, h, tidx, do_r2c, cy, g);
+                                                            stage, h, tidx, do_r2c, cy, g, nullptr);
   }
 }
 
@@ -980,17 +1056,17 @@ template <typename T>
 hipError_t launch_lf_row(int N, bool adjoint, cplx<T>* S0, cplx<T>* S1, T* y, T* acc,
                          const T* const* ptabs, const cplx<T>* tw, int C, long nmaps, int stage,
                          T h, int tidx, int do_r2c, int init, T dk, hipStream_t s, T* zstore,
-                         const GradArgs<T>* ga) {
+                         const GradArgs<T>* ga, T* const* outp) {
   CMBL_DISPATCH_N(N, {
     const long npairs = nmaps * NN_ / 2;
     const T cy = dk / T(NN_);
     constexpr int BS_SMALL = NN_ / 8 > 64 ? NN_ / 8 : 64;
     if (blocks_for(npairs, 256 / (NN_ / 8)) < CMBL_SMALL_GRID)
       row_lf_launch<T, NN_, BS_SMALL>(adjoint, C, ga, S0, S1, y, acc, ptabs, tw, npairs, stage, h,
-                                      tidx, do_r2c, init, cy, zstore, s);
+                                      tidx, do_r2c, init, cy, zstore, s, outp);
     else
       row_lf_launch<T, NN_, 256>(adjoint, C, ga, S0, S1, y, acc, ptabs, tw, npairs, stage, h,
-                                 tidx, do_r2c, init, cy, zstore, s);
+                                 tidx, do_r2c, init, cy, zstore, s, outp);
   });
   return hipGetLastError();
 }
@@ -1039,7 +1115,9 @@ hipError_t launch_scatter(const T* src, T* const* dst, long len, int count, hipS
                                        hipStream_t);                                            \
   template hipError_t launch_lf_row<T>(int, bool, cplx<T>*, cplx<T>*, T*, T*, const T* const*,  \
                                        const cplx<T>*, int, long, int, T, int, int, int, T,     \
-                                       hipStream_t, T*, const GradArgs<T>*);                    \
+                                       hipStream_t, T*, const GradArgs<T>*, T* const*);         \
+  template hipError_t launch_lf_row_r2c_gather<T>(int, const T* const*, int, T*, cplx<T>*,      \
+                                                  long, const cplx<T>*, hipStream_t);           \
   template hipError_t launch_gtab<T>(int, const double*, T*, int, hipStream_t);                 \
   template hipError_t launch_gather<T>(const T* const*, T*, long, int, hipStream_t);            \
   template hipError_t launch_scatter<T>(const T*, T* const*, long, int, hipStream_t);           \

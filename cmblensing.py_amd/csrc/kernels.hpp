@@ -46,11 +46,19 @@ struct GradArgs {
 // Row pass of one stage.  adjoint: divergence rate, zstore (may be NULL)
 // receives the next stage input.  Standard rate with ga != NULL: also
 // accumulates the delta-phi sources of the gradient ODE.
+// outp != NULL (standard rate, the last stage of the serving executor): the
+// new state goes to the per-request maps outp[b] instead of y.
 template <typename T>
 hipError_t launch_lf_row(int N, bool adjoint, cplx<T>* S0, cplx<T>* S1, T* y, T* acc,
                          const T* const* ptabs, const cplx<T>* tw, int C, long nmaps, int stage,
                          T h, int tidx, int do_r2c, int init, T dk, hipStream_t s,
-                         T* zstore = nullptr, const GradArgs<T>* ga = nullptr);
+                         T* zstore = nullptr, const GradArgs<T>* ga = nullptr,
+                         T* const* outp = nullptr);
+// First launch of the serving executor: row R2C of the requests' maps
+// (src[b] + component N^2, NULL = zeros) into S0, also stored to ystage.
+template <typename T>
+hipError_t launch_lf_row_r2c_gather(int N, const T* const* src, int ncomp, T* ystage, cplx<T>* S0,
+                                    long nmaps, const cplx<T>* tw, hipStream_t s);
 template <typename T>
 hipError_t launch_gather(const T* const* src, T* dst, long len, int count, hipStream_t s);
 template <typename T>
