@@ -46,6 +46,9 @@ _SIGS = {
                                  ctypes.c_ulonglong, ctypes.c_ulonglong, c_void_p]),
     "cmbl_gradf_logpdf": (c_int, [c_void_p, ctypes.POINTER(c_void_p), c_void_p, c_void_p, c_void_p,
                                   c_void_p, c_int, c_int, c_int, c_void_p]),
+    "cmbl_lensed_cotangent": (c_int, [c_void_p, ctypes.POINTER(c_void_p), c_void_p, c_void_p,
+                                      c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
+                                      c_void_p]),
     "cmbl_dot_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p,
                              c_void_p]),
     "cmbl_cg_update_xrz": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -455,6 +458,17 @@ def gradf_logpdf(plan: Plan, phis, f, d, ops: "WfOps", out, batch: int, ncomp: i
         check(plan.lib.cmbl_gradf_logpdf(plan.handle, arr, _ptr(f), _ptr(d), ctypes.byref(ops),
                                          out.data_ptr(), batch, ncomp, nsteps, st),
               "cmbl_gradf_logpdf")
+
+
+def lensed_cotangent(plan: Plan, phis, f, d, ops: "WfOps", lensed, cot, batch: int, ncomp: int,
+                     nsteps: int):
+    arr = (c_void_p * batch)(*[p.handle for p in phis])
+    st = stream_ptr(plan.device)
+    note_streams(phis, st)
+    with torch.cuda.device(plan.device):
+        check(plan.lib.cmbl_lensed_cotangent(plan.handle, arr, _ptr(f), _ptr(d), ctypes.byref(ops),
+                                             lensed.data_ptr(), cot.data_ptr(), batch, ncomp,
+                                             nsteps, st), "cmbl_lensed_cotangent")
 
 
 def dot_f64(plan: Plan, a, b, nrhs: int, ncomp: int, out):

@@ -745,9 +745,13 @@ cmbl_status ensure_red(cmbl_plan* p, int nrhs) {
 }
 
 
+// lensed_out / cot_out (C = 1, 2): stop after the data term and return the
+// lensed maps L f and the cotangent maps map(eb2qu(g)) instead of the
+// gradient (the first half of the chain; out is unused then).
 template <typename T>
 cmbl_status run_gradf(cmbl_plan* p, const cmbl_phi* const* phis, const void* f, const void* d,
-                      const cmbl_wf_ops* ops, void* out, int batch, int C, int n, hipStream_t s) {
+                      const cmbl_wf_ops* ops, void* out, int batch, int C, int n, hipStream_t s,
+                      void* lensed_out = nullptr, void* cot_out = nullptr) {
   const int N = p->N;
   const long nm = (long)batch * C;
   const long NN = (long)N * N;
@@ -791,11 +795,12 @@ cmbl_status run_gradf(cmbl_plan* p, const cmbl_phi* const* phis, const void* f, 
     return CMBL_OK;
   }
   // 1. forward model on f (skipped when f = 0: L 0 = 0)
+  T* Ylens = lensed_out ? (T*)lensed_out : Y2;
   if (f) {
     HIP_TRY(cmbl::launch_rotate<T>((const cplx<T>*)f, X1, nullptr, N, C, batch, 1, s));
     CMBL_TRY(cmbl_irfft2(p, X1, Y1, (int)nm, st));
-    CMBL_TRY(cmbl_lenseflow(p, phis, Y1, Y2, batch, C, n, CMBL_FORWARD_LENSE, 0, st));
-    CMBL_TRY(cmbl_rfft2(p, Y2, X2, (int)nm, st));
+    CMBL_TRY(cmbl_lenseflow(p, phis, Y1, Ylens, batch, C, n, CMBL_FORWARD_LENSE, 0, st));
+    CMBL_TRY(cmbl_rfft2(p, Ylens, X2, (int)nm, st));
   }
   if (!pix) {
     // 2. g = B M Cn^-1 (d - M B L f), as eb2qu(g) in X2
@@ -823,6 +828,7 @@ cmbl_status run_gradf(cmbl_plan* p, const cmbl_phi* const* phis, const void* f, 
     HIP_TRY(cmbl::launch_rotate<T>(X2, X2, nullptr, N, C, batch, 0, s));
     HIP_TRY(cmbl::launch_rotate<T>(X2, X2, (const T*)ops->beam, N, C, batch, 1, s));
   }
+  if (cot_out) return cmbl_irfft2(p, X2, cot_out, (int)nm, st);
   // 3. adjoint LenseFlow of map(eb2qu(g))
   CMBL_TRY(cmbl_irfft2(p, X2, Y2, (int)nm, st));
   CMBL_TRY(cmbl_lenseflow(p, phis, Y2, Y2, batch, C, n, CMBL_FORWARD_LENSE, 1, st));
@@ -857,6 +863,27 @@ extern "C" cmbl_status cmbl_gradf_logpdf(cmbl_plan* p, const cmbl_phi* const* ph
   hipStream_t s = (hipStream_t)stream;
   if (p->dtype == CMBL_F32) return run_gradf<float>(p, phis, f, d, ops, out, batch, ncomp, nsteps, s);
   return run_gradf<double>(p, phis, f, d, ops, out, batch, ncomp, nsteps, s);
+}
+
+extern "C" cmbl_status cmbl_lensed_cotangent(cmbl_plan* p, const cmbl_phi* const* phis,
+                                             const void* f, const void* d,
+                                             const cmbl_wf_ops* ops, void* lensed, void* cot,
+                                             int batch, int ncomp, int nsteps, void* stream) {
+  if (!p || !phis || !f || !ops || !lensed || !cot) return fail(CMBL_EINVAL, "NULL argument");
+  if (ncomp != 1 && ncomp != 2) return fail(CMBL_EINVAL, "ncomp must be 1 (T) or 2 (EB)");
+  if ((long)batch * ncomp > p->max_maps)
+    return fail(CMBL_EINVAL, "batch*ncomp exceeds the plan's max_maps");
+  if (ops->mask_pix) {
+    if (!ops->beam || !ops->cninv) return fail(CMBL_EINVAL, "pixel-mask mode needs beam and cninv");
+  } else if (!ops->mb || !ops->bmcn) {
+    return fail(CMBL_EINVAL, "Fourier-mask mode needs mb and bmcn");
+  }
+  if (batch == 0) return CMBL_OK;
+  CMBL_TRY(ensure_wf_scratch(p));
+  hipStream_t s = (hipStream_t)stream;
+  if (p->dtype == CMBL_F32)
+    return run_gradf<float>(p, phis, f, d, ops, nullptr, batch, ncomp, nsteps, s, lensed, cot);
+  return run_gradf<double>(p, phis, f, d, ops, nullptr, batch, ncomp, nsteps, s, lensed, cot);
 }
 
 extern "C" cmbl_status cmbl_dot_f64(cmbl_plan* p, const void* a, const void* b, int b_f64, int nrhs,

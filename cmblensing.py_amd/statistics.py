@@ -16,8 +16,8 @@ import torch
 
 from . import _lib
 from .constants import Basis, DEFAULT_NSTEPS, Parametrization
-from .fields import (Field, FlatS0, dot, eb2qu, fourier, map as to_map, qu2eb)
-from .lense_flow import _phi_matrix, lense_flow_matrices, lense_flow_vjp_matrices
+from .fields import (Field, FlatS0, FlatS02, dot, eb2qu, fourier, map as to_map, qu2eb)
+from .lense_flow import _phi_matrix, lense_flow_matrices, lense_flow_vjp_matrices, phi_handles_for
 from .matrix_operators import PixelMask, log_det, pinv
 
 
@@ -55,6 +55,34 @@ def logpdf(field, phi, data, noise_covariance, phi_covariance, field_covariance,
     return -(data_product + field_product + phi_product + logdets) / 2
 
 
+def _lensed_and_cotangent(f: Field, phi_m: torch.Tensor, data, noise_covariance, mask, beam,
+                          n: int):
+    """Stacked lensed maps y = L f [B, C, N, N] and the cotangent of the data
+    term c = map(eb2qu(B M^T Cn^-1 (d - M B L f))), the lensing and the whole
+    residual chain in one native call (cmbl_lensed_cotangent, T or EB
+    fields), with the phi handles of the given phi basis; and the lead
+    (batch) shape."""
+    from .gradients import WfOperators, stack_matrices, _ncomp
+    C = _ncomp(f)
+    X, lead = stack_matrices(f)
+    B, N = X.shape[0], X.shape[-2]
+    rdt = torch.float32 if X.dtype == torch.complex64 else torch.float64
+    plan = _lib.get_plan(N, f.theta_pix, rdt, B * C, X.device)
+    ops = WfOperators(None, noise_covariance, mask, beam, rdt, X.device)
+    D, _ = stack_matrices(data)
+    D = D.to(X.dtype if not ops.pixel else rdt)
+    if D.shape[0] == 1 and B > 1:
+        D = D.expand(B, *D.shape[1:])
+    D = D.contiguous()
+    phis = phi_handles_for(plan, phi_m, B, n)
+    y = torch.empty((B, C, N, N), dtype=rdt, device=X.device)
+    c = torch.empty_like(y)
+    _lib.lensed_cotangent(plan, phis, torch.view_as_real(X),
+                          torch.view_as_real(D) if D.is_complex() else D, ops.struct(False),
+                          y, c, B, C, n)
+    return y, c, lead
+
+
 def grad_phi_logpdf(unlensed_field, phi, data, noise_covariance, phi_covariance,
                     field_covariance, mask, beam, n: int = DEFAULT_NSTEPS) -> FlatS0:
     """d logpdf / d phi for a FOURIER phi (gradients.py:29-37 with
@@ -63,19 +91,23 @@ def grad_phi_logpdf(unlensed_field, phi, data, noise_covariance, phi_covariance,
     if phi_m.shape[-1] == phi_m.shape[-2]:
         raise ValueError("grad_phi_logpdf takes phi in the FOURIER basis, like the reference")
     f = unlensed_field
-    fq, x, y, names, lead = _lensed_maps(f, phi, n)
-    N = x.shape[-1]
-    lensed = qu2eb(fourier(fq.replace(**{nm: y[:, c].reshape(*lead, N, N)
-                                         for c, nm in enumerate(names)})))
-    r = _residual(lensed, data, mask, beam)
-    if isinstance(mask, PixelMask):
-        s_ = to_map(eb2qu(pinv(noise_covariance) * r))
-        m = mask.matrix.to(dtype=getattr(s_, s_._matrix_names()[0]).dtype)
-        g = beam * qu2eb(fourier(s_ * m))
+    if not isinstance(f, FlatS02):
+        y, c, lead = _lensed_and_cotangent(f, phi_m, data, noise_covariance, mask, beam, n)
+        N = y.shape[-1]
     else:
-        g = beam * (mask * (pinv(noise_covariance) * r))
-    cot = to_map(eb2qu(g))
-    c = torch.stack([getattr(cot, nm).reshape(-1, N, N) for nm in names], dim=1)
+        fq, x, y, names, lead = _lensed_maps(f, phi, n)
+        N = x.shape[-1]
+        lensed = qu2eb(fourier(fq.replace(**{nm: y[:, c].reshape(*lead, N, N)
+                                             for c, nm in enumerate(names)})))
+        r = _residual(lensed, data, mask, beam)
+        if isinstance(mask, PixelMask):
+            s_ = to_map(eb2qu(pinv(noise_covariance) * r))
+            m = mask.matrix.to(dtype=getattr(s_, s_._matrix_names()[0]).dtype)
+            g = beam * qu2eb(fourier(s_ * m))
+        else:
+            g = beam * (mask * (pinv(noise_covariance) * r))
+        cot = to_map(eb2qu(g))
+        c = torch.stack([getattr(cot, nm).reshape(-1, N, N) for nm in names], dim=1)
     _, d_phi = lense_flow_vjp_matrices(y, c, phi_m, f.theta_pix, n, 1)
     # one gradient per batch element (each chain's own phi), or a single one
     d_phi = d_phi.reshape(*lead, N, N // 2 + 1) if len(lead) else d_phi[0]

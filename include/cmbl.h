@@ -167,6 +167,16 @@ cmbl_status cmbl_gradf_logpdf(cmbl_plan* plan, const cmbl_phi* const* phis, cons
                               const void* d, const cmbl_wf_ops* ops, void* out, int batch,
                               int ncomp, int nsteps, void* stream);
 
+/* The first half of gradf_logpdf for ncomp = 1 (T) or 2 (EB): the lensed maps
+ * L f ([batch][ncomp][N][N] MAP, T or Q,U) and the cotangent of logpdf's data
+ * term, map(eb2qu(B M^T Cn^-1 (d - M B L f))) (statistics.py:159; the cotangent
+ * the reference's custom VJP receives, lense_flow.py:23-46).  With
+ * cmbl_lenseflow_vjp from the lensed maps it gives grad_phi_logpdf
+ * (gradients.py:29-37).  f is required; d == NULL means d = 0. */
+cmbl_status cmbl_lensed_cotangent(cmbl_plan* plan, const cmbl_phi* const* phis, const void* f,
+                                  const void* d, const cmbl_wf_ops* ops, void* lensed, void* cot,
+                                  int batch, int ncomp, int nsteps, void* stream);
+
 /* FP64 inner products for nrhs right-hand sides (fields.py:136-142): out[k] =
  * Re sum conj(a_k) b_k w / N^2 over [ncomp][N][N/2+1].  a is complex double;
  * b is complex double (b_f64 = 1) or complex in the plan dtype.  out is a
