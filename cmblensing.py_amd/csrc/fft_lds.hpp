@@ -62,25 +62,52 @@ __device__ __forceinline__ cplx<T> rot90(cplx<T> a) {
   return INV ? mk(-a.y, a.x) : mk(a.y, -a.x);
 }
 
-// LDS padding: one slot every 8 keeps the stride-8 writes of the first pass
-// off a single bank group.
+// LDS layout of a line.  A line is read contiguously (point t + q*N/8 by
+// thread t) and written with the strides of the Stockham passes (8t + q,
+// then (t/8)*64 + t%8 + 8q, ...), in 8-byte (float2) accesses served by
+// 4-byte banks.  The padded layout (one slot every 8 points) keeps the
+// strided writes on distinct banks but puts point 16 of every contiguous
+// run of 16 lanes on the bank of point 0.  The swizzled layout maps point i
+// to i ^ ((i >> 3) & 31), a permutation of each aligned block of 32 under
+// which both the contiguous and the strided accesses of 16 (or 32)
+// consecutive lanes hit distinct banks, only the mirrored N - k accesses of
+// the Hermitian pack/unpack keeping a 2-way conflict, and needs no padding.
+// On the 256^2 row kernels it removes most of the LDS bank-conflict cycles
+// (rocprofv3 SQ_LDS_BANK_CONFLICT).  It is used at N = 256 only: shorter
+// lines, of which several share a wavefront, are served better by the padded
+// layout, and at N = 512 its per-access index registers cost the row kernels
+// a wave per SIMD.
+#ifndef CMBL_LDS_SWIZZLE
+#define CMBL_LDS_SWIZZLE 1
+#endif
 template <int N>
-__host__ __device__ constexpr int lds_len() {
-  return N + N / 8;
+__host__ __device__ constexpr bool lds_swizzled() {
+  return CMBL_LDS_SWIZZLE != 0 && N == 256;
 }
-__device__ __forceinline__ int pad_idx(int i) { return i + (i >> 3); }
+template <int N, bool SW = lds_swizzled<N>()>
+__host__ __device__ constexpr int lds_len() {
+  return SW ? N : N + N / 8;
+}
+template <int N, bool SW = lds_swizzled<N>()>
+__device__ __forceinline__ int lds_idx(int i) {
+  if constexpr (SW)
+    return i ^ ((i >> 3) & 31);
+  else
+    return i + (i >> 3);
+}
 
-// Stride between the LDS buffers of neighbouring lines.  Kernels whose
-// consecutive lanes work on different lines (the column passes) would put all
-// lines on one bank if the stride were a multiple of the bank count; rounding
-// lds_len up to 32 complex and adding 1 spreads 16 float2 (or 8 double2) lines
-// over distinct banks.
+// Stride between the LDS buffers of neighbouring lines in the kernels whose
+// consecutive lanes work on different lines (the column passes, which use
+// the padded layout: lane g of KC consecutive lanes works on line g).  An
+// odd stride, one complex more than a multiple of 32, puts those lines on
+// distinct banks; a stride of 2 more doubled the measured bank-conflict
+// cycles of the 256^2 column pass.
 #ifndef CMBL_LINE_PAD
 #define CMBL_LINE_PAD 1
 #endif
 template <int N>
 __host__ __device__ constexpr int line_stride() {
-  return ((lds_len<N>() + 31) / 32) * 32 + CMBL_LINE_PAD;
+  return ((lds_len<N, false>() + 31) / 32) * 32 + CMBL_LINE_PAD;
 }
 
 template <int N>
@@ -145,7 +172,10 @@ struct Passes {
 // every transform the thread runs reuses them.
 template <int N, typename T>
 struct Twiddles {
+  static constexpr bool kReload = false;
   cplx<T> w[Passes<N>::NTW > 0 ? Passes<N>::NTW : 1];
+  // twiddle number wi of this thread (idx: its index in the table)
+  __device__ __forceinline__ cplx<T> at(int wi, int) const { return w[wi]; }
   __device__ __forceinline__ void load(const cplx<T>* __restrict__ tw, int t) {
     using P = Passes<N>;
     int i = 0;
@@ -166,6 +196,21 @@ struct Twiddles {
       }
     }
   }
+};
+
+// The same twiddles read from an LDS copy of the table (tab[k] = tw[k],
+// k < N) at each use, for kernels whose occupancy is set by register
+// pressure and which run several transforms: the registers are then taken
+// only while a transform runs.  The table lies in its own __shared__ array, so
+// the compiler could prove its values unchanged from one transform to the
+// next and keep them in registers; fft_lines therefore passes the thread
+// index through an empty asm statement first (kReload), which makes every
+// transform's table addresses opaque to it.
+template <int N, typename T>
+struct TwiddlesLds {
+  static constexpr bool kReload = true;
+  const cplx<T>* tab;
+  __device__ __forceinline__ cplx<T> at(int, int idx) const { return tab[idx]; }
 };
 
 // Synchronisation of an LDS exchange.  Block scope: __syncthreads().  Wave
@@ -194,15 +239,19 @@ __device__ __forceinline__ cplx<T> cmul_tw(cplx<T> a, cplx<T> w) {
 
 // NL independent lines, each handled by the same group of N/8 threads
 // (thread index t within the group).  bufs[l] points to that line's LDS
-// buffer (lds_len<N>() entries), tw to the thread's twiddles.  Every thread
-// of the workgroup (WAVE: of the wavefront) must call this the same number
-// of times: the passes synchronise with lds_sync<WAVE>().
-template <int N, bool INV, int NL, bool WAVE = false, typename T>
+// buffer (lds_len<N, SW>() entries in layout SW), tw to the thread's
+// twiddles (Twiddles or TwiddlesLds).  Every thread of the workgroup (WAVE: of the wavefront) must
+// call this the same number of times: the passes synchronise with
+// lds_sync<WAVE>().
+template <int N, bool INV, int NL, bool WAVE = false, bool SW = lds_swizzled<N>(), typename T,
+          typename TW>
 __device__ __forceinline__ void fft_lines(cplx<T> (&v)[NL][8], cplx<T>* const (&bufs)[NL],
-                                          const Twiddles<N, T>& tw, int t) {
+                                          const TW& tw, int t) {
   using P = Passes<N>;
   constexpr int P8 = P::P8, REM = P::REM, G = P::G;
   static_assert(N >= 16 && N <= 512 && (N & (N - 1)) == 0, "N must be 2^m, 16..512");
+  int tt = t;  // thread index for the twiddle addresses
+  if constexpr (TW::kReload) __asm__ volatile("" : "+v"(tt));
 
   int Ns = 1;
   int wi = 0;
@@ -211,9 +260,10 @@ __device__ __forceinline__ void fft_lines(cplx<T> (&v)[NL][8], cplx<T>* const (&
     // radix-8 butterfly j = t on points t + q*N/8
     const int k = t & (Ns - 1);
     if (p > 0) {
+      const int ktw = tt & (Ns - 1);
 #pragma unroll
       for (int q = 1; q < 8; ++q) {
-        const cplx<T> w = tw.w[wi++];
+        const cplx<T> w = tw.at(wi++, (q * ktw) * (N / (Ns * 8)));
 #pragma unroll
         for (int l = 0; l < NL; ++l) v[l][q] = cmul_tw<INV>(v[l][q], w);
       }
@@ -227,14 +277,14 @@ __device__ __forceinline__ void fft_lines(cplx<T> (&v)[NL][8], cplx<T>* const (&
 #pragma unroll
       for (int l = 0; l < NL; ++l)
 #pragma unroll
-        for (int q = 0; q < 8; ++q) bufs[l][pad_idx(base + q * Ns)] = v[l][q];
+        for (int q = 0; q < 8; ++q) bufs[l][lds_idx<N, SW>(base + q * Ns)] = v[l][q];
       lds_sync<WAVE>();
       Ns *= 8;
       if (p < P8 - 1) {
 #pragma unroll
         for (int l = 0; l < NL; ++l)
 #pragma unroll
-          for (int q = 0; q < 8; ++q) v[l][q] = bufs[l][pad_idx(t + q * G)];
+          for (int q = 0; q < 8; ++q) v[l][q] = bufs[l][lds_idx<N, SW>(t + q * G)];
       }
     }
   }
@@ -246,12 +296,13 @@ __device__ __forceinline__ void fft_lines(cplx<T> (&v)[NL][8], cplx<T>* const (&
 #pragma unroll
     for (int l = 0; l < NL; ++l)
 #pragma unroll
-      for (int m = 0; m < 8; ++m) v[l][m] = bufs[l][pad_idx(t + m * G)];
+      for (int m = 0; m < 8; ++m) v[l][m] = bufs[l][lds_idx<N, SW>(t + m * G)];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
+      const int ktw = (tt + b * G) & (Ns - 1);  // Ns * R == N
 #pragma unroll
       for (int q = 1; q < R; ++q) {
-        const cplx<T> w = tw.w[wi++];
+        const cplx<T> w = tw.at(wi++, q * ktw);
 #pragma unroll
         for (int l = 0; l < NL; ++l) v[l][b + q * NB] = cmul_tw<INV>(v[l][b + q * NB], w);
       }

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "fft_lds.hpp"
 #include "kernels.hpp"
@@ -43,6 +44,18 @@ __device__ __forceinline__ cplx<T> imul(cplx<T> F, T s) {
 #define CMBL_WAVE_SYNC 1
 #endif
 constexpr bool kRowWave = CMBL_WAVE_SYNC != 0;
+
+// The stage-2/4 standard-rate row pass (RK = 2) runs four line transforms and
+// needs more registers than 4 waves per SIMD allow when its twiddles stay in
+// registers; at FP32 and N <= 256 it reads them from an LDS table instead (2 KB
+// at N = 256, so 4 blocks of 256 threads still fit a CU's LDS).
+#ifndef CMBL_TW_LDS
+#define CMBL_TW_LDS 1
+#endif
+template <typename T, int N, int RK>
+constexpr bool row_tw_lds() {
+  return CMBL_TW_LDS != 0 && RK == 2 && sizeof(T) == 4 && N <= 256;
+}
 
 // Row pairing used by the fused stage kernels.  Pair index p covers two
 // map-rows: for C == 2 the Q and U rows of the same pixel row (they share the
@@ -79,11 +92,11 @@ __device__ __forceinline__ RowPair row_pair(int p, int C) {
 template <int N, typename T>
 __device__ __forceinline__ void put_herm(cplx<T>* buf, int k, cplx<T> a, cplx<T> b) {
   if (k == 0) {
-    buf[pad_idx(0)] = mk(a.x, b.x);
-    buf[pad_idx(N / 2)] = mk(a.y, b.y);
+    buf[lds_idx<N>(0)] = mk(a.x, b.x);
+    buf[lds_idx<N>(N / 2)] = mk(a.y, b.y);
   } else {
-    buf[pad_idx(k)] = mk(a.x - b.y, a.y + b.x);      // a + i b
-    buf[pad_idx(N - k)] = mk(a.x + b.y, b.x - a.y);  // conj(a) + i conj(b)
+    buf[lds_idx<N>(k)] = mk(a.x - b.y, a.y + b.x);      // a + i b
+    buf[lds_idx<N>(N - k)] = mk(a.x + b.y, b.x - a.y);  // conj(a) + i conj(b)
   }
 }
 
@@ -100,7 +113,7 @@ __device__ __forceinline__ void herm_pair_to_line(const cplx<T>* A, const cplx<T
   }
   lds_sync<WAVE>();
 #pragma unroll
-  for (int q = 0; q < 8; ++q) v[q] = buf[pad_idx(t + q * G)];
+  for (int q = 0; q < 8; ++q) v[q] = buf[lds_idx<N>(t + q * G)];
 }
 
 // After a forward FFT of z = a + i b (v[m] = Z[t + m*G]) split into the two
@@ -111,20 +124,20 @@ __device__ __forceinline__ void split_store(const cplx<T> (&v)[8], cplx<T>* buf,
   constexpr int G = N / 8;
   lds_sync<WAVE>();
 #pragma unroll
-  for (int m = 0; m < 8; ++m) buf[pad_idx(t + m * G)] = v[m];
+  for (int m = 0; m < 8; ++m) buf[lds_idx<N>(t + m * G)] = v[m];
   lds_sync<WAVE>();
   if (!active) return;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int k = t + q * G;
-    cplx<T> zk = buf[pad_idx(k)];
+    cplx<T> zk = buf[lds_idx<N>(k)];
     cplx<T> a, b;
     if (k == 0) {
-      cplx<T> zn = buf[pad_idx(N / 2)];
+      cplx<T> zn = buf[lds_idx<N>(N / 2)];
       a = mk(zk.x, zn.x);
       b = mk(zk.y, zn.y);
     } else {
-      cplx<T> zc = conj(buf[pad_idx(N - k)]);
+      cplx<T> zc = conj(buf[lds_idx<N>(N - k)]);
       const T h = T(0.5);
       a = scale(zk + zc, h);
       cplx<T> d = zk - zc;
@@ -248,11 +261,11 @@ __global__ __launch_bounds__(KC* N / 8) void k_col_fwd_unpack(const cplx<T>* __r
   for (int q = 0; q < 8; ++q) v[0][q] = Pm[(long)(t + q * G) * H + k];
   __syncthreads();
   cplx<T>* const bufs[1] = {lbuf[g]};
-  fft_lines<N, false, 1>(v, bufs, tw, t);
+  fft_lines<N, false, 1, false, false>(v, bufs, tw, t);
   // the packed column needs G[-kx]: go through LDS (uniform for all groups)
   __syncthreads();
 #pragma unroll
-  for (int q = 0; q < 8; ++q) lbuf[g][pad_idx(t + q * G)] = v[0][q];
+  for (int q = 0; q < 8; ++q) lbuf[g][lds_idx<N, false>(t + q * G)] = v[0][q];
   __syncthreads();
   if (k == 0) {
     const T h = T(0.5);
@@ -260,7 +273,7 @@ __global__ __launch_bounds__(KC* N / 8) void k_col_fwd_unpack(const cplx<T>* __r
     for (int q = 0; q < 8; ++q) {
       const int kx = t + q * G;
       cplx<T> z = v[0][q];
-      cplx<T> zc = conj(lbuf[g][pad_idx((N - kx) & (N - 1))]);
+      cplx<T> zc = conj(lbuf[g][lds_idx<N, false>((N - kx) & (N - 1))]);
       cplx<T> a = scale(z + zc, h);
       cplx<T> d = z - zc;
       cplx<T> b = mk(h * d.y, -h * d.x);
@@ -307,7 +320,7 @@ __global__ __launch_bounds__(KC* N / 8) void k_col_inv_pack(const cplx<T>* __res
   }
   __syncthreads();
   cplx<T>* const bufs[1] = {lbuf[g]};
-  fft_lines<N, true, 1>(v, bufs, tw, t);
+  fft_lines<N, true, 1, false, false>(v, bufs, tw, t);
 #pragma unroll
   for (int q = 0; q < 8; ++q) Pm[(long)(t + q * G) * H + k] = scale(v[0][q], scale_);
 }
@@ -454,7 +467,7 @@ __global__ __launch_bounds__(KC* N / 8) void k_col_lf_dx(const cplx<T>* src, cpl
   for (int q = 0; q < 8; ++q) v[0][q] = s0[(long)(t + q * G) * H];
   __syncthreads();
   cplx<T>* const bufs[1] = {lbuf[g]};
-  fft_lines<N, false, 1>(v, bufs, tw, t);
+  fft_lines<N, false, 1, false, false>(v, bufs, tw, t);
   const T sc = T(1) / (T(N) * T(N));
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
@@ -462,30 +475,19 @@ __global__ __launch_bounds__(KC* N / 8) void k_col_lf_dx(const cplx<T>* src, cpl
     const T kxv = (T)kx_signed<N>(kx) * dk * sc;
     v[0][q] = (k == 0 && kx == N / 2) ? mk(T(0), T(0)) : imul(v[0][q], kxv);
   }
-  fft_lines<N, true, 1>(v, bufs, tw, t);
+  fft_lines<N, true, 1, false, false>(v, bufs, tw, t);
 #pragma unroll
   for (int q = 0; q < 8; ++q) d0[(long)(t + q * G) * H] = v[0][q];
 }
 
-// RK4 bookkeeping for one stage (util.py:184-199), per pixel:
+// Classic RK4 bookkeeping of one stage (util.py:184-199), per pixel, with
+// the rate k of the stage:
 //   stage 1: acc = y + h/6 k ; z = y + h/2 k
 //   stage 2: acc += h/3 k    ; z = y + h/2 k
 //   stage 3: acc += h/3 k    ; z = y + h k
 //   stage 4: y = acc + h/6 k ; z = y
-// Register form: y, a (acc) in, a updated in place, z out.  Stage 4 returns
-// the new y in z.
-template <typename T>
-__device__ __forceinline__ void rk4_stage(int stage, T h, T k, T y, T& a, T& z) {
-  if (stage == 1) {
-    a = y + (h / T(6)) * k;
-    z = y + (h / T(2)) * k;
-  } else if (stage == 4) {
-    z = a + (h / T(6)) * k;
-  } else {
-    a = a + (h / T(3)) * k;
-    z = y + (stage == 2 ? h / T(2) : h) * k;
-  }
-}
+// z is the next stage input.  The row kernels branch on the stage once per
+// row (it is uniform over the grid) around straight-line pixel loops.
 
 // Extra operands of the standard-rate row pass inside the gradient ODE
 // (lense_flow.py:156-252): the stage inputs of delta f, the M^-1 tables, and
@@ -536,10 +538,18 @@ __global__ __launch_bounds__(BS) void k_row_lf_std(cplx<T>* __restrict__ S0,
   constexpr int G = N / 8, PAIRS = BS / G, H = N / 2;
   static_assert(BS % G == 0 && BS >= G, "a block holds whole lines");
   constexpr long NN = (long)N * N;
+  constexpr bool TWL = row_tw_lds<T, N, RK>();
   __shared__ cplx<T> lbuf[PAIRS][2][lds_len<N>()];
+  __shared__ cplx<T> twl[TWL ? N : 1];
   const int grp = threadIdx.x / G, t = threadIdx.x % G;
-  Twiddles<N, T> tw;
-  tw.load(twg, t);
+  std::conditional_t<TWL, TwiddlesLds<N, T>, Twiddles<N, T>> tw;
+  if constexpr (TWL) {
+    for (int i = threadIdx.x; i < N; i += BS) twl[i] = twg[i];
+    __syncthreads();
+    tw.tab = twl;
+  } else {
+    tw.load(twg, t);
+  }
   const long p = (long)blockIdx.x * PAIRS + grp;
   const bool active = p < npairs;
   const RowPair rp = row_pair<N>((int)(active ? p : npairs - 1), C);
@@ -566,7 +576,7 @@ __global__ __launch_bounds__(BS) void k_row_lf_std(cplx<T>* __restrict__ S0,
     for (int q = 0; q < 4; ++q) put_herm<N>(bufa, t + q * G, sa[q], sb[q]);
     lds_sync<kRowWave>();
 #pragma unroll
-    for (int q = 0; q < 8; ++q) zr[0][q] = bufa[pad_idx(t + q * G)];
+    for (int q = 0; q < 8; ++q) zr[0][q] = bufa[lds_idx<N>(t + q * G)];
     cplx<T>* const bz[1] = {bufa};
     fft_lines<N, true, 1, kRowWave>(zr, bz, tw, t);
     lds_sync<kRowWave>();
@@ -608,60 +618,85 @@ __global__ __launch_bounds__(BS) void k_row_lf_std(cplx<T>* __restrict__ S0,
     if (r == 0 || C == 1) load_p(r ? rp.ib : rp.ia);
     T yv[8], av[8];
     if constexpr (RK == 0) {
+      // the classic bookkeeping above, one branch on the stage per row
+      T zz[8];
 #pragma unroll
-      for (int m = 0; m < 8; ++m) {
-        const int j = t + m * G;
-        yv[m] = stage == 4 ? T(0) : y[rr * N + j];
-        av[m] = stage == 1 ? T(0) : acc[rr * N + j];
+      for (int m = 0; m < 8; ++m) zz[m] = px[m] * g[m].x + py[m] * g[m].y;  // the rate k
+      if (stage == 4) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) zz[m] = acc[rr * N + t + m * G] + (h / T(6)) * zz[m];
+        if (active) {
+#pragma unroll
+          for (int m = 0; m < 8; ++m) store_y(r, rr, t + m * G, zz[m]);
+        }
+      } else {
+        const T ha = stage == 1 ? h / T(6) : h / T(3);
+        const T hz = stage == 3 ? h : h / T(2);
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+          const int j = t + m * G;
+          yv[m] = y[rr * N + j];
+          av[m] = (stage == 1 ? yv[m] : acc[rr * N + j]) + ha * zz[m];
+          zz[m] = yv[m] + hz * zz[m];
+        }
+        if (active) {
+#pragma unroll
+          for (int m = 0; m < 8; ++m) acc[rr * N + t + m * G] = av[m];
+        }
       }
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
-        const int j = t + m * G;
-        const T k = px[m] * g[m].x + py[m] * g[m].y;
-        T zz;
-        rk4_stage(stage, h, k, yv[m], av[m], zz);
-        if (active) {
-          if (stage == 4)
-            store_y(r, rr, j, zz);
-          else
-            acc[rr * N + j] = av[m];
-        }
         if (r == 0)
-          z[0][m].x = zz;
+          z[0][m].x = zz[m];
         else
-          z[0][m].y = zz;
+          z[0][m].y = zz[m];
       }
     } else {
+      // the stage is uniform over the grid: one branch per row, around
+      // straight-line pixel loops (a branch per pixel keeps every operand of
+      // the row live across all of them and costs occupancy)
+      T zz[8];
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         const int j = t + m * G;
         yv[m] = y[rr * N + j];
-        av[m] = stage == 4 ? acc[rr * N + j] : T(0);
+        zz[m] = px[m] * g[m].x + py[m] * g[m].y;  // the rate k
       }
-      const T inv_n = T(1) / T(N);  // exact: N is a power of two
+      if constexpr (RK == 1) {  // stages 1 and 3
+        const T hk = stage == 1 ? h / T(2) : h;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) zz[m] = yv[m] + hk * zz[m];
+      } else {  // stages 2 and 4
+        const T inv_n = T(1) / T(N);  // exact: N is a power of two
+        if (stage == 2) {
+#pragma unroll
+          for (int m = 0; m < 8; ++m) {
+            const T zin = (r ? zr[0][m].y : zr[0][m].x) * inv_n;
+            av[m] = (T(2) * yv[m] + zin) / T(3) + (h / T(3)) * zz[m];
+            zz[m] = yv[m] + (h / T(2)) * zz[m];
+          }
+          if (active) {
+#pragma unroll
+            for (int m = 0; m < 8; ++m) acc[rr * N + t + m * G] = av[m];
+          }
+        } else {
+#pragma unroll
+          for (int m = 0; m < 8; ++m) {
+            const T zin = (r ? zr[0][m].y : zr[0][m].x) * inv_n;
+            zz[m] = acc[rr * N + t + m * G] + (zin - yv[m]) / T(3) + (h / T(6)) * zz[m];
+          }
+          if (active) {
+#pragma unroll
+            for (int m = 0; m < 8; ++m) store_y(r, rr, t + m * G, zz[m]);
+          }
+        }
+      }
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
-        const int j = t + m * G;
-        const T k = px[m] * g[m].x + py[m] * g[m].y;
-        T zin = T(0);
-        if constexpr (RK == 2) zin = (r ? zr[0][m].y : zr[0][m].x) * inv_n;
-        T zz;
-        if (stage == 1) {
-          zz = yv[m] + (h / T(2)) * k;
-        } else if (stage == 2) {
-          const T a = (T(2) * yv[m] + zin) / T(3) + (h / T(3)) * k;
-          if (active) acc[rr * N + j] = a;
-          zz = yv[m] + (h / T(2)) * k;
-        } else if (stage == 3) {
-          zz = yv[m] + h * k;
-        } else {
-          zz = av[m] + (zin - yv[m]) / T(3) + (h / T(6)) * k;
-          if (active) store_y(r, rr, j, zz);
-        }
         if (r == 0)
-          z[0][m].x = zz;
+          z[0][m].x = zz[m];
         else
-          z[0][m].y = zz;
+          z[0][m].y = zz[m];
       }
     }
   };
@@ -673,7 +708,7 @@ __global__ __launch_bounds__(BS) void k_row_lf_std(cplx<T>* __restrict__ S0,
       cplx<T>* const buf1[1] = {r ? bufb : bufa};
       cplx<T> w[1][8];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) w[0][q] = buf1[0][pad_idx(t + q * G)];
+      for (int q = 0; q < 8; ++q) w[0][q] = buf1[0][lds_idx<N>(t + q * G)];
       fft_lines<N, true, 1, kRowWave>(w, buf1, tw, t);
       row_update(r, w[0]);
     }
@@ -681,8 +716,8 @@ __global__ __launch_bounds__(BS) void k_row_lf_std(cplx<T>* __restrict__ S0,
     cplx<T> v[2][8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      v[0][q] = bufa[pad_idx(t + q * G)];
-      v[1][q] = bufb[pad_idx(t + q * G)];
+      v[0][q] = bufa[lds_idx<N>(t + q * G)];
+      v[1][q] = bufb[lds_idx<N>(t + q * G)];
     }
     cplx<T>* const bufs[2] = {bufa, bufb};
     fft_lines<N, true, 2, kRowWave>(v, bufs, tw, t);
@@ -786,16 +821,17 @@ __global__ __launch_bounds__(BS) void k_row_lf_adj(cplx<T>* __restrict__ S0,
     }
     lds_sync<kRowWave>();
 #pragma unroll
-    for (int q = 0; q < 8; ++q) v[0][q] = bufa[pad_idx(t + q * G)];
+    for (int q = 0; q < 8; ++q) v[0][q] = bufa[lds_idx<N>(t + q * G)];
     cplx<T>* const bufs[1] = {bufa};
     fft_lines<N, true, 1, kRowWave>(v, bufs, tw, t);
   }
+  // the next stage's p = M(t_next)^-1 grad phi, loaded before the update so
+  // that its latency overlaps the update's
   const T* pt = ptabs[rp.b] + (long)(do_r2c ? tidx_next : 0) * 2 * NN;
-  T pxa[8], pya[8], pxb[8], pyb[8], ya[8], yb[8], aa[8], ab[8];
+  T pxa[8], pya[8], pxb[8], pyb[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
-    const int j = t + m * G;
-    const long ja = (long)rp.ia * N + j, jb = (long)rp.ib * N + j;
+    const long ja = (long)rp.ia * N + t + m * G, jb = (long)rp.ib * N + t + m * G;
     pxa[m] = do_r2c ? pt[ja] : T(0);
     pya[m] = do_r2c ? pt[NN + ja] : T(0);
     if constexpr (C == 2) {  // Q and U rows share the pixel row
@@ -805,32 +841,46 @@ __global__ __launch_bounds__(BS) void k_row_lf_adj(cplx<T>* __restrict__ S0,
       pxb[m] = do_r2c ? pt[jb] : T(0);
       pyb[m] = do_r2c ? pt[NN + jb] : T(0);
     }
-    ya[m] = (stage == 4 && !init) ? T(0) : y[rp.ra * N + j];
-    yb[m] = (stage == 4 && !init) ? T(0) : y[rp.rb * N + j];
-    aa[m] = (stage == 1 || init) ? T(0) : acc[rp.ra * N + j];
-    ab[m] = (stage == 1 || init) ? T(0) : acc[rp.rb * N + j];
   }
+  // RK4 update, the classic bookkeeping (one branch on the stage)
   T za[8], zb[8];
   if (init) {
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-      za[m] = ya[m];
-      zb[m] = yb[m];
+      za[m] = y[rp.ra * N + t + m * G];
+      zb[m] = y[rp.rb * N + t + m * G];
+    }
+  } else if (stage == 4) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      za[m] = acc[rp.ra * N + t + m * G] + (h / T(6)) * v[0][m].x;
+      zb[m] = acc[rp.rb * N + t + m * G] + (h / T(6)) * v[0][m].y;
+    }
+    if (active) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        y[rp.ra * N + t + m * G] = za[m];
+        y[rp.rb * N + t + m * G] = zb[m];
+      }
     }
   } else {
+    const T ha = stage == 1 ? h / T(6) : h / T(3);
+    const T hz = stage == 3 ? h : h / T(2);
+    T aa[8], ab[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
       const int j = t + m * G;
-      rk4_stage(stage, h, v[0][m].x, ya[m], aa[m], za[m]);
-      rk4_stage(stage, h, v[0][m].y, yb[m], ab[m], zb[m]);
-      if (active) {
-        if (stage == 4) {
-          y[rp.ra * N + j] = za[m];
-          y[rp.rb * N + j] = zb[m];
-        } else {
-          acc[rp.ra * N + j] = aa[m];
-          acc[rp.rb * N + j] = ab[m];
-        }
+      const T ya = y[rp.ra * N + j], yb = y[rp.rb * N + j];
+      aa[m] = (stage == 1 ? ya : acc[rp.ra * N + j]) + ha * v[0][m].x;
+      ab[m] = (stage == 1 ? yb : acc[rp.rb * N + j]) + ha * v[0][m].y;
+      za[m] = ya + hz * v[0][m].x;
+      zb[m] = yb + hz * v[0][m].y;
+    }
+    if (active) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        acc[rp.ra * N + t + m * G] = aa[m];
+        acc[rp.rb * N + t + m * G] = ab[m];
       }
     }
   }
