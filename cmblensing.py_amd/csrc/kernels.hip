@@ -56,6 +56,17 @@ template <typename T, int N, int RK>
 constexpr bool row_tw_lds() {
   return CMBL_TW_LDS != 0 && RK == 2 && sizeof(T) == 4 && N <= 256;
 }
+// The column pass of the LenseFlow stages (two transforms per column) holds
+// 13 twiddles in registers; at FP32, N = 256 it reads them from an LDS table
+// instead (2 KB beside 37 KB of line buffers), which takes it from 79 to 56
+// VGPRs and from 3 to 4 blocks per CU.
+#ifndef CMBL_COL_TW_LDS
+#define CMBL_COL_TW_LDS 1
+#endif
+template <typename T, int N>
+constexpr bool col_tw_lds() {
+  return CMBL_COL_TW_LDS != 0 && sizeof(T) == 4 && N == 256;
+}
 
 // Row pairing used by the fused stage kernels.  Pair index p covers two
 // map-rows: for C == 2 the Q and U rows of the same pixel row (they share the
@@ -454,10 +465,17 @@ template <typename T, int N, int KC>
 __global__ __launch_bounds__(KC* N / 8) void k_col_lf_dx(const cplx<T>* src, cplx<T>* dst,
                                                          const cplx<T>* __restrict__ twg, T dk) {
   constexpr int G = N / 8, H = N / 2, TILES = H / KC;
+  constexpr bool TWL = col_tw_lds<T, N>();
   __shared__ cplx<T> lbuf[KC][line_stride<N>()];
+  __shared__ cplx<T> twl[TWL ? N : 1];
   const int g = threadIdx.x % KC, t = threadIdx.x / KC;
-  Twiddles<N, T> tw;
-  tw.load(twg, t);
+  std::conditional_t<TWL, TwiddlesLds<N, T>, Twiddles<N, T>> tw;
+  if constexpr (TWL) {
+    for (int i = threadIdx.x; i < N; i += KC * G) twl[i] = twg[i];
+    tw.tab = twl;  // the __syncthreads() below orders these stores
+  } else {
+    tw.load(twg, t);
+  }
   const long m = blockIdx.x / TILES;
   const int k = (blockIdx.x % TILES) * KC + g;
   const cplx<T>* s0 = src + m * N * H + k;
