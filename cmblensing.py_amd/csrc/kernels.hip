@@ -47,14 +47,15 @@ constexpr bool kRowWave = CMBL_WAVE_SYNC != 0;
 
 // The stage-2/4 standard-rate row pass (RK = 2) runs four line transforms and
 // needs more registers than 4 waves per SIMD allow when its twiddles stay in
-// registers; at FP32 and N <= 256 it reads them from an LDS table instead (2 KB
-// at N = 256, so 4 blocks of 256 threads still fit a CU's LDS).
+// registers; at FP32 it reads them from an LDS table instead (2 KB at
+// N = 256, 4 KB at N = 512: 4 blocks of 256 threads still fit a CU's 160 KB
+// of LDS).
 #ifndef CMBL_TW_LDS
 #define CMBL_TW_LDS 1
 #endif
 template <typename T, int N, int RK>
 constexpr bool row_tw_lds() {
-  return CMBL_TW_LDS != 0 && RK == 2 && sizeof(T) == 4 && N <= 256;
+  return CMBL_TW_LDS != 0 && RK == 2 && sizeof(T) == 4;
 }
 // The column pass of the LenseFlow stages (two transforms per column) holds
 // 13 twiddles in registers; at FP32, N = 256 it reads them from an LDS table
