@@ -7,13 +7,16 @@
 // load/store a line with fully coalesced accesses (consecutive threads touch
 // consecutive points), and chain a forward transform, a pointwise multiply and
 // an inverse transform without a trip through memory.  Between radix passes
-// the line goes through an LDS buffer of lds_len<N>() complex values.
+// the line goes through an LDS buffer of lds_len<N>() complex values (padded
+// or swizzled layout, below).
 //
 // Pass plan: floor(m/3) radix-8 passes, then one radix-2 or radix-4 pass when
 // m is not a multiple of 3 (each thread then runs 4 or 2 small butterflies).
 // Twiddles come from a table tw[k] = exp(-2 pi i k / N) computed on the host
 // in double precision and rounded once to T; each thread keeps the ones it
-// needs in registers (Twiddles<N, T>).
+// needs in registers (Twiddles<N, T>), or reads them from an LDS copy of the
+// table at each transform where registers set a kernel's occupancy
+// (TwiddlesLds<N, T>).
 #pragma once
 #include <hip/hip_runtime.h>
 
