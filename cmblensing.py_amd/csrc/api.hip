@@ -848,6 +848,8 @@ extern "C" cmbl_status cmbl_gradf_logpdf(cmbl_plan* p, const cmbl_phi* const* ph
                                          const void* d, const cmbl_wf_ops* ops, void* out,
                                          int batch, int ncomp, int nsteps, void* stream) {
   if (!p || !phis || !ops || !out) return fail(CMBL_EINVAL, "NULL argument");
+  if (batch < 0) return fail(CMBL_EINVAL, "batch < 0");
+  if (nsteps < 1) return fail(CMBL_EINVAL, "nsteps must be >= 1");
   if (ncomp < 1 || ncomp > 3) return fail(CMBL_EINVAL, "ncomp must be 1 (T), 2 (EB) or 3 (TEB)");
   if ((long)batch * ncomp > p->max_maps)
     return fail(CMBL_EINVAL, "batch*ncomp exceeds the plan's max_maps");
@@ -870,6 +872,8 @@ extern "C" cmbl_status cmbl_lensed_cotangent(cmbl_plan* p, const cmbl_phi* const
                                              const cmbl_wf_ops* ops, void* lensed, void* cot,
                                              int batch, int ncomp, int nsteps, void* stream) {
   if (!p || !phis || !f || !ops || !lensed || !cot) return fail(CMBL_EINVAL, "NULL argument");
+  if (batch < 0) return fail(CMBL_EINVAL, "batch < 0");
+  if (nsteps < 1) return fail(CMBL_EINVAL, "nsteps must be >= 1");
   if (ncomp != 1 && ncomp != 2) return fail(CMBL_EINVAL, "ncomp must be 1 (T) or 2 (EB)");
   if ((long)batch * ncomp > p->max_maps)
     return fail(CMBL_EINVAL, "batch*ncomp exceeds the plan's max_maps");
@@ -1048,6 +1052,8 @@ extern "C" cmbl_status cmbl_lenseflow_vjp(cmbl_plan* p, const cmbl_phi* const* p
                                           int nsteps, int direction, void* stream) {
   if (!p || !phis || !lensed || !cotangent || !d_field || !d_phi)
     return fail(CMBL_EINVAL, "NULL argument");
+  if (batch < 0) return fail(CMBL_EINVAL, "batch < 0");
+  if (nsteps < 1) return fail(CMBL_EINVAL, "nsteps must be >= 1");
   if (ncomp != 1 && ncomp != 2) return fail(CMBL_EINVAL, "ncomp must be 1 (T) or 2 (QU)");
   if (direction != CMBL_FORWARD_LENSE && direction != CMBL_INVERSE_LENSE)
     return fail(CMBL_EINVAL, "direction must be +1 or -1");

@@ -1,6 +1,8 @@
 """GPU parity of the HIP LenseFlow path against the FP64 oracle and the
 reference's Julia fixtures.  Tolerances: FP64 1e-11 relative L2, FP32 1e-5
 relative L2 against the FP64 oracle (the workload's stated bars)."""
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -205,6 +207,32 @@ def test_empty_batch_is_a_no_op(pkg, dev):
     _lib.lenseflow(plan, [], x, out, 0, 2, 7, 1, False)
     torch.cuda.synchronize()
     assert torch.all(out == 3.0)
+
+
+def test_negative_batch_and_zero_steps_rejected(pkg, dev):
+    """The gradient, cotangent and VJP entry points refuse batch < 0 and
+    nsteps < 1 with an error, before any launch (a negative batch would
+    otherwise reach a launch as a huge grid)."""
+    from cmblensing_amd import _lib
+    _case("argument validation")
+    lib = _lib.load()
+    plan = _lib.get_plan(64, 2.0, torch.float32, 4, dev)
+    buf = torch.zeros(4 * 2 * 64 * 64 * 2, device=dev)
+    ptr = ctypes.c_void_p(buf.data_ptr())
+    phis = (ctypes.c_void_p * 1)(None)  # never read: the checks come first
+    ops = _lib.WfOps()
+    ops.mb = ops.bmcn = ops.cfinv = buf.data_ptr()
+    for batch, nsteps, msg in [(-1, 7, "batch < 0"), (1, 0, "nsteps must be >= 1")]:
+        for rc in [lib.cmbl_gradf_logpdf(plan.handle, phis, ptr, ptr, ctypes.byref(ops), ptr,
+                                         batch, 2, nsteps, None),
+                   lib.cmbl_lensed_cotangent(plan.handle, phis, ptr, ptr, ctypes.byref(ops), ptr,
+                                             ptr, batch, 2, nsteps, None),
+                   lib.cmbl_lenseflow_vjp(plan.handle, phis, ptr, ptr, ptr, ptr, batch, 2, nsteps,
+                                          1, None),
+                   lib.cmbl_lenseflow(plan.handle, phis, ptr, ptr, batch, 2, nsteps, 1, 0, None)]:
+            assert rc != 0
+            assert msg in lib.cmbl_last_error().decode()
+    torch.cuda.synchronize()
 
 
 @pytest.mark.parametrize("adjoint", [False, True])
