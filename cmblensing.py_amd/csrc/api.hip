@@ -930,6 +930,8 @@ extern "C" cmbl_status cmbl_cg_update_xrz(cmbl_plan* p, void* x, void* r, void* 
 extern "C" cmbl_status cmbl_cg_update_p(cmbl_plan* p, void* pv, const void* z, const double* beta,
                                         void* p_op, int nrhs, int ncomp, void* stream) {
   if (!p || !pv || !z || !beta) return fail(CMBL_EINVAL, "NULL argument");
+  if (nrhs < 1 || ncomp < 1 || ncomp > 3)
+    return fail(CMBL_EINVAL, "nrhs must be >= 1 and ncomp 1, 2 or 3");
   const long len = (long)ncomp * p->N * (p->N / 2 + 1);
   hipStream_t s = (hipStream_t)stream;
   if (p->dtype == CMBL_F32)

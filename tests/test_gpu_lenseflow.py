@@ -212,7 +212,8 @@ def test_empty_batch_is_a_no_op(pkg, dev):
 def test_negative_batch_and_zero_steps_rejected(pkg, dev):
     """The gradient, cotangent and VJP entry points refuse batch < 0 and
     nsteps < 1 with an error, before any launch (a negative batch would
-    otherwise reach a launch as a huge grid)."""
+    otherwise reach a launch as a huge grid); the CG p-update refuses
+    nrhs < 1 and ncomp outside 1..3."""
     from cmblensing_amd import _lib
     _case("argument validation")
     lib = _lib.load()
@@ -232,6 +233,9 @@ def test_negative_batch_and_zero_steps_rejected(pkg, dev):
                    lib.cmbl_lenseflow(plan.handle, phis, ptr, ptr, batch, 2, nsteps, 1, 0, None)]:
             assert rc != 0
             assert msg in lib.cmbl_last_error().decode()
+    for nrhs, ncomp in [(0, 2), (-1, 2), (1, 0), (1, 4)]:
+        assert lib.cmbl_cg_update_p(plan.handle, ptr, ptr, ptr, None, nrhs, ncomp, None) != 0
+        assert "nrhs must be >= 1" in lib.cmbl_last_error().decode()
     torch.cuda.synchronize()
 
 
