@@ -93,6 +93,45 @@ def test_config1_QU(pkg, dev, sims, direction, adjoint, dtype, tol):
     assert rel(got, ref) < tol
 
 
+def test_full_size_linearity_round_trip_and_adjoint(pkg, dev, sims):
+    """At the workload's full shape (256^2 QU FP32, a batch of 128 requests
+    over 32 phis), properties that hold at any size: L is linear in the field
+    (L(a x + b y) = a L x + b L y to FP32 rounding, the 1e-5 bar); the inverse
+    undoes the forward to the RK4 error of n = 7 (below 3e-3 for every
+    request, and request 0's residual field L^-1 L x - x equal to the FP64
+    oracle's to 1e-5 of |x|: the residual is the method's, not rounding); and
+    <L y, x> = <y, L^T x> per request to 1e-4."""
+    N, th, n, B = 256, 2.0, 7, 128
+    _case("full size 256^2 QU fp32 batch 128: linearity, round trip, dot test")
+    phi_np = np.stack([sims.phi_map(N, th, 100 + k) for k in range(32)])
+    P = torch.as_tensor(phi_np[np.arange(B) % 32], device=dev)
+    x = torch.as_tensor(np.stack([sims.qu_map(N, th, 200 + s) for s in range(B)]),
+                        dtype=torch.float32, device=dev)
+    yv = torch.as_tensor(np.stack([sims.qu_map(N, th, 400 + s) for s in range(B)]),
+                         dtype=torch.float32, device=dev)
+    L = lambda f, d=1, adj=False: pkg.lense_flow_matrices(f, P, th, n, d, adj)
+    a, b = 0.7, -1.3
+    lin = L(a * x + b * yv).double()
+    ref = a * L(x).double() + b * L(yv).double()
+    assert float(torch.linalg.vector_norm(lin - ref) / torch.linalg.vector_norm(ref)) < 1e-5
+    Lx = L(x)
+    back = L(Lx, -1)
+    res = (back.double() - x.double()).reshape(B, -1)
+    rt = torch.linalg.vector_norm(res, dim=1) / torch.linalg.vector_norm(x.double().reshape(B, -1),
+                                                                          dim=1)
+    assert float(rt.max()) < 3e-3
+    pw = olf.pix_width_from_theta(th)
+    x0 = x[0].double().cpu().numpy()
+    r0 = olf.lense_flow(olf.lense_flow(x0, phi_np[0], pw, n, 1, False), phi_np[0], pw, n, -1,
+                        False) - x0
+    assert np.linalg.norm(res[0].cpu().numpy().reshape(x0.shape) - r0) / np.linalg.norm(x0) < 1e-5
+    Ly = L(yv).double().reshape(B, -1)
+    LTx = L(x, 1, True).double().reshape(B, -1)
+    lhs = (Ly * x.double().reshape(B, -1)).sum(1)
+    rhs = (yv.double().reshape(B, -1) * LTx).sum(1)
+    assert float(((lhs - rhs).abs() / lhs.abs()).max()) < 1e-4
+
+
 def test_batched_mixed_phi_equals_single(pkg, dev, sims):
     """A batch with a different phi per element gives, element by element,
     exactly what the same element gives alone."""
