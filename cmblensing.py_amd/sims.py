@@ -199,9 +199,16 @@ def square_mask(N, frac_area=0.8, taper_px=8) -> np.ndarray:
     return np.outer(prof(d), prof(d))
 
 
-def point_source_mask(N, theta_pix_arcmin, nholes=200, radius_arcmin=5.0, seed=4) -> np.ndarray:
+HOLES_PER_SQ_ARCMIN = 200 / (512 * 1.5) ** 2   # 200 holes on a 512^2 x 1.5' map
+
+
+def point_source_mask(N, theta_pix_arcmin, nholes=None, radius_arcmin=5.0, seed=4) -> np.ndarray:
     """1 everywhere except `nholes` discs of `radius_arcmin` at uniform random
-    centres (periodic distances)."""
+    centres (periodic distances).  By default the hole count scales with the
+    map's area at the density of 200 holes on a 512^2 map of 1.5 arcmin pixels
+    (about 2.6 % of the area masked at any size), at least one hole."""
+    if nholes is None:
+        nholes = max(1, int(round(HOLES_PER_SQ_ARCMIN * (N * theta_pix_arcmin) ** 2)))
     rng = np.random.default_rng(seed)
     m = np.ones((N, N))
     r_px = radius_arcmin / theta_pix_arcmin

@@ -1,7 +1,7 @@
 """GPU parity of T+EB (FlatS02) fields with BlockTEB operators: LenseFlow,
 gradf_logpdf, logpdf, grad_phi_logpdf and the Wiener filter against the FP64
 oracle and CMBLensing.jl's IP fixtures.  Tolerances: FP64 1e-10 relative L2
-against the oracle (gradients) and 1e-12 relative (logpdf), FP32 2e-5; Julia
+against the oracle (gradients) and 1e-12 relative (logpdf), FP32 1e-5; Julia
 fixtures as pinned for the oracle (interior modes, see test_oracle)."""
 import numpy as np
 import pytest
@@ -66,7 +66,7 @@ def test_lense_flow_IP_julia(pkg, dev):
     assert rel(stack(y), ref) < 1e-11
 
 
-@pytest.mark.parametrize("cdt,tol", [(torch.complex128, 1e-10), (torch.complex64, 2e-5)])
+@pytest.mark.parametrize("cdt,tol", [(torch.complex128, 1e-10), (torch.complex64, 1e-5)])
 def test_gradf_IP(pkg, dev, cdt, tol):
     from cmblensing_amd import gradients
     _case(f"gradf julia IP {cdt}")
@@ -74,7 +74,9 @@ def test_gradf_IP(pkg, dev, cdt, tol):
     out = stack(gradients.gradf_logpdf(g["f"], g["phi"], g["d"], g["Cf"], g["Cn"], g["M"], g["B"],
                                        n=7))
     ref = owf.gradf_logpdf(a["f"], a["phi"], a["d"], a["Cf"], a["Cn"], a["M"], a["B"], 2.5, n=7)
-    assert rel(out, ref) < tol
+    err = rel(out, ref)
+    print(f"[err] gradf vs oracle {err:.2e}", flush=True)
+    assert err < tol
     if cdt == torch.complex128:
         gj = np.stack([L("gradf_t_IP"), L("gradf_e_IP"), L("gradf_b_IP")])
         assert rel(out[:, :, 1:-1], gj[:, :, 1:-1]) < 1e-10
@@ -131,4 +133,4 @@ def test_wiener_filter_IP_mixed_precision(pkg, dev):
                             g64["B"], n=7, tol=0.0, rtol=1e-10, maxiter=1500)
     err = rel(stack(x).astype(np.complex128), stack(x64))
     print("rel err vs FP64 solve", err, flush=True)
-    assert err < 1e-4
+    assert err < 1e-5

@@ -65,7 +65,7 @@ def stack(field):
 
 
 @pytest.mark.parametrize("mask", ["fourier", "square"])
-@pytest.mark.parametrize("cdt,tol", [(torch.complex128, 1e-10), (torch.complex64, 2e-5)])
+@pytest.mark.parametrize("cdt,tol", [(torch.complex128, 1e-10), (torch.complex64, 1e-5)])
 def test_gradf_matches_oracle(pkg, dev, sims, mask, cdt, tol):
     from cmblensing_amd import gradients
     N, th = 64, 2.0
@@ -75,7 +75,9 @@ def test_gradf_matches_oracle(pkg, dev, sims, mask, cdt, tol):
     g = stack(gradients.gradf_logpdf(f, phi, data, Cf, Cn, M, B, n=7))
     ref = owf.gradf_logpdf(prob["f"], prob["phi"], prob["d"], prob["Cf"], prob["Cn"], prob["M"],
                            prob["B"], th, n=7, mask_pix=prob["mask_pix"])
-    assert rel(g, ref) < tol
+    err = rel(g, ref)
+    print(f"[err] gradf vs oracle {err:.2e}", flush=True)
+    assert err < tol
 
 
 def test_gradf_julia_I_fp64(pkg, dev):
@@ -117,7 +119,7 @@ def _julia_P(pkg, dev, cdt=torch.complex128):
     return objs, arrs, L
 
 
-@pytest.mark.parametrize("cdt,tol", [(torch.complex128, 1e-10), (torch.complex64, 2e-5)])
+@pytest.mark.parametrize("cdt,tol", [(torch.complex128, 1e-10), (torch.complex64, 1e-5)])
 def test_gradf_julia_P(pkg, dev, cdt, tol):
     """EB gradient on CMBLensing.jl's polarisation problem: against the FP64
     oracle at the dtype's bar, and (FP64) against the Julia fixture on the
@@ -128,7 +130,9 @@ def test_gradf_julia_P(pkg, dev, cdt, tol):
     g = stack(gradients.gradf_logpdf(o["f"], o["phi"], o["d"], o["Cf"], o["Cn"], o["M"], o["B"],
                                      n=7))
     ref = owf.gradf_logpdf(a["f"], a["phi"], a["d"], a["Cf"], a["Cn"], a["M"], a["B"], 2.5, n=7)
-    assert rel(g, ref) < tol
+    err = rel(g, ref)
+    print(f"[err] gradf vs oracle {err:.2e}", flush=True)
+    assert err < tol
     if cdt == torch.complex128:
         gj = np.stack([L("gradf_e_P"), L("gradf_b_P")])
         assert rel(g[:, :, 1:-1], gj[:, :, 1:-1]) < 1e-10
@@ -213,7 +217,7 @@ def test_wiener_filter_mixed_precision_batch(pkg, dev, sims):
                                 n=7, tol=0.0, rtol=1e-9, maxiter=800, mask_pix=p["mask_pix"])
         err = rel(xs[i], ref)
         print("rhs", i, "rel err vs FP64 oracle WF", err)
-        assert err < 1e-4
+        assert err < 1e-5
 
 
 def test_wiener_filter_active_compaction_is_exact(pkg, dev, sims):
@@ -309,3 +313,62 @@ def test_wiener_filter_julia_start_matrix(pkg, dev, pol, start):
     print(f"{pol} {start}: iterations {info.iterations}, rel L2 {r_all:.3g}, interior {r_int:.3g}",
           flush=True)
     assert r_all < (1e-13 if start.endswith("p0") else 2e-5)
+
+
+def test_wiener_filter_qu512_8rhs_holes_vs_oracle(pkg, dev, sims):
+    """The 512^2 Q/U Wiener-filter workload at its own shape: 8 right-hand
+    sides (signal, noise and phi seeded 100..107), 1.5 arcmin pixels, 1
+    uK-arcmin noise with an l_knee = 100 1/f term, a 1 arcmin beam and 200
+    point-source holes of 5 arcmin (2.6 % of the map); FP32 operator, FP64 CG
+    vectors, dots, alpha/beta and residual refinement, to a relative residual
+    of 1e-6.  Every right-hand side converges with a true FP64 residual below
+    1e-6, and right-hand sides 100 and 105 are within 1e-5 (relative L2) of
+    the FP64 oracle's solution (tests/golden/wf512, written by
+    make_wf512_fixture.py from the oracle's own data; the data built here
+    lens the truth with the FP64 GPU path and are checked against the
+    fixture's checksums first)."""
+    from cmblensing_amd import matrix_operators as mo, wiener_filter as wfm
+    N, th, n, nrhs = 512, 1.5, 7, 8
+    _case(f"wf 512^2 QU 8 RHS holes 1/f, FP32 operator + FP64 CG, rtol 1e-6")
+    fix = np.load(GOLDEN / "wf512" / "wf512_qu_holes.npz")
+    probs = []
+    for s in range(100, 100 + nrhs):
+        pr = sims.wf_problem(N, th, pol=True, seed=s, mask="holes", noise_uk=1.0, ell_knee=100.0,
+                             beam_fwhm=1.0)
+        fq = owf.to_map(owf.eb2qu_field(pr["f"], N, th))
+        x = torch.as_tensor(fq[None], device=dev)
+        lensed = pkg.lense_flow_matrices(x, torch.as_tensor(pr["phi"], device=dev), th, n, 1,
+                                         False)[0].cpu().numpy()
+        probs.append(sims.wf_data(pr, lensed))
+    assert abs(1.0 - probs[0]["mask_pix"].mean() - 0.0261) < 1e-3
+    for s in (int(v) for v in fix["seeds"]):
+        d = probs[s - 100]["d"]
+        assert abs(float(np.sum(d ** 2)) / float(fix[f"d_sumsq_{s}"]) - 1.0) < 1e-9
+        pix = fix["probe_pixels"]
+        np.testing.assert_allclose([[d[c, i, j] for (i, j) in pix] for c in range(2)],
+                                   fix[f"d_probes_{s}"], rtol=1e-8, atol=1e-12)
+    T = lambda a, dt: torch.as_tensor(a, device=dev).to(dt)
+    op = lambda A: mo.DiagonalEB(matrix_EE=T(A[0], torch.float64),
+                                 matrix_BB=T(A[1], torch.float64), theta_pix=th)
+    data = pkg.FlatS2(polar_matrix_1=T(np.stack([p["d"][0] for p in probs]), torch.float32),
+                      polar_matrix_2=T(np.stack([p["d"][1] for p in probs]), torch.float32),
+                      theta_pix=th)
+    z = lambda: torch.zeros((nrhs, N, N // 2 + 1), dtype=torch.complex64, device=dev)
+    x0 = pkg.FlatS2(polar_matrix_1=z(), polar_matrix_2=z(), theta_pix=th)
+    phi = pkg.FlatS0(scalar_matrix=torch.as_tensor(np.stack([p["phi"] for p in probs]),
+                                                   device=dev), theta_pix=th)
+    p0 = probs[0]
+    M = mo.PixelMask(matrix=torch.as_tensor(p0["mask_pix"], device=dev))
+    x, info = wfm.wiener_filter(x0, phi, data, op(p0["Cf"]), op(p0["Cn"]), M, op(p0["B"]), n=n,
+                                tol=0.0, rtol=1e-6, recompute_every=10, maxiter=1000,
+                                return_info=True)
+    print("iterations", info.iterations, "true rel residual",
+          [f"{v:.2e}" for v in info.final_true_rel_residual], flush=True)
+    assert all(info.converged)
+    assert max(info.final_true_rel_residual) < 1e-6
+    xs = np.stack([x.polar_matrix_1.cpu().numpy(), x.polar_matrix_2.cpu().numpy()], axis=1)
+    for s in (int(v) for v in fix["seeds"]):
+        err = rel(xs[s - 100], fix[f"x_{s}"].astype(np.complex128))
+        print(f"rhs {s}: rel L2 vs FP64 oracle WF {err:.2e} (oracle: {int(fix[f'iterations_{s}'])} "
+              f"iterations, true residual {float(fix[f'true_rel_residual_{s}']):.1e})", flush=True)
+        assert err < 1e-5
