@@ -160,7 +160,60 @@ def zero_scalar_field_like(field: Field) -> FlatS0:
 FFT_BATCH_MAPS = 64
 
 
+def _needs_grad(*ts) -> bool:
+    return torch.is_grad_enabled() and any(t.requires_grad for t in ts)
+
+
+# --------------------------------------------------------------------------
+# autograd convention
+# --------------------------------------------------------------------------
+# Gradients of FOURIER matrices are Riesz representers under `dot` (the
+# reference's convention: its phi-prior VJP returns 2 Cphi^-1 phi,
+# statistics.py:196-208, and its LenseFlow VJP the Fourier-accumulated delta
+# phi, lense_flow.py:84-92): g with d logpdf = dot(g, delta) for every
+# perturbation delta.  Gradients of MAP matrices are the plain pixel
+# gradients (d logpdf = sum_x g delta).  Under this pair of inner products
+# the adjoint of rfft2 is irfft2 and the adjoint of irfft2 is rfft2, and every
+# per-mode or per-pixel operation (operators, EB<->QU rotations, masks) is
+# its own torch adjoint; so with fourier, map and dot defined below,
+# torch.autograd.grad(logpdf, (f, phi)) returns gradf_logpdf and
+# grad_phi_logpdf.
+
+class _RFFT2Fn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, theta_pix):
+        ctx.theta_pix = theta_pix
+        return _rfft2_raw(x, theta_pix)
+
+    @staticmethod
+    def backward(ctx, G):
+        return _irfft2_raw(G, ctx.theta_pix), None
+
+
+class _IRFFT2Fn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, X, theta_pix):
+        ctx.theta_pix = theta_pix
+        return _irfft2_raw(X, theta_pix)
+
+    @staticmethod
+    def backward(ctx, g):
+        return _rfft2_raw(g, ctx.theta_pix), None
+
+
 def _rfft2(x: torch.Tensor, theta_pix: float) -> torch.Tensor:
+    if _needs_grad(x):
+        return _RFFT2Fn.apply(x, theta_pix)
+    return _rfft2_raw(x, theta_pix)
+
+
+def _irfft2(X: torch.Tensor, theta_pix: float) -> torch.Tensor:
+    if _needs_grad(X):
+        return _IRFFT2Fn.apply(X, theta_pix)
+    return _irfft2_raw(X, theta_pix)
+
+
+def _rfft2_raw(x: torch.Tensor, theta_pix: float) -> torch.Tensor:
     N = x.shape[-1]
     if x.shape[-2] != N:
         raise ValueError("rfft2 expects square MAP matrices")
@@ -172,7 +225,7 @@ def _rfft2(x: torch.Tensor, theta_pix: float) -> torch.Tensor:
     return out
 
 
-def _irfft2(X: torch.Tensor, theta_pix: float) -> torch.Tensor:
+def _irfft2_raw(X: torch.Tensor, theta_pix: float) -> torch.Tensor:
     N = X.shape[-2]
     if X.shape[-1] != N // 2 + 1:
         raise ValueError("irfft2 expects FOURIER [.., N, N/2+1] matrices")
@@ -247,11 +300,34 @@ def qu2eb(field: Field) -> Field:
 # --------------------------------------------------------------------------
 # inner products (util.py:90-101; fields.py:136-154)
 # --------------------------------------------------------------------------
-def primal_dot(a: torch.Tensor, b: torch.Tensor, nside: int) -> torch.Tensor:
-    """Re sum conj(a) b w / N^2 over the last two axes, accumulated in float64."""
+def _primal_dot_raw(a: torch.Tensor, b: torch.Tensor, nside: int) -> torch.Tensor:
     w = fourier_weights_for(nside, a.device)
     prod = torch.conj(a.to(torch.complex128)) * b.to(torch.complex128)
     return torch.sum(prod.real * w, dim=(-2, -1)) / float(nside * nside)
+
+
+class _DotFn(torch.autograd.Function):
+    """dot with Riesz-representer gradients: d dot(a, b) = dot(b, da) + dot(a, db)."""
+
+    @staticmethod
+    def forward(ctx, a, b, nside):
+        ctx.save_for_backward(a, b)
+        return _primal_dot_raw(a, b, nside)
+
+    @staticmethod
+    def backward(ctx, g):
+        a, b = ctx.saved_tensors
+        g = g[..., None, None]
+        ga = (g * b).to(a.dtype).sum_to_size(a.shape) if ctx.needs_input_grad[0] else None
+        gb = (g * a).to(b.dtype).sum_to_size(b.shape) if ctx.needs_input_grad[1] else None
+        return ga, gb, None
+
+
+def primal_dot(a: torch.Tensor, b: torch.Tensor, nside: int) -> torch.Tensor:
+    """Re sum conj(a) b w / N^2 over the last two axes, accumulated in float64."""
+    if _needs_grad(a, b):
+        return _DotFn.apply(a, b, nside)
+    return _primal_dot_raw(a, b, nside)
 
 
 def dot(field_1: Field, field_2: Field) -> torch.Tensor:

@@ -42,7 +42,7 @@ def phi_handles_for(plan, phi_m: torch.Tensor, batch: int, n: int):
     if count != batch or len(lead) != 1:
         raise ValueError(f"phi batch {tuple(lead)} does not match field batch {batch}")
     # one batched preparation for the whole batch (the cache holds every handle
-    # of the call: an evicted handle is released)
+    # of the call; an entry the cache drops stays valid while a caller holds it)
     return _lib.get_phi_batch(plan, phi_m, n)
 
 
@@ -140,10 +140,15 @@ def _irfft2_t(X: torch.Tensor, theta_pix: float) -> torch.Tensor:
 
 
 class LenseFlowFunction(torch.autograd.Function):
-    """y = LenseFlow(x; phi) on MAP matrices x [B, C, N, N] with a MAP phi
-    ([N, N] shared or [B, N, N]); backward through the hand-written gradient
-    ODE (states recomputed, not stored).  Gradients are the plain ones:
-    dL/dx = L^T dL/dy, dL/dphi(pixel)."""
+    """y = LenseFlow(x; phi) on MAP matrices x [B, C, N, N] with phi MAP
+    ([N, N], [1, N, N] or [B, N, N]) or FOURIER (the same with N//2+1
+    complex columns); backward through the hand-written gradient ODE
+    (lense_flow.py:23-46: states recomputed from y, not stored).  Gradients
+    follow the reference's custom VJP: dL/dx = L^T dL/dy on the maps; for a
+    FOURIER phi the delta-phi rate accumulated in Fourier (lense_flow.py:84-92,
+    which keeps the anti-Hermitian Nyquist content; undo_inner_product leaves
+    FOURIER arrays alone, fields.py:146-154), for a MAP phi its irfft2, the
+    pixel gradient.  A phi shared by the batch gets the sum over the batch."""
 
     @staticmethod
     def forward(ctx, x, phi_m, theta_pix, n, direction):
@@ -159,21 +164,23 @@ class LenseFlowFunction(torch.autograd.Function):
         d_field, d_phi = lense_flow_vjp_matrices(y, gy, phi_m.detach(), theta_pix, n, direction)
         g_phi = None
         if ctx.needs_input_grad[1]:
-            g = _irfft2_t(d_phi, theta_pix).to(phi_m.dtype)        # [B, N, N]
-            g_phi = g.sum(0) if phi_m.dim() == 2 else g.reshape(phi_m.shape)
+            fourier_phi = phi_m.shape[-1] != phi_m.shape[-2]
+            g = d_phi if fourier_phi else _irfft2_t(d_phi, theta_pix)   # [B, N, *]
+            g = g.to(phi_m.dtype)
+            if phi_m.dim() == 2 or phi_m[..., 0, 0].numel() == 1:
+                g = g.sum(0)
+            g_phi = g.reshape(phi_m.shape)
         return d_field if ctx.needs_input_grad[0] else None, g_phi, None, None, None
 
 
 def lense_flow_wrapper(field: Field, phi, n: int = DEFAULT_NSTEPS, direction: int = FORWARD_LENSE,
                        adjoint: bool = False) -> Field:
     """Differentiable lense_flow (lense_flow.py:10-51): torch autograd flows to
-    the field matrices and to a MAP phi through the gradient ODE."""
+    the field matrices and to phi (MAP or FOURIER) through the gradient ODE."""
     if adjoint:
         raise NotImplementedError("the gradient ODE differentiates the non-adjoint LenseFlow "
                                   "(as lense_flow.py:23-46)")
     phi_m = _phi_matrix(phi)
-    if phi_m.shape[-1] != phi_m.shape[-2]:
-        raise ValueError("lense_flow_wrapper takes a MAP phi; use lense_flow_vjp for FOURIER phi")
     names = field._matrix_names()
     m0 = getattr(field, names[0])
     N = m0.shape[-1]
@@ -184,6 +191,8 @@ def lense_flow_wrapper(field: Field, phi, n: int = DEFAULT_NSTEPS, direction: in
         groups.append(["scalar_matrix"])
     if isinstance(field, (FlatS2, FlatS02)):
         groups.append(["polar_matrix_1", "polar_matrix_2"])
+    if not groups:
+        raise TypeError(f"Unsupported type: {type(field).__name__}")
     for g in groups:
         x = torch.stack([getattr(field, nm).reshape(-1, N, N) for nm in g], dim=1)
         y = LenseFlowFunction.apply(x, phi_m, field.theta_pix, n, direction)

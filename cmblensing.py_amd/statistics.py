@@ -4,6 +4,14 @@ gradients.py:29-37 of the reference).
 logpdf = -1/2 Re[ dot(r, Cn^-1 r) + dot(f, Cf^-1 f) + dot(phi, Cphi^-1 phi)
                   + logdet Cn + logdet Cphi + logdet Cf ],   r = d - M B L f
 
+logpdf is built like the reference's: the field is lensed by
+lense_flow_wrapper (the hand-written gradient-ODE backward behind a
+torch.autograd.Function, MAP or FOURIER phi) and the phi prior goes through
+phi_dot_wrapper (its custom VJP 2 Cphi^-1 phi, statistics.py:183-213), so
+torch.autograd.grad(logpdf, (f, phi)) gives d logpdf / d(f, phi) in the
+reference's convention (Riesz representers under dot for FOURIER matrices):
+the same values as gradf_logpdf and grad_phi_logpdf.
+
 grad_phi_logpdf follows the reference's definition for a FOURIER phi: the
 gradient-ODE delta phi accumulated in Fourier (the hand-written backward,
 ``cmbl_lenseflow_vjp``) minus Cphi^-1 phi (the custom VJP of the prior term,
@@ -17,7 +25,9 @@ import torch
 from . import _lib
 from .constants import Basis, DEFAULT_NSTEPS, Parametrization
 from .fields import (Field, FlatS0, FlatS02, dot, eb2qu, fourier, map as to_map, qu2eb)
-from .lense_flow import _phi_matrix, lense_flow_matrices, lense_flow_vjp_matrices, phi_handles_for
+from .fields import _needs_grad, _primal_dot_raw, _real_dtype
+from .lense_flow import (_phi_matrix, lense_flow_matrices, lense_flow_vjp_matrices,
+                         lense_flow_wrapper, phi_handles_for)
 from .matrix_operators import PixelMask, log_det, pinv
 
 
@@ -41,15 +51,43 @@ def _residual(lensed_eb: Field, data: Field, mask, beam) -> Field:
     return data - mask * (beam * lensed_eb)
 
 
+class _PhiDotFn(torch.autograd.Function):
+    """dot(phi, Cphi^-1 phi) with the custom VJP of statistics.py:183-213:
+    the gradient with respect to phi is 2 Cphi^-1 phi times the cotangent (the
+    covariance is not differentiated)."""
+
+    @staticmethod
+    def forward(ctx, phi_m, cinv, nside):
+        ctx.save_for_backward(phi_m, cinv)
+        return _primal_dot_raw(phi_m, cinv * phi_m, nside)
+
+    @staticmethod
+    def backward(ctx, g):
+        phi_m, cinv = ctx.saved_tensors
+        gphi = (2 * g[..., None, None] * (cinv * phi_m)).to(phi_m.dtype).sum_to_size(phi_m.shape)
+        return gphi, None, None
+
+
+def phi_dot_wrapper(phi, phi_covariance) -> torch.Tensor:
+    """dot(phi, pinv(Cphi) * phi) (statistics.py:183-186) with its custom VJP."""
+    if not _needs_grad(phi.scalar_matrix):
+        return dot(phi, pinv(phi_covariance) * phi)
+    pm = phi.scalar_matrix
+    cinv = pinv(phi_covariance).scalar_matrix.to(device=pm.device, dtype=_real_dtype(pm))
+    return _PhiDotFn.apply(pm, cinv, phi.nside)
+
+
 def logpdf(field, phi, data, noise_covariance, phi_covariance, field_covariance, mask, beam,
            n: int = DEFAULT_NSTEPS):
-    """statistics.py:138-165 (same argument order as the reference)."""
+    """statistics.py:138-165 (same argument order as the reference).  phi may be
+    MAP or FOURIER (the reference keeps it FOURIER, statistics.py:152-156);
+    differentiable in field and phi with torch.autograd."""
+    if not isinstance(phi, Field):
+        phi = FlatS0(scalar_matrix=phi, theta_pix=field.theta_pix)
     logdets = log_det(phi_covariance) + log_det(noise_covariance) + log_det(field_covariance)
     field_product = dot(field, pinv(field_covariance) * field)
-    phi_product = dot(phi, pinv(phi_covariance) * phi)
-    fq, x, y, names, lead = _lensed_maps(field, phi, n)
-    lensed = qu2eb(fourier(fq.replace(**{nm: y[:, c].reshape(*lead, *y.shape[-2:])
-                                         for c, nm in enumerate(names)})))
+    phi_product = phi_dot_wrapper(phi, phi_covariance)
+    lensed = qu2eb(fourier(lense_flow_wrapper(to_map(eb2qu(field)), phi, n)))
     r = _residual(lensed, data, mask, beam)
     data_product = dot(r, pinv(noise_covariance) * r)
     return -(data_product + field_product + phi_product + logdets) / 2

@@ -22,7 +22,7 @@ def _case(msg):
 
 @pytest.mark.parametrize("C", [1, 2])
 @pytest.mark.parametrize("basis", ["map", "fourier"])
-@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-10), (torch.float32, 1e-4)])
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-10), (torch.float32, 1e-5)])
 def test_vjp_matches_oracle(pkg, dev, sims, C, basis, dtype, tol):
     from cmblensing_amd.lense_flow import lense_flow_vjp_matrices
     N, th, n = 64, 2.0, 7
@@ -37,11 +37,14 @@ def test_vjp_matches_oracle(pkg, dev, sims, C, basis, dtype, tol):
     T = lambda a: torch.as_tensor(a, device=dev)
     d_field, d_phi = lense_flow_vjp_matrices(T(y[None]).to(dtype), T(cot[None]).to(dtype), T(phin),
                                              th, n, 1)
-    assert rel(d_field[0].double().cpu().numpy(), np.stack(dfs)) < tol
+    e_field = rel(d_field[0].double().cpu().numpy(), np.stack(dfs))
     got = d_phi[0].cpu().numpy()
     if basis == "map":
         got = np.fft.irfft2(got)
-    assert rel(got, dphi) < tol
+    e_phi = rel(got, dphi)
+    print(f"[err] d_field {e_field:.2e} d_phi {e_phi:.2e}", flush=True)
+    assert e_field < tol
+    assert e_phi < tol
 
 
 def test_autograd_matches_finite_differences(pkg, dev, sims):
@@ -164,18 +167,151 @@ def test_differentiation_workload_batch16(pkg, dev, sims):
 
     def lp(ff, pp):
         return statistics.logpdf(ff, pp, data6, Cn6, Cp6, Cf6, M6, B6, n=n)
-    for _ in range(2):
+
+    # 16 seeded directions, each scaled to the norm (under dot) of chain 0's
+    # field or phi, central differences at eps = 1e-3 of that scale.  The f
+    # directions are white; the phi directions are smooth phi-like fields
+    # (white noise in phi is dominated by the nonlinearity of M(t)^-1 in
+    # Hess phi at the pixel scale).
+    f0 = f6.replace(polar_matrix_1=f6.polar_matrix_1[0], polar_matrix_2=f6.polar_matrix_2[0])
+    fnorm = float(pkg.dot(f0, f0)) ** 0.5
+    pnorm = float(pkg.dot(phi6, phi6)) ** 0.5
+    e = 1e-3
+    gp_norm = torch.sqrt(pkg.dot(gp, gp))                      # per chain
+    rel_f, rel_p, cs_p = [], [], []
+    for k in range(16):
         v = np.fft.rfft2(rng.standard_normal((2, N, N)))
         vt = [torch.as_tensor(v[c], device=dev) for c in range(2)]
-        dirf = f6.replace(polar_matrix_1=vt[0].expand(B, -1, -1), polar_matrix_2=vt[1].expand(B, -1, -1))
-        e = 0.1   # logpdf is quadratic in f
+        dirf = f6.replace(polar_matrix_1=vt[0], polar_matrix_2=vt[1])
+        dirf = dirf * (fnorm / float(pkg.dot(dirf, dirf)) ** 0.5)
+        dirf = dirf.replace(polar_matrix_1=dirf.polar_matrix_1.expand(B, -1, -1),
+                            polar_matrix_2=dirf.polar_matrix_2.expand(B, -1, -1))
         fd = (lp(f6 + dirf * e, phi6) - lp(f6 - dirf * e, phi6)) / (2 * e)
         an = pkg.dot(gf, dirf)
-        assert float(torch.max(torch.abs(an - fd) / torch.abs(fd))) < 1e-4
-        w = np.fft.rfft2(sims.phi_map(N, th, int(rng.integers(1000))))
+        rel_f.append(torch.abs(an - fd) / torch.abs(fd))
+        w = np.fft.rfft2(sims.phi_map(N, th, 500 + k))
         wt = phi6.replace(scalar_matrix=torch.as_tensor(w, device=dev))
-        e = 1e-3
+        wt = wt * (pnorm / float(pkg.dot(wt, wt)) ** 0.5)
         fdp = (lp(f6, phi6 + wt * e) - lp(f6, phi6 - wt * e)) / (2 * e)
         anp = pkg.dot(gp, wt.replace(scalar_matrix=wt.scalar_matrix.expand(B, -1, -1)))
-        print("phi: analytic", anp[:3].tolist(), "fd", fdp[:3].tolist(), flush=True)
-        assert float(torch.max(torch.abs(anp - fdp) / torch.abs(fdp))) < 5e-3
+        rel_p.append(torch.abs(anp - fdp) / torch.abs(fdp))
+        # the same gap relative to |g| |w| (Cauchy-Schwarz scale): directions
+        # nearly orthogonal to the gradient make the plain ratio large
+        cs_p.append(torch.abs(anp - fdp) / (gp_norm * pnorm))
+    rel_f, rel_p, cs_p = (torch.stack(a).flatten() for a in (rel_f, rel_p, cs_p))
+    print(f"[err] 16 directions x {B} chains: f max {float(rel_f.max()):.2e}; phi max "
+          f"{float(rel_p.max()):.2e}, median {float(rel_p.median()):.2e}, max relative to "
+          f"|g||w| {float(cs_p.max()):.2e}", flush=True)
+    assert float(rel_f.max()) < 1e-5
+    # the phi gradient is the discrete RK4 adjoint of the gradient ODE (as the
+    # reference's): its gap to the derivative of the discrete logpdf is the
+    # continuous-adjoint gap at n = 7, ~1e-3 of the directional derivative
+    assert float(rel_p.median()) < 5e-3
+    assert float(cs_p.max()) < 1e-5
+
+
+def _autograd_vs_native(pkg, f, phi, d, Cn, Cphi, Cf, M, B, n=7):
+    """torch.autograd.grad of logpdf with respect to every matrix of f and to
+    phi, and the native gradf_logpdf / grad_phi_logpdf of the same problem."""
+    from cmblensing_amd import gradients, statistics
+    names = f._matrix_names()
+    fm = {nm: getattr(f, nm).detach().clone().requires_grad_(True) for nm in names}
+    pm = phi.scalar_matrix.detach().clone().requires_grad_(True)
+    fa, pa = f.replace(**fm), phi.replace(scalar_matrix=pm)
+    lp = statistics.logpdf(fa, pa, d, Cn, Cphi, Cf, M, B, n=n)
+    grads = torch.autograd.grad(lp.sum(), [fm[nm] for nm in names] + [pm])
+    gf_auto = torch.stack(grads[:-1]).cpu().numpy()
+    gp_auto = grads[-1].cpu().numpy()
+    gf = gradients.gradf_logpdf(f, phi, d, Cf, Cn, M, B, n=n)
+    gf_nat = np.stack([getattr(gf, nm).cpu().numpy() for nm in names])
+    gp_nat = statistics.grad_phi_logpdf(f, phi, d, Cn, Cphi, Cf, M, B, n=n).scalar_matrix
+    return gf_auto, gp_auto, gf_nat, gp_nat.cpu().numpy(), float(lp.detach().sum())
+
+
+def test_logpdf_autograd_julia_I(pkg, dev):
+    """torch.autograd.grad(logpdf, (f, phi)) with a FOURIER phi on the
+    reference's I problem equals gradf_logpdf and grad_phi_logpdf, and meets
+    the CMBLensing.jl grad_phi_I fixture like grad_phi_logpdf does."""
+    _case("logpdf autograd julia I fp64")
+    L, S, D = _julia_I(pkg, dev)
+    gfa, gpa, gfn, gpn, lp = _autograd_vs_native(
+        pkg, S(L("t_field_I")), S(L("phi_I")), S(L("data_t_field_I")), D(L("cn_I")),
+        D(L("cphi_I")), D(L("cf_I")), D(L("m_I")), D(L("b_I")))
+    ef, ep, ej = rel(gfa, gfn), rel(gpa, gpn), rel(gpa, L("grad_phi_I"))
+    print(f"[err] autograd vs gradf {ef:.2e}, vs grad_phi {ep:.2e}, vs julia {ej:.2e}", flush=True)
+    assert abs(lp - float((GOLDEN / "julia" / "logpdf_I.txt").read_text())) < 1e-9 * abs(lp)
+    assert ef < 1e-12
+    assert ep < 1e-12
+    assert ej < 2e-3
+
+
+def test_logpdf_autograd_julia_P(pkg, dev):
+    """The same on the reference's polarisation problem (EB field, FOURIER
+    phi), against the grad_phi_t_P fixture."""
+    from test_gpu_wiener import _julia_P
+    _case("logpdf autograd julia P fp64")
+    o, a, L = _julia_P(pkg, dev)
+    gfa, gpa, gfn, gpn, _ = _autograd_vs_native(pkg, o["f"], o["phi"], o["d"], o["Cn"], o["Cphi"],
+                                                o["Cf"], o["M"], o["B"])
+    ef, ep, ej = rel(gfa, gfn), rel(gpa, gpn), rel(gpa, L("grad_phi_t_P"))
+    print(f"[err] autograd vs gradf {ef:.2e}, vs grad_phi {ep:.2e}, vs julia {ej:.2e}", flush=True)
+    assert ef < 1e-12
+    assert ep < 1e-12
+    assert ej < 2e-4
+
+
+def test_logpdf_autograd_pixel_mask_batch(pkg, dev, sims):
+    """Autograd through logpdf with a pixel mask (the residual formed on the
+    Q/U maps) for a batch of 2 chains with a FOURIER phi each, against the
+    native gradients and the FP64 oracle's grad_phi_logpdf."""
+    from cmblensing_amd import matrix_operators as mo
+    from test_gpu_wiener import make_problem
+    N, th = 64, 2.0
+    _case(f"logpdf autograd N={N} QU square mask, 2 chains fp64")
+    probs = [make_problem(sims, N, th, "square", seed=40 + k) for k in range(2)]
+    phi_m, Cphi = sims.phi_map_and_cov(N, th, 41)
+    T = lambda a, dt=torch.complex128: torch.as_tensor(a, device=dev).to(dt)
+    op = lambda A: mo.DiagonalEB(matrix_EE=T(A[0], torch.float64), matrix_BB=T(A[1], torch.float64),
+                                 theta_pix=th)
+    f = pkg.FlatS2(polar_matrix_1=T(np.stack([p["f"][0] for p in probs])),
+                   polar_matrix_2=T(np.stack([p["f"][1] for p in probs])), theta_pix=th)
+    d = pkg.FlatS2(polar_matrix_1=T(np.stack([p["d"][0] for p in probs]), torch.float64),
+                   polar_matrix_2=T(np.stack([p["d"][1] for p in probs]), torch.float64),
+                   theta_pix=th)
+    phis = np.stack([np.fft.rfft2(phi_m), np.fft.rfft2(probs[1]["phi"])])
+    phi = pkg.FlatS0(scalar_matrix=T(phis), theta_pix=th)
+    Cp = mo.DiagonalScalar(scalar_matrix=T(Cphi, torch.float64), theta_pix=th)
+    M = mo.PixelMask(matrix=torch.as_tensor(probs[0]["mask_pix"], device=dev))
+    p0 = probs[0]
+    gfa, gpa, gfn, gpn, _ = _autograd_vs_native(pkg, f, phi, d, op(p0["Cn"]), Cp, op(p0["Cf"]), M,
+                                                op(p0["B"]))
+    ef, ep = rel(gfa, gfn), rel(gpa, gpn)
+    eo = max(rel(gpa[k], owf.grad_phi_logpdf(probs[k]["f"], phis[k], probs[k]["d"], p0["Cn"], Cphi,
+                                             p0["Cf"], None, p0["B"], th, n=7,
+                                             mask_pix=p0["mask_pix"])) for k in range(2))
+    print(f"[err] autograd vs gradf {ef:.2e}, vs grad_phi {ep:.2e}, vs oracle {eo:.2e}", flush=True)
+    assert ef < 1e-12
+    assert ep < 1e-12
+    assert eo < 1e-9
+
+
+def test_lense_flow_wrapper_shared_phi_batch(pkg, dev, sims):
+    """A phi with a leading axis of 1 shared by a field batch of 2: the phi
+    gradient is the sum of the two elements' gradients (MAP and FOURIER phi)."""
+    from cmblensing_amd.lense_flow import LenseFlowFunction
+    N, th, n = 64, 2.0, 7
+    _case("wrapper shared phi [1,N,N] batch 2")
+    T = lambda a: torch.as_tensor(a, device=dev)
+    x = T(np.stack([sims.qu_map(N, th, 60 + k) for k in range(2)]))
+    c = T(np.random.default_rng(61).standard_normal((2, 2, N, N)))
+    for phi0 in (sims.phi_map(N, th, 62), np.fft.rfft2(sims.phi_map(N, th, 62))):
+        phi = T(phi0[None]).requires_grad_(True)
+        (g,) = torch.autograd.grad((LenseFlowFunction.apply(x, phi, th, n, 1) * c).sum(), (phi,))
+        assert g.shape == phi.shape
+        parts = []
+        for k in range(2):
+            pk = T(phi0).requires_grad_(True)
+            (gk,) = torch.autograd.grad((LenseFlowFunction.apply(x[k:k + 1], pk, th, n, 1)
+                                         * c[k:k + 1]).sum(), (pk,))
+            parts.append(gk)
+        assert rel(g[0].cpu().numpy(), (parts[0] + parts[1]).cpu().numpy()) < 1e-13
