@@ -46,6 +46,19 @@ def model_bytes_per_map(N, C, n, s):
     return 4 * n * (C * (12 * H + 5 * P) + 5 * P) * s
 
 
+def design_bytes_per_map(N, C, n, s):
+    """Lower bound of the bytes one LenseFlow of one request moves in this
+    two-pass design (every array read or written once per pass):
+    4n [C (6.75 Q) + 2 Q] with Q = N^2 s.  Per component and stage: the
+    column pass reads the stage spectra S and writes DX (2Q); the row pass
+    reads S and DX and writes the next S (3Q) and moves 1.75Q of RK4 state on
+    average (y read at every stage and written at stage 4, the accumulator
+    written at stage 2 and read at stage 4); per request and stage the two
+    p(t) maps (2Q), shared by the components."""
+    Q = N * N * s
+    return 4 * n * (C * 6.75 * Q + 2 * Q)
+
+
 def row_bytes_per_launch(B, N, C, s):
     """Algorithmic bytes of one fused row-pass launch over B requests,
     averaged over the four RK4 stages: per request 2C packed half planes read
@@ -221,6 +234,18 @@ def serving_run(torch, sims, dev, n_requests, rate, seed=2, nphi=32, budget_gib=
           f"(graphs captured in {warm_s:.1f} s)", file=sys.stderr, flush=True)
     st = server.serve(reqs, list(offs))
     torch.cuda.synchronize()
+    # one served request of every (N, T/QU) against the same request lensed
+    # alone by the plain path: must be bit-identical
+    alone = {}
+    if pkg is not None:
+        for N in sizes:
+            for C in (1, 2):
+                r = next((q for q in reqs if isinstance(q, srv.Request) and q.N == N and q.C == C
+                          and q.state == "done"), None)
+                if r is not None:
+                    a = pkg.lense_flow_matrices(r.x[None], phi_maps[(r.phi_id, N)], th, n,
+                                                r.direction, False)[0]
+                    alone[f"N{N}_{'QU' if C == 2 else 'T'}"] = bool(torch.equal(a, r.out))
     per_n = {}
     for N in sizes:
         lat = np.array([r.latency for r in reqs
@@ -237,7 +262,8 @@ def serving_run(torch, sims, dev, n_requests, rate, seed=2, nphi=32, budget_gib=
            "phi_hit_rate": round(st["phi_hit_rate"], 4), "phi_evictions": st["phi_evictions"],
            "hbm_budget_gib": round(st["hbm_budget_bytes"] / srv.GiB, 2),
            "hbm_peak_gib": round(st["hbm_peak_bytes"] / srv.GiB, 3),
-           "graph_warmup_s": round(warm_s, 2), "per_nside": per_n}
+           "graph_warmup_s": round(warm_s, 2), "per_nside": per_n,
+           "served_equals_alone": alone}
     if wf_every:
         out.update({k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()
                      if k.startswith("wf_")})
@@ -728,12 +754,14 @@ def main():
 
     # HBM traffic per launch of the row pass, from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated on a copy)
-    traffic = None
+    traffic = trace_us = None
     tfile = ROOT / "profiles" / "traffic.json"
     if tfile.exists() and B == 128:
-        traffic = json.loads(tfile.read_text()).get("k_row_lf_std<float,256>", {}).get(
-            "bytes_per_launch")
+        rowprof = json.loads(tfile.read_text()).get("k_row_lf_std<float,256>", {})
+        traffic = rowprof.get("bytes_per_launch")
+        trace_us = rowprof.get("trace_avg_launch_us")
     model_b = model_bytes_per_map(N, C, n, 4)
+    design_b = design_bytes_per_map(N, C, n, 4)
     result = {
         "metric": "LenseFlow maps/s @256^2 QU batched vs plain; p50/p99 latency; err vs FP64",
         "value": round(maps_s, 2),
@@ -757,13 +785,24 @@ def main():
                              "p99": round(float(np.percentile(lat, 99)), 4)},
         "plain_resident_phi_latency_ms": {"p50": round(float(np.percentile(lat_res, 50)), 4),
                                           "p99": round(float(np.percentile(lat_res, 99)), 4)},
-        "model_gbs": round(model_b * maps_s / 1e9, 1),
-        "model_frac_hbm": round(model_b * maps_s / 1e9 / HBM_PEAK_GBS, 4),
+        "byte_models": {
+            "design_bytes_per_map": design_b,
+            "design_gbs": round(design_b * maps_s / 1e9, 1),
+            "design_frac_hbm": round(design_b * maps_s / 1e9 / HBM_PEAK_GBS, 4),
+            "canonical_bytes_per_map": model_b,
+            "canonical_rate_gbs": round(model_b * maps_s / 1e9, 1),
+            "note": "design: 4n [C 6.75 + 2] N^2 s, the bytes this two-pass design moves per "
+                    "map (its lower bound); canonical: 4n [C (12H + 5P) + 5P] s, which charges "
+                    "transfers this design does not make, so its rate is no fraction of peak"},
         "roofline": {"bound": "hbm", "kernel": "k_row_lf_std<float,256>",
                      "achieved": round(row_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(row_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "avg_launch_us": round(row_avg_ms * 1e3, 3),
-                     "algorithmic_bytes_per_launch": row_bytes},
+                     "algorithmic_bytes_per_launch": row_bytes,
+                     "frac_events": round(row_gbs / HBM_PEAK_GBS, 4),
+                     "frac_trace": (round(row_bytes / (trace_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
+                                    if trace_us else None),
+                     "trace_avg_launch_us": trace_us},
         "column_pass": {"kernel": "k_col_lf_dx<float,256,16>",
                         "achieved_gbs": round(col_gbs, 1),
                         "avg_launch_us": round(col_avg_ms * 1e3, 3),
@@ -774,7 +813,8 @@ def main():
         ex.close()
         pk.close()
     if not args.no_serve:
-        result["serving"] = serving_run(torch, sims, dev, args.serve_requests, args.serve_rate)
+        result["serving"] = serving_run(torch, sims, dev, args.serve_requests, args.serve_rate,
+                                        pkg=pkg)
         result["latency_ms"] = {"p50": result["serving"]["p50_ms"],
                                 "p99": result["serving"]["p99_ms"]}
         if args.serve_wf_every:

@@ -165,11 +165,21 @@ class PhiHandle:
     the first time it is asked for."""
 
     @classmethod
-    def _wrap(cls, plan: Plan, handle: c_void_p, nsteps: int, stream: int) -> "PhiHandle":
+    def _wrap(cls, plan: Plan, handle: c_void_p, nsteps: int, stream: int,
+              ready: "torch.cuda.Event") -> "PhiHandle":
         self = cls.__new__(cls)
         self.lib, self.plan, self.nsteps = plan.lib, plan, nsteps
         self.handle, self._min_det, self.streams = handle, None, {stream}
+        self.prep_stream, self.ready = stream, ready
         return self
+
+    def order_on_current_stream(self):
+        """Make the current stream wait for this handle's preparation when it
+        was prepared on another stream (a cached handle used from a second
+        stream would otherwise read tables still being written)."""
+        cur = torch.cuda.current_stream(self.plan.device)
+        if cur.cuda_stream != self.prep_stream:
+            cur.wait_event(self.ready)
 
     @property
     def min_det(self) -> float:
@@ -207,6 +217,8 @@ class PhiHandle:
         self.handle = h
         self._min_det = md.value if with_min_det else None
         self.streams = {st}
+        self.prep_stream = st
+        self.ready = _record_event(plan.device)
 
     def derivatives(self) -> torch.Tensor:
         """The five float64 phi derivative maps as a [5, N, N] tensor (a copy)."""
@@ -301,22 +313,31 @@ PHI_CACHE_SIZE = 64
 
 
 def get_plan(N: int, theta_pix: float, dtype: torch.dtype, nmaps: int, device) -> Plan:
+    """The cached plan of (device, geometry, dtype) for the current stream.
+
+    A plan's scratch (stage spectra, accumulator, phi-preparation and
+    Wiener-filter temporaries) is used by every call on it, so plans are kept
+    per stream: calls issued on different streams never share scratch.  When a
+    larger plan is needed the old one leaves the cache but is not destroyed:
+    whoever still holds it (a Wiener-filter process between its steps, say)
+    keeps a working plan, which is freed with its last reference."""
     device = torch.device(device)
     if device.index is None:
         device = torch.device("cuda", torch.cuda.current_device())
-    key = (device.index, N, round(float(theta_pix), 12), dtype)
+    key = (device.index, N, round(float(theta_pix), 12), dtype,
+           torch.cuda.current_stream(device).cuda_stream)
     plan = _plans.get(key)
     if plan is None or plan.max_maps < nmaps:
         cap = max(nmaps, 2 * plan.max_maps if plan else nmaps)
-        if plan is not None:
-            plan.close()
-        plan = Plan(N, theta_pix, dtype, cap, device)
+        old, plan = plan, Plan(N, theta_pix, dtype, cap, device)
         _plans[key] = plan
-        # phi handles prepared under the old plan object are still valid: the
-        # plan geometry and dtype are unchanged; re-key them to the new plan.
-        for k, v in list(_phis.items()):
-            if k[0] == key:
-                v[2].plan = plan
+        # cached phi handles need no plan of their own (their memory is the
+        # library's): point them at the new plan so they do not keep the old
+        # plan's scratch alive
+        if old is not None:
+            for v in _phis.values():
+                if v[2].plan is old:
+                    v[2].plan = plan
     return plan
 
 
@@ -334,7 +355,14 @@ def prepare_batch(plan: Plan, phis: torch.Tensor, nsteps: int):
     with torch.cuda.device(plan.device):
         check(plan.lib.cmbl_phi_prepare_batch(plan.handle, phis.data_ptr(), basis, nsteps, count,
                                               hs, None, st), "cmbl_phi_prepare_batch")
-    return [PhiHandle._wrap(plan, c_void_p(hs[i]), nsteps, st) for i in range(count)]
+    ready = _record_event(plan.device)
+    return [PhiHandle._wrap(plan, c_void_p(hs[i]), nsteps, st, ready) for i in range(count)]
+
+
+def _record_event(device) -> "torch.cuda.Event":
+    ev = torch.cuda.Event()
+    ev.record(torch.cuda.current_stream(device))
+    return ev
 
 
 def get_phi_batch(plan: Plan, phi: torch.Tensor, nsteps: int):
@@ -352,19 +380,16 @@ def get_phi_batch(plan: Plan, phi: torch.Tensor, nsteps: int):
             break
         hit.append(ent[2])
     if len(hit) == B:
-        for k in keys:
+        for k, h in zip(keys, hit):
             _phis.move_to_end(k)
+            h.order_on_current_stream()
         return hit
     handles = prepare_batch(plan, phi, nsteps)
     ref = weakref.ref(phi)
     for k, h in zip(keys, handles):
-        old = _phis.pop(k, None)
-        if old is not None:
-            old[2].close()
+        _phis.pop(k, None)
         _phis[k] = (ref, phi._version, h)
-    while len(_phis) > PHI_CACHE_SIZE:
-        _, (_, _, old) = _phis.popitem(last=False)
-        old.close()
+    _trim_phi_cache()
     return handles
 
 
@@ -388,15 +413,22 @@ def get_phi(plan: Plan, phi: torch.Tensor, nsteps: int, index=None) -> PhiHandle
         ref, version, h = ent
         if ref() is phi and version == phi._version:
             _phis.move_to_end(key)
+            h.order_on_current_stream()
             return h
         del _phis[key]
-        h.close()
     h = PhiHandle(plan, phi if index is None else phi[index], nsteps)
     _phis[key] = (weakref.ref(phi), phi._version, h)
-    while len(_phis) > PHI_CACHE_SIZE:
-        _, (_, _, old) = _phis.popitem(last=False)
-        old.close()
+    _trim_phi_cache()
     return h
+
+
+def _trim_phi_cache():
+    """Drop the least recently used entries beyond PHI_CACHE_SIZE.  A dropped
+    handle is not released here: a caller still holding it (the phi handles
+    of a Wiener-filter process, say) keeps using it, and the handle is
+    released with its last reference (PhiHandle.__del__)."""
+    while len(_phis) > PHI_CACHE_SIZE:
+        _phis.popitem(last=False)
 
 
 def clear_caches():
