@@ -40,6 +40,8 @@ _SIGS = {
     "cmbl_phi_derivatives": (c_void_p, [c_void_p]),
     "cmbl_lenseflow": (c_int, [c_void_p, ctypes.POINTER(c_void_p), c_void_p, c_void_p, c_int,
                                c_int, c_int, c_int, c_int, c_void_p]),
+    "cmbl_lenseflow_noalloc": (c_int, [c_void_p, ctypes.POINTER(c_void_p), c_void_p, c_void_p,
+                                       c_int, c_int, c_int, c_int, c_int, c_void_p]),
     "cmbl_rfft2": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "cmbl_irfft2": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "cmbl_grf_fourier": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
@@ -449,6 +451,23 @@ def lenseflow(plan: Plan, phis, x: torch.Tensor, out: torch.Tensor, batch: int, 
         check(plan.lib.cmbl_lenseflow(plan.handle, arr, x.data_ptr(), out.data_ptr(), batch, ncomp,
                                       nsteps, int(direction), int(bool(adjoint)), st),
               "cmbl_lenseflow")
+
+
+NOALLOC_SCRATCH_BYTES = 64 << 20
+
+
+def lenseflow_noalloc(plan: Plan, phis, x: torch.Tensor, out: torch.Tensor, batch: int,
+                      ncomp: int, nsteps: int, direction: int, adjoint: bool):
+    """cmbl_lenseflow_noalloc: cmbl_lenseflow with its stage buffers in the
+    library's static per-GPU scratch (NOALLOC_SCRATCH_BYTES); the call itself
+    allocates nothing.  Calls overlapping on one GPU race on that scratch."""
+    arr = (c_void_p * max(batch, 1))(*[p.handle for p in phis])
+    st = stream_ptr(plan.device)
+    note_streams(phis, st)
+    with torch.cuda.device(plan.device):
+        check(plan.lib.cmbl_lenseflow_noalloc(plan.handle, arr, x.data_ptr(), out.data_ptr(),
+                                              batch, ncomp, nsteps, int(direction),
+                                              int(bool(adjoint)), st), "cmbl_lenseflow_noalloc")
 
 
 def rfft2(plan: Plan, x: torch.Tensor, X: torch.Tensor, nmaps: int):

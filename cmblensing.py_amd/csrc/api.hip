@@ -42,6 +42,10 @@ cmbl_status fail(cmbl_status code, const std::string& msg) {
 
 constexpr int kRing = 8;
 
+// Device address of the allocation-free entry point's static scratch on the
+// current device (defined with cmbl_lenseflow_noalloc).
+cmbl_status scratch_address(void** out);
+
 bool valid_n(int N) {
   return N >= 16 && N <= 512 && (N & (N - 1)) == 0;
 }
@@ -418,6 +422,10 @@ cmbl_status cmbl_plan_create(cmbl_plan** out, int N, double theta_pix_arcmin, in
   for (int i = 0; i < kRing; ++i)
     if ((e = hipEventCreateWithFlags(&p->ring_ev[i], hipEventDisableTiming)) != hipSuccess)
       return cleanup(fail(CMBL_EHIP, std::string("event: ") + hipGetErrorString(e)));
+  // set-up of the allocation-free entry point on this device
+  void* sp = nullptr;
+  const cmbl_status sst = scratch_address(&sp);
+  if (sst != CMBL_OK) return cleanup(sst);
   *out = p;
   return CMBL_OK;
 }
@@ -554,9 +562,10 @@ namespace {
 struct Timer {
   cmbl_plan* p;
   hipStream_t s;
+  bool on = true;  // false: never time (no events are created or recorded)
   size_t used = 0;
   cmbl_status begin(int kind) {
-    if (!p->timing) return CMBL_OK;
+    if (!on || !p->timing) return CMBL_OK;
     while (p->tev.size() < 2 * (used + 1)) {
       hipEvent_t e;
       HIP_TRY(hipEventCreate(&e));
@@ -568,13 +577,13 @@ struct Timer {
     return CMBL_OK;
   }
   cmbl_status end() {
-    if (!p->timing) return CMBL_OK;
+    if (!on || !p->timing) return CMBL_OK;
     HIP_TRY(hipEventRecord(p->tev[2 * used + 1], s));
     ++used;
     return CMBL_OK;
   }
   cmbl_status flush() {
-    if (!p->timing || used == 0) return CMBL_OK;
+    if (!on || !p->timing || used == 0) return CMBL_OK;
     HIP_TRY(hipEventSynchronize(p->tev[2 * used - 1]));
     for (size_t i = 0; i < used; ++i) {
       float ms = 0;
@@ -596,19 +605,29 @@ struct Timer {
     if (ts_ != CMBL_OK) return ts_;       \
   } while (0)
 
+// Stage buffers of one LenseFlow: the two packed spectra stacks S0, S1
+// [nmaps][N][N/2] complex and the RK4 accumulator [nmaps][N][N].
+struct StageBufs {
+  void* s0;
+  void* s1;
+  void* acc;
+};
+
 // src / outp (serving executor, standard rate only): the input maps are read
 // through the request pointers by the first launch (and copied into y, the
 // staging state), and the last stage writes the result through the output
-// pointers -- no separate gather and scatter passes.
+// pointers -- no separate gather and scatter passes.  bufs: stage buffers
+// other than the plan's (then the plan's timing mode is ignored).
 template <typename T>
 cmbl_status run_lenseflow(cmbl_plan* p, const void* const* ptabs, T* y, long nmaps, int C,
                           int n, int direction, int adjoint, hipStream_t s,
-                          const T* const* src = nullptr, T* const* outp = nullptr) {
+                          const T* const* src = nullptr, T* const* outp = nullptr,
+                          const StageBufs* bufs = nullptr) {
   const int N = p->N;
   const cplx<T>* tw = (const cplx<T>*)p->tw;
-  cplx<T>* S0 = (cplx<T>*)p->s0;
-  cplx<T>* S1 = (cplx<T>*)p->s1;
-  T* acc = (T*)p->acc;
+  cplx<T>* S0 = (cplx<T>*)(bufs ? bufs->s0 : p->s0);
+  cplx<T>* S1 = (cplx<T>*)(bufs ? bufs->s1 : p->s1);
+  T* acc = (T*)(bufs ? bufs->acc : p->acc);
   const T* const* pt = (const T* const*)ptabs;
   // truth table of lense_flow.py:317-322
   const bool fwd_time = ((direction == CMBL_FORWARD_LENSE) != (adjoint != 0));
@@ -621,7 +640,7 @@ cmbl_status run_lenseflow(cmbl_plan* p, const void* const* ptabs, T* y, long nma
   };
   static const int stage_half[5] = {0, 0, 1, 1, 2};
   const bool adj = adjoint != 0;
-  Timer tm{p, s};
+  Timer tm{p, s, bufs == nullptr};
   if (!adj && src) {
     TIMED(2, cmbl::launch_lf_row_r2c_gather<T>(N, src, C, y, S0, nmaps, tw, s));
   } else if (!adj) {
@@ -695,6 +714,115 @@ extern "C" cmbl_status cmbl_lenseflow(cmbl_plan* p, const cmbl_phi* const* phis,
   HIP_TRY(hipEventRecord(p->ring_ev[slot], s));
   p->ring_used[slot] = true;
   return CMBL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// LenseFlow without allocation: the stage buffers and the per-call phi table
+// pointers live in a file-scope device array of fixed size instead of the
+// plan's scratch.  Each device holds its own copy of the array for the life
+// of the process; its address is resolved once per device
+// (hipGetSymbolAddress, at plan creation or on the entry point's first call
+// there) and only that address is ever handed to kernels or copies.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t kScratchBytes = size_t(64) << 20;  // CMBL_NOALLOC_SCRATCH_BYTES
+static_assert(kScratchBytes == CMBL_NOALLOC_SCRATCH_BYTES, "header and library disagree");
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device.
+cmbl_status scratch_address(void** out) {
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(CMBL_EINVAL, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(CMBL_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return CMBL_OK;
+}
+
+constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Bytes of scratch a call needs: the phi table pointers, then S0, S1 and the
+// accumulator, each region 256-byte aligned.
+size_t noalloc_bytes(long maps, int batch, int N, size_t tsize) {
+  const size_t plane = align256((size_t)maps * N * N * tsize);
+  return align256((size_t)batch * sizeof(void*)) + 3 * plane;
+}
+
+// Up to 64 phi table pointers passed by value and stored to the device
+// array (a kernel rather than a host-to-device copy: no host staging).
+struct PtrPack {
+  const void* p[64];
+};
+
+__global__ void k_store_ptrs(PtrPack pk, const void** dst, int count) {
+  const int i = threadIdx.x;
+  if (i < count) dst[i] = pk.p[i];
+}
+
+}  // namespace
+
+extern "C" cmbl_status cmbl_lenseflow_noalloc(cmbl_plan* p, const cmbl_phi* const* phis,
+                                              const void* in, void* out, int batch, int ncomp,
+                                              int nsteps, int direction, int adjoint,
+                                              void* stream) {
+  if (!p || !phis || !in || !out) return fail(CMBL_EINVAL, "NULL argument");
+  if (ncomp != 1 && ncomp != 2) return fail(CMBL_EINVAL, "ncomp must be 1 (T) or 2 (QU)");
+  if (batch < 0) return fail(CMBL_EINVAL, "batch < 0");
+  if (direction != CMBL_FORWARD_LENSE && direction != CMBL_INVERSE_LENSE)
+    return fail(CMBL_EINVAL, "direction must be +1 or -1");
+  if (nsteps < 1) return fail(CMBL_EINVAL, "nsteps must be >= 1");
+  for (int b = 0; b < batch; ++b) {
+    if (!phis[b]) return fail(CMBL_EINVAL, "phi handle is NULL");
+    if (phis[b]->N != p->N || phis[b]->dtype != p->dtype)
+      return fail(CMBL_EINVAL, "phi handle was prepared for another geometry or dtype");
+    if (phis[b]->nsteps != nsteps) return fail(CMBL_EINVAL, "phi handle prepared for another nsteps");
+  }
+  const long maps = (long)batch * ncomp;
+  const size_t need = noalloc_bytes(maps, batch, p->N, p->tsize);
+  if (need > kScratchBytes) {
+    const size_t per_map = 3 * (size_t)p->N * p->N * p->tsize;
+    return fail(CMBL_EINVAL,
+                "cmbl_lenseflow_noalloc: " + std::to_string(maps) + " maps of " +
+                    std::to_string(p->N) + "^2 need " + std::to_string(need) +
+                    " bytes of scratch; the static scratch holds " +
+                    std::to_string(kScratchBytes) + " bytes (about " +
+                    std::to_string(kScratchBytes / per_map) + " maps at this size and dtype)");
+  }
+  if (batch == 0) return CMBL_OK;
+  void* base = nullptr;
+  CMBL_TRY(scratch_address(&base));
+  hipStream_t s = (hipStream_t)stream;
+  unsigned char* b0 = (unsigned char*)base;
+  const void** ptrs = (const void**)b0;
+  const size_t plane = align256((size_t)maps * p->N * p->N * p->tsize);
+  unsigned char* bufs0 = b0 + align256((size_t)batch * sizeof(void*));
+  const StageBufs bufs{bufs0, bufs0 + plane, bufs0 + 2 * plane};
+  const size_t bytes = (size_t)maps * p->N * p->N * p->tsize;
+  if (in != out) HIP_TRY(hipMemcpyAsync(out, in, bytes, hipMemcpyDeviceToDevice, s));
+  for (int off = 0; off < batch; off += 64) {
+    PtrPack pk{};
+    const int cnt = batch - off < 64 ? batch - off : 64;
+    for (int k = 0; k < cnt; ++k) pk.p[k] = phis[off + k]->ptab;
+    k_store_ptrs<<<1, 64, 0, s>>>(pk, ptrs + off, cnt);
+    HIP_TRY(hipGetLastError());
+  }
+  if (p->dtype == CMBL_F32)
+    return run_lenseflow<float>(p, (const void* const*)ptrs, (float*)out, maps, ncomp, nsteps,
+                                direction, adjoint, s, nullptr, nullptr, &bufs);
+  return run_lenseflow<double>(p, (const void* const*)ptrs, (double*)out, maps, ncomp, nsteps,
+                               direction, adjoint, s, nullptr, nullptr, &bufs);
 }
 
 extern "C" cmbl_status cmbl_plan_set_timing(cmbl_plan* p, int enable) {

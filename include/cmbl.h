@@ -103,6 +103,27 @@ cmbl_status cmbl_lenseflow(cmbl_plan* plan, const cmbl_phi* const* phis, const v
                            void* out, int batch, int ncomp, int nsteps, int direction,
                            int adjoint, void* stream);
 
+/* LenseFlow that allocates nothing: the same computation, kernels and
+ * results (bit for bit) as cmbl_lenseflow, but the stage spectra, the RK4
+ * accumulator and the per-call phi table pointers live in a static device
+ * array of CMBL_NOALLOC_SCRATCH_BYTES per GPU instead of the plan's scratch,
+ * and the call makes no allocation, no host-to-device staging copy and no
+ * timing events.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under stream capture, tight serving
+ * loops).  The plan supplies geometry and twiddles only (it is created
+ * beforehand, which also resolves the array's address on that device); the
+ * caller passes the inputs and outputs.  A call needs
+ * 3 * batch * ncomp * N^2 * sizeof(dtype) bytes (each region rounded to 256
+ * bytes) plus 8 bytes per phi pointer; larger calls fail with CMBL_EINVAL
+ * before any work.  Every call on one GPU uses that GPU's one array: calls
+ * that overlap on a GPU (concurrent streams or host threads) race, so order
+ * them on one stream.  The array keeps the last call's contents; nothing in
+ * it must start at zero. */
+#define CMBL_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+cmbl_status cmbl_lenseflow_noalloc(cmbl_plan* plan, const cmbl_phi* const* phis, const void* in,
+                                   void* out, int batch, int ncomp, int nsteps, int direction,
+                                   int adjoint, void* stream);
+
 /* Backward rule of lense_flow_wrapper (lense_flow.py:10-51, 53-126, 156-252):
  * the joint RK4 of (f, delta f, delta phi) integrated from the transformed
  * field `lensed` (= lense_flow(f, phi, nsteps, direction)) and the cotangent
