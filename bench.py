@@ -4,7 +4,8 @@ Workload: forward LenseFlow of Q/U map pairs, N = 256, 2 arcmin pixels, float32,
 7 RK4 steps (the 256^2 QU configuration).  One "step" is one batched pass over
 `--batch` requests; request r uses phi number r % 32 of 32 seeded lensing
 potentials whose operators are prepared once and stay resident in HBM (the
-serving form).  The plain path is one request per call with its phi prepared
+serving form), and each stream's batch is ordered by phi, as the serving
+backend orders the batches it forms.  The plain path is one request per call with its phi prepared
 in the same call, as the reference's lense_flow does.
 
 The timed step is the served form: the batch split over the serving
@@ -300,7 +301,16 @@ def _lensed_truth(torch, pkg, dev, prob, n):
     return y[0].cpu().numpy()
 
 
-def wf_run(torch, pkg, sims, dev, N=512, th=1.5, nrhs=8, n=7, rtol=1e-6):
+def _marker(torch, dev):
+    """A launch that appears nowhere else in these workloads (row R2C of one
+    16^2 map: k_row_r2c<float, 16>), to cut a leg out of a kernel trace."""
+    import importlib
+    fields = importlib.import_module("cmblensing_amd.fields")
+    fields._rfft2(torch.zeros((16, 16), dtype=torch.float32, device=dev), 2.0)
+
+
+def wf_run(torch, pkg, sims, dev, N=512, th=1.5, nrhs=8, n=7, rtol=1e-6, markers=False,
+           fp64_solve=True):
     """Lensed Wiener filter, Q/U N=512 at 1.5 arcmin, 8 right-hand sides
     (signal, noise and phi seeded 100..107), 1 uK-arcmin white noise with a
     l_knee = 100 1/f term, 200 point-source holes of 5 arcmin, 1 arcmin beam.
@@ -332,11 +342,24 @@ def wf_run(torch, pkg, sims, dev, N=512, th=1.5, nrhs=8, n=7, rtol=1e-6):
     # shape the solve uses, including the shrinking active batches)
     wfm.wiener_filter(*args32, n=n, tol=0.0, rtol=rtol, recompute_every=10, maxiter=1000)
     torch.cuda.synchronize()
+    if markers:
+        _marker(torch, dev)
+        torch.cuda.synchronize()
     t0 = time.perf_counter()
     x32, info = wfm.wiener_filter(*args32, n=n, tol=0.0, rtol=rtol, recompute_every=10,
                                   maxiter=1000, return_info=True)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    if markers:
+        _marker(torch, dev)
+        torch.cuda.synchronize()
+    its = info.iterations
+    leg = {"workload": "wf_QU_512_1.5arcmin_8rhs_holes_1overf", "nrhs": nrhs,
+           "iterations": its, "converged": all(info.converged),
+           "final_true_rel_residual_fp64": max(info.final_true_rel_residual),
+           "time_s": round(dt, 4), "ms_per_iteration": round(dt / max(its) * 1e3, 3)}
+    if not fp64_solve:
+        return leg
     args64 = batched(torch.complex128)
     t1 = time.perf_counter()
     x64, info64 = wfm.wiener_filter(*args64, n=n, tol=0.0, rtol=1e-10, maxiter=3000,
@@ -348,14 +371,10 @@ def wf_run(torch, pkg, sims, dev, N=512, th=1.5, nrhs=8, n=7, rtol=1e-6):
         a = torch.stack([x32.polar_matrix_1[i], x32.polar_matrix_2[i]]).to(torch.complex128)
         b = torch.stack([x64.polar_matrix_1[i], x64.polar_matrix_2[i]])
         errs.append(float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)))
-    its = info.iterations
-    return {"workload": "wf_QU_512_1.5arcmin_8rhs_holes_1overf", "nrhs": nrhs,
-            "iterations": its, "converged": all(info.converged),
-            "final_true_rel_residual_fp64": max(info.final_true_rel_residual),
-            "time_s": round(dt, 3), "ms_per_iteration": round(dt / max(its) * 1e3, 3),
-            "fp64_solve": {"iterations": info64.iterations, "time_s": round(dt64, 3),
-                           "final_true_rel_residual": max(info64.final_true_rel_residual)},
-            "rel_l2_vs_fp64_solve_max": max(errs)}
+    leg.update({"fp64_solve": {"iterations": info64.iterations, "time_s": round(dt64, 3),
+                               "final_true_rel_residual": max(info64.final_true_rel_residual)},
+                "rel_l2_vs_fp64_solve_max": max(errs)})
+    return leg
 
 
 def grad_run(torch, pkg, sims, dev, N=256, th=2.0, chains=16, n=7, reps=5):
@@ -630,6 +649,9 @@ def main():
     ap.add_argument("--no-sim", action="store_true")
     ap.add_argument("--no-config0", action="store_true")
     ap.add_argument("--no-map-joint", action="store_true")
+    ap.add_argument("--wf-only", action="store_true",
+                    help="only the 512^2 Q/U Wiener-filter leg, its timed solve between two "
+                         "marker launches (k_row_r2c<float, 16>) for a kernel trace")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -649,30 +671,43 @@ def main():
     sims = ge._load_module("cmbl_sims_b", ROOT / "cmblensing.py_amd" / "sims.py")
 
     dev = torch.device("cuda:0")
+    if args.wf_only:
+        print(json.dumps({"wiener_filter": wf_run(torch, pkg, sims, dev, markers=True,
+                                                  fp64_solve=False)}), flush=True)
+        return
     N, th, n, C, B = 256, 2.0, 7, 2, args.batch
     NPHI = 32
     dtype = torch.float32
     print(f"[bench] workload QU N={N} theta={th} n={n} fp32 batch={B}", file=sys.stderr, flush=True)
 
-    # synthetic inputs: 32 phi, B distinct Q/U requests (cycled from 8 draws)
+    # synthetic inputs: 32 phi; request r (in arrival order) lenses Q/U draw
+    # r % 8 of 8 by phi r % 32.  The serving backend forms a batch of each
+    # stream's share of the arrivals and orders it by phi (requests that share
+    # a phi then read the same p(t) rows together), so the batch positions
+    # hold the requests in that order; every path below uses it.
+    NS = args.streams
+    if B % NS:
+        sys.exit(f"--batch {B} must be a multiple of --streams {NS}")
+    Bs = B // NS
+    order = []
+    for k in range(NS):
+        order += sorted(range(k * Bs, (k + 1) * Bs), key=lambda r: r % NPHI)
+    req_f = [r % 8 for r in order]          # Q/U draw of batch position j
+    req_p = [r % NPHI for r in order]       # phi of batch position j
     phis = [sims.phi_map(N, th, 1000 + i) for i in range(NPHI)]
     fq = [sims.qu_map(N, th, i) for i in range(8)]
-    x = torch.as_tensor(np.stack([fq[r % 8] for r in range(B)]), dtype=dtype, device=dev)
+    x = torch.as_tensor(np.stack([fq[req_f[j]] for j in range(B)]), dtype=dtype, device=dev)
     phi_t = [torch.as_tensor(p, device=dev) for p in phis]
 
     plan = _lib.get_plan(N, th, dtype, B * C, dev)
     handles = [_lib.get_phi(plan, p, n) for p in phi_t]
-    req_phi = [handles[r % NPHI] for r in range(B)]
+    req_phi = [handles[req_p[j]] for j in range(B)]
     out = torch.empty_like(x)
 
     # the served form, as the serving backend runs it: the B requests of a
     # step split over its streams (default 2, default priority), each half one
     # replay of a captured batched graph (gather, 57 stage kernels, scatter)
     # on a plan of its own, so the two batches overlap on the device
-    NS = args.streams
-    if B % NS:
-        sys.exit(f"--batch {B} must be a multiple of --streams {NS}")
-    Bs = B // NS
     lanes = []
     for k in range(NS):
         pk = _lib.Plan(N, th, dtype, Bs * C, dev)
@@ -777,7 +812,7 @@ def main():
         "data": "synthetic (seeded Gaussian random fields, 32 resident phi)",
         "config": {"workload": "lenseflow_fwd_QU_256_2arcmin_n7_fp32", "N": N, "theta_pix": th,
                    "nsteps": n, "components": "QU", "batch": B, "phi_resident": NPHI,
-                   "streams": NS},
+                   "streams": NS, "batch_order": "each stream's batch ordered by phi"},
         "direct_batched_maps_s": round(direct_maps_s, 2),
         "plain_maps_s": round(plain_maps_s, 2),
         "batched_over_plain": round(maps_s / plain_maps_s, 2),
@@ -845,18 +880,19 @@ def main():
     if not args.no_cpu:
         # the CPU leg: the FP64 oracle checks one batched result, then is timed
         from oracle import lenseflow as olf
-        ref = olf.lense_flow(fq[r_chk % 8], phis[r_chk % NPHI], olf.pix_width_from_theta(th), n)
+        f_chk, p_chk = fq[req_f[r_chk]], phis[req_p[r_chk]]
+        ref = olf.lense_flow(f_chk, p_chk, olf.pix_width_from_theta(th), n)
         result["rel_l2_vs_fp64_oracle"] = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
         # the four modes of the 256^2 QU configuration against the oracle, and
         # the oracle's own round trip (the RK4 discretisation at n = 7)
         pw = olf.pix_width_from_theta(th)
         for name, (d, a) in LF_MODES.items():
-            ref_m = olf.lense_flow(fq[r_chk % 8], phis[r_chk % NPHI], pw, n, d, a)
+            ref_m = olf.lense_flow(f_chk, p_chk, pw, n, d, a)
             c1_leg[name]["rel_l2_vs_fp64_oracle"] = float(
                 np.linalg.norm(c1_chk[name] - ref_m) / np.linalg.norm(ref_m))
-        back = olf.lense_flow(ref, phis[r_chk % NPHI], pw, n, -1, False)
+        back = olf.lense_flow(ref, p_chk, pw, n, -1, False)
         c1_leg["oracle_fp64_round_trip_rel_l2"] = float(
-            np.linalg.norm(back - fq[r_chk % 8]) / np.linalg.norm(fq[r_chk % 8]))
+            np.linalg.norm(back - f_chk) / np.linalg.norm(f_chk))
         # the same workload on the host: Q/U 256^2 requests (8 seeded pairs,
         # one resident phi), forward, n = 7, FP64 oracle, threads over requests
         nth = cpu_threads()

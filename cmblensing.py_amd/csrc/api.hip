@@ -5,6 +5,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -83,6 +84,12 @@ struct cmbl_plan {
   hipEvent_t ring_ev[kRing] = {};
   bool ring_used[kRing] = {};
   int ring_next = 0;
+  // second stream of the two-half LenseFlow schedule (default priority) and
+  // the fork / join events that order it with the caller's stream
+  hipStream_t aux = nullptr;
+  hipEvent_t fork_ev = nullptr;
+  hipEvent_t join_ev = nullptr;
+  bool split = true;
   // optional per-kernel timing (HIP events around every stage launch)
   bool timing = false;
   std::vector<hipEvent_t> tev;   // pool, 2 per launch
@@ -422,6 +429,15 @@ cmbl_status cmbl_plan_create(cmbl_plan** out, int N, double theta_pix_arcmin, in
   for (int i = 0; i < kRing; ++i)
     if ((e = hipEventCreateWithFlags(&p->ring_ev[i], hipEventDisableTiming)) != hipSuccess)
       return cleanup(fail(CMBL_EHIP, std::string("event: ") + hipGetErrorString(e)));
+  if ((e = hipStreamCreateWithFlags(&p->aux, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&p->fork_ev, hipEventDisableTiming)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&p->join_ev, hipEventDisableTiming)) != hipSuccess)
+    return cleanup(fail(CMBL_EHIP, std::string("second stream: ") + hipGetErrorString(e)));
+  {
+    // CMBL_SPLIT=0 runs every batch as one part (for A/B measurements)
+    const char* sp_env = std::getenv("CMBL_SPLIT");
+    p->split = !(sp_env && sp_env[0] == '0');
+  }
   // set-up of the allocation-free entry point on this device
   void* sp = nullptr;
   const cmbl_status sst = scratch_address(&sp);
@@ -433,6 +449,12 @@ cmbl_status cmbl_plan_create(cmbl_plan** out, int N, double theta_pix_arcmin, in
 cmbl_status cmbl_plan_destroy(cmbl_plan* p) {
   if (!p) return CMBL_OK;
   for (hipEvent_t e : p->tev) (void)hipEventDestroy(e);
+  if (p->aux) {
+    (void)hipStreamSynchronize(p->aux);
+    (void)hipStreamDestroy(p->aux);
+  }
+  if (p->fork_ev) (void)hipEventDestroy(p->fork_ev);
+  if (p->join_ev) (void)hipEventDestroy(p->join_ev);
   for (int i = 0; i < kRing; ++i)
     if (p->ring_ev[i]) {
       (void)hipEventSynchronize(p->ring_ev[i]);
@@ -613,6 +635,25 @@ struct StageBufs {
   void* acc;
 };
 
+// One part of a LenseFlow batch: a contiguous range of maps, its phi table
+// pointers, its slices of the stage buffers and the stream it runs on.
+template <typename T>
+struct LfPart {
+  const T* const* pt;
+  T* y;
+  cplx<T>* S0;
+  cplx<T>* S1;
+  T* acc;
+  long nmaps;
+  hipStream_t s;
+};
+
+// Batches of at least this many map pixels (nmaps * N^2) run as two halves on
+// the caller's stream and the plan's second stream, launch by launch: one
+// half's column pass (few large blocks at N = 512, bound by its
+// load-transform-store latency) then overlaps the other half's row pass.
+constexpr long kSplitMinPixels = 1L << 21;
+
 // src / outp (serving executor, standard rate only): the input maps are read
 // through the request pointers by the first launch (and copied into y, the
 // staging state), and the last stage writes the result through the output
@@ -624,6 +665,7 @@ cmbl_status run_lenseflow(cmbl_plan* p, const void* const* ptabs, T* y, long nma
                           const T* const* src = nullptr, T* const* outp = nullptr,
                           const StageBufs* bufs = nullptr) {
   const int N = p->N;
+  const long NN = (long)N * N, NH = (long)N * (N / 2);
   const cplx<T>* tw = (const cplx<T>*)p->tw;
   cplx<T>* S0 = (cplx<T>*)(bufs ? bufs->s0 : p->s0);
   cplx<T>* S1 = (cplx<T>*)(bufs ? bufs->s1 : p->s1);
@@ -641,20 +683,42 @@ cmbl_status run_lenseflow(cmbl_plan* p, const void* const* ptabs, T* y, long nma
   static const int stage_half[5] = {0, 0, 1, 1, 2};
   const bool adj = adjoint != 0;
   Timer tm{p, s, bufs == nullptr};
-  if (!adj && src) {
-    TIMED(2, cmbl::launch_lf_row_r2c_gather<T>(N, src, C, y, S0, nmaps, tw, s));
-  } else if (!adj) {
-    TIMED(2, cmbl::launch_lf_row_r2c<T>(N, y, S0, nmaps, tw, s));
-  } else {
-    TIMED(2, cmbl::launch_lf_row<T>(N, true, S0, S1, y, acc, pt, tw, C, nmaps, 1, h,
-                                    tindex(0, 0), 1, 1, (T)p->dk, s));
+  // one part, or two halves of the batch on two streams (not for the
+  // executor's sequences, stage buffers of the caller, timed runs or a
+  // stream under capture)
+  const long batch = nmaps / C;
+  bool split = p->split && p->aux && !src && !outp && !bufs && !p->timing && batch >= 2 &&
+               nmaps * NN >= kSplitMinPixels;
+  if (split) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(s, &cs));
+    split = cs == hipStreamCaptureStatusNone;
+  }
+  LfPart<T> parts[2] = {{pt, y, S0, S1, acc, nmaps, s}, {}};
+  int np = 1;
+  if (split) {
+    const long mA = ((batch + 1) / 2) * C;
+    parts[0].nmaps = mA;
+    parts[1] = {pt + mA / C, y + mA * NN, S0 + mA * NH, S1 + mA * NH, acc + mA * NN, nmaps - mA,
+                p->aux};
+    np = 2;
+    HIP_TRY(hipEventRecord(p->fork_ev, s));
+    HIP_TRY(hipStreamWaitEvent(p->aux, p->fork_ev, 0));
+  }
+  for (int k = 0; k < np; ++k) {
+    const LfPart<T>& q = parts[k];
+    if (!adj && src) {
+      TIMED(2, cmbl::launch_lf_row_r2c_gather<T>(N, src, C, q.y, q.S0, q.nmaps, tw, q.s));
+    } else if (!adj) {
+      TIMED(2, cmbl::launch_lf_row_r2c<T>(N, q.y, q.S0, q.nmaps, tw, q.s));
+    } else {
+      TIMED(2, cmbl::launch_lf_row<T>(N, true, q.S0, q.S1, q.y, q.acc, q.pt, tw, C, q.nmaps, 1, h,
+                                      tindex(0, 0), 1, 1, (T)p->dk, q.s));
+    }
   }
   for (int i = 0; i < n; ++i) {
     for (int st = 1; st <= 4; ++st) {
       const bool last = (i == n - 1) && (st == 4);
-      // standard rate: d/dx of S0 into S1 (S0 is kept for d/dy); adjoint:
-      // d/dx of U in place (V in S1 supplies the y part)
-      TIMED(0, cmbl::launch_lf_col<T>(N, S0, adj ? S0 : S1, nmaps, tw, (T)p->dk, s));
       int tidx;
       if (!adj) {
         tidx = tindex(i, stage_half[st]);
@@ -664,10 +728,23 @@ cmbl_status run_lenseflow(cmbl_plan* p, const void* const* ptabs, T* y, long nma
         tidx = st < 4 ? tindex(i, stage_half[st + 1]) : tindex(i + 1, 0);
         if (last) tidx = 0;
       }
-      TIMED(1, cmbl::launch_lf_row<T>(N, adj, S0, S1, y, acc, pt, tw, C, nmaps, st, h, tidx,
-                                      last ? 0 : 1, 0, (T)p->dk, s, nullptr, nullptr,
-                                      (last && !adj) ? outp : nullptr));
+      for (int k = 0; k < np; ++k) {
+        const LfPart<T>& q = parts[k];
+        // standard rate: d/dx of S0 into S1 (S0 is kept for d/dy); adjoint:
+        // d/dx of U in place (V in S1 supplies the y part)
+        TIMED(0, cmbl::launch_lf_col<T>(N, q.S0, adj ? q.S0 : q.S1, q.nmaps, tw, (T)p->dk, q.s));
+      }
+      for (int k = 0; k < np; ++k) {
+        const LfPart<T>& q = parts[k];
+        TIMED(1, cmbl::launch_lf_row<T>(N, adj, q.S0, q.S1, q.y, q.acc, q.pt, tw, C, q.nmaps, st,
+                                        h, tidx, last ? 0 : 1, 0, (T)p->dk, q.s, nullptr, nullptr,
+                                        (last && !adj) ? outp : nullptr));
+      }
     }
+  }
+  if (split) {
+    HIP_TRY(hipEventRecord(p->join_ev, p->aux));
+    HIP_TRY(hipStreamWaitEvent(s, p->join_ev, 0));
   }
   return tm.flush();
 }

@@ -71,7 +71,12 @@ constexpr bool col_tw_lds() {
 
 // Row pairing used by the fused stage kernels.  Pair index p covers two
 // map-rows: for C == 2 the Q and U rows of the same pixel row (they share the
-// deflection operator rows), for C == 1 the pixel rows 2r and 2r+1.
+// deflection operator rows), for C == 1 the pixel rows 2r and 2r+1.  The
+// pairs of a workgroup are consecutive rows of one batch element (numbering
+// the batch element fastest, so that a workgroup takes one pixel row of
+// consecutive elements, measured slower: 58.2k against 60.7k maps/s at
+// 256^2 QU batch 128 ordered by phi -- the rows of one element are
+// contiguous in HBM).
 struct RowPair {
   int b;       // batch element
   int ia, ib;  // pixel rows
@@ -977,6 +982,13 @@ static inline unsigned blocks_for(long n, long per) { return (unsigned)((n + per
 #ifndef CMBL_SMALL_GRID
 #define CMBL_SMALL_GRID 256
 #endif
+// The column pass takes its quarter-width blocks only below this many
+// default-shape blocks: narrow blocks read row segments of 32 bytes, which
+// lose to the full-width shape as soon as a quarter of the CUs have a block
+// (12 maps of 512^2: 16.6 us with 4-column blocks against ~11 us with 16).
+#ifndef CMBL_SMALL_COL_GRID
+#define CMBL_SMALL_COL_GRID 64
+#endif
 
 
 template <typename T>
@@ -1050,7 +1062,7 @@ hipError_t launch_lf_col(int N, const cplx<T>* src, cplx<T>* dst, long nmaps, co
     constexpr int KC = col_kc<T>(NN_);
     constexpr int KCS = KC / 4 > 2 ? KC / 4 : 2;
     const unsigned nb = (unsigned)(nmaps * (NN_ / 2 / KC));
-    if (nb < CMBL_SMALL_GRID)
+    if (nb < CMBL_SMALL_COL_GRID)
       k_col_lf_dx<T, NN_, KCS><<<(unsigned)(nmaps * (NN_ / 2 / KCS)), KCS * NN_ / 8, 0, s>>>(
           src, dst, tw, dk);
     else

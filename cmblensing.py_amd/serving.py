@@ -19,6 +19,8 @@ a set of shared lensing potentials -- and runs many requests together:
   request arrived first is launched as one batch of up to the bucket's batch
   size, so requests that arrive while others run are packed into the next
   launch and batches of different buckets overlap on the device's streams;
+  a batch's positions are ordered by phi, so requests sharing a phi read its
+  p(t) rows together;
 * each batch is one replay of a hipGraph captured once per (stream, bucket,
   batch class), with the request maps gathered into and scattered out of a
   staging buffer by the graph's own kernels;
@@ -332,7 +334,12 @@ class Server:
             if q is None:
                 break
             nb = min(len(q), self.backend.max_batch(key))
-            reqs = [q.popleft() for _ in range(nb)]
+            # batch positions ordered by phi: requests that share a phi then
+            # read the same rows of its p(t) table at about the same time
+            # (their row-pass workgroups lie a fixed stride apart, on one
+            # XCD), so its L2 serves the repeats; a request's result does not
+            # depend on its position
+            reqs = sorted((q.popleft() for _ in range(nb)), key=lambda r: r.phi_id)
             handles = [self.phis[(r.phi_id, r.N)].handle for r in reqs]
             for r in reqs:
                 r.state = "running"
