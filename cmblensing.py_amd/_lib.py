@@ -57,6 +57,9 @@ _SIGS = {
                                    c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "cmbl_cg_update_p": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
                                  c_void_p]),
+    "cmbl_cg_step": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                             c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
+                             c_void_p]),
     "cmbl_lenseflow_vjp": (c_int, [c_void_p, ctypes.POINTER(c_void_p), c_void_p, c_void_p,
                                    c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p]),
     "cmbl_exec_create": (c_int, [ctypes.POINTER(c_void_p), c_void_p, c_int, c_int, c_int, c_int,
@@ -312,8 +315,22 @@ def _memcpy_d2d(dst: int, src: int, nbytes: int):
 _plans: dict = {}
 _phis: "OrderedDict" = OrderedDict()
 PHI_CACHE_SIZE = 64
+# the caches are shared by host threads (the serving Wiener-filter lane runs on
+# one of its own)
+_cache_lock = threading.RLock()
 
 
+def _locked(fn):
+    import functools
+
+    @functools.wraps(fn)
+    def wrapper(*a, **k):
+        with _cache_lock:
+            return fn(*a, **k)
+    return wrapper
+
+
+@_locked
 def get_plan(N: int, theta_pix: float, dtype: torch.dtype, nmaps: int, device) -> Plan:
     """The cached plan of (device, geometry, dtype) for the current stream.
 
@@ -367,6 +384,7 @@ def _record_event(device) -> "torch.cuda.Event":
     return ev
 
 
+@_locked
 def get_phi_batch(plan: Plan, phi: torch.Tensor, nsteps: int):
     """Prepared phis for every element of a batched phi [B, ...] (cached like
     get_phi per element; any miss prepares the whole batch in one call)."""
@@ -402,6 +420,7 @@ def reserve_phi_cache(count: int):
     PHI_CACHE_SIZE = max(PHI_CACHE_SIZE, int(count))
 
 
+@_locked
 def get_phi(plan: Plan, phi: torch.Tensor, nsteps: int, index=None) -> PhiHandle:
     """Prepared phi for (plan geometry, the tensor object and its version,
     nsteps) -- or for ``phi[index]`` of a batched phi tensor.  Entries are
@@ -433,6 +452,7 @@ def _trim_phi_cache():
         _phis.popitem(last=False)
 
 
+@_locked
 def clear_caches():
     for v in _phis.values():
         v[2].close()
@@ -542,6 +562,17 @@ def cg_update_p(plan: Plan, p, z, beta, p_op, nrhs: int, ncomp: int):
         check(plan.lib.cmbl_cg_update_p(plan.handle, p.data_ptr(), z.data_ptr(), beta.data_ptr(),
                                         _ptr(p_op), nrhs, ncomp, stream_ptr(plan.device)),
               "cmbl_cg_update_p")
+
+
+def cg_step(plan: Plan, x, r, z, p, Ap, pinv, p_op, res, act, bb, rel, nrhs: int, ncomp: int):
+    """cmbl_cg_step: one PCG iteration after the operator, scalars on the
+    device (res updated in place; act a bool device tensor or None; rel and
+    bb together or None)."""
+    with torch.cuda.device(plan.device):
+        check(plan.lib.cmbl_cg_step(plan.handle, x.data_ptr(), r.data_ptr(), z.data_ptr(),
+                                    p.data_ptr(), Ap.data_ptr(), pinv.data_ptr(), _ptr(p_op),
+                                    res.data_ptr(), _ptr(act), _ptr(bb), _ptr(rel), nrhs, ncomp,
+                                    stream_ptr(plan.device)), "cmbl_cg_step")
 
 
 def lenseflow_vjp(plan: Plan, phis, lensed, cot, d_field, d_phi, batch: int, ncomp: int,

@@ -47,6 +47,27 @@ constexpr int kRing = 8;
 // current device (defined with cmbl_lenseflow_noalloc).
 cmbl_status scratch_address(void** out);
 
+// The second stream of the two-half LenseFlow schedule on the current device:
+// one per device, created by the first plan there (early in a process, before
+// most other streams exist) and shared by every plan (splits issued from
+// several caller streams meet on it, ordered by their events).  Non-blocking,
+// default priority.  HIP multiplexes a process's streams over a few hardware
+// queues; a second stream that shares the caller's queue serialises the two
+// halves (a per-plan stream created after a serving run's many streams made
+// the 512^2 8-RHS Wiener filter 25 % slower).
+std::mutex g_aux_mu;
+hipStream_t g_aux[64] = {};
+
+cmbl_status aux_stream(hipStream_t* out) {
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(CMBL_EINVAL, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_aux_mu);
+  if (!g_aux[dev]) HIP_TRY(hipStreamCreateWithFlags(&g_aux[dev], hipStreamNonBlocking));
+  *out = g_aux[dev];
+  return CMBL_OK;
+}
+
 bool valid_n(int N) {
   return N >= 16 && N <= 512 && (N & (N - 1)) == 0;
 }
@@ -84,8 +105,9 @@ struct cmbl_plan {
   hipEvent_t ring_ev[kRing] = {};
   bool ring_used[kRing] = {};
   int ring_next = 0;
-  // second stream of the two-half LenseFlow schedule (default priority) and
-  // the fork / join events that order it with the caller's stream
+  // second stream of the two-half LenseFlow schedule (the device's shared
+  // one, see aux_stream) and the fork / join events that order it with the
+  // caller's stream
   hipStream_t aux = nullptr;
   hipEvent_t fork_ev = nullptr;
   hipEvent_t join_ev = nullptr;
@@ -104,6 +126,7 @@ struct cmbl_plan {
   void* wx2 = nullptr;
   double* red = nullptr;
   int red_rhs = 0;
+  double* cgs = nullptr;  // PCG scalars of cmbl_cg_step: pAp, alpha, rz, beta, rr [5][red_rhs]
   // gradient-ODE scratch, allocated on first use
   void* gy = nullptr;    // f state copy [max_maps][N][N]
   void* gacc = nullptr;  // delta-f accumulator [max_maps][N][N]
@@ -429,8 +452,11 @@ cmbl_status cmbl_plan_create(cmbl_plan** out, int N, double theta_pix_arcmin, in
   for (int i = 0; i < kRing; ++i)
     if ((e = hipEventCreateWithFlags(&p->ring_ev[i], hipEventDisableTiming)) != hipSuccess)
       return cleanup(fail(CMBL_EHIP, std::string("event: ") + hipGetErrorString(e)));
-  if ((e = hipStreamCreateWithFlags(&p->aux, hipStreamNonBlocking)) != hipSuccess ||
-      (e = hipEventCreateWithFlags(&p->fork_ev, hipEventDisableTiming)) != hipSuccess ||
+  {
+    const cmbl_status ast = aux_stream(&p->aux);
+    if (ast != CMBL_OK) return cleanup(ast);
+  }
+  if ((e = hipEventCreateWithFlags(&p->fork_ev, hipEventDisableTiming)) != hipSuccess ||
       (e = hipEventCreateWithFlags(&p->join_ev, hipEventDisableTiming)) != hipSuccess)
     return cleanup(fail(CMBL_EHIP, std::string("second stream: ") + hipGetErrorString(e)));
   {
@@ -449,10 +475,7 @@ cmbl_status cmbl_plan_create(cmbl_plan** out, int N, double theta_pix_arcmin, in
 cmbl_status cmbl_plan_destroy(cmbl_plan* p) {
   if (!p) return CMBL_OK;
   for (hipEvent_t e : p->tev) (void)hipEventDestroy(e);
-  if (p->aux) {
-    (void)hipStreamSynchronize(p->aux);
-    (void)hipStreamDestroy(p->aux);
-  }
+  if (p->aux) (void)hipStreamSynchronize(p->aux);  // the device's shared second stream
   if (p->fork_ev) (void)hipEventDestroy(p->fork_ev);
   if (p->join_ev) (void)hipEventDestroy(p->join_ev);
   for (int i = 0; i < kRing; ++i)
@@ -944,7 +967,8 @@ cmbl_status ensure_red(cmbl_plan* p, int nrhs) {
     (void)hipFree(p->red);
     p->red = nullptr;
   }
-  HIP_TRY(hipMalloc(&p->red, (size_t)nrhs * cmbl::red_parts() * sizeof(double)));
+  HIP_TRY(hipMalloc(&p->red, (size_t)nrhs * (cmbl::red_parts() + 5) * sizeof(double)));
+  p->cgs = p->red + (size_t)nrhs * cmbl::red_parts();
   p->red_rhs = nrhs;
   return CMBL_OK;
 }
@@ -1129,6 +1153,54 @@ extern "C" cmbl_status cmbl_cg_update_xrz(cmbl_plan* p, void* x, void* r, void* 
     HIP_TRY(cmbl::launch_cg_xrz<double>((cplx<double>*)x, (cplx<double>*)r, (cplx<double>*)z,
                                         (const cplx<double>*)pv, (const cplx<double>*)Ap, pinv,
                                         alpha, p->N, ncomp, len, nrhs, p->red, rz, s));
+  return CMBL_OK;
+}
+
+extern "C" cmbl_status cmbl_cg_step(cmbl_plan* p, void* x, void* r, void* z, void* pv,
+                                    const void* Ap, const double* pinv, void* p_op, double* res,
+                                    const unsigned char* act, const double* bb, double* rel,
+                                    int nrhs, int ncomp, void* stream) {
+  if (!p || !x || !r || !z || !pv || !Ap || !pinv || !res)
+    return fail(CMBL_EINVAL, "NULL argument");
+  if (nrhs < 1 || ncomp < 1 || ncomp > 3)
+    return fail(CMBL_EINVAL, "nrhs must be >= 1 and ncomp 1, 2 or 3");
+  if (!rel != !bb) return fail(CMBL_EINVAL, "rel and bb go together");
+  CMBL_TRY(ensure_red(p, nrhs));
+  const long len = (long)ncomp * p->N * (p->N / 2 + 1);
+  hipStream_t s = (hipStream_t)stream;
+  double* pAp = p->cgs;
+  double* alpha = pAp + nrhs;
+  double* rz = alpha + nrhs;
+  double* beta = rz + nrhs;
+  double* rr = beta + nrhs;
+  const bool f32 = p->dtype == CMBL_F32;
+  if (f32)
+    HIP_TRY(cmbl::launch_dot<float>((const cplx<double>*)pv, (const cplx<float>*)Ap, p->N, len,
+                                    nrhs, p->red, pAp, s));
+  else
+    HIP_TRY(cmbl::launch_dot<double>((const cplx<double>*)pv, (const cplx<double>*)Ap, p->N, len,
+                                     nrhs, p->red, pAp, s));
+  HIP_TRY(cmbl::launch_cg_alpha(res, pAp, act, alpha, nrhs, s));
+  if (f32)
+    HIP_TRY(cmbl::launch_cg_xrz<float>((cplx<double>*)x, (cplx<double>*)r, (cplx<double>*)z,
+                                       (const cplx<double>*)pv, (const cplx<float>*)Ap, pinv, alpha,
+                                       p->N, ncomp, len, nrhs, p->red, rz, s));
+  else
+    HIP_TRY(cmbl::launch_cg_xrz<double>((cplx<double>*)x, (cplx<double>*)r, (cplx<double>*)z,
+                                        (const cplx<double>*)pv, (const cplx<double>*)Ap, pinv,
+                                        alpha, p->N, ncomp, len, nrhs, p->red, rz, s));
+  HIP_TRY(cmbl::launch_cg_beta(res, rz, act, beta, nrhs, s));
+  if (f32)
+    HIP_TRY(cmbl::launch_cg_p<float>((cplx<double>*)pv, (const cplx<double>*)z, beta,
+                                     (cplx<float>*)p_op, len, nrhs, s));
+  else
+    HIP_TRY(cmbl::launch_cg_p<double>((cplx<double>*)pv, (const cplx<double>*)z, beta,
+                                      (cplx<double>*)p_op, len, nrhs, s));
+  if (rel) {
+    HIP_TRY(cmbl::launch_dot<double>((const cplx<double>*)r, (const cplx<double>*)r, p->N, len,
+                                     nrhs, p->red, rr, s));
+    HIP_TRY(cmbl::launch_cg_rel(rr, bb, rel, nrhs, s));
+  }
   return CMBL_OK;
 }
 

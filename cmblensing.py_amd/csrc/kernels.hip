@@ -957,9 +957,19 @@ __global__ void k_scatter(const T* __restrict__ src, T* const* __restrict__ dst,
 // ---------------------------------------------------------------------------
 // host-side launchers (dispatch on N)
 // ---------------------------------------------------------------------------
+// Columns per column-pass workgroup: 16 (float) / 8 (double) complex values,
+// 128-byte row segments.
+#ifndef CMBL_COL_KC_F32
+#define CMBL_COL_KC_F32 16
+#endif
+#ifndef CMBL_COL_KC_F64
+#define CMBL_COL_KC_F64 8
+#endif
 template <typename T>
 constexpr int col_kc(int N) {
-  return (sizeof(T) == 4 ? 16 : 8) < N / 2 ? (sizeof(T) == 4 ? 16 : 8) : N / 2;
+  return (sizeof(T) == 4 ? CMBL_COL_KC_F32 : CMBL_COL_KC_F64) < N / 2
+             ? (sizeof(T) == 4 ? CMBL_COL_KC_F32 : CMBL_COL_KC_F64)
+             : N / 2;
 }
 
 #define CMBL_DISPATCH_N(N, ...)                                \

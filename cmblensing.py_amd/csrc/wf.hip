@@ -449,6 +449,38 @@ __global__ void k_cg_p(cplx<double>* __restrict__ p, const cplx<double>* __restr
   }
 }
 
+// PCG scalars of one iteration (wiener_filter.py:84, 94), per right-hand
+// side, with an optional active mask (inactive: alpha = beta = 0 and res kept):
+//   alpha = res / (pAp != 0 ? pAp : 1)
+__global__ void k_cg_alpha(const double* __restrict__ res, const double* __restrict__ pAp,
+                           const unsigned char* __restrict__ act, double* __restrict__ alpha,
+                           int nrhs) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nrhs) return;
+  const double d = pAp[k];
+  alpha[k] = (!act || act[k]) ? res[k] / (d != 0.0 ? d : 1.0) : 0.0;
+}
+
+//   beta = rz / (res != 0 ? res : 1) ; res = rz   (active right-hand sides)
+__global__ void k_cg_beta(double* __restrict__ res, const double* __restrict__ rz,
+                          const unsigned char* __restrict__ act, double* __restrict__ beta,
+                          int nrhs) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nrhs) return;
+  const bool on = !act || act[k];
+  const double r0 = res[k];
+  beta[k] = on ? rz[k] / (r0 != 0.0 ? r0 : 1.0) : 0.0;
+  if (on) res[k] = rz[k];
+}
+
+//   rel = sqrt(rr / bb)
+__global__ void k_cg_rel(const double* __restrict__ rr, const double* __restrict__ bb,
+                         double* __restrict__ rel, int nrhs) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nrhs) return;
+  rel[k] = sqrt(rr[k] / bb[k]);
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -553,6 +585,24 @@ hipError_t launch_wf_final_teb(const cplx<T>* XLT, const cplx<T>* XLP, const cpl
                                const T* cfinv, cplx<T>* out, int N, long nfields, hipStream_t s) {
   k_wf_final_teb<T><<<grid_for(nfields * (long)N * (N / 2 + 1)), 256, 0, s>>>(
       XLT, XLP, F, cfinv, out, N, nfields);
+  return hipGetLastError();
+}
+
+hipError_t launch_cg_alpha(const double* res, const double* pAp, const unsigned char* act,
+                           double* alpha, int nrhs, hipStream_t s) {
+  k_cg_alpha<<<(nrhs + 63) / 64, 64, 0, s>>>(res, pAp, act, alpha, nrhs);
+  return hipGetLastError();
+}
+
+hipError_t launch_cg_beta(double* res, const double* rz, const unsigned char* act, double* beta,
+                          int nrhs, hipStream_t s) {
+  k_cg_beta<<<(nrhs + 63) / 64, 64, 0, s>>>(res, rz, act, beta, nrhs);
+  return hipGetLastError();
+}
+
+hipError_t launch_cg_rel(const double* rr, const double* bb, double* rel, int nrhs,
+                         hipStream_t s) {
+  k_cg_rel<<<(nrhs + 63) / 64, 64, 0, s>>>(rr, bb, rel, nrhs);
   return hipGetLastError();
 }
 

@@ -42,6 +42,11 @@ hipError_t launch_wf_data_teb(const cplx<T>* XT, const cplx<T>* XP, const cplx<T
 template <typename T>
 hipError_t launch_wf_final_teb(const cplx<T>* XLT, const cplx<T>* XLP, const cplx<T>* F,
                                const T* cfinv, cplx<T>* out, int N, long nfields, hipStream_t s);
+hipError_t launch_cg_alpha(const double* res, const double* pAp, const unsigned char* act,
+                           double* alpha, int nrhs, hipStream_t s);
+hipError_t launch_cg_beta(double* res, const double* rz, const unsigned char* act, double* beta,
+                          int nrhs, hipStream_t s);
+hipError_t launch_cg_rel(const double* rr, const double* bb, double* rel, int nrhs, hipStream_t s);
 int red_parts();
 
 }  // namespace cmbl

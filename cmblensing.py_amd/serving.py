@@ -41,6 +41,8 @@ use a host-side one) can stand in for it.
 from __future__ import annotations
 
 import math
+import sys
+import threading
 import time
 from collections import OrderedDict, deque
 from dataclasses import dataclass, field
@@ -132,12 +134,11 @@ class WFRequest:
 @dataclass(eq=False)
 class _WFToken:
     gen: object
-    event: object = None
-    started: bool = False
     finished: bool = False
     result: object = None
     error: Optional[str] = None
     end: object = None
+    thread: object = None
 
 
 @dataclass(eq=False)
@@ -385,18 +386,19 @@ class Server:
         return finished
 
     def step(self) -> int:
-        """One scheduling pass: retire finished batches, admit, launch."""
+        """One scheduling pass: retire finished batches, admit, launch.
+        Returns the number of batches retired plus launched."""
         n = self._poll()
         self._admit()
-        self._launch()
-        return n
+        return n + self._launch()
 
     def busy(self) -> bool:
         return bool(self.waiting or self.inflight or self.wf_queue or any(self.buckets.values()))
 
     def drain(self):
         while self.busy():
-            self.step()
+            if not self.step():
+                time.sleep(0)
 
     def serve(self, requests: List[Request], offsets: List[float]) -> dict:
         """Open-loop run: request i arrives `offsets[i]` seconds after the
@@ -413,7 +415,10 @@ class Server:
                 r.arrival = t0 + offsets[order[i]]
                 self.submit(r)
                 i += 1
-            self.step()
+            if not self.step():
+                # an idle pass hands the interpreter lock to the Wiener-filter
+                # lane's thread, which needs it between its launches
+                time.sleep(0)
         return self.stats()
 
     # -- reporting -------------------------------------------------------------
@@ -474,6 +479,8 @@ class GpuBackend:
         self._anchor_ev = None
         self._anchor_t = 0.0
         self._wf_stream = None
+        self._switch_interval = None    # CPython's switch interval while a lane thread runs
+        self._warm = False              # every default bucket's executors captured
         self.wf_slots = 1
 
     # memory model ---------------------------------------------------------------
@@ -507,6 +514,13 @@ class GpuBackend:
         ek = (si, key, cls)
         ex = self.execs.get(ek)
         if ex is None:
+            # a graph capture must not overlap the Wiener-filter lane's device
+            # work (its synchronisations would invalidate the capture): wait
+            # for a running lane first (only for a bucket warmup() did not
+            # cover -- launch_wf warms the default buckets before the lane's
+            # first solve)
+            for t in getattr(self, "_wf_threads", []):
+                t.join()
             N, C, direction, adjoint = key
             ex = self._lib.Executor(self._plan(si, N), cls, C, self.nsteps, direction, adjoint)
             self.execs[ek] = ex
@@ -515,6 +529,7 @@ class GpuBackend:
     def warmup(self, keys=None):
         """Create every executor (all batch classes) for `keys` (default: every
         bucket of the served sizes), so no graph is captured while serving."""
+        default = keys is None
         if keys is None:
             keys = [(N, C, d, False) for N in self.sizes for C in (1, 2)
                     for d in (FORWARD_LENSE, INVERSE_LENSE)]
@@ -525,6 +540,7 @@ class GpuBackend:
                 for cls in classes:
                     self._exec(si, key, cls)
         torch.cuda.synchronize(self.device)
+        self._warm = self._warm or default
 
     def prepare_phi(self, N: int, phi: torch.Tensor):
         # on a stream of its own (preparation waits for its own work only); the
@@ -581,11 +597,15 @@ class GpuBackend:
         return nrhs * C * (6 * 16 * H + 10 * 8 * P)
 
     def launch_wf(self, req: WFRequest, phi: torch.Tensor):
-        """Start the solve as a process (wiener_filter_process) on the lane's
-        own stream; ready() advances it whenever the event it waits on has
-        completed, so the host never blocks on it."""
+        """Start the solve (wiener_filter_process) on a host thread of its own
+        that drives the lane's stream: the scheduler thread never spends time
+        enqueueing a solve's ~170 launches per CG iteration, so LenseFlow
+        batches keep being formed and launched while it runs (the library's
+        calls and torch's launches release the interpreter lock)."""
         from .fields import FlatS0
         from .wiener_filter import wiener_filter_process
+        if not self._warm:
+            self.warmup()
         if self._anchor_ev is None:
             self.start_clock()
         if self._wf_stream is None:
@@ -594,27 +614,33 @@ class GpuBackend:
         Cf, Cn, M, B = req.ops
         phif = FlatS0(scalar_matrix=phi, theta_pix=self.theta_pix)
         tok = _WFToken(wiener_filter_process(req.data * 0, phif, req.data, Cf, Cn, M, B, n=req.n,
-                                             tol=0.0, rtol=req.rtol, recompute_every=10,
+                                             tol=0.0, rtol=req.rtol, recompute_every=20,
                                              maxiter=req.maxiter, return_info=True))
-        self._advance(tok)
+        tok.thread = threading.Thread(target=self._run_wf, args=(tok,), daemon=True)
+        self._wf_threads = [t for t in getattr(self, "_wf_threads", []) if t.is_alive()]
+        self._wf_threads.append(tok.thread)
+        # the lane's thread and the scheduler share the interpreter lock; a
+        # short switch interval (0.2 ms instead of CPython's 5 ms) bounds how
+        # long either waits for it while both run Python between launches
+        if self._switch_interval is None:
+            self._switch_interval = sys.getswitchinterval()
+            sys.setswitchinterval(min(self._switch_interval, 2e-4))
+        tok.thread.start()
         return tok
 
-    def _advance(self, tok: "_WFToken"):
-        """Run the process until it waits on work that has not finished."""
+    def _run_wf(self, tok: "_WFToken"):
+        """The lane's thread: run the process to completion on the lane's
+        stream, then record the end event (read by finish_time)."""
+        from .wiener_filter import run_process
         with torch.cuda.device(self.device), torch.cuda.stream(self._wf_stream):
-            while not tok.finished and (tok.event is None or tok.event.query()):
-                try:
-                    tok.event = tok.gen.send(None) if tok.started else next(tok.gen)
-                    tok.started = True
-                except StopIteration as stop:
-                    tok.result = stop.value
-                    tok.finished = True
-                except Exception as e:  # reported on the request; the server keeps running
-                    tok.error = f"{type(e).__name__}: {e}"
-                    tok.finished = True
-                if tok.finished:
-                    tok.end = torch.cuda.Event(enable_timing=True)
-                    tok.end.record(self._wf_stream)
+            try:
+                tok.result = run_process(tok.gen)
+            except Exception as e:  # reported on the request; the server keeps running
+                tok.error = f"{type(e).__name__}: {e}"
+            end = torch.cuda.Event(enable_timing=True)
+            end.record(self._wf_stream)
+        tok.end = end
+        tok.finished = True
 
     def wf_result(self, token, req: WFRequest) -> Optional[str]:
         if token.error is None:
@@ -623,7 +649,6 @@ class GpuBackend:
 
     def ready(self, token) -> bool:
         if isinstance(token, _WFToken):
-            self._advance(token)
             return token.finished and token.end.query()
         return token.query()
 
@@ -636,6 +661,11 @@ class GpuBackend:
         torch.cuda.synchronize(self.device)
 
     def close(self):
+        for t in getattr(self, "_wf_threads", []):
+            t.join()
+        if self._switch_interval is not None:
+            sys.setswitchinterval(self._switch_interval)
+            self._switch_interval = None
         torch.cuda.synchronize(self.device)
         for ex in self.execs.values():
             ex.close()

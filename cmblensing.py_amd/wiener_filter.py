@@ -144,6 +144,14 @@ def A_matrix_operator(field, phi, data, field_covariance, noise_covariance, mask
     return unstack_matrices(field, prob.A(X.to(prob.cdt)), lead)
 
 
+def _cg_step(prob, x, r, z, p, Ap, pinv_arr, p_op, res_d, act_d, bb_d, rel_d, B):
+    """One PCG iteration after the operator on the device (cmbl_cg_step):
+    res_d is updated in place."""
+    V = torch.view_as_real
+    _lib.cg_step(prob.plan, V(x), V(r), V(z), V(p), V(Ap), pinv_arr, V(p_op), res_d, act_d, bb_d,
+                 rel_d, B, prob.C)
+
+
 def _host(t: torch.Tensor):
     """Wait point of a solve process: yields an event recorded after the work
     that produces t, then returns t on the host as float64."""
@@ -272,19 +280,10 @@ def conjugate_gradient_process(x0, data, phi, mask, beam, noise_covariance, fiel
         pending = None
         while True:
             Ap = prob.A(p_op)                                 # :82
-            pAp = prob.dot(p, Ap)
-            alpha = torch.where(act_d, res_d / torch.where(pAp != 0, pAp, torch.ones_like(pAp)),
-                                torch.zeros_like(res_d))      # :84
-            rz = torch.empty(B, dtype=torch.float64, device=dev)
-            _lib.cg_update_xrz(prob.plan, torch.view_as_real(x), torch.view_as_real(r),
-                               torch.view_as_real(z), torch.view_as_real(p),
-                               torch.view_as_real(Ap), pinv_arr, alpha, B, prob.C, rz)  # :86-92
-            beta = torch.where(act_d, rz / torch.where(res_d != 0, res_d, torch.ones_like(res_d)),
-                               torch.zeros_like(res_d))       # :94
-            _lib.cg_update_p(prob.plan, torch.view_as_real(p), torch.view_as_real(z), beta,
-                             torch.view_as_real(p_op), B, prob.C)              # :95
-            cur_d = torch.where(act_d, rz, cur_d)
-            res_d = torch.where(act_d, rz, res_d)
+            # :84-95 in one call: alpha, x, r, z, dot(r, z), beta, p, and
+            # res = dot(r, z) for the active right-hand sides
+            _cg_step(prob, x, r, z, p, Ap, pinv_arr, p_op, res_d, act_d, None, None, B)
+            cur_d = res_d.clone()
             k_d = k_d + act_d.long()
             hist.append((act_d, cur_d))
             act_d = (cur_d >= tol) & (k_d < maxiter)          # :75-77
@@ -317,32 +316,40 @@ def conjugate_gradient_process(x0, data, phi, mask, beam, noise_covariance, fiel
     def rel_dev():
         return torch.sqrt(prob.dot(r, r) / bb_d)
 
+    def record(a_row, cur_row, rel_row):
+        for i in range(B):
+            if bool(a_row[i]):
+                info.res_history[i].append(float(cur_row[i]))
+                if rel_row is not None:
+                    info.rel_history[i].append(float(rel_row[i]))
+
+    # the host's bookkeeping of an iteration (histories) is done while the
+    # device runs the next one, after that one has been enqueued
+    pending = None
+    res_d = res_h.to(dev)
+    rel_d = torch.empty(B, dtype=torch.float64, device=dev)
     while bool(act.any()):
         idx = None if (bool(act.all()) or not COMPACT_ACTIVE) else \
             [i for i in range(B) if bool(act[i])]
         act_d = act.to(dev)
-        res_d = res_h.to(dev)
         Ap = prob.A_active(p_op, idx)                         # :82
-        pAp = prob.dot(p, Ap)
-        alpha = torch.where(act_d, res_d / torch.where(pAp != 0, pAp, torch.ones_like(pAp)),
-                            torch.zeros_like(res_d))          # :84
-        rz = torch.empty(B, dtype=torch.float64, device=dev)
-        _lib.cg_update_xrz(prob.plan, torch.view_as_real(x), torch.view_as_real(r),
-                           torch.view_as_real(z), torch.view_as_real(p), torch.view_as_real(Ap),
-                           pinv_arr, alpha, B, prob.C, rz)   # :86-92
-        beta = torch.where(act_d, rz / torch.where(res_d != 0, res_d, torch.ones_like(res_d)),
-                           torch.zeros_like(res_d))           # :94
-        _lib.cg_update_p(prob.plan, torch.view_as_real(p), torch.view_as_real(z), beta,
-                         torch.view_as_real(p_op), B, prob.C)            # :95
+        # :84-95 in one call (alpha, x, r, z, dot(r, z), beta, p, res), with
+        # sqrt(dot(r, r) / dot(b, b)) for the relative test
+        _cg_step(prob, x, r, z, p, Ap, pinv_arr, p_op, res_d, act_d,
+                 bb_d if rtol is not None else None, rel_d if rtol is not None else None, B)
+        rel_cur = rel_d
         for i in (range(B) if idx is None else idx):
             true_r[i] = False
-        res_d = torch.where(act_d, rz, res_d)
         k += act.long()
         it += 1
         if recompute_every and it % recompute_every == 0:
             res_d = torch.where(act_d, residual(x, idx), res_d)
+            rel_cur = rel_dev() if rtol is not None else None
+        if pending is not None:
+            record(*pending)
+            pending = None
         if rtol is not None:
-            res_h, rel = (yield from _host(torch.stack([res_d, rel_dev()])))
+            res_h, rel = (yield from _host(torch.stack([res_d, rel_cur])))
             # before a right-hand side stops on the relative test, replace its
             # recursive residual by the recomputed b - A x (FP64 arithmetic);
             # it stops only if that one passes too
@@ -352,12 +359,10 @@ def conjugate_gradient_process(x0, data, phi, mask, beam, noise_covariance, fiel
         else:
             res_h = yield from _host(res_d)
         res_curr = torch.where(act, res_h, res_curr)
-        for i in range(B):
-            if bool(act[i]):
-                info.res_history[i].append(float(res_curr[i]))
-                if rel is not None:
-                    info.rel_history[i].append(float(rel[i]))
+        pending = (act, res_curr, rel)
         act = active_mask()
+    if pending is not None:
+        record(*pending)
 
     info.iterations = [int(v) for v in k]
     conv = res_curr < tol

@@ -217,6 +217,21 @@ cmbl_status cmbl_cg_update_xrz(cmbl_plan* plan, void* x, void* r, void* z, const
 cmbl_status cmbl_cg_update_p(cmbl_plan* plan, void* p, const void* z, const double* beta,
                              void* p_op, int nrhs, int ncomp, void* stream);
 
+/* One PCG iteration after the operator (wiener_filter.py:84-95) with every
+ * scalar kept on the device, for nrhs systems (FP64 vectors as above):
+ *   pAp = dot(p, Ap) ; alpha = res / (pAp != 0 ? pAp : 1)
+ *   x += alpha p ; r -= alpha Ap ; z = pinv .* r ; rz = dot(r, z)
+ *   beta = rz / (res != 0 ? res : 1) ; p = z + beta p (p_op as in cmbl_cg_update_p)
+ *   res = rz                                  (res: device array, updated in place)
+ * for the right-hand sides with act[k] != 0 (act: device bytes, NULL = all);
+ * the others get alpha = beta = 0 and keep res.  With rel (and bb, device
+ * arrays) also rel = sqrt(dot(r, r) / bb).  The same float64 operations as
+ * cmbl_dot_f64 + cmbl_cg_update_xrz + cmbl_cg_update_p with the scalars
+ * formed on the host, in one call. */
+cmbl_status cmbl_cg_step(cmbl_plan* plan, void* x, void* r, void* z, void* p, const void* Ap,
+                         const double* pinv, void* p_op, double* res, const unsigned char* act,
+                         const double* bb, double* rel, int nrhs, int ncomp, void* stream);
+
 /* ---- serving executor ------------------------------------------------------
  * No counterpart in the reference (it serves one call at a time): a batched
  * LenseFlow of fixed capacity (max_batch requests of ncomp components, nsteps,

@@ -197,7 +197,7 @@ def test_server_wiener_filter_requests(pkg, srv, sims, dev):
         wf_reqs.append(srv.WFRequest(data=data, phi_id=k, ops=ops, n=NSTEPS, rtol=1e-6))
         phif = pkg.FlatS0(scalar_matrix=phi_maps[(k, N)], theta_pix=th)
         direct.append(wfm.wiener_filter(data * 0, phif, data, *ops, n=NSTEPS, tol=0.0, rtol=1e-6,
-                                        recompute_every=10, maxiter=500))
+                                        recompute_every=20, maxiter=500))
     be = srv.GpuBackend(dev, sizes=sizes, nstreams=2, max_batch={64: 16, 128: 16})
     server = srv.Server(be, lambda k, N_: phi_maps[(k, N_)], hbm_budget=8 * srv.GiB)
     allreq = reqs[:40] + wf_reqs + reqs[40:]

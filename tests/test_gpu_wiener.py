@@ -360,7 +360,7 @@ def test_wiener_filter_qu512_8rhs_holes_vs_oracle(pkg, dev, sims):
     p0 = probs[0]
     M = mo.PixelMask(matrix=torch.as_tensor(p0["mask_pix"], device=dev))
     x, info = wfm.wiener_filter(x0, phi, data, op(p0["Cf"]), op(p0["Cn"]), M, op(p0["B"]), n=n,
-                                tol=0.0, rtol=1e-6, recompute_every=10, maxiter=1000,
+                                tol=0.0, rtol=1e-6, recompute_every=20, maxiter=1000,
                                 return_info=True)
     print("iterations", info.iterations, "true rel residual",
           [f"{v:.2e}" for v in info.final_true_rel_residual], flush=True)
