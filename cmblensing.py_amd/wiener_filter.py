@@ -253,10 +253,15 @@ def conjugate_gradient_process(x0, data, phi, mask, beam, noise_covariance, fiel
 
     rel = (yield from rel_now()) if rtol is not None else None
 
+    # right-hand sides whose recursive residual has passed the relative test
+    # and that wait, paused (alpha = beta = 0, no iteration counted), for the
+    # FP64 recomputation that confirms or reactivates them
+    verify = torch.zeros(B, dtype=torch.bool)
+
     def active_mask():
         act = (res_curr >= tol) & (k < maxiter)               # :75-77
         if rtol is not None:
-            act &= rel >= rtol
+            act &= (rel >= rtol) & ~verify
         return act
 
     info = CGInfo(iterations=[0] * B, converged=[False] * B,
@@ -348,17 +353,29 @@ def conjugate_gradient_process(x0, data, phi, mask, beam, noise_covariance, fiel
         if pending is not None:
             record(*pending)
             pending = None
+        upd = act
         if rtol is not None:
             res_h, rel = (yield from _host(torch.stack([res_d, rel_cur])))
-            # before a right-hand side stops on the relative test, replace its
-            # recursive residual by the recomputed b - A x (FP64 arithmetic);
-            # it stops only if that one passes too
-            if bool((act & (rel < rtol)).any()):
-                res_d = torch.where(act_d, residual(x, idx), res_d)
+            # before a right-hand side stops on the relative test, its
+            # recursive residual is replaced by the recomputed b - A x (FP64
+            # arithmetic), and it stops only if that one passes too.  The
+            # recomputation is batched: a crossing right-hand side pauses
+            # until half the batch waits or none is left iterating, and only
+            # the waiting ones are recomputed (a paused right-hand side's
+            # iterates do not change, so this only saves FP64 operator
+            # applications).
+            verify |= act & (rel < rtol)
+            going = act & ~verify & (res_h >= tol) & (k < maxiter)
+            if bool(verify.any()) and (not bool(going.any())
+                                       or int(verify.sum()) >= max(1, B // 2)):
+                vidx = [i for i in range(B) if bool(verify[i])]
+                res_d = torch.where(verify.to(dev), residual(x, vidx), res_d)
                 res_h, rel = (yield from _host(torch.stack([res_d, rel_dev()])))
+                upd = act | verify
+                verify[:] = False
         else:
             res_h = yield from _host(res_d)
-        res_curr = torch.where(act, res_h, res_curr)
+        res_curr = torch.where(upd, res_h, res_curr)
         pending = (act, res_curr, rel)
         act = active_mask()
     if pending is not None:
