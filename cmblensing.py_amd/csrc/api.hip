@@ -1035,31 +1035,39 @@ cmbl_status run_gradf(cmbl_plan* p, const cmbl_phi* const* phis, const void* f, 
     // 2. g = B M Cn^-1 (d - M B L f), as eb2qu(g) in X2
     HIP_TRY(cmbl::launch_wf_data<T>(f ? X2 : nullptr, (const cplx<T>*)d, (const T*)ops->mb,
                                     (const T*)ops->bmcn, X2, N, C, batch, s));
+    if (cot_out) return cmbl_irfft2(p, X2, cot_out, (int)nm, st);
+    // 3. adjoint LenseFlow of map(eb2qu(g))
+    CMBL_TRY(cmbl_irfft2(p, X2, Y2, (int)nm, st));
   } else {
     // 2'. pixel mask: r = d - M map(B L f) in pixel space, then
-    //     g = B fourier(M map(Cn^-1 fourier(r)))
+    //     g = B fourier(M map(Cn^-1 fourier(r))).  Each map -> Fourier
+    //     operator -> map step is a row R2C, one column pass applying the
+    //     operator between the column FFTs (k_col_op), and a row C2R that
+    //     also applies the pixel-space step that follows.
+    cplx<T>* S = (cplx<T>*)p->s0;
+    const cplx<T>* tw = (const cplx<T>*)p->tw;
+    auto op_chain = [&](const T* x, const T* diag, T* y, const T* dd, int ep) -> hipError_t {
+      hipError_t e = cmbl::launch_lf_row_r2c<T>(N, x, S, nm, tw, s);
+      if (e == hipSuccess) e = cmbl::launch_col_op<T>(N, S, diag, batch, C, tw, s);
+      if (e == hipSuccess) e = cmbl::launch_row_c2r<T>(N, S, y, nm, tw, dd, mask, ep, s);
+      return e;
+    };
     if (f) {
-      HIP_TRY(cmbl::launch_rotate<T>(X2, X2, nullptr, N, C, batch, 0, s));
-      HIP_TRY(cmbl::launch_rotate<T>(X2, X2, (const T*)ops->beam, N, C, batch, 1, s));
-      CMBL_TRY(cmbl_irfft2(p, X2, Y2, (int)nm, st));
-      HIP_TRY(cmbl::launch_pixel_resid<T>((const T*)d, mask, Y2, NN, nm, s));
+      HIP_TRY(op_chain(Ylens, (const T*)ops->beam, Y2, (const T*)d, 2));
     } else if (d) {
       HIP_TRY(hipMemcpyAsync(Y2, d, nm * NN * sizeof(T), hipMemcpyDeviceToDevice, s));
     } else {
       HIP_TRY(cmbl::launch_fill_zero<T>(Y2, nm * NN, s));
     }
-    CMBL_TRY(cmbl_rfft2(p, Y2, X2, (int)nm, st));
-    HIP_TRY(cmbl::launch_rotate<T>(X2, X2, nullptr, N, C, batch, 0, s));
-    HIP_TRY(cmbl::launch_rotate<T>(X2, X2, (const T*)ops->cninv, N, C, batch, 1, s));
-    CMBL_TRY(cmbl_irfft2(p, X2, Y2, (int)nm, st));
-    HIP_TRY(cmbl::launch_pixel_mul<T>(Y2, mask, NN, nm, s));
-    CMBL_TRY(cmbl_rfft2(p, Y2, X2, (int)nm, st));
-    HIP_TRY(cmbl::launch_rotate<T>(X2, X2, nullptr, N, C, batch, 0, s));
-    HIP_TRY(cmbl::launch_rotate<T>(X2, X2, (const T*)ops->beam, N, C, batch, 1, s));
+    HIP_TRY(op_chain(Y2, (const T*)ops->cninv, Y2, nullptr, 1));
+    // the beam, to the cotangent maps or into the adjoint LenseFlow's input
+    HIP_TRY(cmbl::launch_lf_row_r2c<T>(N, Y2, S, nm, tw, s));
+    HIP_TRY(cmbl::launch_col_op<T>(N, S, (const T*)ops->beam, batch, C, tw, s));
+    HIP_TRY(cmbl::launch_row_c2r<T>(N, S, cot_out ? (T*)cot_out : Y2, nm, tw, nullptr, nullptr, 0,
+                                    s));
+    if (cot_out) return CMBL_OK;
   }
-  if (cot_out) return cmbl_irfft2(p, X2, cot_out, (int)nm, st);
   // 3. adjoint LenseFlow of map(eb2qu(g))
-  CMBL_TRY(cmbl_irfft2(p, X2, Y2, (int)nm, st));
   CMBL_TRY(cmbl_lenseflow(p, phis, Y2, Y2, batch, C, n, CMBL_FORWARD_LENSE, 1, st));
   CMBL_TRY(cmbl_rfft2(p, Y2, X2, (int)nm, st));
   // 4. out = qu2eb(fourier(L^T g)) - Cf^-1 qu2eb(fourier(map(eb2qu f))); the

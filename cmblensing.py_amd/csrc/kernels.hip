@@ -233,9 +233,15 @@ __global__ __launch_bounds__(256) void k_row_r2c_gather(const T* const* __restri
 }
 
 // Row C2R of row pairs: packed P[nrows][N/2] -> x[nrows][N] (no scaling).
-template <typename T, int N>
+// EP, a pixel-space epilogue with w [N][N] shared by all maps:
+//   0: x;  1: x = w .* x;  2: x = d - w .* x  (d NULL: zeros)
+// (the pixel mask's product and the data residual of the pixel-mask
+// posterior, applied as the rows are stored).
+template <typename T, int N, int EP = 0>
 __global__ __launch_bounds__(256) void k_row_c2r(const cplx<T>* __restrict__ P, T* __restrict__ x,
-                                                 const cplx<T>* __restrict__ twg, long npairs) {
+                                                 const cplx<T>* __restrict__ twg, long npairs,
+                                                 const T* __restrict__ d = nullptr,
+                                                 const T* __restrict__ w = nullptr) {
   constexpr int G = N / 8, PAIRS = 256 / G, H = N / 2;
   __shared__ cplx<T> lbuf[PAIRS][lds_len<N>()];
   const int grp = threadIdx.x / G, t = threadIdx.x % G;
@@ -252,6 +258,22 @@ __global__ __launch_bounds__(256) void k_row_c2r(const cplx<T>* __restrict__ P, 
   if (!active) return;
   T* xa = x + (2 * pp) * N;
   T* xb = xa + N;
+  if constexpr (EP != 0) {
+    const T* wa = w + ((2 * pp) % N) * N;
+    const T* da = (EP == 2 && d) ? d + (2 * pp) * N : nullptr;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int i = t + m * G;
+      T a = v[0][m].x * wa[i], b = v[0][m].y * wa[N + i];
+      if constexpr (EP == 2) {
+        a = (da ? da[i] : T(0)) - a;
+        b = (da ? da[N + i] : T(0)) - b;
+      }
+      xa[i] = a;
+      xb[i] = b;
+    }
+    return;
+  }
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     xa[t + m * G] = v[0][m].x;
@@ -1055,6 +1077,23 @@ hipError_t launch_lf_row_r2c(int N, const T* y, cplx<T>* S0, long nmaps, const c
 }
 
 template <typename T>
+hipError_t launch_row_c2r(int N, const cplx<T>* P, T* x, long nmaps, const cplx<T>* tw,
+                          const T* d, const T* w, int ep, hipStream_t s) {
+  if (ep < 0 || ep > 2 || (ep != 0 && !w)) return hipErrorInvalidValue;
+  CMBL_DISPATCH_N(N, {
+    const long npairs = nmaps * NN_ / 2;
+    const unsigned nb = blocks_for(npairs, 256 / (NN_ / 8));
+    if (ep == 0)
+      k_row_c2r<T, NN_, 0><<<nb, 256, 0, s>>>(P, x, tw, npairs);
+    else if (ep == 1)
+      k_row_c2r<T, NN_, 1><<<nb, 256, 0, s>>>(P, x, tw, npairs, d, w);
+    else
+      k_row_c2r<T, NN_, 2><<<nb, 256, 0, s>>>(P, x, tw, npairs, d, w);
+  });
+  return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_lf_row_r2c_gather(int N, const T* const* src, int ncomp, T* ystage, cplx<T>* S0,
                                     long nmaps, const cplx<T>* tw, hipStream_t s) {
   CMBL_DISPATCH_N(N, {
@@ -1207,6 +1246,8 @@ hipError_t launch_scatter(const T* src, T* const* dst, long len, int count, hipS
   template hipError_t launch_lf_row<T>(int, bool, cplx<T>*, cplx<T>*, T*, T*, const T* const*,  \
                                        const cplx<T>*, int, long, int, T, int, int, int, T,     \
                                        hipStream_t, T*, const GradArgs<T>*, T* const*);         \
+  template hipError_t launch_row_c2r<T>(int, const cplx<T>*, T*, long, const cplx<T>*, const T*, \
+                                        const T*, int, hipStream_t);                            \
   template hipError_t launch_lf_row_r2c_gather<T>(int, const T* const*, int, T*, cplx<T>*,      \
                                                   long, const cplx<T>*, hipStream_t);           \
   template hipError_t launch_gtab<T>(int, const double*, T*, int, hipStream_t);                 \

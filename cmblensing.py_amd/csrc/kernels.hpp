@@ -30,6 +30,12 @@ hipError_t launch_ptab(int N, const double* D, T* ptab, int nsteps, double* minp
 template <typename T>
 hipError_t launch_lf_row_r2c(int N, const T* y, cplx<T>* S0, long nmaps, const cplx<T>* tw,
                              hipStream_t s);
+// Row C2R of packed row spectra P [nmaps][N][N/2] -> x [nmaps][N][N] (no
+// scaling), with a pixel epilogue, w [N][N] for every map:
+// ep 0: x; 1: w .* x; 2: d - w .* x (d NULL: zeros).
+template <typename T>
+hipError_t launch_row_c2r(int N, const cplx<T>* P, T* x, long nmaps, const cplx<T>* tw,
+                          const T* d, const T* w, int ep, hipStream_t s);
 // x-derivative column pass (src may alias dst).
 template <typename T>
 hipError_t launch_lf_col(int N, const cplx<T>* src, cplx<T>* dst, long nmaps, const cplx<T>* tw,

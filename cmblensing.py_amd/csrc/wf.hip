@@ -279,29 +279,10 @@ __global__ void k_wf_final_teb(const cplx<T>* XLT, const cplx<T>* XLP,
   }
 }
 
-// Pixel-space multiply x[m][p] *= w[p] (a pixel mask shared by all maps).
-template <typename T>
-__global__ void k_pixel_mul(T* x, const T* __restrict__ w, long npix, long nmaps) {
-  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < nmaps * npix;
-       g += (long)gridDim.x * blockDim.x)
-    x[g] *= w[g % npix];
-}
-
 template <typename T>
 __global__ void k_fill_zero(T* x, long n) {
   for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x)
     x[g] = T(0);
-}
-
-// residual in pixel space: r = d - w .* x
-template <typename T>
-__global__ void k_pixel_resid(const T* __restrict__ d, const T* __restrict__ w, T* x, long npix,
-                              long nmaps) {
-  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < nmaps * npix;
-       g += (long)gridDim.x * blockDim.x) {
-    const T v = x[g] * w[g % npix];
-    x[g] = (d ? d[g] : T(0)) - v;
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -481,6 +462,155 @@ __global__ void k_cg_rel(const double* __restrict__ rr, const double* __restrict
   rel[k] = sqrt(rr[k] / bb[k]);
 }
 
+// A Fourier-diagonal operator applied between two basis changes, in one
+// column pass: for packed row spectra P [nfields][C][N][N/2] (after a row R2C,
+// k_row_r2c), the column FFT, per mode (kx, ky)
+//   C = 2:  (Q, U) -> eb2qu(diag .* qu2eb(Q, U))     (util.py:140-161)
+//   C = 1:  T -> diag .* T
+// (matrix_operators.py:54-90), then the inverse column FFT times scale_ (1/N^2),
+// in place; a row C2R (k_row_c2r) then gives irfft2(op(rfft2(x))) with numpy's
+// conventions.  It replaces rfft2 -> two rotations -> irfft2 (six passes over
+// the data) by one column pass between the row passes.  The packed DC|Nyquist
+// column (k = 0) is unpacked into the ky = 0 and ky = N/2 modes, transformed,
+// and repacked as Herm(Y0) + i Herm(Yn) -- the parts the row C2R keeps -- as the
+// generic basis-change passes do; the workgroups of tile 0 hold that column and
+// take the extra LDS exchanges (uniformly over their threads).
+template <typename T, int N, int KC, int C>
+__global__ __launch_bounds__(KC * N / 8) void k_col_op(cplx<T>* P, const T* __restrict__ diag,
+                                                       const cplx<T>* __restrict__ twg,
+                                                       T scale_) {
+  constexpr int G = N / 8, H = N / 2, TILES = H / KC;
+  constexpr long PN = (long)N * (H + 1);  // natural half-plane size (diag planes)
+  __shared__ cplx<T> lbuf[C][KC][line_stride<N>()];
+  const int g = threadIdx.x % KC, t = threadIdx.x / KC;
+  Twiddles<N, T> tw;
+  tw.load(twg, t);
+  const long f = blockIdx.x / TILES;
+  const int tile = blockIdx.x % TILES;
+  const int k = tile * KC + g;
+  cplx<T>* Pc[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) Pc[c] = P + (f * C + c) * (long)N * H;
+  cplx<T> v[C][8];
+#pragma unroll
+  for (int c = 0; c < C; ++c)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[c][q] = Pc[c][(long)(t + q * G) * H + k];
+  __syncthreads();
+  cplx<T>* bufs[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) bufs[c] = lbuf[c][g];
+  cplx<T>* const(&cb)[C] = bufs;
+  fft_lines<N, false, C, false, false>(v, cb, tw, t);
+  // the operator at mode (kx, ky) on the C values a[c]
+  auto op = [&](int kx, int ky, cplx<T>(&a)[C]) {
+    const long idx = (long)kx * (H + 1) + ky;
+    if constexpr (C == 1) {
+      a[0] = scale(a[0], diag[idx]);
+    } else {
+      T cs, sn;
+      rot2psi<T>(N, idx, cs, sn);
+      cplx<T> e, b;
+      qu2eb_mode(a[0], a[1], cs, sn, e, b);
+      e = scale(e, diag[idx]);
+      b = scale(b, diag[PN + idx]);
+      eb2qu_mode(e, b, cs, sn, a[0], a[1]);
+    }
+  };
+  if (tile != 0) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      cplx<T> a[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) a[c] = v[c][q];
+      op(t + q * G, k, a);
+#pragma unroll
+      for (int c = 0; c < C; ++c) v[c][q] = a[c];
+    }
+  } else {
+    // uniform over the workgroup: g == 0 holds the packed column
+    const T h = T(0.5);
+    cplx<T> y0[C][8], yn[C][8];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) lbuf[c][g][lds_idx<N, false>(t + q * G)] = v[c][q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int kx = t + q * G;
+      if (k == 0) {
+        cplx<T> a0[C], an[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          const cplx<T> z = v[c][q];
+          const cplx<T> zc = conj(lbuf[c][g][lds_idx<N, false>((N - kx) & (N - 1))]);
+          a0[c] = scale(z + zc, h);
+          const cplx<T> d = z - zc;
+          an[c] = mk(h * d.y, -h * d.x);
+        }
+        op(kx, 0, a0);
+        op(kx, H, an);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          y0[c][q] = a0[c];
+          yn[c][q] = an[c];
+        }
+      } else {
+        cplx<T> a[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) a[c] = v[c][q];
+        op(kx, k, a);
+#pragma unroll
+        for (int c = 0; c < C; ++c) v[c][q] = a[c];
+      }
+    }
+    // repack: Herm(Y0) + i Herm(Yn), Herm(Y)[kx] = (Y[kx] + conj(Y[-kx])) / 2
+    __syncthreads();
+    if (k == 0) {
+#pragma unroll
+      for (int c = 0; c < C; ++c)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) lbuf[c][g][lds_idx<N, false>(t + q * G)] = y0[c][q];
+    }
+    __syncthreads();
+    if (k == 0) {
+#pragma unroll
+      for (int c = 0; c < C; ++c)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int kx = t + q * G;
+          y0[c][q] = scale(y0[c][q] + conj(lbuf[c][g][lds_idx<N, false>((N - kx) & (N - 1))]), h);
+        }
+    }
+    __syncthreads();
+    if (k == 0) {
+#pragma unroll
+      for (int c = 0; c < C; ++c)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) lbuf[c][g][lds_idx<N, false>(t + q * G)] = yn[c][q];
+    }
+    __syncthreads();
+    if (k == 0) {
+#pragma unroll
+      for (int c = 0; c < C; ++c)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int kx = t + q * G;
+          const cplx<T> n0 =
+              scale(yn[c][q] + conj(lbuf[c][g][lds_idx<N, false>((N - kx) & (N - 1))]), h);
+          v[c][q] = mk(y0[c][q].x - n0.y, y0[c][q].y + n0.x);
+        }
+    }
+  }
+  fft_lines<N, true, C, false, false>(v, cb, tw, t);
+#pragma unroll
+  for (int c = 0; c < C; ++c)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) Pc[c][(long)(t + q * G) * H + k] = scale(v[c][q], scale_);
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -514,23 +644,11 @@ hipError_t launch_wf_final(const cplx<T>* XL, const cplx<T>* F, const T* cfinv, 
 }
 
 template <typename T>
-hipError_t launch_pixel_mul(T* x, const T* w, long npix, long nmaps, hipStream_t s) {
-  k_pixel_mul<T><<<grid_for(npix * nmaps), 256, 0, s>>>(x, w, npix, nmaps);
-  return hipGetLastError();
-}
-
-template <typename T>
 hipError_t launch_fill_zero(T* x, long n, hipStream_t s) {
   k_fill_zero<T><<<grid_for(n), 256, 0, s>>>(x, n);
   return hipGetLastError();
 }
 
-template <typename T>
-hipError_t launch_pixel_resid(const T* d, const T* w, T* x, long npix, long nmaps,
-                              hipStream_t s) {
-  k_pixel_resid<T><<<grid_for(npix * nmaps), 256, 0, s>>>(d, w, x, npix, nmaps);
-  return hipGetLastError();
-}
 
 template <typename Tb>
 hipError_t launch_dot(const cplx<double>* a, const cplx<Tb>* b, int N, long len, int nrhs,
@@ -588,6 +706,46 @@ hipError_t launch_wf_final_teb(const cplx<T>* XLT, const cplx<T>* XLP, const cpl
   return hipGetLastError();
 }
 
+// Columns per k_col_op workgroup: 128-byte row segments (16 float / 8 double
+// complex), halved for Q/U pairs at N = 512, where two 16-column sets of line
+// buffers would not fit the LDS.
+template <typename T, int N, int C>
+constexpr int col_op_kc() {
+  constexpr int base = sizeof(T) == 4 ? 16 : 8;
+  constexpr int kc = (C == 2 && N >= 512) ? base / 2 : base;
+  return kc < N / 2 ? kc : N / 2;
+}
+
+template <typename T>
+hipError_t launch_col_op(int N, cplx<T>* P, const T* diag, long nfields, int C,
+                         const cplx<T>* tw, hipStream_t s) {
+#define CMBL_COL_OP(NN_)                                                                  \
+  case NN_: {                                                                             \
+    const T sc = T(1) / (T(NN_) * T(NN_));                                                \
+    if (C == 2) {                                                                         \
+      constexpr int KC = col_op_kc<T, NN_, 2>();                                          \
+      k_col_op<T, NN_, KC, 2><<<(unsigned)(nfields * (NN_ / 2 / KC)), KC * NN_ / 8, 0, s>>>( \
+          P, diag, tw, sc);                                                               \
+    } else {                                                                              \
+      constexpr int KC = col_op_kc<T, NN_, 1>();                                          \
+      k_col_op<T, NN_, KC, 1><<<(unsigned)(nfields * (NN_ / 2 / KC)), KC * NN_ / 8, 0, s>>>( \
+          P, diag, tw, sc);                                                               \
+    }                                                                                     \
+  } break;
+  switch (N) {
+    CMBL_COL_OP(16)
+    CMBL_COL_OP(32)
+    CMBL_COL_OP(64)
+    CMBL_COL_OP(128)
+    CMBL_COL_OP(256)
+    CMBL_COL_OP(512)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef CMBL_COL_OP
+  return hipGetLastError();
+}
+
 hipError_t launch_cg_alpha(const double* res, const double* pAp, const unsigned char* act,
                            double* alpha, int nrhs, hipStream_t s) {
   k_cg_alpha<<<(nrhs + 63) / 64, 64, 0, s>>>(res, pAp, act, alpha, nrhs);
@@ -611,13 +769,13 @@ int red_parts() { return kRedParts; }
 #define CMBL_WF_INST(T)                                                                         \
   template hipError_t launch_rotate<T>(const cplx<T>*, cplx<T>*, const T*, int, int, long, int, \
                                        hipStream_t);                                            \
+  template hipError_t launch_col_op<T>(int, cplx<T>*, const T*, long, int, const cplx<T>*,       \
+                                       hipStream_t);                                            \
   template hipError_t launch_wf_data<T>(const cplx<T>*, const cplx<T>*, const T*, const T*,     \
                                         cplx<T>*, int, int, long, hipStream_t);                 \
   template hipError_t launch_wf_final<T>(const cplx<T>*, const cplx<T>*, const T*, cplx<T>*,    \
                                          int, int, long, hipStream_t);                          \
-  template hipError_t launch_pixel_mul<T>(T*, const T*, long, long, hipStream_t);               \
   template hipError_t launch_fill_zero<T>(T*, long, hipStream_t);                               \
-  template hipError_t launch_pixel_resid<T>(const T*, const T*, T*, long, long, hipStream_t);   \
   template hipError_t launch_dot<T>(const cplx<double>*, const cplx<T>*, int, long, int,        \
                                     double*, double*, hipStream_t);                             \
   template hipError_t launch_cg_xrz<T>(cplx<double>*, cplx<double>*, cplx<double>*,             \

@@ -16,11 +16,7 @@ template <typename T>
 hipError_t launch_wf_final(const cplx<T>* XL, const cplx<T>* XF, const T* cfinv, cplx<T>* out,
                            int N, int C, long nfields, hipStream_t s);
 template <typename T>
-hipError_t launch_pixel_mul(T* x, const T* w, long npix, long nmaps, hipStream_t s);
-template <typename T>
 hipError_t launch_fill_zero(T* x, long n, hipStream_t s);
-template <typename T>
-hipError_t launch_pixel_resid(const T* d, const T* w, T* x, long npix, long nmaps, hipStream_t s);
 template <typename Tb>
 hipError_t launch_dot(const cplx<double>* a, const cplx<Tb>* b, int N, long len, int nrhs,
                       double* part, double* out, hipStream_t s);
@@ -42,6 +38,13 @@ hipError_t launch_wf_data_teb(const cplx<T>* XT, const cplx<T>* XP, const cplx<T
 template <typename T>
 hipError_t launch_wf_final_teb(const cplx<T>* XLT, const cplx<T>* XLP, const cplx<T>* F,
                                const T* cfinv, cplx<T>* out, int N, long nfields, hipStream_t s);
+// In place on packed row spectra P [nfields][C][N][N/2] (a row R2C of the
+// maps): column FFT, per mode C = 2: eb2qu(diag .* qu2eb(.)), C = 1:
+// diag .* (.), inverse column FFT, times 1/N^2; a row C2R then gives
+// irfft2(op(rfft2(x))).  diag [C][N][N/2+1].
+template <typename T>
+hipError_t launch_col_op(int N, cplx<T>* P, const T* diag, long nfields, int C,
+                         const cplx<T>* tw, hipStream_t s);
 hipError_t launch_cg_alpha(const double* res, const double* pAp, const unsigned char* act,
                            double* alpha, int nrhs, hipStream_t s);
 hipError_t launch_cg_beta(double* res, const double* rz, const unsigned char* act, double* beta,
