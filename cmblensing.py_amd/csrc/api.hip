@@ -1023,29 +1023,35 @@ cmbl_status run_gradf(cmbl_plan* p, const cmbl_phi* const* phis, const void* f, 
                                          batch, s));
     return CMBL_OK;
   }
-  // 1. forward model on f (skipped when f = 0: L 0 = 0)
+  // The map -> Fourier -> map steps are row R2C / C2R passes around one
+  // column pass that applies the step's per-mode operator (wf.hip).
+  cplx<T>* S = (cplx<T>*)p->s0;
+  const cplx<T>* tw = (const cplx<T>*)p->tw;
+  // 1. forward model on f (skipped when f = 0: L 0 = 0): map(eb2qu(f)), lensed
   T* Ylens = lensed_out ? (T*)lensed_out : Y2;
   if (f) {
-    HIP_TRY(cmbl::launch_rotate<T>((const cplx<T>*)f, X1, nullptr, N, C, batch, 1, s));
-    CMBL_TRY(cmbl_irfft2(p, X1, Y1, (int)nm, st));
+    HIP_TRY(cmbl::launch_col_inv_eb2qu<T>(N, (const cplx<T>*)f, S, batch, C, tw, s));
+    HIP_TRY(cmbl::launch_row_c2r<T>(N, S, Y1, nm, tw, nullptr, nullptr, 0, s));
     CMBL_TRY(cmbl_lenseflow(p, phis, Y1, Ylens, batch, C, n, CMBL_FORWARD_LENSE, 0, st));
-    CMBL_TRY(cmbl_rfft2(p, Ylens, X2, (int)nm, st));
   }
   if (!pix) {
-    // 2. g = B M Cn^-1 (d - M B L f), as eb2qu(g) in X2
-    HIP_TRY(cmbl::launch_wf_data<T>(f ? X2 : nullptr, (const cplx<T>*)d, (const T*)ops->mb,
-                                    (const T*)ops->bmcn, X2, N, C, batch, s));
-    if (cot_out) return cmbl_irfft2(p, X2, cot_out, (int)nm, st);
-    // 3. adjoint LenseFlow of map(eb2qu(g))
-    CMBL_TRY(cmbl_irfft2(p, X2, Y2, (int)nm, st));
+    // 2. g = B M Cn^-1 (d - M B L f), as map(eb2qu(g)) in Y2 (or the cotangent)
+    T* gmap = cot_out ? (T*)cot_out : Y2;
+    if (f) {
+      HIP_TRY(cmbl::launch_lf_row_r2c<T>(N, Ylens, S, nm, tw, s));
+      HIP_TRY(cmbl::launch_col_data<T>(N, S, (const cplx<T>*)d, (const T*)ops->mb,
+                                       (const T*)ops->bmcn, batch, C, tw, s));
+      HIP_TRY(cmbl::launch_row_c2r<T>(N, S, gmap, nm, tw, nullptr, nullptr, 0, s));
+    } else {
+      HIP_TRY(cmbl::launch_wf_data<T>(nullptr, (const cplx<T>*)d, (const T*)ops->mb,
+                                      (const T*)ops->bmcn, X2, N, C, batch, s));
+      CMBL_TRY(cmbl_irfft2(p, X2, gmap, (int)nm, st));
+    }
+    if (cot_out) return CMBL_OK;
   } else {
     // 2'. pixel mask: r = d - M map(B L f) in pixel space, then
-    //     g = B fourier(M map(Cn^-1 fourier(r))).  Each map -> Fourier
-    //     operator -> map step is a row R2C, one column pass applying the
-    //     operator between the column FFTs (k_col_op), and a row C2R that
-    //     also applies the pixel-space step that follows.
-    cplx<T>* S = (cplx<T>*)p->s0;
-    const cplx<T>* tw = (const cplx<T>*)p->tw;
+    //     g = B fourier(M map(Cn^-1 fourier(r))); the row C2R of each step
+    //     also applies the pixel-space step that follows it.
     auto op_chain = [&](const T* x, const T* diag, T* y, const T* dd, int ep) -> hipError_t {
       hipError_t e = cmbl::launch_lf_row_r2c<T>(N, x, S, nm, tw, s);
       if (e == hipSuccess) e = cmbl::launch_col_op<T>(N, S, diag, batch, C, tw, s);
@@ -1069,13 +1075,13 @@ cmbl_status run_gradf(cmbl_plan* p, const cmbl_phi* const* phis, const void* f, 
   }
   // 3. adjoint LenseFlow of map(eb2qu(g))
   CMBL_TRY(cmbl_lenseflow(p, phis, Y2, Y2, batch, C, n, CMBL_FORWARD_LENSE, 1, st));
-  CMBL_TRY(cmbl_rfft2(p, Y2, X2, (int)nm, st));
   // 4. out = qu2eb(fourier(L^T g)) - Cf^-1 qu2eb(fourier(map(eb2qu f))); the
   //    round trip of f is a Hermitian projection applied per mode in-kernel
   const bool prior = f && ops->cfinv;
-  HIP_TRY(cmbl::launch_wf_final<T>(X2, prior ? (const cplx<T>*)f : nullptr,
-                                   prior ? (const T*)ops->cfinv : nullptr, (cplx<T>*)out, N, C,
-                                   batch, s));
+  HIP_TRY(cmbl::launch_lf_row_r2c<T>(N, Y2, S, nm, tw, s));
+  HIP_TRY(cmbl::launch_col_fwd_final<T>(N, S, prior ? (const cplx<T>*)f : nullptr,
+                                        prior ? (const T*)ops->cfinv : nullptr, (cplx<T>*)out,
+                                        batch, C, tw, s));
   return CMBL_OK;
 }
 

@@ -47,39 +47,6 @@ __device__ __forceinline__ void qu2eb_mode(cplx<T> q, cplx<T> u, T c, T s, cplx<
   b = mk(q.x * s - u.x * c, q.y * s - u.y * c);
 }
 
-// out = rot(diag .* in): per-mode diagonal (per component, may be NULL), then
-// EB->QU (to_qu) or QU->EB.  C = 1 skips the rotation.  in may alias out.
-template <typename T>
-__global__ void k_rotate(const cplx<T>* in, cplx<T>* out, const T* __restrict__ diag, int N,
-                         int C, long nfields, int to_qu) {
-  const long P = (long)N * (N / 2 + 1);
-  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < nfields * P;
-       g += (long)gridDim.x * blockDim.x) {
-    const long f = g / P, idx = g % P;
-    const long base = f * C * P + idx;
-    if (C == 1) {
-      cplx<T> a = in[base];
-      if (diag) a = scale(a, diag[idx]);
-      out[base] = a;
-      continue;
-    }
-    cplx<T> a = in[base], b = in[base + P];
-    if (diag) {
-      a = scale(a, diag[idx]);
-      b = scale(b, diag[P + idx]);
-    }
-    T c, s;
-    rot2psi<T>(N, idx, c, s);
-    cplx<T> o0, o1;
-    if (to_qu)
-      eb2qu_mode(a, b, c, s, o0, o1);
-    else
-      qu2eb_mode(a, b, c, s, o0, o1);
-    out[base] = o0;
-    out[base + P] = o1;
-  }
-}
-
 // Data term of gradf_logpdf (gradients.py:18-22) from the rfft2 of the lensed
 // QU maps X:  g = bmcn .* (d - mb .* qu2eb(X)) in EB, returned as eb2qu(g).
 // mb = M B and bmcn = B M Cn^-1 are per-mode diagonals [C][N][N/2+1]; d may be
@@ -125,10 +92,10 @@ __global__ void k_wf_data(const cplx<T>* X, const cplx<T>* __restrict__ d,
 // field.  fourier(map(.)) of a half plane only replaces the two
 // self-conjugate columns (ky = 0 and ky = N/2) by their Hermitian parts
 // (Y[kx] + conj(Y[-kx]))/2, and the EB<->QU rotation is the same at kx and -kx
-// on those columns, so the prior term is cf_inv .* Herm(F) computed per mode:
-// identical to the reference's round trip, without leaking FFT rounding of
-// large modes into small ones (which cf_inv, spanning ~1e10, would amplify).
-// F and cf_inv may be NULL (no prior term).  out may alias XL.
+// on those columns, so the prior term is cf_inv .* Herm(F) computed per mode
+// (herm_col; OpFinal below and k_wf_final_teb): identical to the reference's
+// round trip, without leaking FFT rounding of large modes into small ones
+// (which cf_inv, spanning ~1e10, would amplify).
 template <typename T>
 __device__ __forceinline__ cplx<T> herm_col(const cplx<T>* F, long base_f, int N, int i, int j) {
   const int H1 = N / 2 + 1;
@@ -136,36 +103,6 @@ __device__ __forceinline__ cplx<T> herm_col(const cplx<T>* F, long base_f, int N
   if (j != 0 && j != N / 2) return a;
   const cplx<T> b = F[base_f + (long)((N - i) & (N - 1)) * H1 + j];
   return mk(T(0.5) * (a.x + b.x), T(0.5) * (a.y - b.y));
-}
-
-template <typename T>
-__global__ void k_wf_final(const cplx<T>* XL, const cplx<T>* __restrict__ F,
-                           const T* __restrict__ cfinv, cplx<T>* out, int N, int C, long nfields) {
-  const long P = (long)N * (N / 2 + 1);
-  const int H1 = N / 2 + 1;
-  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < nfields * P;
-       g += (long)gridDim.x * blockDim.x) {
-    const long f = g / P, idx = g % P;
-    const long base = f * C * P + idx;
-    const int i = (int)(idx / H1), j = (int)(idx % H1);
-    const bool prior = F && cfinv;
-    if (C == 1) {
-      cplx<T> o = XL[base];
-      if (prior) o = o - scale(herm_col(F, f * C * P, N, i, j), cfinv[idx]);
-      out[base] = o;
-      continue;
-    }
-    T c, s;
-    rot2psi<T>(N, idx, c, s);
-    cplx<T> e, b;
-    qu2eb_mode(XL[base], XL[base + P], c, s, e, b);
-    if (prior) {
-      e = e - scale(herm_col(F, f * C * P, N, i, j), cfinv[idx]);
-      b = b - scale(herm_col(F, f * C * P + P, N, i, j), cfinv[P + idx]);
-    }
-    out[base] = e;
-    out[base + P] = b;
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -247,7 +184,7 @@ __global__ void k_wf_data_teb(const cplx<T>* XT, const cplx<T>* XP, const cplx<T
 }
 
 // Last line of gradf for T + EB: out [nfields][3] = (XLT, qu2eb(XLP)) minus
-// cfinv (Herm F_T, Herm F_E, Herm F_B) (the prior term as in k_wf_final).
+// cfinv (Herm F_T, Herm F_E, Herm F_B) (the prior term as in OpFinal).
 // F and cfinv may be NULL.
 template <typename T>
 __global__ void k_wf_final_teb(const cplx<T>* XLT, const cplx<T>* XLP,
@@ -462,49 +399,43 @@ __global__ void k_cg_rel(const double* __restrict__ rr, const double* __restrict
   rel[k] = sqrt(rr[k] / bb[k]);
 }
 
-// A Fourier-diagonal operator applied between two basis changes, in one
-// column pass: for packed row spectra P [nfields][C][N][N/2] (after a row R2C,
-// k_row_r2c), the column FFT, per mode (kx, ky)
-//   C = 2:  (Q, U) -> eb2qu(diag .* qu2eb(Q, U))     (util.py:140-161)
-//   C = 1:  T -> diag .* T
-// (matrix_operators.py:54-90), then the inverse column FFT times scale_ (1/N^2),
-// in place; a row C2R (k_row_c2r) then gives irfft2(op(rfft2(x))) with numpy's
-// conventions.  It replaces rfft2 -> two rotations -> irfft2 (six passes over
-// the data) by one column pass between the row passes.  The packed DC|Nyquist
-// column (k = 0) is unpacked into the ky = 0 and ky = N/2 modes, transformed,
-// and repacked as Herm(Y0) + i Herm(Yn) -- the parts the row C2R keeps -- as the
-// generic basis-change passes do; the workgroups of tile 0 hold that column and
-// take the extra LDS exchanges (uniformly over their threads).
-template <typename T, int N, int KC, int C>
-__global__ __launch_bounds__(KC * N / 8) void k_col_op(cplx<T>* P, const T* __restrict__ diag,
-                                                       const cplx<T>* __restrict__ twg,
-                                                       T scale_) {
-  constexpr int G = N / 8, H = N / 2, TILES = H / KC;
-  constexpr long PN = (long)N * (H + 1);  // natural half-plane size (diag planes)
-  __shared__ cplx<T> lbuf[C][KC][line_stride<N>()];
-  const int g = threadIdx.x % KC, t = threadIdx.x / KC;
-  Twiddles<N, T> tw;
-  tw.load(twg, t);
-  const long f = blockIdx.x / TILES;
-  const int tile = blockIdx.x % TILES;
-  const int k = tile * KC + g;
-  cplx<T>* Pc[C];
-#pragma unroll
-  for (int c = 0; c < C; ++c) Pc[c] = P + (f * C + c) * (long)N * H;
-  cplx<T> v[C][8];
-#pragma unroll
-  for (int c = 0; c < C; ++c)
-#pragma unroll
-    for (int q = 0; q < 8; ++q) v[c][q] = Pc[c][(long)(t + q * G) * H + k];
-  __syncthreads();
-  cplx<T>* bufs[C];
-#pragma unroll
-  for (int c = 0; c < C; ++c) bufs[c] = lbuf[c][g];
-  cplx<T>* const(&cb)[C] = bufs;
-  fft_lines<N, false, C, false, false>(v, cb, tw, t);
-  // the operator at mode (kx, ky) on the C values a[c]
-  auto op = [&](int kx, int ky, cplx<T>(&a)[C]) {
-    const long idx = (long)kx * (H + 1) + ky;
+// ---------------------------------------------------------------------------
+// Column passes with the per-mode operators at their edges.  The posterior's
+// chains map -> rfft2 -> per-mode operator -> irfft2 -> map and field ->
+// per-mode operator -> irfft2 (and back) are run as row R2C / C2R passes
+// (k_row_r2c, k_row_c2r) around ONE column pass that applies the operator
+// between its column FFT and the layout change, instead of separate rfft2,
+// rotation and irfft2 passes over the half planes.
+//
+// Packed row spectra P [nfields][C][N][N/2]: the row R2C of real maps, with
+// column 0 holding DC + i Nyquist (k_row_r2c).  Natural half planes X
+// [nfields][C][N][N/2+1] (numpy's rfft2 layout).  The column FFT of the packed
+// column 0 gives Z = Y0 + i Yn; Y0[kx] = (Z[kx] + conj Z[-kx]) / 2 and
+// Yn[kx] = (Z[kx] - conj Z[-kx]) / 2i are the ky = 0 and ky = N/2 modes, and the
+// inverse way a column enters as Herm(Y0) + i Herm(Yn), Herm(Y)[kx] =
+// (Y[kx] + conj Y[-kx]) / 2: the part the row C2R keeps (k_col_fwd_unpack,
+// k_col_inv_pack).  Workgroups of tile 0 hold column 0 and take the LDS
+// exchanges this needs; the branch is uniform over a workgroup.
+//
+// An operator acts in place on the C values of one mode (field f, kx, ky),
+// a[c] = component c:
+//   OpDiagRot  C = 2: eb2qu(diag .* qu2eb(a)); C = 1: diag .* a
+//              (the noise / beam operators of the pixel-mask posterior,
+//               matrix_operators.py:54-90 with util.py:140-161)
+//   OpData     C = 2: eb2qu(bmcn .* (d - mb .* qu2eb(a))); C = 1: likewise
+//              without the rotations (k_wf_data; gradients.py:16-22)
+//   OpEb2Qu    eb2qu(a) (util.py:140-150)
+//   OpFinal    C = 2: qu2eb(a) - cfinv .* Herm(F); C = 1: a - cfinv .* Herm(F)
+//              (gradients.py:25; the prior term through herm_col)
+// ---------------------------------------------------------------------------
+
+template <typename T, int C>
+struct OpDiagRot {
+  const T* diag;
+  int N;
+  __device__ __forceinline__ void operator()(long, int kx, int ky, cplx<T> (&a)[C]) const {
+    const long PN = (long)N * (N / 2 + 1);
+    const long idx = (long)kx * (N / 2 + 1) + ky;
     if constexpr (C == 1) {
       a[0] = scale(a[0], diag[idx]);
     } else {
@@ -516,99 +447,329 @@ __global__ __launch_bounds__(KC * N / 8) void k_col_op(cplx<T>* P, const T* __re
       b = scale(b, diag[PN + idx]);
       eb2qu_mode(e, b, cs, sn, a[0], a[1]);
     }
-  };
+  }
+};
+
+template <typename T, int C>
+struct OpData {
+  const cplx<T>* d;  // [nfields][C][N][N/2+1] or NULL
+  const T* mb;
+  const T* bmcn;
+  int N;
+  __device__ __forceinline__ void operator()(long f, int kx, int ky, cplx<T> (&a)[C]) const {
+    const long PN = (long)N * (N / 2 + 1);
+    const long idx = (long)kx * (N / 2 + 1) + ky;
+    const long base = f * C * PN + idx;
+    if constexpr (C == 1) {
+      cplx<T> r = scale(a[0], -mb[idx]);
+      if (d) r = r + d[base];
+      a[0] = scale(r, bmcn[idx]);
+    } else {
+      T cs, sn;
+      rot2psi<T>(N, idx, cs, sn);
+      cplx<T> e, b;
+      qu2eb_mode(a[0], a[1], cs, sn, e, b);
+      cplx<T> re = scale(e, -mb[idx]), rb = scale(b, -mb[PN + idx]);
+      if (d) {
+        re = re + d[base];
+        rb = rb + d[base + PN];
+      }
+      re = scale(re, bmcn[idx]);
+      rb = scale(rb, bmcn[PN + idx]);
+      eb2qu_mode(re, rb, cs, sn, a[0], a[1]);
+    }
+  }
+};
+
+template <typename T, int C>
+struct OpEb2Qu {
+  int N;
+  __device__ __forceinline__ void operator()(long, int kx, int ky, cplx<T> (&a)[C]) const {
+    if constexpr (C == 2) {
+      T cs, sn;
+      rot2psi<T>(N, (long)kx * (N / 2 + 1) + ky, cs, sn);
+      const cplx<T> e = a[0], b = a[1];
+      eb2qu_mode(e, b, cs, sn, a[0], a[1]);
+    }
+  }
+};
+
+template <typename T, int C>
+struct OpFinal {
+  const cplx<T>* F;  // [nfields][C][N][N/2+1] or NULL (no prior term)
+  const T* cfinv;
+  int N;
+  __device__ __forceinline__ void operator()(long f, int kx, int ky, cplx<T> (&a)[C]) const {
+    const long PN = (long)N * (N / 2 + 1);
+    const long idx = (long)kx * (N / 2 + 1) + ky;
+    const bool prior = F && cfinv;
+    if constexpr (C == 2) {
+      T cs, sn;
+      rot2psi<T>(N, idx, cs, sn);
+      const cplx<T> q = a[0], u = a[1];
+      qu2eb_mode(q, u, cs, sn, a[0], a[1]);
+    }
+    if (prior) {
+#pragma unroll
+      for (int c = 0; c < C; ++c)
+        a[c] = a[c] - scale(herm_col(F, (f * C + c) * PN, N, kx, ky), cfinv[c * PN + idx]);
+    }
+  }
+};
+
+// v [C][8] (column k = tile * KC + g of every component, rows t + q G) after
+// the forward column FFT: apply op to every mode.  Column 0 (g == 0 of tile
+// 0) is unpacked into its two modes, which are returned in y0 / yn (the other
+// lanes' results stay in v).  Uniform over the workgroup: `tile` is.
+template <typename T, int N, int KC, int C, typename Op>
+__device__ __forceinline__ void col_modes_apply(cplx<T> (&v)[C][8], cplx<T> (&y0)[C][8],
+                                                cplx<T> (&yn)[C][8],
+                                                cplx<T> (*lbuf)[KC][line_stride<N>()], int g,
+                                                int t, int tile, long f, const Op& op) {
+  constexpr int G = N / 8, H = N / 2;
+  const int k = tile * KC + g;
   if (tile != 0) {
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       cplx<T> a[C];
 #pragma unroll
       for (int c = 0; c < C; ++c) a[c] = v[c][q];
-      op(t + q * G, k, a);
+      op(f, t + q * G, k, a);
 #pragma unroll
       for (int c = 0; c < C; ++c) v[c][q] = a[c];
     }
-  } else {
-    // uniform over the workgroup: g == 0 holds the packed column
-    const T h = T(0.5);
-    cplx<T> y0[C][8], yn[C][8];
-    __syncthreads();
+    return;
+  }
+  const T h = T(0.5);
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < C; ++c)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) lbuf[c][g][lds_idx<N, false>(t + q * G)] = v[c][q];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int kx = t + q * G;
+    if (k == 0) {
+      cplx<T> a0[C], an[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const cplx<T> z = v[c][q];
+        const cplx<T> zc = conj(lbuf[c][g][lds_idx<N, false>((N - kx) & (N - 1))]);
+        a0[c] = scale(z + zc, h);
+        const cplx<T> dd = z - zc;
+        an[c] = mk(h * dd.y, -h * dd.x);
+      }
+      op(f, kx, 0, a0);
+      op(f, kx, H, an);
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        y0[c][q] = a0[c];
+        yn[c][q] = an[c];
+      }
+    } else {
+      cplx<T> a[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) a[c] = v[c][q];
+      op(f, kx, k, a);
+#pragma unroll
+      for (int c = 0; c < C; ++c) v[c][q] = a[c];
+    }
+  }
+}
+
+// Column 0 of tile 0 back into the packed form Herm(Y0) + i Herm(Yn), into
+// v of the g == 0 lanes.  Uniform over the workgroup.
+template <typename T, int N, int KC, int C>
+__device__ __forceinline__ void col0_repack(cplx<T> (&v)[C][8], cplx<T> (&y0)[C][8],
+                                            const cplx<T> (&yn)[C][8],
+                                            cplx<T> (*lbuf)[KC][line_stride<N>()], int g, int t,
+                                            bool col0) {
+  constexpr int G = N / 8;
+  const T h = T(0.5);
+  __syncthreads();
+  if (col0) {
 #pragma unroll
     for (int c = 0; c < C; ++c)
 #pragma unroll
-      for (int q = 0; q < 8; ++q) lbuf[c][g][lds_idx<N, false>(t + q * G)] = v[c][q];
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int kx = t + q * G;
-      if (k == 0) {
-        cplx<T> a0[C], an[C];
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-          const cplx<T> z = v[c][q];
-          const cplx<T> zc = conj(lbuf[c][g][lds_idx<N, false>((N - kx) & (N - 1))]);
-          a0[c] = scale(z + zc, h);
-          const cplx<T> d = z - zc;
-          an[c] = mk(h * d.y, -h * d.x);
-        }
-        op(kx, 0, a0);
-        op(kx, H, an);
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-          y0[c][q] = a0[c];
-          yn[c][q] = an[c];
-        }
-      } else {
-        cplx<T> a[C];
-#pragma unroll
-        for (int c = 0; c < C; ++c) a[c] = v[c][q];
-        op(kx, k, a);
-#pragma unroll
-        for (int c = 0; c < C; ++c) v[c][q] = a[c];
-      }
-    }
-    // repack: Herm(Y0) + i Herm(Yn), Herm(Y)[kx] = (Y[kx] + conj(Y[-kx])) / 2
-    __syncthreads();
-    if (k == 0) {
-#pragma unroll
-      for (int c = 0; c < C; ++c)
-#pragma unroll
-        for (int q = 0; q < 8; ++q) lbuf[c][g][lds_idx<N, false>(t + q * G)] = y0[c][q];
-    }
-    __syncthreads();
-    if (k == 0) {
-#pragma unroll
-      for (int c = 0; c < C; ++c)
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const int kx = t + q * G;
-          y0[c][q] = scale(y0[c][q] + conj(lbuf[c][g][lds_idx<N, false>((N - kx) & (N - 1))]), h);
-        }
-    }
-    __syncthreads();
-    if (k == 0) {
-#pragma unroll
-      for (int c = 0; c < C; ++c)
-#pragma unroll
-        for (int q = 0; q < 8; ++q) lbuf[c][g][lds_idx<N, false>(t + q * G)] = yn[c][q];
-    }
-    __syncthreads();
-    if (k == 0) {
-#pragma unroll
-      for (int c = 0; c < C; ++c)
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const int kx = t + q * G;
-          const cplx<T> n0 =
-              scale(yn[c][q] + conj(lbuf[c][g][lds_idx<N, false>((N - kx) & (N - 1))]), h);
-          v[c][q] = mk(y0[c][q].x - n0.y, y0[c][q].y + n0.x);
-        }
-    }
+      for (int q = 0; q < 8; ++q) lbuf[c][g][lds_idx<N, false>(t + q * G)] = y0[c][q];
   }
+  __syncthreads();
+  if (col0) {
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int kx = t + q * G;
+        y0[c][q] = scale(y0[c][q] + conj(lbuf[c][g][lds_idx<N, false>((N - kx) & (N - 1))]), h);
+      }
+  }
+  __syncthreads();
+  if (col0) {
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) lbuf[c][g][lds_idx<N, false>(t + q * G)] = yn[c][q];
+  }
+  __syncthreads();
+  if (col0) {
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int kx = t + q * G;
+        const cplx<T> n0 =
+            scale(yn[c][q] + conj(lbuf[c][g][lds_idx<N, false>((N - kx) & (N - 1))]), h);
+        v[c][q] = mk(y0[c][q].x - n0.y, y0[c][q].y + n0.x);
+      }
+  }
+}
+
+// packed -> op -> packed, in place: column FFT, op, inverse column FFT times
+// scale_ (1 / N^2).  A row C2R afterwards gives irfft2(op(rfft2(x))).
+template <typename T, int N, int KC, int C, typename Op>
+__global__ __launch_bounds__(KC * N / 8) void k_col_op(cplx<T>* P, Op op,
+                                                       const cplx<T>* __restrict__ twg,
+                                                       T scale_) {
+  constexpr int G = N / 8, H = N / 2, TILES = H / KC;
+  __shared__ cplx<T> lbuf[C][KC][line_stride<N>()];
+  const int g = threadIdx.x % KC, t = threadIdx.x / KC;
+  Twiddles<N, T> tw;
+  tw.load(twg, t);
+  const long f = blockIdx.x / TILES;
+  const int tile = blockIdx.x % TILES;
+  const int k = tile * KC + g;
+  cplx<T>* Pc = P + f * C * (long)N * H + k;
+  cplx<T> v[C][8];
+#pragma unroll
+  for (int c = 0; c < C; ++c)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[c][q] = Pc[(c * (long)N + t + q * G) * H];
+  __syncthreads();
+  cplx<T>* bufs[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) bufs[c] = lbuf[c][g];
+  cplx<T>* const(&cb)[C] = bufs;
+  fft_lines<N, false, C, false, false>(v, cb, tw, t);
+  cplx<T> y0[C][8], yn[C][8];
+  col_modes_apply<T, N, KC, C>(v, y0, yn, lbuf, g, t, tile, f, op);
+  if (tile == 0) col0_repack<T, N, KC, C>(v, y0, yn, lbuf, g, t, k == 0);
   fft_lines<N, true, C, false, false>(v, cb, tw, t);
 #pragma unroll
   for (int c = 0; c < C; ++c)
 #pragma unroll
-    for (int q = 0; q < 8; ++q) Pc[c][(long)(t + q * G) * H + k] = scale(v[c][q], scale_);
+    for (int q = 0; q < 8; ++q) Pc[(c * (long)N + t + q * G) * H] = scale(v[c][q], scale_);
+}
+
+// natural X -> op -> packed P: the column half of irfft2(op(X)), times
+// scale_ (1 / N^2); a row C2R follows.  Column 0 reads its four modes
+// (kx, 0), (-kx, 0), (kx, N/2), (-kx, N/2) and applies op to each.
+template <typename T, int N, int KC, int C, typename Op>
+__global__ __launch_bounds__(KC * N / 8) void k_col_inv_op(const cplx<T>* __restrict__ X,
+                                                           cplx<T>* __restrict__ P, Op op,
+                                                           const cplx<T>* __restrict__ twg,
+                                                           T scale_) {
+  constexpr int G = N / 8, H = N / 2, TILES = H / KC;
+  constexpr long PN = (long)N * (H + 1);
+  __shared__ cplx<T> lbuf[C][KC][line_stride<N>()];
+  const int g = threadIdx.x % KC, t = threadIdx.x / KC;
+  Twiddles<N, T> tw;
+  tw.load(twg, t);
+  const long f = blockIdx.x / TILES;
+  const int k = (blockIdx.x % TILES) * KC + g;
+  const cplx<T>* Xf = X + f * C * PN;
+  cplx<T> v[C][8];
+  if (k == 0) {
+    const T h = T(0.5);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int kx = t + q * G, mx = (N - kx) & (N - 1);
+      cplx<T> a0[C], b0[C], an[C], bn[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        a0[c] = Xf[c * PN + (long)kx * (H + 1)];
+        b0[c] = Xf[c * PN + (long)mx * (H + 1)];
+        an[c] = Xf[c * PN + (long)kx * (H + 1) + H];
+        bn[c] = Xf[c * PN + (long)mx * (H + 1) + H];
+      }
+      op(f, kx, 0, a0);
+      op(f, mx, 0, b0);
+      op(f, kx, H, an);
+      op(f, mx, H, bn);
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const cplx<T> d0 = scale(a0[c] + conj(b0[c]), h);
+        const cplx<T> n0 = scale(an[c] + conj(bn[c]), h);
+        v[c][q] = mk(d0.x - n0.y, d0.y + n0.x);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int kx = t + q * G;
+      cplx<T> a[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) a[c] = Xf[c * PN + (long)kx * (H + 1) + k];
+      op(f, kx, k, a);
+#pragma unroll
+      for (int c = 0; c < C; ++c) v[c][q] = a[c];
+    }
+  }
+  __syncthreads();
+  cplx<T>* bufs[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) bufs[c] = lbuf[c][g];
+  cplx<T>* const(&cb)[C] = bufs;
+  fft_lines<N, true, C, false, false>(v, cb, tw, t);
+  cplx<T>* Pc = P + f * C * (long)N * H + k;
+#pragma unroll
+  for (int c = 0; c < C; ++c)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) Pc[(c * (long)N + t + q * G) * H] = scale(v[c][q], scale_);
+}
+
+// packed P -> op -> natural out: the column half of op(rfft2(x)) after a row
+// R2C (no scaling, numpy's forward convention).
+template <typename T, int N, int KC, int C, typename Op>
+__global__ __launch_bounds__(KC * N / 8) void k_col_fwd_op(const cplx<T>* __restrict__ P,
+                                                           cplx<T>* __restrict__ out, Op op,
+                                                           const cplx<T>* __restrict__ twg) {
+  constexpr int G = N / 8, H = N / 2, TILES = H / KC;
+  constexpr long PN = (long)N * (H + 1);
+  __shared__ cplx<T> lbuf[C][KC][line_stride<N>()];
+  const int g = threadIdx.x % KC, t = threadIdx.x / KC;
+  Twiddles<N, T> tw;
+  tw.load(twg, t);
+  const long f = blockIdx.x / TILES;
+  const int tile = blockIdx.x % TILES;
+  const int k = tile * KC + g;
+  const cplx<T>* Pc = P + f * C * (long)N * H + k;
+  cplx<T> v[C][8];
+#pragma unroll
+  for (int c = 0; c < C; ++c)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[c][q] = Pc[(c * (long)N + t + q * G) * H];
+  __syncthreads();
+  cplx<T>* bufs[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) bufs[c] = lbuf[c][g];
+  cplx<T>* const(&cb)[C] = bufs;
+  fft_lines<N, false, C, false, false>(v, cb, tw, t);
+  cplx<T> y0[C][8], yn[C][8];
+  col_modes_apply<T, N, KC, C>(v, y0, yn, lbuf, g, t, tile, f, op);
+  cplx<T>* of = out + f * C * PN;
+#pragma unroll
+  for (int c = 0; c < C; ++c)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const long r = (long)(t + q * G) * (H + 1);
+      if (k == 0) {
+        of[c * PN + r] = y0[c][q];
+        of[c * PN + r + H] = yn[c][q];
+      } else {
+        of[c * PN + r + k] = v[c][q];
+      }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -620,26 +781,10 @@ static inline unsigned grid_for(long n) {
 }
 
 template <typename T>
-hipError_t launch_rotate(const cplx<T>* in, cplx<T>* out, const T* diag, int N, int C,
-                         long nfields, int to_qu, hipStream_t s) {
-  const long n = nfields * (long)N * (N / 2 + 1);
-  k_rotate<T><<<grid_for(n), 256, 0, s>>>(in, out, diag, N, C, nfields, to_qu);
-  return hipGetLastError();
-}
-
-template <typename T>
 hipError_t launch_wf_data(const cplx<T>* X, const cplx<T>* d, const T* mb, const T* bmcn,
                           cplx<T>* out, int N, int C, long nfields, hipStream_t s) {
   const long n = nfields * (long)N * (N / 2 + 1);
   k_wf_data<T><<<grid_for(n), 256, 0, s>>>(X, d, mb, bmcn, out, N, C, nfields);
-  return hipGetLastError();
-}
-
-template <typename T>
-hipError_t launch_wf_final(const cplx<T>* XL, const cplx<T>* F, const T* cfinv, cplx<T>* out,
-                           int N, int C, long nfields, hipStream_t s) {
-  const long n = nfields * (long)N * (N / 2 + 1);
-  k_wf_final<T><<<grid_for(n), 256, 0, s>>>(XL, F, cfinv, out, N, C, nfields);
   return hipGetLastError();
 }
 
@@ -706,9 +851,9 @@ hipError_t launch_wf_final_teb(const cplx<T>* XLT, const cplx<T>* XLP, const cpl
   return hipGetLastError();
 }
 
-// Columns per k_col_op workgroup: 128-byte row segments (16 float / 8 double
-// complex), halved for Q/U pairs at N = 512, where two 16-column sets of line
-// buffers would not fit the LDS.
+// Columns per workgroup of the operator column passes: 128-byte row segments
+// (16 float / 8 double complex), halved for Q/U pairs at N = 512, where two
+// 16-column sets of line buffers would not fit the LDS.
 template <typename T, int N, int C>
 constexpr int col_op_kc() {
   constexpr int base = sizeof(T) == 4 ? 16 : 8;
@@ -716,35 +861,86 @@ constexpr int col_op_kc() {
   return kc < N / 2 ? kc : N / 2;
 }
 
+#define CMBL_COL_DISPATCH(N, ...)                              \
+  switch (N) {                                                 \
+    case 16: { constexpr int NN_ = 16; __VA_ARGS__; } break;   \
+    case 32: { constexpr int NN_ = 32; __VA_ARGS__; } break;   \
+    case 64: { constexpr int NN_ = 64; __VA_ARGS__; } break;   \
+    case 128: { constexpr int NN_ = 128; __VA_ARGS__; } break; \
+    case 256: { constexpr int NN_ = 256; __VA_ARGS__; } break; \
+    case 512: { constexpr int NN_ = 512; __VA_ARGS__; } break; \
+    default: return hipErrorInvalidValue;                      \
+  }
+
+template <typename T, int C, template <typename, int> class OpT>
+static hipError_t col_op_c(int N, cplx<T>* P, const OpT<T, C>& op, long nfields,
+                           const cplx<T>* tw, hipStream_t s) {
+  CMBL_COL_DISPATCH(N, {
+    constexpr int KC = col_op_kc<T, NN_, C>();
+    const T sc = T(1) / (T(NN_) * T(NN_));
+    k_col_op<T, NN_, KC, C, OpT<T, C>>
+        <<<(unsigned)(nfields * (NN_ / 2 / KC)), KC * NN_ / 8, 0, s>>>(P, op, tw, sc);
+  });
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_col_op(int N, cplx<T>* P, const T* diag, long nfields, int C,
                          const cplx<T>* tw, hipStream_t s) {
-#define CMBL_COL_OP(NN_)                                                                  \
-  case NN_: {                                                                             \
-    const T sc = T(1) / (T(NN_) * T(NN_));                                                \
-    if (C == 2) {                                                                         \
-      constexpr int KC = col_op_kc<T, NN_, 2>();                                          \
-      k_col_op<T, NN_, KC, 2><<<(unsigned)(nfields * (NN_ / 2 / KC)), KC * NN_ / 8, 0, s>>>( \
-          P, diag, tw, sc);                                                               \
-    } else {                                                                              \
-      constexpr int KC = col_op_kc<T, NN_, 1>();                                          \
-      k_col_op<T, NN_, KC, 1><<<(unsigned)(nfields * (NN_ / 2 / KC)), KC * NN_ / 8, 0, s>>>( \
-          P, diag, tw, sc);                                                               \
-    }                                                                                     \
-  } break;
-  switch (N) {
-    CMBL_COL_OP(16)
-    CMBL_COL_OP(32)
-    CMBL_COL_OP(64)
-    CMBL_COL_OP(128)
-    CMBL_COL_OP(256)
-    CMBL_COL_OP(512)
-    default:
-      return hipErrorInvalidValue;
-  }
-#undef CMBL_COL_OP
-  return hipGetLastError();
+  if (C == 2) return col_op_c<T, 2, OpDiagRot>(N, P, OpDiagRot<T, 2>{diag, N}, nfields, tw, s);
+  if (C == 1) return col_op_c<T, 1, OpDiagRot>(N, P, OpDiagRot<T, 1>{diag, N}, nfields, tw, s);
+  return hipErrorInvalidValue;
 }
+
+template <typename T>
+hipError_t launch_col_data(int N, cplx<T>* P, const cplx<T>* d, const T* mb, const T* bmcn,
+                           long nfields, int C, const cplx<T>* tw, hipStream_t s) {
+  if (C == 2)
+    return col_op_c<T, 2, OpData>(N, P, OpData<T, 2>{d, mb, bmcn, N}, nfields, tw, s);
+  if (C == 1)
+    return col_op_c<T, 1, OpData>(N, P, OpData<T, 1>{d, mb, bmcn, N}, nfields, tw, s);
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t launch_col_inv_eb2qu(int N, const cplx<T>* X, cplx<T>* P, long nfields, int C,
+                                const cplx<T>* tw, hipStream_t s) {
+  auto go = [&](auto cc) -> hipError_t {
+    constexpr int CC = decltype(cc)::value;
+    CMBL_COL_DISPATCH(N, {
+      constexpr int KC = col_op_kc<T, NN_, CC>();
+      const T sc = T(1) / (T(NN_) * T(NN_));
+      k_col_inv_op<T, NN_, KC, CC, OpEb2Qu<T, CC>>
+          <<<(unsigned)(nfields * (NN_ / 2 / KC)), KC * NN_ / 8, 0, s>>>(
+              X, P, OpEb2Qu<T, CC>{N}, tw, sc);
+    });
+    return hipGetLastError();
+  };
+  if (C == 2) return go(std::integral_constant<int, 2>{});
+  if (C == 1) return go(std::integral_constant<int, 1>{});
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t launch_col_fwd_final(int N, const cplx<T>* P, const cplx<T>* F, const T* cfinv,
+                                cplx<T>* out, long nfields, int C, const cplx<T>* tw,
+                                hipStream_t s) {
+  auto go = [&](auto cc) -> hipError_t {
+    constexpr int CC = decltype(cc)::value;
+    CMBL_COL_DISPATCH(N, {
+      constexpr int KC = col_op_kc<T, NN_, CC>();
+      k_col_fwd_op<T, NN_, KC, CC, OpFinal<T, CC>>
+          <<<(unsigned)(nfields * (NN_ / 2 / KC)), KC * NN_ / 8, 0, s>>>(
+              P, out, OpFinal<T, CC>{F, cfinv, N}, tw);
+    });
+    return hipGetLastError();
+  };
+  if (C == 2) return go(std::integral_constant<int, 2>{});
+  if (C == 1) return go(std::integral_constant<int, 1>{});
+  return hipErrorInvalidValue;
+}
+
+#undef CMBL_COL_DISPATCH
 
 hipError_t launch_cg_alpha(const double* res, const double* pAp, const unsigned char* act,
                            double* alpha, int nrhs, hipStream_t s) {
@@ -767,14 +963,16 @@ hipError_t launch_cg_rel(const double* rr, const double* bb, double* rel, int nr
 int red_parts() { return kRedParts; }
 
 #define CMBL_WF_INST(T)                                                                         \
-  template hipError_t launch_rotate<T>(const cplx<T>*, cplx<T>*, const T*, int, int, long, int, \
-                                       hipStream_t);                                            \
   template hipError_t launch_col_op<T>(int, cplx<T>*, const T*, long, int, const cplx<T>*,       \
                                        hipStream_t);                                            \
+  template hipError_t launch_col_data<T>(int, cplx<T>*, const cplx<T>*, const T*, const T*, long, \
+                                         int, const cplx<T>*, hipStream_t);                     \
+  template hipError_t launch_col_inv_eb2qu<T>(int, const cplx<T>*, cplx<T>*, long, int,          \
+                                              const cplx<T>*, hipStream_t);                     \
+  template hipError_t launch_col_fwd_final<T>(int, const cplx<T>*, const cplx<T>*, const T*,     \
+                                              cplx<T>*, long, int, const cplx<T>*, hipStream_t); \
   template hipError_t launch_wf_data<T>(const cplx<T>*, const cplx<T>*, const T*, const T*,     \
                                         cplx<T>*, int, int, long, hipStream_t);                 \
-  template hipError_t launch_wf_final<T>(const cplx<T>*, const cplx<T>*, const T*, cplx<T>*,    \
-                                         int, int, long, hipStream_t);                          \
   template hipError_t launch_fill_zero<T>(T*, long, hipStream_t);                               \
   template hipError_t launch_dot<T>(const cplx<double>*, const cplx<T>*, int, long, int,        \
                                     double*, double*, hipStream_t);                             \

@@ -7,14 +7,8 @@
 namespace cmbl {
 
 template <typename T>
-hipError_t launch_rotate(const cplx<T>* in, cplx<T>* out, const T* diag, int N, int C,
-                         long nfields, int to_qu, hipStream_t s);
-template <typename T>
 hipError_t launch_wf_data(const cplx<T>* X, const cplx<T>* d, const T* mb, const T* bmcn,
                           cplx<T>* out, int N, int C, long nfields, hipStream_t s);
-template <typename T>
-hipError_t launch_wf_final(const cplx<T>* XL, const cplx<T>* XF, const T* cfinv, cplx<T>* out,
-                           int N, int C, long nfields, hipStream_t s);
 template <typename T>
 hipError_t launch_fill_zero(T* x, long n, hipStream_t s);
 template <typename Tb>
@@ -45,6 +39,23 @@ hipError_t launch_wf_final_teb(const cplx<T>* XLT, const cplx<T>* XLP, const cpl
 template <typename T>
 hipError_t launch_col_op(int N, cplx<T>* P, const T* diag, long nfields, int C,
                          const cplx<T>* tw, hipStream_t s);
+// The same pass with the data term of the Fourier-mask posterior per mode:
+// eb2qu(bmcn .* (d - mb .* qu2eb(.))) (d natural [nfields][C][N][N/2+1] or
+// NULL).
+template <typename T>
+hipError_t launch_col_data(int N, cplx<T>* P, const cplx<T>* d, const T* mb, const T* bmcn,
+                           long nfields, int C, const cplx<T>* tw, hipStream_t s);
+// Natural EB (or T) half planes X -> eb2qu per mode -> inverse column FFT ->
+// packed P, times 1/N^2 (the column half of irfft2(eb2qu(X))).
+template <typename T>
+hipError_t launch_col_inv_eb2qu(int N, const cplx<T>* X, cplx<T>* P, long nfields, int C,
+                                const cplx<T>* tw, hipStream_t s);
+// Packed row spectra P -> column FFT -> per mode qu2eb(.) - cfinv .* Herm(F)
+// -> natural out (the column half of rfft2 and the last line of gradf_logpdf).
+template <typename T>
+hipError_t launch_col_fwd_final(int N, const cplx<T>* P, const cplx<T>* F, const T* cfinv,
+                                cplx<T>* out, long nfields, int C, const cplx<T>* tw,
+                                hipStream_t s);
 hipError_t launch_cg_alpha(const double* res, const double* pAp, const unsigned char* act,
                            double* alpha, int nrhs, hipStream_t s);
 hipError_t launch_cg_beta(double* res, const double* rz, const unsigned char* act, double* beta,
