@@ -16,21 +16,34 @@
 
 namespace cmbl {
 
+// cos 2psi, sin 2psi at mode (i, j) = (kx, ky) of the half plane.  The
+// frequencies are integers, so l^2 and the numerators are exact; FP64 divides,
+// FP32 multiplies by the hardware reciprocal of l^2 (1 ulp).
 template <typename T>
-__device__ __forceinline__ void rot2psi(int N, long idx, T& c, T& s) {
-  const int H1 = N / 2 + 1;
-  const int i = (int)(idx / H1), j = (int)(idx % H1);
-  const double ly = (i < N / 2) ? i : i - N;
-  double lx = j;
-  if (j == N / 2 && i <= N / 2) lx = -lx;
-  const double l2 = lx * lx + ly * ly;
+__device__ __forceinline__ void rot2psi_ij(int N, int i, int j, T& c, T& s) {
+  const int ly = (i < N / 2) ? i : i - N;
+  const int lx = (j == N / 2 && i <= N / 2) ? -j : j;
+  const int l2 = lx * lx + ly * ly;
   if (l2 == 0) {
     c = T(1);
     s = T(0);
-  } else {
-    c = (T)((ly * ly - lx * lx) / l2);
-    s = (T)(2.0 * lx * ly / l2);
+    return;
   }
+  const int nc = ly * ly - lx * lx, ns = 2 * lx * ly;
+  if constexpr (sizeof(T) == 4) {
+    const float r = __builtin_amdgcn_rcpf((float)l2);
+    c = (float)nc * r;
+    s = (float)ns * r;
+  } else {
+    c = (double)nc / (double)l2;
+    s = (double)ns / (double)l2;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void rot2psi(int N, long idx, T& c, T& s) {
+  const int H1 = N / 2 + 1;
+  rot2psi_ij<T>(N, (int)(idx / H1), (int)(idx % H1), c, s);
 }
 
 template <typename T>
@@ -440,7 +453,7 @@ struct OpDiagRot {
       a[0] = scale(a[0], diag[idx]);
     } else {
       T cs, sn;
-      rot2psi<T>(N, idx, cs, sn);
+      rot2psi_ij<T>(N, kx, ky, cs, sn);
       cplx<T> e, b;
       qu2eb_mode(a[0], a[1], cs, sn, e, b);
       e = scale(e, diag[idx]);
@@ -466,7 +479,7 @@ struct OpData {
       a[0] = scale(r, bmcn[idx]);
     } else {
       T cs, sn;
-      rot2psi<T>(N, idx, cs, sn);
+      rot2psi_ij<T>(N, kx, ky, cs, sn);
       cplx<T> e, b;
       qu2eb_mode(a[0], a[1], cs, sn, e, b);
       cplx<T> re = scale(e, -mb[idx]), rb = scale(b, -mb[PN + idx]);
@@ -487,7 +500,7 @@ struct OpEb2Qu {
   __device__ __forceinline__ void operator()(long, int kx, int ky, cplx<T> (&a)[C]) const {
     if constexpr (C == 2) {
       T cs, sn;
-      rot2psi<T>(N, (long)kx * (N / 2 + 1) + ky, cs, sn);
+      rot2psi_ij<T>(N, kx, ky, cs, sn);
       const cplx<T> e = a[0], b = a[1];
       eb2qu_mode(e, b, cs, sn, a[0], a[1]);
     }
@@ -502,14 +515,13 @@ struct OpFinal {
   __device__ __forceinline__ void operator()(long f, int kx, int ky, cplx<T> (&a)[C]) const {
     const long PN = (long)N * (N / 2 + 1);
     const long idx = (long)kx * (N / 2 + 1) + ky;
-    const bool prior = F && cfinv;
     if constexpr (C == 2) {
       T cs, sn;
-      rot2psi<T>(N, idx, cs, sn);
+      rot2psi_ij<T>(N, kx, ky, cs, sn);
       const cplx<T> q = a[0], u = a[1];
       qu2eb_mode(q, u, cs, sn, a[0], a[1]);
     }
-    if (prior) {
+    if (F && cfinv) {
 #pragma unroll
       for (int c = 0; c < C; ++c)
         a[c] = a[c] - scale(herm_col(F, (f * C + c) * PN, N, kx, ky), cfinv[c * PN + idx]);
@@ -517,259 +529,147 @@ struct OpFinal {
   }
 };
 
-// v [C][8] (column k = tile * KC + g of every component, rows t + q G) after
-// the forward column FFT: apply op to every mode.  Column 0 (g == 0 of tile
-// 0) is unpacked into its two modes, which are returned in y0 / yn (the other
-// lanes' results stay in v).  Uniform over the workgroup: `tile` is.
-template <typename T, int N, int KC, int C, typename Op>
-__device__ __forceinline__ void col_modes_apply(cplx<T> (&v)[C][8], cplx<T> (&y0)[C][8],
-                                                cplx<T> (&yn)[C][8],
-                                                cplx<T> (*lbuf)[KC][line_stride<N>()], int g,
-                                                int t, int tile, long f, const Op& op) {
-  constexpr int G = N / 8, H = N / 2;
-  const int k = tile * KC + g;
-  if (tile != 0) {
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      cplx<T> a[C];
-#pragma unroll
-      for (int c = 0; c < C; ++c) a[c] = v[c][q];
-      op(f, t + q * G, k, a);
-#pragma unroll
-      for (int c = 0; c < C; ++c) v[c][q] = a[c];
-    }
-    return;
-  }
-  const T h = T(0.5);
-  __syncthreads();
-#pragma unroll
-  for (int c = 0; c < C; ++c)
-#pragma unroll
-    for (int q = 0; q < 8; ++q) lbuf[c][g][lds_idx<N, false>(t + q * G)] = v[c][q];
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int kx = t + q * G;
-    if (k == 0) {
-      cplx<T> a0[C], an[C];
-#pragma unroll
-      for (int c = 0; c < C; ++c) {
-        const cplx<T> z = v[c][q];
-        const cplx<T> zc = conj(lbuf[c][g][lds_idx<N, false>((N - kx) & (N - 1))]);
-        a0[c] = scale(z + zc, h);
-        const cplx<T> dd = z - zc;
-        an[c] = mk(h * dd.y, -h * dd.x);
-      }
-      op(f, kx, 0, a0);
-      op(f, kx, H, an);
-#pragma unroll
-      for (int c = 0; c < C; ++c) {
-        y0[c][q] = a0[c];
-        yn[c][q] = an[c];
-      }
-    } else {
-      cplx<T> a[C];
-#pragma unroll
-      for (int c = 0; c < C; ++c) a[c] = v[c][q];
-      op(f, kx, k, a);
-#pragma unroll
-      for (int c = 0; c < C; ++c) v[c][q] = a[c];
-    }
-  }
+template <typename T, int C>
+__device__ __forceinline__ cplx<T> pick(const cplx<T> (&a)[C], int c) {
+  if constexpr (C == 1)
+    return a[0];
+  else
+    return c == 0 ? a[0] : a[1];
 }
 
-// Column 0 of tile 0 back into the packed form Herm(Y0) + i Herm(Yn), into
-// v of the g == 0 lanes.  Uniform over the workgroup.
-template <typename T, int N, int KC, int C>
-__device__ __forceinline__ void col0_repack(cplx<T> (&v)[C][8], cplx<T> (&y0)[C][8],
-                                            const cplx<T> (&yn)[C][8],
-                                            cplx<T> (*lbuf)[KC][line_stride<N>()], int g, int t,
-                                            bool col0) {
-  constexpr int G = N / 8;
-  const T h = T(0.5);
-  __syncthreads();
-  if (col0) {
-#pragma unroll
-    for (int c = 0; c < C; ++c)
-#pragma unroll
-      for (int q = 0; q < 8; ++q) lbuf[c][g][lds_idx<N, false>(t + q * G)] = y0[c][q];
-  }
-  __syncthreads();
-  if (col0) {
-#pragma unroll
-    for (int c = 0; c < C; ++c)
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const int kx = t + q * G;
-        y0[c][q] = scale(y0[c][q] + conj(lbuf[c][g][lds_idx<N, false>((N - kx) & (N - 1))]), h);
-      }
-  }
-  __syncthreads();
-  if (col0) {
-#pragma unroll
-    for (int c = 0; c < C; ++c)
-#pragma unroll
-      for (int q = 0; q < 8; ++q) lbuf[c][g][lds_idx<N, false>(t + q * G)] = yn[c][q];
-  }
-  __syncthreads();
-  if (col0) {
-#pragma unroll
-    for (int c = 0; c < C; ++c)
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const int kx = t + q * G;
-        const cplx<T> n0 =
-            scale(yn[c][q] + conj(lbuf[c][g][lds_idx<N, false>((N - kx) & (N - 1))]), h);
-        v[c][q] = mk(y0[c][q].x - n0.y, y0[c][q].y + n0.x);
-      }
-  }
-}
-
-// packed -> op -> packed, in place: column FFT, op, inverse column FFT times
-// scale_ (1 / N^2).  A row C2R afterwards gives irfft2(op(rfft2(x))).
-template <typename T, int N, int KC, int C, typename Op>
-__global__ __launch_bounds__(KC * N / 8) void k_col_op(cplx<T>* P, Op op,
-                                                       const cplx<T>* __restrict__ twg,
-                                                       T scale_) {
+// One column pass of an operator edge.  A workgroup holds KC columns of all C
+// components of one field; thread (c, t, g) -- g fastest, so that consecutive
+// lanes read consecutive columns -- transforms line (component c, column
+// tile KC + g) with points t + q N/8, q < 8.  The per-mode operator mixes the
+// components, which meet through LDS: after the forward transform every line
+// is written to its LDS buffer and each thread reads the C values of its
+// modes there.  Column 0 (packed DC + i Nyquist) reads Z[kx] and Z[-kx] and
+// evaluates the operator at (kx, 0), (kx, N/2) -- and, for the repacked
+// output, at (-kx, 0), (-kx, N/2) -- so no second exchange of Z is needed.
+//   EDGE 0: packed -> op -> packed, in place (times scale_)
+//   EDGE 1: natural -> op -> packed (times scale_)
+//   EDGE 2: packed -> op -> natural (forward; scale_ unused)
+template <typename T, int N, int KC, int C, int EDGE, typename Op>
+__global__ __launch_bounds__(C * KC * N / 8) void k_col_edge(const cplx<T>* in, cplx<T>* out,
+                                                             Op op,
+                                                             const cplx<T>* __restrict__ twg,
+                                                             T scale_) {
   constexpr int G = N / 8, H = N / 2, TILES = H / KC;
+  constexpr long PN = (long)N * (H + 1), PP = (long)N * H;
   __shared__ cplx<T> lbuf[C][KC][line_stride<N>()];
-  const int g = threadIdx.x % KC, t = threadIdx.x / KC;
+  __shared__ cplx<T> col0[EDGE == 2 ? 1 : C][EDGE == 2 ? 1 : lds_len<N, false>()];
+  const int g = threadIdx.x % KC, t = (threadIdx.x / KC) % G, c = threadIdx.x / (KC * G);
   Twiddles<N, T> tw;
   tw.load(twg, t);
   const long f = blockIdx.x / TILES;
   const int tile = blockIdx.x % TILES;
   const int k = tile * KC + g;
-  cplx<T>* Pc = P + f * C * (long)N * H + k;
-  cplx<T> v[C][8];
+  auto L = [&](int cc, int kx) -> cplx<T>& { return lbuf[cc][g][lds_idx<N, false>(kx)]; };
+  cplx<T>* const bufs[1] = {lbuf[c][g]};
+  const T h = T(0.5);
+  cplx<T> v[1][8];
+  if constexpr (EDGE != 1) {
+    const cplx<T>* src = in + (f * C + c) * PP + k;
 #pragma unroll
-  for (int c = 0; c < C; ++c)
+    for (int q = 0; q < 8; ++q) v[0][q] = src[(long)(t + q * G) * H];
+    fft_lines<N, false, 1, false, false>(v, bufs, tw, t);
+    __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 8; ++q) v[c][q] = Pc[(c * (long)N + t + q * G) * H];
-  __syncthreads();
-  cplx<T>* bufs[C];
+    for (int q = 0; q < 8; ++q) L(c, t + q * G) = v[0][q];
+    __syncthreads();
+    if (k != 0) {
 #pragma unroll
-  for (int c = 0; c < C; ++c) bufs[c] = lbuf[c][g];
-  cplx<T>* const(&cb)[C] = bufs;
-  fft_lines<N, false, C, false, false>(v, cb, tw, t);
-  cplx<T> y0[C][8], yn[C][8];
-  col_modes_apply<T, N, KC, C>(v, y0, yn, lbuf, g, t, tile, f, op);
-  if (tile == 0) col0_repack<T, N, KC, C>(v, y0, yn, lbuf, g, t, k == 0);
-  fft_lines<N, true, C, false, false>(v, cb, tw, t);
+      for (int q = 0; q < 8; ++q) {
+        cplx<T> a[C];
 #pragma unroll
-  for (int c = 0; c < C; ++c)
-#pragma unroll
-    for (int q = 0; q < 8; ++q) Pc[(c * (long)N + t + q * G) * H] = scale(v[c][q], scale_);
-}
-
-// natural X -> op -> packed P: the column half of irfft2(op(X)), times
-// scale_ (1 / N^2); a row C2R follows.  Column 0 reads its four modes
-// (kx, 0), (-kx, 0), (kx, N/2), (-kx, N/2) and applies op to each.
-template <typename T, int N, int KC, int C, typename Op>
-__global__ __launch_bounds__(KC * N / 8) void k_col_inv_op(const cplx<T>* __restrict__ X,
-                                                           cplx<T>* __restrict__ P, Op op,
-                                                           const cplx<T>* __restrict__ twg,
-                                                           T scale_) {
-  constexpr int G = N / 8, H = N / 2, TILES = H / KC;
-  constexpr long PN = (long)N * (H + 1);
-  __shared__ cplx<T> lbuf[C][KC][line_stride<N>()];
-  const int g = threadIdx.x % KC, t = threadIdx.x / KC;
-  Twiddles<N, T> tw;
-  tw.load(twg, t);
-  const long f = blockIdx.x / TILES;
-  const int k = (blockIdx.x % TILES) * KC + g;
-  const cplx<T>* Xf = X + f * C * PN;
-  cplx<T> v[C][8];
-  if (k == 0) {
-    const T h = T(0.5);
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int kx = t + q * G, mx = (N - kx) & (N - 1);
-      cplx<T> a0[C], b0[C], an[C], bn[C];
-#pragma unroll
-      for (int c = 0; c < C; ++c) {
-        a0[c] = Xf[c * PN + (long)kx * (H + 1)];
-        b0[c] = Xf[c * PN + (long)mx * (H + 1)];
-        an[c] = Xf[c * PN + (long)kx * (H + 1) + H];
-        bn[c] = Xf[c * PN + (long)mx * (H + 1) + H];
-      }
-      op(f, kx, 0, a0);
-      op(f, mx, 0, b0);
-      op(f, kx, H, an);
-      op(f, mx, H, bn);
-#pragma unroll
-      for (int c = 0; c < C; ++c) {
-        const cplx<T> d0 = scale(a0[c] + conj(b0[c]), h);
-        const cplx<T> n0 = scale(an[c] + conj(bn[c]), h);
-        v[c][q] = mk(d0.x - n0.y, d0.y + n0.x);
+        for (int cc = 0; cc < C; ++cc) a[cc] = L(cc, t + q * G);
+        op(f, t + q * G, k, a);
+        v[0][q] = pick<T, C>(a, c);
       }
     }
   } else {
+    const cplx<T>* src = in + (f * C + c) * PN + k;
+    if (k != 0) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int kx = t + q * G;
-      cplx<T> a[C];
-#pragma unroll
-      for (int c = 0; c < C; ++c) a[c] = Xf[c * PN + (long)kx * (H + 1) + k];
-      op(f, kx, k, a);
-#pragma unroll
-      for (int c = 0; c < C; ++c) v[c][q] = a[c];
+      for (int q = 0; q < 8; ++q) v[0][q] = src[(long)(t + q * G) * (H + 1)];
     }
-  }
-  __syncthreads();
-  cplx<T>* bufs[C];
+    __syncthreads();
+    if (k != 0) {
 #pragma unroll
-  for (int c = 0; c < C; ++c) bufs[c] = lbuf[c][g];
-  cplx<T>* const(&cb)[C] = bufs;
-  fft_lines<N, true, C, false, false>(v, cb, tw, t);
-  cplx<T>* Pc = P + f * C * (long)N * H + k;
+      for (int q = 0; q < 8; ++q) L(c, t + q * G) = v[0][q];
+    }
+    __syncthreads();
+    if (k != 0) {
 #pragma unroll
-  for (int c = 0; c < C; ++c)
+      for (int q = 0; q < 8; ++q) {
+        cplx<T> a[C];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) Pc[(c * (long)N + t + q * G) * H] = scale(v[c][q], scale_);
-}
-
-// packed P -> op -> natural out: the column half of op(rfft2(x)) after a row
-// R2C (no scaling, numpy's forward convention).
-template <typename T, int N, int KC, int C, typename Op>
-__global__ __launch_bounds__(KC * N / 8) void k_col_fwd_op(const cplx<T>* __restrict__ P,
-                                                           cplx<T>* __restrict__ out, Op op,
-                                                           const cplx<T>* __restrict__ twg) {
-  constexpr int G = N / 8, H = N / 2, TILES = H / KC;
-  constexpr long PN = (long)N * (H + 1);
-  __shared__ cplx<T> lbuf[C][KC][line_stride<N>()];
-  const int g = threadIdx.x % KC, t = threadIdx.x / KC;
-  Twiddles<N, T> tw;
-  tw.load(twg, t);
-  const long f = blockIdx.x / TILES;
-  const int tile = blockIdx.x % TILES;
-  const int k = tile * KC + g;
-  const cplx<T>* Pc = P + f * C * (long)N * H + k;
-  cplx<T> v[C][8];
-#pragma unroll
-  for (int c = 0; c < C; ++c)
-#pragma unroll
-    for (int q = 0; q < 8; ++q) v[c][q] = Pc[(c * (long)N + t + q * G) * H];
-  __syncthreads();
-  cplx<T>* bufs[C];
-#pragma unroll
-  for (int c = 0; c < C; ++c) bufs[c] = lbuf[c][g];
-  cplx<T>* const(&cb)[C] = bufs;
-  fft_lines<N, false, C, false, false>(v, cb, tw, t);
-  cplx<T> y0[C][8], yn[C][8];
-  col_modes_apply<T, N, KC, C>(v, y0, yn, lbuf, g, t, tile, f, op);
-  cplx<T>* of = out + f * C * PN;
-#pragma unroll
-  for (int c = 0; c < C; ++c)
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const long r = (long)(t + q * G) * (H + 1);
-      if (k == 0) {
-        of[c * PN + r] = y0[c][q];
-        of[c * PN + r + H] = yn[c][q];
-      } else {
-        of[c * PN + r + k] = v[c][q];
+        for (int cc = 0; cc < C; ++cc) a[cc] = L(cc, t + q * G);
+        op(f, t + q * G, k, a);
+        v[0][q] = pick<T, C>(a, c);
       }
     }
+  }
+  // Column 0, spread over the workgroup's KC column lanes of tile 0: lane g
+  // takes kx = t + q N/8 for q = g, g + KC, ... (so no lane chains more than
+  // 8 / KC of its operator loads); its results go through LDS (col0) to the
+  // column's own lanes, or straight out.
+  if (tile == 0) {
+    const cplx<T>* nat = in + f * C * PN;  // EDGE 1: natural input
+    for (int q = g; q < 8; q += KC) {
+      const int kx = t + q * G, mx = (N - kx) & (N - 1);
+      cplx<T> a0[C], an[C], b0[C], bn[C];
+#pragma unroll
+      for (int cc = 0; cc < C; ++cc) {
+        if constexpr (EDGE == 1) {
+          a0[cc] = nat[cc * PN + (long)kx * (H + 1)];
+          b0[cc] = nat[cc * PN + (long)mx * (H + 1)];
+          an[cc] = nat[cc * PN + (long)kx * (H + 1) + H];
+          bn[cc] = nat[cc * PN + (long)mx * (H + 1) + H];
+        } else {
+          const cplx<T> z = lbuf[cc][0][lds_idx<N, false>(kx)];
+          const cplx<T> zm = lbuf[cc][0][lds_idx<N, false>(mx)];
+          const cplx<T> d = z - conj(zm), dm = zm - conj(z);
+          a0[cc] = scale(z + conj(zm), h);
+          an[cc] = mk(h * d.y, -h * d.x);
+          b0[cc] = scale(zm + conj(z), h);
+          bn[cc] = mk(h * dm.y, -h * dm.x);
+        }
+      }
+      op(f, kx, 0, a0);
+      op(f, kx, H, an);
+      if constexpr (EDGE == 2) {
+        cplx<T>* o = out + (f * C + c) * PN + (long)kx * (H + 1);
+        o[0] = pick<T, C>(a0, c);
+        o[H] = pick<T, C>(an, c);
+      } else {
+        op(f, mx, 0, b0);
+        op(f, mx, H, bn);
+        const cplx<T> y0 = scale(pick<T, C>(a0, c) + conj(pick<T, C>(b0, c)), h);
+        const cplx<T> yn = scale(pick<T, C>(an, c) + conj(pick<T, C>(bn, c)), h);
+        col0[c][lds_idx<N, false>(kx)] = mk(y0.x - yn.y, y0.y + yn.x);
+      }
+    }
+  }
+  if constexpr (EDGE == 2) {
+    if (k != 0) {
+      cplx<T>* dst = out + (f * C + c) * PN + k;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) dst[(long)(t + q * G) * (H + 1)] = v[0][q];
+    }
+  } else {
+    if (tile == 0) {  // uniform over the workgroup
+      __syncthreads();
+      if (k == 0) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[0][q] = col0[c][lds_idx<N, false>(t + q * G)];
+      }
+    }
+    fft_lines<N, true, 1, false, false>(v, bufs, tw, t);
+    cplx<T>* dst = out + (f * C + c) * PP + k;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) dst[(long)(t + q * G) * H] = scale(v[0][q], scale_);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -852,8 +752,9 @@ hipError_t launch_wf_final_teb(const cplx<T>* XLT, const cplx<T>* XLP, const cpl
 }
 
 // Columns per workgroup of the operator column passes: 128-byte row segments
-// (16 float / 8 double complex), halved for Q/U pairs at N = 512, where two
-// 16-column sets of line buffers would not fit the LDS.
+// (16 float / 8 double complex), halved for Q/U pairs at N = 512 so that a
+// workgroup (C KC N / 8 threads, C KC line buffers) stays within 1,024
+// threads and half the LDS.
 template <typename T, int N, int C>
 constexpr int col_op_kc() {
   constexpr int base = sizeof(T) == 4 ? 16 : 8;
@@ -872,14 +773,14 @@ constexpr int col_op_kc() {
     default: return hipErrorInvalidValue;                      \
   }
 
-template <typename T, int C, template <typename, int> class OpT>
-static hipError_t col_op_c(int N, cplx<T>* P, const OpT<T, C>& op, long nfields,
-                           const cplx<T>* tw, hipStream_t s) {
+template <typename T, int C, int EDGE, template <typename, int> class OpT>
+static hipError_t col_edge(int N, const cplx<T>* in, cplx<T>* out, const OpT<T, C>& op,
+                           long nfields, const cplx<T>* tw, hipStream_t s) {
   CMBL_COL_DISPATCH(N, {
     constexpr int KC = col_op_kc<T, NN_, C>();
     const T sc = T(1) / (T(NN_) * T(NN_));
-    k_col_op<T, NN_, KC, C, OpT<T, C>>
-        <<<(unsigned)(nfields * (NN_ / 2 / KC)), KC * NN_ / 8, 0, s>>>(P, op, tw, sc);
+    k_col_edge<T, NN_, KC, C, EDGE, OpT<T, C>>
+        <<<(unsigned)(nfields * (NN_ / 2 / KC)), C * KC * NN_ / 8, 0, s>>>(in, out, op, tw, sc);
   });
   return hipGetLastError();
 }
@@ -887,37 +788,24 @@ static hipError_t col_op_c(int N, cplx<T>* P, const OpT<T, C>& op, long nfields,
 template <typename T>
 hipError_t launch_col_op(int N, cplx<T>* P, const T* diag, long nfields, int C,
                          const cplx<T>* tw, hipStream_t s) {
-  if (C == 2) return col_op_c<T, 2, OpDiagRot>(N, P, OpDiagRot<T, 2>{diag, N}, nfields, tw, s);
-  if (C == 1) return col_op_c<T, 1, OpDiagRot>(N, P, OpDiagRot<T, 1>{diag, N}, nfields, tw, s);
+  if (C == 2) return col_edge<T, 2, 0, OpDiagRot>(N, P, P, {diag, N}, nfields, tw, s);
+  if (C == 1) return col_edge<T, 1, 0, OpDiagRot>(N, P, P, {diag, N}, nfields, tw, s);
   return hipErrorInvalidValue;
 }
 
 template <typename T>
 hipError_t launch_col_data(int N, cplx<T>* P, const cplx<T>* d, const T* mb, const T* bmcn,
                            long nfields, int C, const cplx<T>* tw, hipStream_t s) {
-  if (C == 2)
-    return col_op_c<T, 2, OpData>(N, P, OpData<T, 2>{d, mb, bmcn, N}, nfields, tw, s);
-  if (C == 1)
-    return col_op_c<T, 1, OpData>(N, P, OpData<T, 1>{d, mb, bmcn, N}, nfields, tw, s);
+  if (C == 2) return col_edge<T, 2, 0, OpData>(N, P, P, {d, mb, bmcn, N}, nfields, tw, s);
+  if (C == 1) return col_edge<T, 1, 0, OpData>(N, P, P, {d, mb, bmcn, N}, nfields, tw, s);
   return hipErrorInvalidValue;
 }
 
 template <typename T>
 hipError_t launch_col_inv_eb2qu(int N, const cplx<T>* X, cplx<T>* P, long nfields, int C,
                                 const cplx<T>* tw, hipStream_t s) {
-  auto go = [&](auto cc) -> hipError_t {
-    constexpr int CC = decltype(cc)::value;
-    CMBL_COL_DISPATCH(N, {
-      constexpr int KC = col_op_kc<T, NN_, CC>();
-      const T sc = T(1) / (T(NN_) * T(NN_));
-      k_col_inv_op<T, NN_, KC, CC, OpEb2Qu<T, CC>>
-          <<<(unsigned)(nfields * (NN_ / 2 / KC)), KC * NN_ / 8, 0, s>>>(
-              X, P, OpEb2Qu<T, CC>{N}, tw, sc);
-    });
-    return hipGetLastError();
-  };
-  if (C == 2) return go(std::integral_constant<int, 2>{});
-  if (C == 1) return go(std::integral_constant<int, 1>{});
+  if (C == 2) return col_edge<T, 2, 1, OpEb2Qu>(N, X, P, {N}, nfields, tw, s);
+  if (C == 1) return col_edge<T, 1, 1, OpEb2Qu>(N, X, P, {N}, nfields, tw, s);
   return hipErrorInvalidValue;
 }
 
@@ -925,18 +813,8 @@ template <typename T>
 hipError_t launch_col_fwd_final(int N, const cplx<T>* P, const cplx<T>* F, const T* cfinv,
                                 cplx<T>* out, long nfields, int C, const cplx<T>* tw,
                                 hipStream_t s) {
-  auto go = [&](auto cc) -> hipError_t {
-    constexpr int CC = decltype(cc)::value;
-    CMBL_COL_DISPATCH(N, {
-      constexpr int KC = col_op_kc<T, NN_, CC>();
-      k_col_fwd_op<T, NN_, KC, CC, OpFinal<T, CC>>
-          <<<(unsigned)(nfields * (NN_ / 2 / KC)), KC * NN_ / 8, 0, s>>>(
-              P, out, OpFinal<T, CC>{F, cfinv, N}, tw);
-    });
-    return hipGetLastError();
-  };
-  if (C == 2) return go(std::integral_constant<int, 2>{});
-  if (C == 1) return go(std::integral_constant<int, 1>{});
+  if (C == 2) return col_edge<T, 2, 2, OpFinal>(N, P, out, {F, cfinv, N}, nfields, tw, s);
+  if (C == 1) return col_edge<T, 1, 2, OpFinal>(N, P, out, {F, cfinv, N}, nfields, tw, s);
   return hipErrorInvalidValue;
 }
 
