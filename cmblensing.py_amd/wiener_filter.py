@@ -153,12 +153,16 @@ def _cg_step(prob, x, r, z, p, Ap, pinv_arr, p_op, res_d, act_d, bb_d, rel_d, B)
 
 
 def _host(t: torch.Tensor):
-    """Wait point of a solve process: yields an event recorded after the work
-    that produces t, then returns t on the host as float64."""
+    """Wait point of a solve process: copies t into pinned host memory behind
+    the work that produces it, yields an event recorded after the copy, then
+    returns the host copy as float64 (one wait, no second device round
+    trip)."""
+    h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+    h.copy_(t, non_blocking=True)
     ev = torch.cuda.Event()
     ev.record(torch.cuda.current_stream(t.device))
     yield ev
-    return t.cpu().double()
+    return h.double()
 
 
 def run_process(gen):
@@ -333,10 +337,12 @@ def conjugate_gradient_process(x0, data, phi, mask, beam, noise_covariance, fiel
     pending = None
     res_d = res_h.to(dev)
     rel_d = torch.empty(B, dtype=torch.float64, device=dev)
+    act_prev, act_d = None, None
     while bool(act.any()):
         idx = None if (bool(act.all()) or not COMPACT_ACTIVE) else \
             [i for i in range(B) if bool(act[i])]
-        act_d = act.to(dev)
+        if act_prev is None or not torch.equal(act, act_prev):
+            act_prev, act_d = act.clone(), act.to(dev)   # uploaded when it changes
         Ap = prob.A_active(p_op, idx)                         # :82
         # :84-95 in one call (alpha, x, r, z, dot(r, z), beta, p, res), with
         # sqrt(dot(r, r) / dot(b, b)) for the relative test
