@@ -80,6 +80,43 @@ def test_gradf_matches_oracle(pkg, dev, sims, mask, cdt, tol):
     assert err < tol
 
 
+_ORACLE_GRADF = {}
+
+
+@pytest.mark.parametrize("N", [16, 32, 128, 512])
+@pytest.mark.parametrize("pol", [True, False])
+@pytest.mark.parametrize("mask", ["fourier", "holes"])
+@pytest.mark.parametrize("cdt,tol", [(torch.complex128, 1e-10), (torch.complex64, 1e-5)])
+def test_gradf_sizes_components_masks(pkg, dev, sims, N, pol, mask, cdt, tol):
+    """The operator's column passes with the basis changes and diagonal
+    operators at their edges, at every tile shape they take: each N has its
+    own columns per workgroup and its own spread of the packed column 0 over
+    them, T (C = 1) and Q/U (C = 2) differ in the exchange, the Fourier mask
+    runs the data-term pass and the pixel mask the beam / Cn^-1 passes with
+    the mask in the row C2R.  For Q/U the prior term is ~99.9 % of the
+    gradient's norm (the B-mode power is 1e-3 of the E-mode's), so the data
+    term -- everything the lensing and the edge passes compute -- is also
+    checked alone, with the prior switched off (Cf scaled by 1e30)."""
+    from cmblensing_amd import gradients
+    th = 1.5 if N == 512 else 2.0
+    _case(f"gradf N={N} {'QU' if pol else 'T'} mask={mask} {cdt}")
+    key = (N, pol, mask)
+    if key not in _ORACLE_GRADF:
+        prob = make_problem(sims, N, th, mask, pol=pol, seed=3)
+        refs = [owf.gradf_logpdf(prob["f"], prob["phi"], prob["d"], prob["Cf"] * scale,
+                                 prob["Cn"], prob["M"], prob["B"], th, n=7,
+                                 mask_pix=prob["mask_pix"]) for scale in (1.0, 1e30)]
+        _ORACLE_GRADF[key] = (prob, refs)
+    prob, refs = _ORACLE_GRADF[key]
+    for part, scale, ref in (("full", 1.0, refs[0]), ("data term", 1e30, refs[1])):
+        pr = dict(prob, Cf=prob["Cf"] * scale)
+        f, phi, data, Cf, Cn, M, B = gpu_objects(pkg, pr, dev, cdt)
+        g = stack(gradients.gradf_logpdf(f, phi, data, Cf, Cn, M, B, n=7))
+        err = rel(g, ref)
+        print(f"[err] gradf {part} N={N} pol={pol} {mask} vs oracle {err:.2e}", flush=True)
+        assert err < tol
+
+
 def test_gradf_julia_I_fp64(pkg, dev):
     from cmblensing_amd import gradients, matrix_operators as mo
     _case("gradf julia I N=256 fp64")
