@@ -233,24 +233,10 @@ def get_d_matrix(cf_tt, cf_te, cf_ee, cf_bb, cn_tt, cn_te, cn_ee, cn_bb):
     return d_tt, d_te, d_ee, torch.sqrt(d_bb)
 
 
-def get_d_tt_matrix(cfs_tt, cft_tt, cn_tt, r, r_fid):
-    """simulate.py:170-176"""
-    pre = math.radians(5 / ARCMIN_PER_DEGREE) ** 2
-    cf_r = cfs_tt + (r / r_fid) * cft_tt
-    return torch.sqrt((cf_r + pre + 2 * cn_tt) * reciprocal_matrix(cf_r))
-
-
 def get_g_matrix(cphi_fid, nphi, a_phi_fid, a_phi=1):
     """simulate.py:180-185"""
     g0 = torch.sqrt(1 + 2 * nphi * reciprocal_matrix(cphi_fid))
     g = torch.sqrt(1 + 2 * nphi * reciprocal_matrix((a_phi / a_phi_fid) * cphi_fid))
-    return reciprocal_matrix(g0) * g
-
-
-def get_g_matrix_lcdm(cphi_fid, cphi_curr, nphi):
-    """simulate.py:187-192"""
-    g0 = torch.sqrt(1 + 2 * nphi * reciprocal_matrix(cphi_fid))
-    g = torch.sqrt(1 + 2 * nphi * reciprocal_matrix(cphi_curr))
     return reciprocal_matrix(g0) * g
 
 
@@ -679,14 +665,3 @@ def batch_simulated_trials(num_trials: int, nside: int, theta_pix: float, cls: d
     streams 0..num_trials-1 of one master seed."""
     cfg = sim_config(nside, theta_pix, pol, cls, lmax, uk_arcmin_t, device=device)
     return build_dataset(cfg, *simulate(cfg, master_seed, num_trials, dtype, n))
-
-
-def f_sky(nside: int, theta_pix: float) -> float:
-    """simulate.py:796-799"""
-    dx = np.deg2rad(theta_pix / ARCMIN_PER_DEGREE)
-    return nside ** 2 * dx ** 2 * (180 / np.pi) ** 2 / 40_000
-
-
-def log_c_ell_variance(ell, f_sky_value, delta_l):
-    """simulate.py:802-803"""
-    return 2 / (delta_l * (2 * ell + 1) * f_sky_value)

@@ -263,24 +263,10 @@ def get_d_matrix(cf_tt, cf_te, cf_ee, cf_bb, cn_tt, cn_te, cn_ee, cn_bb):
     return d_tt, d_te, d_ee, np.sqrt(d_bb)
 
 
-def get_d_tt_matrix(cfs_tt, cft_tt, cn_tt, r, r_fid):
-    """simulate.py:170-176"""
-    pre = np.deg2rad(5 / ARCMIN_PER_DEGREE) ** 2
-    cf_r = cfs_tt + (r / r_fid) * cft_tt
-    return np.sqrt((cf_r + pre * np.ones(cn_tt.shape) + 2 * cn_tt) * reciprocal(cf_r))
-
-
 def get_g_matrix(cphi_fid, nphi, a_phi_fid, a_phi=1):
     """simulate.py:180-185"""
     g0 = np.sqrt(1 + 2 * nphi * reciprocal(cphi_fid))
     g = np.sqrt(1 + 2 * nphi * reciprocal((a_phi / a_phi_fid) * cphi_fid))
-    return reciprocal(g0) * g
-
-
-def get_g_matrix_lcdm(cphi_fid, cphi_curr, nphi):
-    """simulate.py:187-192"""
-    g0 = np.sqrt(1 + 2 * nphi * reciprocal(cphi_fid))
-    g = np.sqrt(1 + 2 * nphi * reciprocal(cphi_curr))
     return reciprocal(g0) * g
 
 

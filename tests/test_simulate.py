@@ -366,9 +366,6 @@ def test_package_mixing_matrices_match_oracle(psim):
     assert np.allclose(psim.get_g_matrix(torch.as_tensor(cphi), torch.as_tensor(nphi), 1.0,
                                          2.0).numpy(),
                        osim.get_g_matrix(cphi, nphi, 1.0, 2.0), rtol=1e-14)
-    assert np.allclose(psim.get_d_tt_matrix(*[torch.as_tensor(a) for a in (cf_tt, cf_ee, cn_tt)],
-                                            0.1, 0.2).numpy(),
-                       osim.get_d_tt_matrix(cf_tt, cf_ee, cn_tt, 0.1, 0.2), rtol=1e-14)
 
 
 def test_package_qe_legs_match_oracle(psim):
