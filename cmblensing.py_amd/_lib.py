@@ -51,6 +51,8 @@ _SIGS = {
     "cmbl_lensed_cotangent": (c_int, [c_void_p, ctypes.POINTER(c_void_p), c_void_p, c_void_p,
                                       c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
                                       c_void_p]),
+    "cmbl_gradf_final": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
+                                 c_void_p]),
     "cmbl_dot_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p,
                              c_void_p]),
     "cmbl_cg_update_xrz": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -540,6 +542,15 @@ def lensed_cotangent(plan: Plan, phis, f, d, ops: "WfOps", lensed, cot, batch: i
         check(plan.lib.cmbl_lensed_cotangent(plan.handle, arr, _ptr(f), _ptr(d), ctypes.byref(ops),
                                              lensed.data_ptr(), cot.data_ptr(), batch, ncomp,
                                              nsteps, st), "cmbl_lensed_cotangent")
+
+
+def gradf_final(plan: Plan, adj, f, cfinv, out, batch: int, ncomp: int):
+    """cmbl_gradf_final: out = qu2eb(fourier(adj)) - cfinv .* Herm(f) (f or
+    cfinv None: no prior term)."""
+    with torch.cuda.device(plan.device):
+        check(plan.lib.cmbl_gradf_final(plan.handle, adj.data_ptr(), _ptr(f), _ptr(cfinv),
+                                        out.data_ptr(), batch, ncomp, stream_ptr(plan.device)),
+              "cmbl_gradf_final")
 
 
 def dot_f64(plan: Plan, a, b, nrhs: int, ncomp: int, out):

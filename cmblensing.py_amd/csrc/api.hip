@@ -1133,6 +1133,35 @@ extern "C" cmbl_status cmbl_lensed_cotangent(cmbl_plan* p, const cmbl_phi* const
   return run_gradf<double>(p, phis, f, d, ops, nullptr, batch, ncomp, nsteps, s, lensed, cot);
 }
 
+extern "C" cmbl_status cmbl_gradf_final(cmbl_plan* p, const void* adj, const void* f,
+                                        const void* cfinv, void* out, int batch, int ncomp,
+                                        void* stream) {
+  if (!p || !adj || !out) return fail(CMBL_EINVAL, "NULL argument");
+  if (batch < 0) return fail(CMBL_EINVAL, "batch < 0");
+  if (ncomp != 1 && ncomp != 2) return fail(CMBL_EINVAL, "ncomp must be 1 (T) or 2 (EB)");
+  if ((long)batch * ncomp > p->max_maps)
+    return fail(CMBL_EINVAL, "batch*ncomp exceeds the plan's max_maps");
+  if (batch == 0) return CMBL_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const long nm = (long)batch * ncomp;
+  const bool prior = f && cfinv;
+  if (p->dtype == CMBL_F32) {
+    const cplx<float>* tw = (const cplx<float>*)p->tw;
+    HIP_TRY(cmbl::launch_lf_row_r2c<float>(p->N, (const float*)adj, (cplx<float>*)p->s0, nm, tw, s));
+    HIP_TRY(cmbl::launch_col_fwd_final<float>(
+        p->N, (const cplx<float>*)p->s0, prior ? (const cplx<float>*)f : nullptr,
+        prior ? (const float*)cfinv : nullptr, (cplx<float>*)out, batch, ncomp, tw, s));
+  } else {
+    const cplx<double>* tw = (const cplx<double>*)p->tw;
+    HIP_TRY(cmbl::launch_lf_row_r2c<double>(p->N, (const double*)adj, (cplx<double>*)p->s0, nm, tw,
+                                            s));
+    HIP_TRY(cmbl::launch_col_fwd_final<double>(
+        p->N, (const cplx<double>*)p->s0, prior ? (const cplx<double>*)f : nullptr,
+        prior ? (const double*)cfinv : nullptr, (cplx<double>*)out, batch, ncomp, tw, s));
+  }
+  return CMBL_OK;
+}
+
 extern "C" cmbl_status cmbl_dot_f64(cmbl_plan* p, const void* a, const void* b, int b_f64, int nrhs,
                                     int ncomp, double* out, void* stream) {
   if (!p || !a || !b || !out) return fail(CMBL_EINVAL, "NULL argument");

@@ -76,12 +76,11 @@ def mixed_grad_phi_logpdf(mixed_field, mixed_phi, data, noise_covariance, phi_co
                           field_covariance, mask, beam, mixing_d, mixing_g,
                           n: int = DEFAULT_NSTEPS) -> FlatS0:
     """gradients.py:40-65: G^-1 (grad_phi + J^T grad_f) at the unmixed point."""
-    from .gradients import gradf_logpdf
-    from .statistics import grad_phi_logpdf
+    from .statistics import grad_logpdf
     field, phi = unmix(mixed_field, mixed_phi, mixing_d, mixing_g, n)
-    gphi = grad_phi_logpdf(field, phi, data, noise_covariance, phi_covariance, field_covariance,
+    # both gradients from one forward LenseFlow and one gradient ODE
+    gf, gphi = grad_logpdf(field, phi, data, noise_covariance, phi_covariance, field_covariance,
                            mask, beam, n=n)
-    gf = gradf_logpdf(field, phi, data, field_covariance, noise_covariance, mask, beam, n=n)
     chain = mixing_jacobian_phi_component(mixed_field, phi, gf, mixing_d, n)
     out = gphi.replace(scalar_matrix=gphi.scalar_matrix + chain.to(gphi.scalar_matrix.dtype))
     return pinv(mixing_g) * out

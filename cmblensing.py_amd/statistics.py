@@ -147,9 +147,47 @@ def grad_phi_logpdf(unlensed_field, phi, data, noise_covariance, phi_covariance,
         cot = to_map(eb2qu(g))
         c = torch.stack([getattr(cot, nm).reshape(-1, N, N) for nm in names], dim=1)
     _, d_phi = lense_flow_vjp_matrices(y, c, phi_m, f.theta_pix, n, 1)
-    # one gradient per batch element (each chain's own phi), or a single one
+    return _phi_gradient(d_phi, phi, phi_m, phi_covariance, lead, N, f.theta_pix)
+
+
+def _phi_gradient(d_phi, phi, phi_m, phi_covariance, lead, N, theta_pix) -> FlatS0:
+    """The phi gradient from the gradient ODE's Fourier delta-phi: one per
+    batch element (each chain's own phi) or a single one, minus the prior's
+    Cphi^-1 phi (statistics.py:196-208)."""
     d_phi = d_phi.reshape(*lead, N, N // 2 + 1) if len(lead) else d_phi[0]
     prior = pinv(phi_covariance) * (phi if isinstance(phi, Field) else FlatS0(scalar_matrix=phi_m))
     out = d_phi.to(phi_m.dtype) - prior.scalar_matrix
-    return FlatS0(scalar_matrix=out, theta_pix=f.theta_pix, basis=Basis.FOURIER,
+    return FlatS0(scalar_matrix=out, theta_pix=theta_pix, basis=Basis.FOURIER,
                   parametrization=Parametrization.T)
+
+
+def grad_logpdf(unlensed_field, phi, data, noise_covariance, phi_covariance, field_covariance,
+                mask, beam, n: int = DEFAULT_NSTEPS):
+    """Both gradients of logpdf, (d/d f, d/d phi), as gradf_logpdf and
+    grad_phi_logpdf return them (EB or T FOURIER; FOURIER phi), from one
+    forward LenseFlow and one gradient ODE -- what one reverse pass of the
+    reference's jax.grad(logpdf, argnums=(0, 1)) computes (gradients.py:12-37,
+    statistics.py:138-213).  The gradient ODE's delta f is L^T applied to the
+    data term's cotangent, the adjoint LenseFlow gradf_logpdf runs, so
+    d/d f = qu2eb(fourier(delta f)) - Cf^-1 f (cmbl_gradf_final) needs no
+    second forward and no adjoint LenseFlow.  T + EB fields take the two
+    separate calls."""
+    from .gradients import gradf_logpdf, per_mode, stack_matrices, unstack_matrices
+    f = unlensed_field
+    phi_m = _phi_matrix(phi)
+    if phi_m.shape[-1] == phi_m.shape[-2]:
+        raise ValueError("grad_logpdf takes phi in the FOURIER basis, like the reference")
+    if isinstance(f, FlatS02):
+        return (gradf_logpdf(f, phi, data, field_covariance, noise_covariance, mask, beam, n=n),
+                grad_phi_logpdf(f, phi, data, noise_covariance, phi_covariance, field_covariance,
+                                mask, beam, n=n))
+    y, c, lead = _lensed_and_cotangent(f, phi_m, data, noise_covariance, mask, beam, n)
+    B, C, N = y.shape[0], y.shape[1], y.shape[-1]
+    d_field, d_phi = lense_flow_vjp_matrices(y, c, phi_m, f.theta_pix, n, 1)
+    X, lead_f = stack_matrices(f)
+    plan = _lib.get_plan(N, f.theta_pix, y.dtype, B * C, X.device)
+    cfinv = per_mode(pinv(field_covariance), y.dtype, X.device)
+    out = torch.empty((B, C, N, N // 2 + 1), dtype=X.dtype, device=X.device)
+    _lib.gradf_final(plan, d_field, torch.view_as_real(X), cfinv, torch.view_as_real(out), B, C)
+    return (unstack_matrices(f, out, lead_f),
+            _phi_gradient(d_phi, phi, phi_m, phi_covariance, lead, N, f.theta_pix))

@@ -198,6 +198,19 @@ cmbl_status cmbl_lensed_cotangent(cmbl_plan* plan, const cmbl_phi* const* phis, 
                                   const void* d, const cmbl_wf_ops* ops, void* lensed, void* cot,
                                   int batch, int ncomp, int nsteps, void* stream);
 
+/* The last line of gradf_logpdf (gradients.py:25) from the adjoint-lensed
+ * cotangent instead of from f: out = qu2eb(fourier(adj)) - Cf^-1 f, with
+ * adj = L^T map(eb2qu(g)) [batch][ncomp][N][N] MAP (T or Q,U) -- the d_field
+ * of cmbl_lenseflow_vjp run on cmbl_lensed_cotangent's two outputs -- f the
+ * EB (or T) FOURIER field and cfinv Cf^-1 per mode [ncomp][N][N/2+1] in the
+ * plan dtype (either NULL drops the prior term; the prior is applied per mode
+ * to the Hermitian part of f, as cmbl_gradf_logpdf does).  So one forward
+ * LenseFlow and one gradient ODE give both gradients of logpdf, as one
+ * reverse pass of the reference's jax.grad does (gradients.py:29-37).
+ * ncomp 1 (T) or 2 (EB); out may not alias f. */
+cmbl_status cmbl_gradf_final(cmbl_plan* plan, const void* adj, const void* f, const void* cfinv,
+                             void* out, int batch, int ncomp, void* stream);
+
 /* FP64 inner products for nrhs right-hand sides (fields.py:136-142): out[k] =
  * Re sum conj(a_k) b_k w / N^2 over [ncomp][N][N/2+1].  a is complex double;
  * b is complex double (b_f64 = 1) or complex in the plan dtype.  out is a

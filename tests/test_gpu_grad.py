@@ -295,6 +295,85 @@ def test_logpdf_autograd_pixel_mask_batch(pkg, dev, sims):
     assert eo < 1e-9
 
 
+@pytest.mark.parametrize("pol", [True, False])
+@pytest.mark.parametrize("mask", ["fourier", "square"])
+@pytest.mark.parametrize("cdt,tol", [(torch.complex128, 1e-10), (torch.complex64, 1e-5)])
+def test_grad_logpdf_equals_separate_gradients(pkg, dev, sims, pol, mask, cdt, tol):
+    """statistics.grad_logpdf -- both gradients from one forward LenseFlow and
+    one gradient ODE, d/d f from the ODE's delta f -- against gradf_logpdf
+    (forward and adjoint LenseFlow) and grad_phi_logpdf, for 3 fields with a
+    FOURIER phi each, with the prior and (Cf scaled by 1e30) without it, so
+    that the data term is checked where the prior does not dominate: equal in
+    FP64 (1e-10), the FP32 result within 1e-5 of the FP64 one, and the FP64
+    result against the oracle."""
+    from cmblensing_amd import gradients, statistics, matrix_operators as mo
+    from test_gpu_wiener import make_problem
+    N, th, nb = 64, 2.0, 3
+    _case(f"grad_logpdf vs separate N={N} {'QU' if pol else 'T'} {mask} {cdt}")
+    probs = [make_problem(sims, N, th, mask, pol=pol, seed=70 + k) for k in range(nb)]
+    _, Cphi = sims.phi_map_and_cov(N, th, 71)
+    rdt = torch.float32 if cdt == torch.complex64 else torch.float64
+    T = lambda a, dt=cdt: torch.as_tensor(a, device=dev).to(dt)
+    C = probs[0]["C"]
+    names = ("polar_matrix_1", "polar_matrix_2") if C == 2 else ("scalar_matrix",)
+    Fld = pkg.FlatS2 if C == 2 else pkg.FlatS0
+    pixel = probs[0]["mask_pix"] is not None
+    f = Fld(**{nm: T(np.stack([p["f"][c] for p in probs])) for c, nm in enumerate(names)},
+            theta_pix=th)
+    d = Fld(**{nm: T(np.stack([p["d"][c] for p in probs]), rdt if pixel else cdt)
+               for c, nm in enumerate(names)}, theta_pix=th)
+    phis = np.stack([np.fft.rfft2(p["phi"]) for p in probs])
+    phi = pkg.FlatS0(scalar_matrix=T(phis), theta_pix=th)
+    Cp = mo.DiagonalScalar(scalar_matrix=T(Cphi, torch.float64), theta_pix=th)
+
+    def op(A):
+        if C == 2:
+            return mo.DiagonalEB(matrix_EE=T(A[0], torch.float64),
+                                 matrix_BB=T(A[1], torch.float64), theta_pix=th)
+        return mo.DiagonalScalar(scalar_matrix=T(A[0], torch.float64), theta_pix=th)
+
+    p0 = probs[0]
+    M = mo.PixelMask(matrix=torch.as_tensor(p0["mask_pix"], device=dev)) if pixel else op(p0["M"])
+    stk = lambda fl: np.stack([getattr(fl, nm).cpu().numpy() for nm in names], axis=1)
+    for scale in (1.0, 1e30):
+        Cf = op(p0["Cf"] * scale)
+        gf, gp = statistics.grad_logpdf(f, phi, d, op(p0["Cn"]), Cp, Cf, M, op(p0["B"]), n=7)
+        gf_s = gradients.gradf_logpdf(f, phi, d, Cf, op(p0["Cn"]), M, op(p0["B"]), n=7)
+        gp_s = statistics.grad_phi_logpdf(f, phi, d, op(p0["Cn"]), Cp, Cf, M, op(p0["B"]), n=7)
+        ef = rel(stk(gf), stk(gf_s))
+        ep = rel(gp.scalar_matrix.cpu().numpy(), gp_s.scalar_matrix.cpu().numpy())
+        print(f"[err] Cf x{scale:g}: grad_logpdf vs gradf {ef:.2e}, vs grad_phi {ep:.2e}",
+              flush=True)
+        assert ep < tol
+        if cdt == torch.complex128:
+            assert ef < tol
+        else:
+            # two FP32 routes to d/d f (the adjoint LenseFlow, the gradient
+            # ODE's delta f); the data term at the truth is a small residual
+            # of large maps, so their FP32 rounding differs by up to ~2e-5 of
+            # it for T (printed above): the combined route is held to the FP32
+            # bar against FP64
+            f64 = Fld(**{nm: getattr(f, nm).to(torch.complex128) for nm in names}, theta_pix=th)
+            d64 = Fld(**{nm: getattr(d, nm).to(torch.float64 if pixel else torch.complex128)
+                         for nm in names}, theta_pix=th)
+            phi64 = pkg.FlatS0(scalar_matrix=T(phis, torch.complex128), theta_pix=th)
+            op64 = lambda A: op(A)
+            g64, p64 = statistics.grad_logpdf(f64, phi64, d64, op64(p0["Cn"]), Cp, Cf, M,
+                                              op64(p0["B"]), n=7)
+            e64 = rel(stk(gf), stk(g64))
+            e64p = rel(gp.scalar_matrix.cpu().numpy(), p64.scalar_matrix.cpu().numpy())
+            print(f"[err] Cf x{scale:g}: FP32 grad_logpdf vs FP64: f {e64:.2e}, phi {e64p:.2e}",
+                  flush=True)
+            assert e64 < tol and e64p < tol
+        if cdt == torch.complex128:
+            for k in (0, 2):
+                pk = probs[k]
+                mk = dict(mask_pix=pk["mask_pix"]) if pixel else {}
+                rf = owf.gradf_logpdf(pk["f"], pk["phi"], pk["d"], p0["Cf"] * scale, p0["Cn"],
+                                      None if pixel else p0["M"], p0["B"], th, n=7, **mk)
+                assert rel(stk(gf)[k], rf) < 1e-9
+
+
 def test_lense_flow_wrapper_shared_phi_batch(pkg, dev, sims):
     """A phi with a leading axis of 1 shared by a field batch of 2: the phi
     gradient is the sum of the two elements' gradients (MAP and FOURIER phi)."""
