@@ -43,6 +43,8 @@ def test_validation_errors_without_device(pkg):
     assert lib.cmbl_plan_create(ctypes.byref(h), 256, -1.0, 0, 4) != 0     # bad pixel size
     assert lib.cmbl_plan_create(None, 256, 2.0, 0, 4) != 0
     assert lib.cmbl_lenseflow(None, None, None, None, 1, 2, 7, 1, 0, None) != 0
+    assert lib.cmbl_gradf_final(None, None, None, None, None, 1, 2, None) != 0
+    assert "NULL argument" in lib.cmbl_last_error().decode()
     assert lib.cmbl_plan_destroy(None) == 0
 
 

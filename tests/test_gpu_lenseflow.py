@@ -252,7 +252,8 @@ def test_negative_batch_and_zero_steps_rejected(pkg, dev):
     """The gradient, cotangent and VJP entry points refuse batch < 0 and
     nsteps < 1 with an error, before any launch (a negative batch would
     otherwise reach a launch as a huge grid); the CG p-update refuses
-    nrhs < 1 and ncomp outside 1..3."""
+    nrhs < 1 and ncomp outside 1..3; the gradf last line refuses bad batch,
+    ncomp and capacity."""
     from cmblensing_amd import _lib
     _case("argument validation")
     lib = _lib.load()
@@ -275,6 +276,12 @@ def test_negative_batch_and_zero_steps_rejected(pkg, dev):
     for nrhs, ncomp in [(0, 2), (-1, 2), (1, 0), (1, 4)]:
         assert lib.cmbl_cg_update_p(plan.handle, ptr, ptr, ptr, None, nrhs, ncomp, None) != 0
         assert "nrhs must be >= 1" in lib.cmbl_last_error().decode()
+    # the last line of gradf from the gradient ODE's delta f: batch, ncomp
+    # and the plan's capacity are checked before any launch
+    for batch, ncomp, msg in [(-1, 2, "batch < 0"), (1, 3, "ncomp must be 1 (T) or 2 (EB)"),
+                              (plan.max_maps // 2 + 1, 2, "exceeds the plan's max_maps")]:
+        assert lib.cmbl_gradf_final(plan.handle, ptr, ptr, ptr, ptr, batch, ncomp, None) != 0
+        assert msg in lib.cmbl_last_error().decode()
     torch.cuda.synchronize()
 
 
