@@ -17,6 +17,14 @@ default here.  ``rtol`` adds a relative test sqrt(dot(r, r) / dot(b, b)) <
 rtol that is only passed by a recomputed residual b - A x (so the reported
 residual is the true one, to the operator's precision), and
 ``recompute_every`` also replaces the recursive residual every k iterations.
+``refine_drop`` (with ``rtol``) replaces a right-hand side's recursive
+residual whenever its relative residual has fallen by that factor since its
+last replacement (a reliable-update rule): with a float32 operator the
+recursion drifts from b - A x as the residual shrinks below the operator's
+error, and without a replacement within ~5-6 decades it can break down
+(alpha from a noise-dominated p.Ap).  The default for a float32 operator
+with ``rtol`` is 1e-4, or 1e-5 beside a ``recompute_every`` cadence, so every
+right-hand side stays within that reach whatever its convergence rate.
 """
 from __future__ import annotations
 
@@ -37,6 +45,16 @@ from .matrix_operators import BlockTEB, DiagonalEB, DiagonalScalar, PixelMask, p
 # active only (False: to every right-hand side, whose inactive ones then get
 # alpha = beta = 0; the iterates are the same either way).
 COMPACT_ACTIVE = True
+
+# Default reliable-update factors for a float32 operator with a relative stop:
+# a right-hand side's recursive residual is replaced by b - A x (float64) once
+# its relative residual has fallen by this factor since the last replacement.
+# With a fixed cadence (recompute_every) as well, the rule is a safety net for
+# right-hand sides that converge faster than the cadence expects: 1e-5 (the
+# recursion broke down between 5 and 6 decades past a replacement on the
+# 512^2 Wiener filter; a cadence of 20 moves about 4).
+REFINE_DROP_FP32 = 1e-4
+REFINE_DROP_FP32_CADENCE = 1e-5
 
 
 @dataclass
@@ -179,7 +197,8 @@ def run_process(gen):
 def conjugate_gradient_process(x0, data, phi, mask, beam, noise_covariance, field_covariance,
                                b_vector, preconditioner, maxiter: int = 500, tol: float = 1e-1, *,
                                n: int = DEFAULT_NSTEPS, rtol: Optional[float] = None,
-                               recompute_every: Optional[int] = None, return_info: bool = False):
+                               recompute_every: Optional[int] = None, return_info: bool = False,
+                               refine_drop="auto"):
     """PCG of wiener_filter.py:51-105, batched over right-hand sides, float64
     vectors and scalars around a float32 (or float64) operator, written as a
     process: a generator that enqueues device work and yields a HIP event at
@@ -190,6 +209,9 @@ def conjugate_gradient_process(x0, data, phi, mask, beam, noise_covariance, fiel
     B = X0.shape[0]
     prob = _Problem(x0, phi, field_covariance, noise_covariance, mask, beam, n, B)
     dev = prob.device
+    if refine_drop == "auto":
+        refine_drop = (None if (rtol is None or not prob.mixed) else
+                       REFINE_DROP_FP32_CADENCE if recompute_every else REFINE_DROP_FP32)
     x = X0.to(torch.complex128).contiguous().clone()
     bvec, _ = stack_matrices(b_vector)
     bvec = bvec.to(torch.complex128)
@@ -338,6 +360,12 @@ def conjugate_gradient_process(x0, data, phi, mask, beam, noise_covariance, fiel
     res_d = res_h.to(dev)
     rel_d = torch.empty(B, dtype=torch.float64, device=dev)
     act_prev, act_d = None, None
+    # reliable updates: the relative residual at each right-hand side's last
+    # recomputed residual, and those due for one (replaced at the next
+    # iteration's recomputation point, so no extra host wait)
+    rel_ref = rel.clone() if (refine_drop and rtol is not None) else None
+    due = torch.zeros(B, dtype=torch.bool)
+    pending_ref = None
     while bool(act.any()):
         idx = None if (bool(act.all()) or not COMPACT_ACTIVE) else \
             [i for i in range(B) if bool(act[i])]
@@ -349,12 +377,28 @@ def conjugate_gradient_process(x0, data, phi, mask, beam, noise_covariance, fiel
         _cg_step(prob, x, r, z, p, Ap, pinv_arr, p_op, res_d, act_d,
                  bb_d if rtol is not None else None, rel_d if rtol is not None else None, B)
         rel_cur = rel_d
+        if pending_ref is not None:
+            # the last read's reliable-update bookkeeping, done while the
+            # device runs this iteration
+            rel_p, tr_p, act_p = pending_ref
+            pending_ref = None
+            rel_ref = torch.where(torch.tensor(tr_p), rel_p, rel_ref)
+            due = act_p & (rel_p < rel_ref * refine_drop) & (rel_p >= rtol)
+            if bool(due.any()):
+                # batched: with one right-hand side due, every active one at
+                # least halfway there (in decades) is refined with it
+                due = act_p & (rel_p < rel_ref * refine_drop ** 0.5) & (rel_p >= rtol)
         for i in (range(B) if idx is None else idx):
             true_r[i] = False
         k += act.long()
         it += 1
-        if recompute_every and it % recompute_every == 0:
-            res_d = torch.where(act_d, residual(x, idx), res_d)
+        ref = act.clone() if (recompute_every and it % recompute_every == 0) else None
+        if rel_ref is not None and bool((due & act).any()):
+            ref = (due & act) if ref is None else (ref | (due & act))
+        if ref is not None:
+            ridx = None if bool(ref.all()) else [i for i in range(B) if bool(ref[i])]
+            ref_d = act_d if torch.equal(ref, act) else ref.to(dev)
+            res_d = torch.where(ref_d, residual(x, ridx), res_d)
             rel_cur = rel_dev() if rtol is not None else None
         if pending is not None:
             record(*pending)
@@ -379,6 +423,8 @@ def conjugate_gradient_process(x0, data, phi, mask, beam, noise_covariance, fiel
                 res_h, rel = (yield from _host(torch.stack([res_d, rel_dev()])))
                 upd = act | verify
                 verify[:] = False
+            if rel_ref is not None:
+                pending_ref = (rel, list(true_r), act)
         else:
             res_h = yield from _host(res_d)
         res_curr = torch.where(upd, res_h, res_curr)
@@ -407,31 +453,35 @@ def conjugate_gradient_process(x0, data, phi, mask, beam, noise_covariance, fiel
 def conjugate_gradient(x0, data, phi, mask, beam, noise_covariance, field_covariance, b_vector,
                        preconditioner, maxiter: int = 500, tol: float = 1e-1, *,
                        n: int = DEFAULT_NSTEPS, rtol: Optional[float] = None,
-                       recompute_every: Optional[int] = None, return_info: bool = False):
+                       recompute_every: Optional[int] = None, return_info: bool = False,
+                       refine_drop="auto"):
     """PCG of wiener_filter.py:51-105 (see conjugate_gradient_process)."""
     return run_process(conjugate_gradient_process(
         x0, data, phi, mask, beam, noise_covariance, field_covariance, b_vector, preconditioner,
-        maxiter, tol, n=n, rtol=rtol, recompute_every=recompute_every, return_info=return_info))
+        maxiter, tol, n=n, rtol=rtol, recompute_every=recompute_every, return_info=return_info,
+        refine_drop=refine_drop))
 
 
 def wiener_filter(field, phi, data, field_covariance, noise_covariance, mask, beam, *,
                   n: int = DEFAULT_NSTEPS, maxiter: int = 500, tol: float = 1e-1,
                   rtol: Optional[float] = None, recompute_every: Optional[int] = None,
-                  return_info: bool = False):
+                  return_info: bool = False, refine_drop="auto"):
     """wiener_filter.py:10-26: b = -gradf(0, d), P = Cf^-1 + B M Cn^-1 M B,
     PCG from `field`."""
     return run_process(wiener_filter_process(
         field, phi, data, field_covariance, noise_covariance, mask, beam, n=n, maxiter=maxiter,
-        tol=tol, rtol=rtol, recompute_every=recompute_every, return_info=return_info))
+        tol=tol, rtol=rtol, recompute_every=recompute_every, return_info=return_info,
+        refine_drop=refine_drop))
 
 
 def wiener_filter_process(field, phi, data, field_covariance, noise_covariance, mask, beam, *,
                           n: int = DEFAULT_NSTEPS, maxiter: int = 500, tol: float = 1e-1,
                           rtol: Optional[float] = None, recompute_every: Optional[int] = None,
-                          return_info: bool = False):
+                          return_info: bool = False, refine_drop="auto"):
     """wiener_filter as a process (see conjugate_gradient_process)."""
     b = get_b_vector(phi, data, field_covariance, noise_covariance, mask, beam, n=n, like=field)
     P = hessian_logpdf_preconditioner(field_covariance, beam, mask, noise_covariance)
     return (yield from conjugate_gradient_process(
         field, data, phi, mask, beam, noise_covariance, field_covariance, b, P, maxiter, tol, n=n,
-        rtol=rtol, recompute_every=recompute_every, return_info=return_info))
+        rtol=rtol, recompute_every=recompute_every, return_info=return_info,
+        refine_drop=refine_drop))
