@@ -431,7 +431,8 @@ __global__ void k_cg_rel(const double* __restrict__ rr, const double* __restrict
 // exchanges this needs; the branch is uniform over a workgroup.
 //
 // An operator acts in place on the C values of one mode (field f, kx, ky),
-// a[c] = component c:
+// a[c] = component c; the caller keeps component cu only, so an operator may
+// skip work that feeds the other components alone:
 //   OpDiagRot  C = 2: eb2qu(diag .* qu2eb(a)); C = 1: diag .* a
 //              (the noise / beam operators of the pixel-mask posterior,
 //               matrix_operators.py:54-90 with util.py:140-161)
@@ -446,7 +447,7 @@ template <typename T, int C>
 struct OpDiagRot {
   const T* diag;
   int N;
-  __device__ __forceinline__ void operator()(long, int kx, int ky, cplx<T> (&a)[C]) const {
+  __device__ __forceinline__ void operator()(long, int kx, int ky, cplx<T> (&a)[C], int) const {
     const long PN = (long)N * (N / 2 + 1);
     const long idx = (long)kx * (N / 2 + 1) + ky;
     if constexpr (C == 1) {
@@ -469,7 +470,7 @@ struct OpData {
   const T* mb;
   const T* bmcn;
   int N;
-  __device__ __forceinline__ void operator()(long f, int kx, int ky, cplx<T> (&a)[C]) const {
+  __device__ __forceinline__ void operator()(long f, int kx, int ky, cplx<T> (&a)[C], int) const {
     const long PN = (long)N * (N / 2 + 1);
     const long idx = (long)kx * (N / 2 + 1) + ky;
     const long base = f * C * PN + idx;
@@ -497,7 +498,7 @@ struct OpData {
 template <typename T, int C>
 struct OpEb2Qu {
   int N;
-  __device__ __forceinline__ void operator()(long, int kx, int ky, cplx<T> (&a)[C]) const {
+  __device__ __forceinline__ void operator()(long, int kx, int ky, cplx<T> (&a)[C], int) const {
     if constexpr (C == 2) {
       T cs, sn;
       rot2psi_ij<T>(N, kx, ky, cs, sn);
@@ -512,7 +513,7 @@ struct OpFinal {
   const cplx<T>* F;  // [nfields][C][N][N/2+1] or NULL (no prior term)
   const T* cfinv;
   int N;
-  __device__ __forceinline__ void operator()(long f, int kx, int ky, cplx<T> (&a)[C]) const {
+  __device__ __forceinline__ void operator()(long f, int kx, int ky, cplx<T> (&a)[C], int cu) const {
     const long PN = (long)N * (N / 2 + 1);
     const long idx = (long)kx * (N / 2 + 1) + ky;
     if constexpr (C == 2) {
@@ -522,9 +523,11 @@ struct OpFinal {
       qu2eb_mode(q, u, cs, sn, a[0], a[1]);
     }
     if (F && cfinv) {
+      // the prior per component: only the component the caller keeps (cu)
 #pragma unroll
       for (int c = 0; c < C; ++c)
-        a[c] = a[c] - scale(herm_col(F, (f * C + c) * PN, N, kx, ky), cfinv[c * PN + idx]);
+        if (c == cu)
+          a[c] = a[c] - scale(herm_col(F, (f * C + c) * PN, N, kx, ky), cfinv[c * PN + idx]);
     }
   }
 };
@@ -583,7 +586,7 @@ __global__ __launch_bounds__(C * KC * N / 8) void k_col_edge(const cplx<T>* in, 
         cplx<T> a[C];
 #pragma unroll
         for (int cc = 0; cc < C; ++cc) a[cc] = L(cc, t + q * G);
-        op(f, t + q * G, k, a);
+        op(f, t + q * G, k, a, c);
         v[0][q] = pick<T, C>(a, c);
       }
     }
@@ -605,7 +608,7 @@ __global__ __launch_bounds__(C * KC * N / 8) void k_col_edge(const cplx<T>* in, 
         cplx<T> a[C];
 #pragma unroll
         for (int cc = 0; cc < C; ++cc) a[cc] = L(cc, t + q * G);
-        op(f, t + q * G, k, a);
+        op(f, t + q * G, k, a, c);
         v[0][q] = pick<T, C>(a, c);
       }
     }
@@ -636,15 +639,15 @@ __global__ __launch_bounds__(C * KC * N / 8) void k_col_edge(const cplx<T>* in, 
           bn[cc] = mk(h * dm.y, -h * dm.x);
         }
       }
-      op(f, kx, 0, a0);
-      op(f, kx, H, an);
+      op(f, kx, 0, a0, c);
+      op(f, kx, H, an, c);
       if constexpr (EDGE == 2) {
         cplx<T>* o = out + (f * C + c) * PN + (long)kx * (H + 1);
         o[0] = pick<T, C>(a0, c);
         o[H] = pick<T, C>(an, c);
       } else {
-        op(f, mx, 0, b0);
-        op(f, mx, H, bn);
+        op(f, mx, 0, b0, c);
+        op(f, mx, H, bn, c);
         const cplx<T> y0 = scale(pick<T, C>(a0, c) + conj(pick<T, C>(b0, c)), h);
         const cplx<T> yn = scale(pick<T, C>(an, c) + conj(pick<T, C>(bn, c)), h);
         col0[c][lds_idx<N, false>(kx)] = mk(y0.x - yn.y, y0.y + yn.x);
