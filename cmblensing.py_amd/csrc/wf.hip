@@ -427,8 +427,9 @@ __global__ void k_cg_rel(const double* __restrict__ rr, const double* __restrict
 // Yn[kx] = (Z[kx] - conj Z[-kx]) / 2i are the ky = 0 and ky = N/2 modes, and the
 // inverse way a column enters as Herm(Y0) + i Herm(Yn), Herm(Y)[kx] =
 // (Y[kx] + conj Y[-kx]) / 2: the part the row C2R keeps (k_col_fwd_unpack,
-// k_col_inv_pack).  Workgroups of tile 0 hold column 0 and take the LDS
-// exchanges this needs; the branch is uniform over a workgroup.
+// k_col_inv_pack).  Workgroups of tile 0 hold column 0; its work is spread
+// over their column lanes and handed to column 0's own lanes through LDS
+// (col0); the branch on the tile is uniform over a workgroup.
 //
 // An operator acts in place on the C values of one mode (field f, kx, ky),
 // a[c] = component c; the caller keeps component cu only, so an operator may
