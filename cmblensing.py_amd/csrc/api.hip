@@ -112,6 +112,9 @@ struct cmbl_plan {
   hipEvent_t fork_ev = nullptr;
   hipEvent_t join_ev = nullptr;
   bool split = true;
+  // adjoint rate with the classic RK4 accumulator (k_row_lf_adj) instead of
+  // the low-storage bookkeeping (k_row_lf_adj_ls); for A/B measurements
+  bool adj_classic = false;
   // optional per-kernel timing (HIP events around every stage launch)
   bool timing = false;
   std::vector<hipEvent_t> tev;   // pool, 2 per launch
@@ -463,6 +466,9 @@ cmbl_status cmbl_plan_create(cmbl_plan** out, int N, double theta_pix_arcmin, in
     // CMBL_SPLIT=0 runs every batch as one part (for A/B measurements)
     const char* sp_env = std::getenv("CMBL_SPLIT");
     p->split = !(sp_env && sp_env[0] == '0');
+    // CMBL_ADJ_CLASSIC=1 runs the adjoint rate with the classic accumulator
+    const char* ac_env = std::getenv("CMBL_ADJ_CLASSIC");
+    p->adj_classic = ac_env && ac_env[0] == '1';
   }
   // set-up of the allocation-free entry point on this device
   void* sp = nullptr;
@@ -734,9 +740,12 @@ cmbl_status run_lenseflow(cmbl_plan* p, const void* const* ptabs, T* y, long nma
       TIMED(2, cmbl::launch_lf_row_r2c_gather<T>(N, src, C, q.y, q.S0, q.nmaps, tw, q.s));
     } else if (!adj) {
       TIMED(2, cmbl::launch_lf_row_r2c<T>(N, q.y, q.S0, q.nmaps, tw, q.s));
-    } else {
+    } else if (p->adj_classic) {
       TIMED(2, cmbl::launch_lf_row<T>(N, true, q.S0, q.S1, q.y, q.acc, q.pt, tw, C, q.nmaps, 1, h,
                                       tindex(0, 0), 1, 1, (T)p->dk, q.s));
+    } else {
+      TIMED(2, cmbl::launch_lf_row_adj_ls<T>(N, q.S0, (T*)q.S1, q.y, q.acc, q.pt, tw, C, q.nmaps,
+                                             1, h, 0, tindex(0, 0), 1, 1, (T)p->dk, q.s));
     }
   }
   for (int i = 0; i < n; ++i) {
@@ -754,14 +763,22 @@ cmbl_status run_lenseflow(cmbl_plan* p, const void* const* ptabs, T* y, long nma
       for (int k = 0; k < np; ++k) {
         const LfPart<T>& q = parts[k];
         // standard rate: d/dx of S0 into S1 (S0 is kept for d/dy); adjoint:
-        // d/dx of U in place (V in S1 supplies the y part)
+        // d/dx of U in place (S1 holds the stage input z_s, whose y part the
+        // row pass forms, or with the classic accumulator the spectra V)
         TIMED(0, cmbl::launch_lf_col<T>(N, q.S0, adj ? q.S0 : q.S1, q.nmaps, tw, (T)p->dk, q.s));
       }
       for (int k = 0; k < np; ++k) {
         const LfPart<T>& q = parts[k];
-        TIMED(1, cmbl::launch_lf_row<T>(N, adj, q.S0, q.S1, q.y, q.acc, q.pt, tw, C, q.nmaps, st,
-                                        h, tidx, last ? 0 : 1, 0, (T)p->dk, q.s, nullptr, nullptr,
-                                        (last && !adj) ? outp : nullptr));
+        if (adj && !p->adj_classic) {
+          // the S1 stack holds the stage input z_s as a real map
+          TIMED(1, cmbl::launch_lf_row_adj_ls<T>(N, q.S0, (T*)q.S1, q.y, q.acc, q.pt, tw, C,
+                                                 q.nmaps, st, h, tindex(i, stage_half[st]), tidx,
+                                                 last ? 0 : 1, 0, (T)p->dk, q.s));
+        } else {
+          TIMED(1, cmbl::launch_lf_row<T>(N, adj, q.S0, q.S1, q.y, q.acc, q.pt, tw, C, q.nmaps,
+                                          st, h, tidx, last ? 0 : 1, 0, (T)p->dk, q.s, nullptr,
+                                          nullptr, (last && !adj) ? outp : nullptr));
+        }
       }
     }
   }

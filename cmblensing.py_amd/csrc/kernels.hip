@@ -950,6 +950,144 @@ __global__ __launch_bounds__(BS) void k_row_lf_adj(cplx<T>* __restrict__ S0,
   split_store<N, kRowWave>(uv[1], bufb, S0 + rp.rb * H, S1 + rp.rb * H, t, active);
 }
 
+// Row pass, adjoint rate, with the low-storage RK4 bookkeeping of
+// k_row_lf_std (RK = 1, 2).  Between launches S0 holds U, the row spectra of
+// u = p_x(t_s) z_s (after the column pass: their x-derivative, in place), and
+// Z the stage input z_s itself as a real map (stage 1: z_1 = y, read from y).
+// Storing z_s where k_row_lf_adj stores the spectra of v = p_y z_s moves the
+// same bytes, and makes z_s available to rebuild the accumulator.  Per row
+// pair: v = p_y(t_s) z_s of both rows as the line v_a + i v_b, forward FFT,
+// multiply by i ky / N (its y-derivative: i ky keeps each row's Hermitian
+// symmetry, so both rows go through one line; DC and the Nyquist bin drop out
+// as in the C2R), add the Hermitian line of the two x-derivative half rows,
+// one inverse FFT: the rates div(p z) of both rows as (Re, Im).  Then
+//   stage 1: z_2 = y + h/2 k                                  -> Z
+//   stage 2: acc = (2 y + z_2)/3 + h/3 k, z_3 = y + h/2 k     -> acc, Z
+//   stage 3: z_4 = y + h k                                    -> Z
+//   stage 4: y' = acc + (z_4 - y)/3 + h/6 k                   -> y (= z_1 next)
+// and, unless this is the very last stage, u = p_x(t_next) z_next is row-R2C'd
+// (two rows per line) into S0.  init != 0: no rate, U of z_1 = y only.
+// Per launch (5.25 C + 2) N^2 s against the classic kernel's (6.5 C + 2).
+template <typename T, int N, int C, int BS>
+__global__ __launch_bounds__(BS) void k_row_lf_adj_ls(cplx<T>* __restrict__ S0,
+                                                       T* __restrict__ Z, T* __restrict__ y,
+                                                       T* __restrict__ acc,
+                                                       const T* const* __restrict__ ptabs,
+                                                       const cplx<T>* __restrict__ twg,
+                                                       long npairs, int stage, T h, int tidx,
+                                                       int tidx_next, int do_r2c, int init,
+                                                       T cy) {
+  constexpr int G = N / 8, PAIRS = BS / G, H = N / 2;
+  static_assert(BS % G == 0 && BS >= G, "a block holds whole lines");
+  constexpr long NN = (long)N * N;
+  __shared__ cplx<T> lbuf[PAIRS][2][lds_len<N>()];
+  const int grp = threadIdx.x / G, t = threadIdx.x % G;
+  Twiddles<N, T> tw;
+  tw.load(twg, t);
+  const long p = (long)blockIdx.x * PAIRS + grp;
+  const bool active = p < npairs;
+  const RowPair rp = row_pair<N>((int)(active ? p : npairs - 1), C);
+  cplx<T>* bufa = lbuf[grp][0];
+  cplx<T>* bufb = lbuf[grp][1];
+  cplx<T>* const bl[1] = {bufa};
+  const T* pt = ptabs[rp.b];
+  // the stage input of both rows
+  const T* zsrc = (init || stage == 1) ? y : Z;
+  T za[8], zb[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    za[m] = zsrc[rp.ra * N + t + m * G];
+    zb[m] = zsrc[rp.rb * N + t + m * G];
+  }
+  cplx<T> w[1][8];
+  if (!init) {
+    // the x-derivative half rows as the Hermitian line W_a + i W_b (bufb)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = t + q * G;
+      put_herm<N>(bufb, k, S0[rp.ra * H + k], S0[rp.rb * H + k]);
+    }
+    const T* py = pt + (long)tidx * 2 * NN + NN;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int j = t + m * G;
+      const T pa = py[(long)rp.ia * N + j];
+      const T pb = C == 2 ? pa : py[(long)rp.ib * N + j];
+      w[0][m] = mk(pa * za[m], pb * zb[m]);
+    }
+    fft_lines<N, false, 1, kRowWave>(w, bl, tw, t);
+    lds_sync<kRowWave>();
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int k = t + m * G;  // bin k of the full line; signed ky, 0 at DC and N/2
+      const T c = m < 4 ? (T)k * cy : (m == 4 && t == 0 ? T(0) : (T)(k - N) * cy);
+      w[0][m] = bufb[lds_idx<N>(k)] + imul(w[0][m], c);
+    }
+    fft_lines<N, true, 1, kRowWave>(w, bl, tw, t);
+    // RK4 update, one branch on the (grid-uniform) stage around straight-line
+    // pixel loops
+    if (stage == 1) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        za[m] = za[m] + (h / T(2)) * w[0][m].x;
+        zb[m] = zb[m] + (h / T(2)) * w[0][m].y;
+      }
+    } else if (stage == 3) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const int j = t + m * G;
+        za[m] = y[rp.ra * N + j] + h * w[0][m].x;
+        zb[m] = y[rp.rb * N + j] + h * w[0][m].y;
+      }
+    } else if (stage == 2) {
+      T aa[8], ab[8];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const int j = t + m * G;
+        const T ya = y[rp.ra * N + j], yb = y[rp.rb * N + j];
+        aa[m] = (T(2) * ya + za[m]) / T(3) + (h / T(3)) * w[0][m].x;
+        ab[m] = (T(2) * yb + zb[m]) / T(3) + (h / T(3)) * w[0][m].y;
+        za[m] = ya + (h / T(2)) * w[0][m].x;
+        zb[m] = yb + (h / T(2)) * w[0][m].y;
+      }
+      if (active) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+          acc[rp.ra * N + t + m * G] = aa[m];
+          acc[rp.rb * N + t + m * G] = ab[m];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const int j = t + m * G;
+        const T ya = y[rp.ra * N + j], yb = y[rp.rb * N + j];
+        za[m] = acc[rp.ra * N + j] + (za[m] - ya) / T(3) + (h / T(6)) * w[0][m].x;
+        zb[m] = acc[rp.rb * N + j] + (zb[m] - yb) / T(3) + (h / T(6)) * w[0][m].y;
+      }
+    }
+    if (active) {
+      T* zdst = stage == 4 ? y : Z;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        zdst[rp.ra * N + t + m * G] = za[m];
+        zdst[rp.rb * N + t + m * G] = zb[m];
+      }
+    }
+  }
+  if (!do_r2c) return;  // uniform over the grid
+  const T* px = pt + (long)tidx_next * 2 * NN;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int j = t + m * G;
+    const T pa = px[(long)rp.ia * N + j];
+    const T pb = C == 2 ? pa : px[(long)rp.ib * N + j];
+    w[0][m] = mk(pa * za[m], pb * zb[m]);
+  }
+  fft_lines<N, false, 1, kRowWave>(w, bl, tw, t);
+  split_store<N, kRowWave>(w[0], bufa, S0 + rp.ra * H, S0 + rp.rb * H, t, active);
+}
+
 // Gather / scatter of whole maps through device pointer arrays (serving
 // executor), `len` elements per slot b = blockIdx.y.  A NULL source fills the
 // slot with zeros (padding); a NULL destination skips it.
@@ -1202,6 +1340,32 @@ hipError_t launch_lf_row(int N, bool adjoint, cplx<T>* S0, cplx<T>* S1, T* y, T*
 }
 
 template <typename T>
+hipError_t launch_lf_row_adj_ls(int N, cplx<T>* S0, T* Z, T* y, T* acc, const T* const* ptabs,
+                                const cplx<T>* tw, int C, long nmaps, int stage, T h, int tidx,
+                                int tidx_next, int do_r2c, int init, T dk, hipStream_t s) {
+  CMBL_DISPATCH_N(N, {
+    const long npairs = nmaps * NN_ / 2;
+    const T cy = dk / T(NN_);
+    constexpr int BS_SMALL = NN_ / 8 > 64 ? NN_ / 8 : 64;
+    auto go = [&](auto bs) {
+      constexpr int BS = decltype(bs)::value;
+      const unsigned nb = blocks_for(npairs, BS / (NN_ / 8));
+      if (C == 2)
+        k_row_lf_adj_ls<T, NN_, 2, BS><<<nb, BS, 0, s>>>(S0, Z, y, acc, ptabs, tw, npairs, stage,
+                                                         h, tidx, tidx_next, do_r2c, init, cy);
+      else
+        k_row_lf_adj_ls<T, NN_, 1, BS><<<nb, BS, 0, s>>>(S0, Z, y, acc, ptabs, tw, npairs, stage,
+                                                         h, tidx, tidx_next, do_r2c, init, cy);
+    };
+    if (blocks_for(npairs, 256 / (NN_ / 8)) < CMBL_SMALL_GRID)
+      go(std::integral_constant<int, BS_SMALL>{});
+    else
+      go(std::integral_constant<int, 256>{});
+  });
+  return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_gtab(int N, const double* D, T* gtab, int nsteps, hipStream_t s) {
   const long P = (long)N * N;
   k_gtab<T><<<blocks_for(P, 256), 256, 0, s>>>(D, gtab, N, nsteps);
@@ -1250,6 +1414,9 @@ hipError_t launch_scatter(const T* src, T* const* dst, long len, int count, hipS
                                         const T*, int, hipStream_t);                            \
   template hipError_t launch_lf_row_r2c_gather<T>(int, const T* const*, int, T*, cplx<T>*,      \
                                                   long, const cplx<T>*, hipStream_t);           \
+  template hipError_t launch_lf_row_adj_ls<T>(int, cplx<T>*, T*, T*, T*, const T* const*,       \
+                                              const cplx<T>*, int, long, int, T, int, int, int, \
+                                              int, T, hipStream_t);                             \
   template hipError_t launch_gtab<T>(int, const double*, T*, int, hipStream_t);                 \
   template hipError_t launch_gather<T>(const T* const*, T*, long, int, hipStream_t);            \
   template hipError_t launch_scatter<T>(const T*, T* const*, long, int, hipStream_t);           \

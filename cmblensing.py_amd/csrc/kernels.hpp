@@ -60,6 +60,14 @@ hipError_t launch_lf_row(int N, bool adjoint, cplx<T>* S0, cplx<T>* S1, T* y, T*
                          T h, int tidx, int do_r2c, int init, T dk, hipStream_t s,
                          T* zstore = nullptr, const GradArgs<T>* ga = nullptr,
                          T* const* outp = nullptr);
+// Row pass of one adjoint-rate stage with the low-storage RK4 bookkeeping:
+// S0 = U (x-derivative after the column pass), Z = the stage input z_s as a
+// real map [nmaps][N][N] (stage 1 reads z_1 = y).  tidx: time index of this
+// stage (p_y), tidx_next: of the next stage (p_x of u).  init: U of y only.
+template <typename T>
+hipError_t launch_lf_row_adj_ls(int N, cplx<T>* S0, T* Z, T* y, T* acc, const T* const* ptabs,
+                                const cplx<T>* tw, int C, long nmaps, int stage, T h, int tidx,
+                                int tidx_next, int do_r2c, int init, T dk, hipStream_t s);
 // First launch of the serving executor: row R2C of the requests' maps
 // (src[b] + component N^2, NULL = zeros) into S0, also stored to ystage.
 template <typename T>
