@@ -14,7 +14,11 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 INCLUDE = PKG.parent / "include"
 LIB = PKG / "libcmbl.so"
-SOURCES = ["kernels.hip", "wf.hip", "sim.hip", "api.hip"]
+# (source, object, extra flags): kernels.hip is compiled twice, its FP32 and
+# FP64 instantiations in separate objects, so the two compile concurrently
+SOURCES = [("kernels.hip", "kernels_f32.o", ["-DCMBL_KERNELS_PART=1"]),
+           ("kernels.hip", "kernels_f64.o", ["-DCMBL_KERNELS_PART=2"]),
+           ("wf.hip", "wf.o", []), ("sim.hip", "sim.o", []), ("api.hip", "api.o", [])]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 ops are disabled for all device code
@@ -38,16 +42,23 @@ def build(verbose: bool = False) -> Path:
     cc = hipcc()
     objdir = PKG / "_build"
     objdir.mkdir(exist_ok=True)
-    objs = []
-    for src in SOURCES:
-        obj = objdir / (Path(src).stem + ".o")
-        cmd = [cc, *HIPCC_FLAGS, f"-I{INCLUDE}", "-c", str(CSRC / src), "-o", str(obj)]
+    objs, procs = [], []
+    # the sources compile concurrently (one hipcc process each)
+    for src, oname, extra in SOURCES:
+        obj = objdir / oname
+        cmd = [cc, *HIPCC_FLAGS, *extra, f"-I{INCLUDE}", "-c", str(CSRC / src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        if r.returncode != 0:
-            raise RuntimeError(f"hipcc failed on {src}:\n{r.stderr}")
+        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                                            text=True)))
         objs.append(str(obj))
+    failed = []
+    for src, pr in procs:
+        _, err = pr.communicate()
+        if pr.returncode != 0:
+            failed.append(f"hipcc failed on {src}:\n{err}")
+    if failed:
+        raise RuntimeError("\n".join(failed))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", str(tmp)]
     if verbose:

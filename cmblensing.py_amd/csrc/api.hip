@@ -1347,11 +1347,20 @@ cmbl_status run_vjp(cmbl_plan* p, const void* const* ptabs, const void* const* g
   HIP_TRY(hipMemcpyAsync(yf, lensed, nm * NN * sizeof(T), hipMemcpyDeviceToDevice, s));
   if (dfield != cot)
     HIP_TRY(hipMemcpyAsync(dfield, cot, nm * NN * sizeof(T), hipMemcpyDeviceToDevice, s));
-  HIP_TRY(hipMemcpyAsync(zd, cot, nm * NN * sizeof(T), hipMemcpyDeviceToDevice, s));
+  // the delta-f rate: low-storage bookkeeping (the Vd stack holds its stage
+  // input z_s, which is also what the delta-phi sources read; z_1 is the state
+  // itself), or the classic accumulator with the stage inputs copied to zd
+  const bool ls = !p->adj_classic;
+  T* const zs = (T*)Vd;
+  if (!ls) HIP_TRY(hipMemcpyAsync(zd, cot, nm * NN * sizeof(T), hipMemcpyDeviceToDevice, s));
   HIP_TRY(cmbl::launch_fill_zero<T>(src, (long)batch * 5 * NN, s));
   HIP_TRY(cmbl::launch_lf_row_r2c<T>(N, yf, Sf, nm, tw, s));
-  HIP_TRY(cmbl::launch_lf_row<T>(N, true, Ud, Vd, dfield, accd, pt, tw, C, nm, 1, h, tindex(0, 0),
-                                 1, 1, (T)p->dk, s));
+  if (ls)
+    HIP_TRY(cmbl::launch_lf_row_adj_ls<T>(N, Ud, zs, dfield, accd, pt, tw, C, nm, 1, h, 0,
+                                          tindex(0, 0), 1, 1, (T)p->dk, s));
+  else
+    HIP_TRY(cmbl::launch_lf_row<T>(N, true, Ud, Vd, dfield, accd, pt, tw, C, nm, 1, h,
+                                   tindex(0, 0), 1, 1, (T)p->dk, s));
   for (int i = 0; i < n; ++i) {
     for (int st = 1; st <= 4; ++st) {
       const bool last = (i == n - 1) && (st == 4);
@@ -1359,7 +1368,7 @@ cmbl_status run_vjp(cmbl_plan* p, const void* const* ptabs, const void* const* g
       HIP_TRY(cmbl::launch_lf_col<T>(N, Ud, Ud, nm, tw, (T)p->dk, s));
       const int tidx = tindex(i, stage_half[st]);
       cmbl::GradArgs<T> ga;
-      ga.zd = zd;
+      ga.zd = !ls ? zd : (st == 1 ? dfield : zs);
       ga.gtabs = (const T* const*)gtabs;
       ga.sacc = src;
       ga.cs = (T)(hd * bw[st]);
@@ -1368,8 +1377,12 @@ cmbl_status run_vjp(cmbl_plan* p, const void* const* ptabs, const void* const* g
                                      last ? 0 : 1, 0, (T)p->dk, s, nullptr, &ga));
       int tn = st < 4 ? tindex(i, stage_half[st + 1]) : tindex(i + 1, 0);
       if (last) tn = 0;
-      HIP_TRY(cmbl::launch_lf_row<T>(N, true, Ud, Vd, dfield, accd, pt, tw, C, nm, st, h, tn,
-                                     last ? 0 : 1, 0, (T)p->dk, s, last ? nullptr : zd));
+      if (ls)
+        HIP_TRY(cmbl::launch_lf_row_adj_ls<T>(N, Ud, zs, dfield, accd, pt, tw, C, nm, st, h, tidx,
+                                              tn, last ? 0 : 1, 0, (T)p->dk, s));
+      else
+        HIP_TRY(cmbl::launch_lf_row<T>(N, true, Ud, Vd, dfield, accd, pt, tw, C, nm, st, h, tn,
+                                       last ? 0 : 1, 0, (T)p->dk, s, last ? nullptr : zd));
     }
   }
   // delta phi = sum of multipliers times rfft2 of the accumulated sources

@@ -22,6 +22,13 @@
 #include "fft_lds.hpp"
 #include "kernels.hpp"
 
+// What this translation unit instantiates, so that the build can compile the
+// FP32 and FP64 kernels concurrently: 0 everything, 1 the float kernels and
+// the phi preparation, 2 the double kernels.
+#ifndef CMBL_KERNELS_PART
+#define CMBL_KERNELS_PART 0
+#endif
+
 namespace cmbl {
 
 // ---------------------------------------------------------------------------
@@ -368,6 +375,7 @@ __global__ __launch_bounds__(KC* N / 8) void k_col_inv_pack(const cplx<T>* __res
 // phi preparation (double precision)
 // ---------------------------------------------------------------------------
 
+#if CMBL_KERNELS_PART != 2
 // Five derivative spectra of F (natural layout): i KX F, i KY F, -KX^2 F,
 // -KX KY F, -KY^2 F, with KY's Nyquist entry negated (util.py:103-107).
 __global__ void k_phi_mult(const cplx<double>* __restrict__ F, cplx<double>* __restrict__ D, int N,
@@ -389,6 +397,7 @@ __global__ void k_phi_mult(const cplx<double>* __restrict__ F, cplx<double>* __r
     D[4 * n + idx] = scale(f, -(KY * KY));
   }
 }
+#endif
 
 // Deflection-operator table: for t_j = j/(2n), j = 0..2n,
 //   p(t_j) = (I + t_j Hess phi)^-1 grad phi        (lense_flow.py:372-381, 397-415)
@@ -1186,6 +1195,7 @@ hipError_t launch_irfft2(int N, const cplx<T>* X, cplx<T>* P, T* x, long nmaps,
   return hipGetLastError();
 }
 
+#if CMBL_KERNELS_PART != 2
 hipError_t launch_phi_mult(int N, const cplx<double>* F, cplx<double>* D, double dk,
                            hipStream_t s, int count) {
   if (count < 1) return hipSuccess;
@@ -1193,6 +1203,7 @@ hipError_t launch_phi_mult(int N, const cplx<double>* F, cplx<double>* D, double
   k_phi_mult<<<dim3(blocks_for(n, 256), count), 256, 0, s>>>(F, D, N, dk);
   return hipGetLastError();
 }
+#endif
 
 template <typename T>
 hipError_t launch_ptab(int N, const double* D, T* ptab, int nsteps, double* minpart,
@@ -1422,7 +1433,11 @@ hipError_t launch_scatter(const T* src, T* const* dst, long len, int count, hipS
   template hipError_t launch_scatter<T>(const T*, T* const*, long, int, hipStream_t);           \
   template hipError_t launch_dphi_combine<T>(int, const cplx<T>*, cplx<T>*, long, double,       \
                                              hipStream_t);
+#if CMBL_KERNELS_PART != 2
 CMBL_INST(float)
+#endif
+#if CMBL_KERNELS_PART != 1
 CMBL_INST(double)
+#endif
 
 }  // namespace cmbl
