@@ -723,11 +723,15 @@ __global__ __launch_bounds__(BS) void k_row_lf_std(cplx<T>* __restrict__ S0,
         for (int m = 0; m < 8; ++m) zz[m] = yv[m] + hk * zz[m];
       } else {  // stages 2 and 4
         const T inv_n = T(1) / T(N);  // exact: N is a power of two
+        // thirds as products with the rounded constants (a division per pixel
+        // costs ~10 VALU instructions)
+        constexpr T c13 = T(1) / T(3), c23 = T(2) / T(3);
         if (stage == 2) {
+          const T h3 = h / T(3);
 #pragma unroll
           for (int m = 0; m < 8; ++m) {
             const T zin = (r ? zr[0][m].y : zr[0][m].x) * inv_n;
-            av[m] = (T(2) * yv[m] + zin) / T(3) + (h / T(3)) * zz[m];
+            av[m] = c23 * yv[m] + (c13 * zin + h3 * zz[m]);
             zz[m] = yv[m] + (h / T(2)) * zz[m];
           }
           if (active) {
@@ -735,10 +739,11 @@ __global__ __launch_bounds__(BS) void k_row_lf_std(cplx<T>* __restrict__ S0,
             for (int m = 0; m < 8; ++m) acc[rr * N + t + m * G] = av[m];
           }
         } else {
+          const T h6 = h / T(6);
 #pragma unroll
           for (int m = 0; m < 8; ++m) {
             const T zin = (r ? zr[0][m].y : zr[0][m].x) * inv_n;
-            zz[m] = acc[rr * N + t + m * G] + (zin - yv[m]) / T(3) + (h / T(6)) * zz[m];
+            zz[m] = acc[rr * N + t + m * G] + (c13 * (zin - yv[m]) + h6 * zz[m]);
           }
           if (active) {
 #pragma unroll
@@ -1034,7 +1039,8 @@ __global__ __launch_bounds__(BS) void k_row_lf_adj_ls(cplx<T>* __restrict__ S0,
     }
     fft_lines<N, true, 1, kRowWave>(w, bl, tw, t);
     // RK4 update, one branch on the (grid-uniform) stage around straight-line
-    // pixel loops
+    // pixel loops; thirds as products with the rounded constants
+    constexpr T c13 = T(1) / T(3), c23 = T(2) / T(3);
     if (stage == 1) {
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
@@ -1050,12 +1056,13 @@ __global__ __launch_bounds__(BS) void k_row_lf_adj_ls(cplx<T>* __restrict__ S0,
       }
     } else if (stage == 2) {
       T aa[8], ab[8];
+      const T h3 = h / T(3);
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         const int j = t + m * G;
         const T ya = y[rp.ra * N + j], yb = y[rp.rb * N + j];
-        aa[m] = (T(2) * ya + za[m]) / T(3) + (h / T(3)) * w[0][m].x;
-        ab[m] = (T(2) * yb + zb[m]) / T(3) + (h / T(3)) * w[0][m].y;
+        aa[m] = c23 * ya + (c13 * za[m] + h3 * w[0][m].x);
+        ab[m] = c23 * yb + (c13 * zb[m] + h3 * w[0][m].y);
         za[m] = ya + (h / T(2)) * w[0][m].x;
         zb[m] = yb + (h / T(2)) * w[0][m].y;
       }
@@ -1067,12 +1074,13 @@ __global__ __launch_bounds__(BS) void k_row_lf_adj_ls(cplx<T>* __restrict__ S0,
         }
       }
     } else {
+      const T h6 = h / T(6);
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         const int j = t + m * G;
         const T ya = y[rp.ra * N + j], yb = y[rp.rb * N + j];
-        za[m] = acc[rp.ra * N + j] + (za[m] - ya) / T(3) + (h / T(6)) * w[0][m].x;
-        zb[m] = acc[rp.rb * N + j] + (zb[m] - yb) / T(3) + (h / T(6)) * w[0][m].y;
+        za[m] = acc[rp.ra * N + j] + (c13 * (za[m] - ya) + h6 * w[0][m].x);
+        zb[m] = acc[rp.rb * N + j] + (c13 * (zb[m] - yb) + h6 * w[0][m].y);
       }
     }
     if (active) {
