@@ -70,6 +70,17 @@ def row_bytes_per_launch(B, N, C, s):
     return B * (4.75 * C + 2) * N * N * s
 
 
+def adj_row_bytes_per_launch(B, N, C, s):
+    """Algorithmic bytes of one adjoint-rate row-pass launch (low-storage
+    bookkeeping) over B requests, averaged over the four RK4 stages: per
+    component the x-derivative spectra U read and the next U written (2Q),
+    the stage input z read and the next one written (2Q, the stage-1 read
+    being y itself: -0.25Q), y read (1Q) and written at stage 4 (0.25Q), the
+    accumulator written at stage 2 and read at stage 4 (0.5Q); per request
+    the two p(t) maps (2Q).  (5.25 C + 2) Q with Q = N^2 s."""
+    return B * (5.25 * C + 2) * N * N * s
+
+
 def col_bytes_per_launch(B, N, C, s):
     """x-derivative column pass: per component one packed half plane (N^2 s
     bytes) read and one written."""
@@ -766,7 +777,14 @@ def main():
     for _ in range(ksteps):
         direct_step()
     tim = plan.get_timing(reset=True)
+    # the same for the adjoint mode (its own row kernel, the same column pass)
+    for _ in range(ksteps):
+        _lib.lenseflow(plan, req_phi, x, out, B, C, n, 1, True)
+    tim_adj = plan.get_timing(reset=True)
     plan.set_timing(False)
+    adj_row_avg_ms = tim_adj["row_ms"] / tim_adj["row_n"]
+    adj_row_bytes = adj_row_bytes_per_launch(B, N, C, 4)
+    adj_row_gbs = adj_row_bytes / (adj_row_avg_ms * 1e-3) / 1e9
     row_avg_ms = tim["row_ms"] / tim["row_n"]
     col_avg_ms = tim["col_ms"] / tim["col_n"]
     row_bytes = row_bytes_per_launch(B, N, C, 4)
@@ -805,12 +823,15 @@ def main():
 
     # HBM traffic per launch of the row pass, from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated on a copy)
-    traffic = trace_us = None
+    traffic = trace_us = col_trace_us = adj_trace_us = None
     tfile = ROOT / "profiles" / "traffic.json"
     if tfile.exists() and B == 128:
-        rowprof = json.loads(tfile.read_text()).get("k_row_lf_std<float,256>", {})
+        prof = json.loads(tfile.read_text())
+        rowprof = prof.get("k_row_lf_std<float,256>", {})
         traffic = rowprof.get("bytes_per_launch")
         trace_us = rowprof.get("trace_avg_launch_us")
+        col_trace_us = prof.get("k_col_lf_dx<float,256,16>", {}).get("trace_avg_launch_us")
+        adj_trace_us = prof.get("k_row_lf_adj_ls<float,256>", {}).get("trace_avg_launch_us")
     model_b = model_bytes_per_map(N, C, n, 4)
     design_b = design_bytes_per_map(N, C, n, 4)
     result = {
@@ -857,7 +878,19 @@ def main():
         "column_pass": {"kernel": "k_col_lf_dx<float,256,16>",
                         "achieved_gbs": round(col_gbs, 1),
                         "avg_launch_us": round(col_avg_ms * 1e3, 3),
-                        "algorithmic_bytes_per_launch": col_bytes},
+                        "algorithmic_bytes_per_launch": col_bytes,
+                        "frac_events": round(col_gbs / HBM_PEAK_GBS, 4),
+                        "frac_trace": (round(col_bytes / (col_trace_us * 1e-6) / 1e9
+                                             / HBM_PEAK_GBS, 4) if col_trace_us else None),
+                        "trace_avg_launch_us": col_trace_us},
+        "adjoint_row_pass": {"kernel": "k_row_lf_adj_ls<float,256>",
+                             "achieved_gbs": round(adj_row_gbs, 1),
+                             "avg_launch_us": round(adj_row_avg_ms * 1e3, 3),
+                             "algorithmic_bytes_per_launch": adj_row_bytes,
+                             "frac_events": round(adj_row_gbs / HBM_PEAK_GBS, 4),
+                             "frac_trace": (round(adj_row_bytes / (adj_trace_us * 1e-6) / 1e9
+                                                  / HBM_PEAK_GBS, 4) if adj_trace_us else None),
+                             "trace_avg_launch_us": adj_trace_us},
         "config1": c1_leg,
     }
     for pk, ex, *_ in lanes:
