@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 
 #include "fft_lds.hpp"
@@ -498,7 +499,14 @@ __global__ void k_dphi_combine(const cplx<T>* __restrict__ F, cplx<T>* __restric
 // In the packed DC|Nyquist column only the DC part of d/dx survives the C2R
 // (the Nyquist part's d/dy drops out) and the self-conjugate kx = N/2 bin of
 // d/dx drops out, so that bin is zeroed.
-template <typename T, int N, int KC>
+//
+// XP (XCD-paired half tiles): each workgroup takes KC columns, half of a
+// 2 KC-column tile whose row segments are 128 bytes, and the two halves go to
+// workgroups 8 apart -- dispatched to the same XCD, so a segment's cache line
+// is fetched once into that XCD's L2 and both halves' partial-line stores
+// merge there.  Twice the workgroups of the full-tile shape.  The grid must
+// be a multiple of 16 workgroups.
+template <typename T, int N, int KC, bool XP = false>
 __global__ __launch_bounds__(KC* N / 8) void k_col_lf_dx(const cplx<T>* src, cplx<T>* dst,
                                                          const cplx<T>* __restrict__ twg, T dk) {
   constexpr int G = N / 8, H = N / 2, TILES = H / KC;
@@ -513,8 +521,18 @@ __global__ __launch_bounds__(KC* N / 8) void k_col_lf_dx(const cplx<T>* src, cpl
   } else {
     tw.load(twg, t);
   }
-  const long m = blockIdx.x / TILES;
-  const int k = (blockIdx.x % TILES) * KC + g;
+  long m;
+  int k;
+  if constexpr (XP) {
+    const long b = blockIdx.x;
+    const long seg = (b / 16) * 8 + b % 8;  // 2 KC-column tile
+    constexpr int SEGS = H / (2 * KC);
+    m = seg / SEGS;
+    k = (int)(seg % SEGS) * 2 * KC + (int)((b % 16) / 8) * KC + g;
+  } else {
+    m = blockIdx.x / TILES;
+    k = (blockIdx.x % TILES) * KC + g;
+  }
   const cplx<T>* s0 = src + m * N * H + k;
   cplx<T>* d0 = dst + m * N * H + k;
   cplx<T> v[1][8];
@@ -1261,6 +1279,19 @@ hipError_t launch_lf_row_r2c_gather(int N, const T* const* src, int ncomp, T* ys
   return hipGetLastError();
 }
 
+// The FP32 column pass at N = 512 runs on XCD-paired half tiles (two
+// workgroups of 8 columns per CU instead of one of 16: 8-RHS 512^2 Wiener
+// filter 2.11 -> 2.06 ms per iteration); at 256 the full tiles are faster
+// (27.7 against 30.7 us per launch at batch 128).  CMBL_COL_PAIR=0 turns the
+// pairing off (A/B runs).
+static int col_pair_mode() {
+  static const int mode = [] {
+    const char* e = std::getenv("CMBL_COL_PAIR");
+    return e ? std::atoi(e) : 1;
+  }();
+  return mode;
+}
+
 template <typename T>
 hipError_t launch_lf_col(int N, const cplx<T>* src, cplx<T>* dst, long nmaps, const cplx<T>* tw,
                          T dk, hipStream_t s) {
@@ -1268,6 +1299,13 @@ hipError_t launch_lf_col(int N, const cplx<T>* src, cplx<T>* dst, long nmaps, co
     constexpr int KC = col_kc<T>(NN_);
     constexpr int KCS = KC / 4 > 2 ? KC / 4 : 2;
     const unsigned nb = (unsigned)(nmaps * (NN_ / 2 / KC));
+    constexpr bool PAIRABLE = sizeof(T) == 4 && NN_ == 512 && KC == 16;
+    if constexpr (PAIRABLE) {
+      if (col_pair_mode() && nb >= CMBL_SMALL_COL_GRID && (2 * nb) % 16 == 0) {
+        k_col_lf_dx<T, NN_, KC / 2, true><<<2 * nb, KC / 2 * NN_ / 8, 0, s>>>(src, dst, tw, dk);
+        break;
+      }
+    }
     if (nb < CMBL_SMALL_COL_GRID)
       k_col_lf_dx<T, NN_, KCS><<<(unsigned)(nmaps * (NN_ / 2 / KCS)), KCS * NN_ / 8, 0, s>>>(
           src, dst, tw, dk);

@@ -394,3 +394,66 @@ def test_lense_flow_wrapper_shared_phi_batch(pkg, dev, sims):
                                          * c[k:k + 1]).sum(), (pk,))
             parts.append(gk)
         assert rel(g[0].cpu().numpy(), (parts[0] + parts[1]).cpu().numpy()) < 1e-13
+
+
+@pytest.mark.parametrize("dtype,tol,agree", [(torch.float64, 1e-11, 1e-12),
+                                             (torch.float32, 1e-5, 1e-5)])
+def test_adjoint_bookkeepings_agree(pkg, dev, sims, monkeypatch, dtype, tol, agree):
+    """The adjoint rate's row pass with the low-storage RK4 bookkeeping (the
+    default: the stage input stored in place of the V spectra) and with the
+    classic accumulator (CMBL_ADJ_CLASSIC=1, read when a plan is created) run
+    the same RK4: both adjoint directions and the gradient ODE (whose delta-f
+    rate uses the same row pass) agree with each other and with the oracle."""
+    from cmblensing_amd import _lib
+    N, th, n, C, B = 64, 2.0, 7, 2, 3
+    _case(f"adjoint bookkeepings N={N} QU batch {B} {dtype}")
+    pw = olf.pix_width_from_theta(th)
+    fs = np.stack([sims.qu_map(N, th, s) for s in range(B)])
+    phi = sims.phi_map(N, th, 4)
+    x = torch.as_tensor(fs, dtype=dtype, device=dev)
+    P = torch.as_tensor(phi, device=dev)
+    plans = {}
+    for classic in (False, True):
+        if classic:
+            monkeypatch.setenv("CMBL_ADJ_CLASSIC", "1")
+        else:
+            monkeypatch.delenv("CMBL_ADJ_CLASSIC", raising=False)
+        plans[classic] = _lib.Plan(N, th, dtype, B * C, dev)
+    monkeypatch.delenv("CMBL_ADJ_CLASSIC", raising=False)
+    handles = {k: _lib.PhiHandle(p, P, n) for k, p in plans.items()}
+    for direction in (1, -1):
+        got = {}
+        for k, plan in plans.items():
+            out = torch.empty_like(x)
+            _lib.lenseflow(plan, [handles[k]] * B, x, out, B, C, n, direction, True)
+            got[k] = out.double().cpu().numpy()
+        ref = olf.lense_flow(fs[1], phi, pw, n, direction, True)
+        e_ls, e_cl, e_ab = rel(got[False][1], ref), rel(got[True][1], ref), rel(got[False], got[True])
+        print(f"[err] adjoint dir={direction}: low-storage {e_ls:.2e} classic {e_cl:.2e} "
+              f"between {e_ab:.2e}", flush=True)
+        assert e_ls < tol and e_cl < tol and e_ab < agree
+    # the gradient ODE (lense_flow_wrapper backward) both ways
+    y = np.stack([np.stack(olf.lense_flow(fs[b], phi, pw, n)) for b in range(B)])
+    cot = np.random.default_rng(5).standard_normal(y.shape)
+    cdt = torch.complex64 if dtype == torch.float32 else torch.complex128
+    res = {}
+    for k, plan in plans.items():
+        yl = torch.as_tensor(y, dtype=dtype, device=dev)
+        ct = torch.as_tensor(cot, dtype=dtype, device=dev)
+        d_field = torch.empty_like(yl)
+        d_phi = torch.empty((B, N, N // 2 + 1), dtype=cdt, device=dev)
+        _lib.lenseflow_vjp(plan, [handles[k]] * B, yl, ct, d_field, torch.view_as_real(d_phi), B,
+                           C, n, 1)
+        res[k] = (d_field.double().cpu().numpy(), d_phi.cpu().numpy())
+    _, dfs, dphi = olf.lensing_operator_gradients(phi, list(y[0]), list(cot[0]), -1, n, pw)
+    for k in plans:
+        e_f = rel(res[k][0][0], np.stack(dfs))
+        e_p = rel(np.fft.irfft2(res[k][1][0]), dphi)
+        print(f"[err] vjp classic={k}: d_field {e_f:.2e} d_phi {e_p:.2e}", flush=True)
+        assert e_f < tol and e_p < tol
+    assert rel(res[False][0], res[True][0]) < agree
+    assert rel(res[False][1], res[True][1]) < agree
+    for h in handles.values():
+        h.close()
+    for p in plans.values():
+        p.close()
