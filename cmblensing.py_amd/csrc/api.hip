@@ -133,7 +133,7 @@ struct cmbl_plan {
   // gradient-ODE scratch, allocated on first use
   void* gy = nullptr;    // f state copy [max_maps][N][N]
   void* gacc = nullptr;  // delta-f accumulator [max_maps][N][N]
-  void* gzd = nullptr;   // delta-f stage input [max_maps][N][N]
+  void* gzd = nullptr;   // delta-f stage input [max_maps][N][N] (classic adjoint only)
   void* gu = nullptr;    // U, V packed spectra [max_maps][N][N/2]
   void* gv = nullptr;
   void* gsrc = nullptr;  // delta-phi sources [max_maps][5][N][N]
@@ -1293,7 +1293,8 @@ cmbl_status ensure_grad_scratch(cmbl_plan* p) {
   hipError_t e;
   if ((e = hipMalloc(&p->gy, M * NN * ts)) != hipSuccess ||
       (e = hipMalloc(&p->gacc, M * NN * ts)) != hipSuccess ||
-      (e = hipMalloc(&p->gzd, M * NN * ts)) != hipSuccess ||
+      // the classic adjoint bookkeeping's copy of each delta-f stage input
+      (p->adj_classic && (e = hipMalloc(&p->gzd, M * NN * ts)) != hipSuccess) ||
       (e = hipMalloc(&p->gu, M * NH * 2 * ts)) != hipSuccess ||
       (e = hipMalloc(&p->gv, M * NH * 2 * ts)) != hipSuccess ||
       (e = hipMalloc(&p->gsrc, M * 5 * NN * ts)) != hipSuccess ||
