@@ -506,10 +506,25 @@ __global__ void k_dphi_combine(const cplx<T>* __restrict__ F, cplx<T>* __restric
 // is fetched once into that XCD's L2 and both halves' partial-line stores
 // merge there.  Twice the workgroups of the full-tile shape.  The grid must
 // be a multiple of 16 workgroups.
+// Tile b of the column pass -> map m and the column k of lane g.
+template <int N, int KC, bool XP>
+__device__ __forceinline__ void col_tile(long b, int g, long& m, int& k) {
+  constexpr int H = N / 2, TILES = H / KC;
+  if constexpr (XP) {
+    const long seg = (b / 16) * 8 + b % 8;  // 2 KC-column tile
+    constexpr int SEGS = H / (2 * KC);
+    m = seg / SEGS;
+    k = (int)(seg % SEGS) * 2 * KC + (int)((b % 16) / 8) * KC + g;
+  } else {
+    m = b / TILES;
+    k = (int)(b % TILES) * KC + g;
+  }
+}
+
 template <typename T, int N, int KC, bool XP = false>
 __global__ __launch_bounds__(KC* N / 8) void k_col_lf_dx(const cplx<T>* src, cplx<T>* dst,
                                                          const cplx<T>* __restrict__ twg, T dk) {
-  constexpr int G = N / 8, H = N / 2, TILES = H / KC;
+  constexpr int G = N / 8, H = N / 2;
   constexpr bool TWL = col_tw_lds<T, N>();
   __shared__ cplx<T> lbuf[KC][line_stride<N>()];
   __shared__ cplx<T> twl[TWL ? N : 1];
@@ -523,16 +538,7 @@ __global__ __launch_bounds__(KC* N / 8) void k_col_lf_dx(const cplx<T>* src, cpl
   }
   long m;
   int k;
-  if constexpr (XP) {
-    const long b = blockIdx.x;
-    const long seg = (b / 16) * 8 + b % 8;  // 2 KC-column tile
-    constexpr int SEGS = H / (2 * KC);
-    m = seg / SEGS;
-    k = (int)(seg % SEGS) * 2 * KC + (int)((b % 16) / 8) * KC + g;
-  } else {
-    m = blockIdx.x / TILES;
-    k = (blockIdx.x % TILES) * KC + g;
-  }
+  col_tile<N, KC, XP>(blockIdx.x, g, m, k);
   const cplx<T>* s0 = src + m * N * H + k;
   cplx<T>* d0 = dst + m * N * H + k;
   cplx<T> v[1][8];
