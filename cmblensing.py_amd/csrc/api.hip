@@ -115,6 +115,9 @@ struct cmbl_plan {
   // adjoint rate with the classic RK4 accumulator (k_row_lf_adj) instead of
   // the low-storage bookkeeping (k_row_lf_adj_ls); for A/B measurements
   bool adj_classic = false;
+  // small FP32 maps (N <= 64) lensed by the one-launch resident kernel
+  // (k_lf_resident) instead of the two-pass stage sequence
+  bool resident = true;
   // optional per-kernel timing (HIP events around every stage launch)
   bool timing = false;
   std::vector<hipEvent_t> tev;   // pool, 2 per launch
@@ -469,6 +472,9 @@ cmbl_status cmbl_plan_create(cmbl_plan** out, int N, double theta_pix_arcmin, in
     // CMBL_ADJ_CLASSIC=1 runs the adjoint rate with the classic accumulator
     const char* ac_env = std::getenv("CMBL_ADJ_CLASSIC");
     p->adj_classic = ac_env && ac_env[0] == '1';
+    // CMBL_RESIDENT=0 runs small FP32 maps through the two-pass sequence too
+    const char* rs_env = std::getenv("CMBL_RESIDENT");
+    p->resident = !(rs_env && rs_env[0] == '0');
   }
   // set-up of the allocation-free entry point on this device
   void* sp = nullptr;
@@ -712,6 +718,13 @@ cmbl_status run_lenseflow(cmbl_plan* p, const void* const* ptabs, T* y, long nma
   static const int stage_half[5] = {0, 0, 1, 1, 2};
   const bool adj = adjoint != 0;
   Timer tm{p, s, bufs == nullptr};
+  // small FP32 maps, standard rate: the whole LenseFlow in one launch with the
+  // stage state of each component in one workgroup's LDS
+  if (std::is_same<T, float>::value && !adj && N <= 64 && p->resident) {
+    TIMED(2, cmbl::launch_lf_resident<T>(N, src, outp, y, C, pt, tw, nmaps, n, fwd_time ? 1 : 0,
+                                         h, (T)p->dk, s));
+    return tm.flush();
+  }
   // one part, or two halves of the batch on two streams (not for the
   // executor's sequences, stage buffers of the caller, timed runs or a
   // stream under capture)

@@ -1129,6 +1129,181 @@ __global__ __launch_bounds__(BS) void k_row_lf_adj_ls(cplx<T>* __restrict__ S0,
   split_store<N, kRowWave>(w[0], bufa, S0 + rp.ra * H, S0 + rp.rb * H, t, active);
 }
 
+// ---------------------------------------------------------------------------
+// Resident LenseFlow of small maps (N <= 64, FP32)
+// ---------------------------------------------------------------------------
+// One workgroup runs the whole LenseFlow of one component (standard rate):
+// its stage state -- the packed row spectra S, their x-derivative DX, the
+// state y and the RK4 accumulator, 64 KB at N = 64 -- stays in LDS for all 4n
+// stages, and only the p(t) rows are read from memory (from L2 when several
+// components use one phi).  Per stage a column phase (every packed column:
+// FFT, i KX / N^2, inverse FFT, into DX) and a row phase (per row pair: the
+// lines DX + i DY, inverse FFTs, rate p(t) . grad z, classic RK4 update, the
+// next stage input R2C'd into S), separated by workgroup barriers: one launch
+// instead of 4n + 1 pairs of launches, for requests whose launches would be
+// latency-bound.  The line transforms and the per-pixel arithmetic are those
+// of the two-pass kernels; the bookkeeping is the classic accumulator (the
+// state lives on chip, so rebuilding it would save no memory traffic).
+// src / outp: the serving executor's request pointers (src[b] + c N^2, NULL:
+// zeros; outp[b] + c N^2, NULL: not written); without them y [nmaps][N][N]
+// is lensed in place.  One workgroup per component (map m = b C + c).
+template <typename T, int N, int BS>
+__global__ __launch_bounds__(BS) void k_lf_resident(const T* const* __restrict__ src,
+                                                     T* const* __restrict__ outp,
+                                                     T* __restrict__ y, int C,
+                                                     const T* const* __restrict__ ptabs,
+                                                     const cplx<T>* __restrict__ twg,
+                                                     int nsteps, int fwd_time, T h, T dk) {
+  constexpr int G = N / 8, H = N / 2, L = BS / G;  // line groups
+  static_assert(BS % 64 == 0 && L >= H, "whole wavefronts, a group per pair and per column");
+  constexpr int NN = N * N;
+  __shared__ cplx<T> S[N * H];
+  __shared__ cplx<T> DX[N * H];
+  __shared__ T Y[NN];
+  __shared__ T ACC[NN];
+  __shared__ cplx<T> lbuf[L][2][lds_len<N>()];
+  const int j = threadIdx.x / G, t = threadIdx.x % G;
+  // groups beyond the H pairs / columns repeat one (their results are not
+  // stored), so every lane of a wavefront runs the same line transforms
+  const int jj = j % H;
+  const bool owner = j < H;
+  Twiddles<N, T> tw;
+  tw.load(twg, t);
+  const long m = blockIdx.x;
+  const long b = m / C;
+  const int c = (int)(m % C);
+  const T* in = src ? (src[b] ? src[b] + (long)c * NN : nullptr) : y + m * NN;
+  T* out = outp ? (outp[b] ? outp[b] + (long)c * NN : nullptr) : y + m * NN;
+  const T* pt = ptabs[b];
+  cplx<T>* bufa = lbuf[j][0];
+  cplx<T>* bufb = lbuf[j][1];
+  cplx<T>* const bla[1] = {bufa};
+  const int ra = 2 * jj, rb = ra + 1;  // the pair's rows
+  // state in, and the row spectra of the first stage input
+  for (int i = threadIdx.x; i < NN; i += BS) Y[i] = in ? in[i] : T(0);
+  __syncthreads();
+  cplx<T> z[1][8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) z[0][q] = mk(Y[ra * N + t + q * G], Y[rb * N + t + q * G]);
+  fft_lines<N, false, 1, kRowWave>(z, bla, tw, t);
+  split_store<N, kRowWave>(z[0], bufa, S + ra * H, S + rb * H, t, owner);
+  const T sc = T(1) / (T(N) * T(N));
+  const T cy = dk / T(N);
+  static const int stage_half[5] = {0, 0, 1, 1, 2};
+  for (int i = 0; i < nsteps; ++i) {
+    for (int st = 1; st <= 4; ++st) {
+      const bool last = (i == nsteps - 1) && (st == 4);
+      const int jt = 2 * i + stage_half[st];
+      const int tidx = fwd_time ? jt : 2 * nsteps - jt;
+      // this stage's p(t) rows, loaded before the column phase so that their
+      // memory latency is spent under it
+      T pxr[2][8], pyr[2][8];
+      {
+        const T* px = pt + (long)tidx * 2 * NN;
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const int jp = (r ? rb : ra) * N + t + q * G;
+            pxr[r][q] = px[jp];
+            pyr[r][q] = px[NN + jp];
+          }
+      }
+      __syncthreads();  // S of this stage complete
+      // column phase: column jj of S -> d/dx into DX
+      {
+        cplx<T> v[1][8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[0][q] = S[(t + q * G) * H + jj];
+        fft_lines<N, false, 1, kRowWave>(v, bla, tw, t);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int kx = t + q * G;
+          const T kxv = (T)kx_signed<N>(kx) * dk * sc;
+          v[0][q] = (jj == 0 && kx == N / 2) ? mk(T(0), T(0)) : imul(v[0][q], kxv);
+        }
+        fft_lines<N, true, 1, kRowWave>(v, bla, tw, t);
+        if (owner) {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) DX[(t + q * G) * H + jj] = v[0][q];
+        }
+      }
+      __syncthreads();  // DX complete
+      // row phase: rows ra, rb
+      cplx<T> dxa[4], sa[4], dxb[4], sb[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int k = t + q * G;
+        dxa[q] = DX[ra * H + k];
+        sa[q] = S[ra * H + k];
+        dxb[q] = DX[rb * H + k];
+        sb[q] = S[rb * H + k];
+      }
+      lds_sync<kRowWave>();
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int k = t + q * G;
+        const T cc = (T)k * cy;
+        put_herm<N>(bufa, k, dxa[q], k == 0 ? mk(T(0), T(0)) : imul(sa[q], cc));
+        put_herm<N>(bufb, k, dxb[q], k == 0 ? mk(T(0), T(0)) : imul(sb[q], cc));
+      }
+      lds_sync<kRowWave>();
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        cplx<T>* const b1[1] = {r ? bufb : bufa};
+        cplx<T> w[1][8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) w[0][q] = b1[0][lds_idx<N>(t + q * G)];
+        fft_lines<N, true, 1, kRowWave>(w, b1, tw, t);
+        const int row = r ? rb : ra;
+        // the classic RK4 bookkeeping (util.py:184-199)
+        T zz[8];
+        if (st == 4) {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const int jp = row * N + t + q * G;
+            zz[q] = ACC[jp] + (h / T(6)) * (pxr[r][q] * w[0][q].x + pyr[r][q] * w[0][q].y);
+          }
+          if (owner) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) Y[row * N + t + q * G] = zz[q];
+          }
+        } else {
+          const T ha = st == 1 ? h / T(6) : h / T(3);
+          const T hz = st == 3 ? h : h / T(2);
+          T av[8];
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const int jp = row * N + t + q * G;
+            const T k = pxr[r][q] * w[0][q].x + pyr[r][q] * w[0][q].y;
+            const T yv = Y[jp];
+            av[q] = (st == 1 ? yv : ACC[jp]) + ha * k;
+            zz[q] = yv + hz * k;
+          }
+          if (owner) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) ACC[row * N + t + q * G] = av[q];
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          if (r == 0)
+            z[0][q].x = zz[q];
+          else
+            z[0][q].y = zz[q];
+        }
+      }
+      if (last) break;
+      fft_lines<N, false, 1, kRowWave>(z, bla, tw, t);
+      split_store<N, kRowWave>(z[0], bufa, S + ra * H, S + rb * H, t, owner);
+    }
+  }
+  __syncthreads();  // Y complete
+  if (out) {
+    for (int i = threadIdx.x; i < NN; i += BS) out[i] = Y[i];
+  }
+}
+
 // Gather / scatter of whole maps through device pointer arrays (serving
 // executor), `len` elements per slot b = blockIdx.y.  A NULL source fills the
 // slot with zeros (padding); a NULL destination skips it.
@@ -1429,6 +1604,34 @@ hipError_t launch_lf_row_adj_ls(int N, cplx<T>* S0, T* Z, T* y, T* acc, const T*
 }
 
 template <typename T>
+hipError_t launch_lf_resident(int N, const T* const* src, T* const* outp, T* y, int C,
+                              const T* const* ptabs, const cplx<T>* tw, long nmaps, int nsteps,
+                              int fwd_time, T h, T dk, hipStream_t s) {
+  if constexpr (sizeof(T) != 4) {
+    return hipErrorInvalidValue;
+  } else {
+    if (nmaps < 1) return hipSuccess;
+    switch (N) {
+      case 16:
+        k_lf_resident<T, 16, 64><<<(unsigned)nmaps, 64, 0, s>>>(src, outp, y, C, ptabs, tw,
+                                                                nsteps, fwd_time, h, dk);
+        break;
+      case 32:
+        k_lf_resident<T, 32, 64><<<(unsigned)nmaps, 64, 0, s>>>(src, outp, y, C, ptabs, tw,
+                                                                nsteps, fwd_time, h, dk);
+        break;
+      case 64:
+        k_lf_resident<T, 64, 256><<<(unsigned)nmaps, 256, 0, s>>>(src, outp, y, C, ptabs, tw,
+                                                                  nsteps, fwd_time, h, dk);
+        break;
+      default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
+}
+
+template <typename T>
 hipError_t launch_gtab(int N, const double* D, T* gtab, int nsteps, hipStream_t s) {
   const long P = (long)N * N;
   k_gtab<T><<<blocks_for(P, 256), 256, 0, s>>>(D, gtab, N, nsteps);
@@ -1480,6 +1683,9 @@ hipError_t launch_scatter(const T* src, T* const* dst, long len, int count, hipS
   template hipError_t launch_lf_row_adj_ls<T>(int, cplx<T>*, T*, T*, T*, const T* const*,       \
                                               const cplx<T>*, int, long, int, T, int, int, int, \
                                               int, T, hipStream_t);                             \
+  template hipError_t launch_lf_resident<T>(int, const T* const*, T* const*, T*, int,          \
+                                            const T* const*, const cplx<T>*, long, int, int, T, \
+                                            T, hipStream_t);                                    \
   template hipError_t launch_gtab<T>(int, const double*, T*, int, hipStream_t);                 \
   template hipError_t launch_gather<T>(const T* const*, T*, long, int, hipStream_t);            \
   template hipError_t launch_scatter<T>(const T*, T* const*, long, int, hipStream_t);           \

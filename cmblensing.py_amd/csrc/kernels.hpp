@@ -68,6 +68,13 @@ template <typename T>
 hipError_t launch_lf_row_adj_ls(int N, cplx<T>* S0, T* Z, T* y, T* acc, const T* const* ptabs,
                                 const cplx<T>* tw, int C, long nmaps, int stage, T h, int tidx,
                                 int tidx_next, int do_r2c, int init, T dk, hipStream_t s);
+// The whole standard-rate LenseFlow of small FP32 maps (N <= 64) in one
+// launch, one workgroup per component with its stage state in LDS.  src /
+// outp: request pointers of the serving executor (else y in place).
+template <typename T>
+hipError_t launch_lf_resident(int N, const T* const* src, T* const* outp, T* y, int C,
+                              const T* const* ptabs, const cplx<T>* tw, long nmaps, int nsteps,
+                              int fwd_time, T h, T dk, hipStream_t s);
 // First launch of the serving executor: row R2C of the requests' maps
 // (src[b] + component N^2, NULL = zeros) into S0, also stored to ystage.
 template <typename T>

@@ -390,3 +390,61 @@ def test_julia_lensing_matrix(pkg, dev, pol, mode):
     r = rel(st(out), st(ref))
     print(f"{pol} {mode}: rel L2 {r:.3g}", flush=True)
     assert r < tol
+
+
+@pytest.mark.parametrize("N", [16, 32, 64])
+@pytest.mark.parametrize("C", [1, 2])
+def test_resident_small_maps(pkg, dev, sims, monkeypatch, N, C):
+    """FP32 LenseFlows at N <= 64 run in one launch with each component's
+    stage state in LDS (k_lf_resident); CMBL_RESIDENT=0, read when a plan is
+    created, runs them through the two-pass stage sequence.  Both directions,
+    n = 7 and n = 1, a batch of 3 with a phi per request: each request within
+    1e-5 of the FP64 oracle both ways and the two ways within FP32 rounding of
+    each other; the allocation-free entry point, and a request alone, equal
+    the batched result bit for bit."""
+    from cmblensing_amd import _lib
+    th, B = 2.0, 3
+    _case(f"resident N={N} C={C} batch {B}")
+    pw = olf.pix_width_from_theta(th)
+    fs = np.stack([(sims.t_map if C == 1 else sims.qu_map)(N, th, s).reshape(C, N, N)
+                   for s in range(B)])
+    phis = [sims.phi_map(N, th, 10 + s) for s in range(B)]
+    x = torch.as_tensor(fs, dtype=torch.float32, device=dev)
+    plans = {}
+    for resident in (True, False):
+        if resident:
+            monkeypatch.delenv("CMBL_RESIDENT", raising=False)
+        else:
+            monkeypatch.setenv("CMBL_RESIDENT", "0")
+        plans[resident] = _lib.Plan(N, th, torch.float32, B * C, dev)
+    monkeypatch.delenv("CMBL_RESIDENT", raising=False)
+    for n in (7, 1):
+        hs = {k: [_lib.PhiHandle(p, torch.as_tensor(ph, device=dev), n) for ph in phis]
+              for k, p in plans.items()}
+        for direction in (1, -1):
+            got = {}
+            for k, plan in plans.items():
+                out = torch.empty_like(x)
+                _lib.lenseflow(plan, hs[k], x, out, B, C, n, direction, False)
+                got[k] = out.double().cpu().numpy()
+            for b in range(B):
+                ref = olf.lense_flow(fs[b][0] if C == 1 else fs[b], phis[b], pw, n, direction,
+                                     False)
+                ref = np.asarray(ref).reshape(C, N, N)
+                assert rel(got[True][b], ref) < 1e-5, (n, direction, b)
+                assert rel(got[False][b], ref) < 1e-5, (n, direction, b)
+            assert rel(got[True], got[False]) < 1e-5
+            # the allocation-free entry point takes the same one-launch path
+            na = torch.empty_like(x)
+            _lib.lenseflow_noalloc(plans[True], hs[True], x, na, B, C, n, direction, False)
+            assert torch.equal(na.cpu(), torch.as_tensor(got[True], dtype=torch.float32))
+            # request 1 alone on the resident path equals its place in the batch
+            one = torch.empty_like(x[1:2])
+            _lib.lenseflow(plans[True], hs[True][1:2], x[1:2].contiguous(), one, 1, C, n,
+                           direction, False)
+            assert torch.equal(one[0].cpu(), torch.as_tensor(got[True][1], dtype=torch.float32))
+        for k in hs:
+            for h in hs[k]:
+                h.close()
+    for p in plans.values():
+        p.close()
