@@ -4,8 +4,9 @@ Workload: forward LenseFlow of Q/U map pairs, N = 256, 2 arcmin pixels, float32,
 7 RK4 steps (the 256^2 QU configuration).  One "step" is one batched pass over
 `--batch` requests; request r uses phi number r % 32 of 32 seeded lensing
 potentials whose operators are prepared once and stay resident in HBM (the
-serving form), and each stream's batch is ordered by phi, as the serving
-backend orders the batches it forms.  The plain path is one request per call with its phi prepared
+serving form), and the requests of a step are ordered by phi before they
+are split over the streams, as the serving backend forms the batches of a
+bucket that become ready together.  The plain path is one request per call with its phi prepared
 in the same call, as the reference's lense_flow does.
 
 The timed step is the served form: the batch split over the serving
@@ -708,17 +709,19 @@ def main():
     print(f"[bench] workload QU N={N} theta={th} n={n} fp32 batch={B}", file=sys.stderr, flush=True)
 
     # synthetic inputs: 32 phi; request r (in arrival order) lenses Q/U draw
-    # r % 8 of 8 by phi r % 32.  The serving backend forms a batch of each
-    # stream's share of the arrivals and orders it by phi (requests that share
+    # r % 8 of 8 by phi r % 32.  The serving backend forms the batches of a
+    # step together and orders them by phi (requests that share
     # a phi then read the same p(t) rows together), so the batch positions
     # hold the requests in that order; every path below uses it.
     NS = args.streams
     if B % NS:
         sys.exit(f"--batch {B} must be a multiple of --streams {NS}")
     Bs = B // NS
-    order = []
-    for k in range(NS):
-        order += sorted(range(k * Bs, (k + 1) * Bs), key=lambda r: r % NPHI)
+    # the requests of a step ordered by phi across the whole step, then split
+    # over the streams (the serving scheduler forms the batches of a bucket
+    # that become ready together this way; stable, so arrival order among the
+    # requests of one phi)
+    order = sorted(range(B), key=lambda r: r % NPHI)
     req_f = [r % 8 for r in order]          # Q/U draw of batch position j
     req_p = [r % NPHI for r in order]       # phi of batch position j
     phis = [sims.phi_map(N, th, 1000 + i) for i in range(NPHI)]
@@ -849,7 +852,7 @@ def main():
         "data": "synthetic (seeded Gaussian random fields, 32 resident phi)",
         "config": {"workload": "lenseflow_fwd_QU_256_2arcmin_n7_fp32", "N": N, "theta_pix": th,
                    "nsteps": n, "components": "QU", "batch": B, "phi_resident": NPHI,
-                   "streams": NS, "batch_order": "each stream's batch ordered by phi"},
+                   "streams": NS, "batch_order": "the step's requests ordered by phi, then split over the streams"},
         "direct_batched_maps_s": round(direct_maps_s, 2),
         "plain_maps_s": round(plain_maps_s, 2),
         "batched_over_plain": round(maps_s / plain_maps_s, 2),

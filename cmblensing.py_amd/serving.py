@@ -19,8 +19,11 @@ a set of shared lensing potentials -- and runs many requests together:
   request arrived first is launched as one batch of up to the bucket's batch
   size, so requests that arrive while others run are packed into the next
   launch and batches of different buckets overlap on the device's streams;
-  a batch's positions are ordered by phi, so requests sharing a phi read its
-  p(t) rows together;
+  when a bucket holds more than one batch while several stream slots are
+  free, its next batches are formed together and ordered by phi across all
+  of them before the split, so requests sharing a phi fall into one batch;
+  a batch's positions are ordered by phi, so those requests read its p(t)
+  rows together;
 * each batch is one replay of a hipGraph captured once per (stream, bucket,
   batch class), with the request maps gathered into and scattered out of a
   staging buffer by the graph's own kernels;
@@ -308,13 +311,6 @@ class Server:
             self.phis[(req.phi_id, req.N)].refs -= 1
 
     # -- batching ------------------------------------------------------------
-    def _free_stream(self) -> Optional[int]:
-        best = None
-        for s, n in enumerate(self.per_stream):
-            if n < self.max_inflight and (best is None or n < self.per_stream[best]):
-                best = s
-        return best
-
     def _launch(self) -> int:
         launched = 0
         while self.wf_queue and self.wf_running < self.wf_slots:
@@ -325,8 +321,8 @@ class Server:
             self.wf_running += 1
             launched += 1
         while True:
-            s = self._free_stream()
-            if s is None:
+            slots = self._free_slots()
+            if not slots:
                 break
             key, q = None, None
             for k, dq in self.buckets.items():
@@ -334,22 +330,46 @@ class Server:
                     key, q = k, dq
             if q is None:
                 break
-            nb = min(len(q), self.backend.max_batch(key))
-            # batch positions ordered by phi: requests that share a phi then
-            # read the same rows of its p(t) table at about the same time
-            # (their row-pass workgroups lie a fixed stride apart, on one
-            # XCD), so its L2 serves the repeats; a request's result does not
-            # depend on its position
-            reqs = sorted((q.popleft() for _ in range(nb)), key=lambda r: r.phi_id)
-            handles = [self.phis[(r.phi_id, r.N)].handle for r in reqs]
-            for r in reqs:
-                r.state = "running"
-            token = self.backend.launch(key, reqs, handles, s)
-            self.inflight.append(_Batch(token, reqs, s, key))
-            self.per_stream[s] += 1
-            self.batch_sizes.append(nb)
-            launched += 1
+            cap = self.backend.max_batch(key)
+            # a bucket holding more than one batch while several slots are
+            # free: its next batches are formed together and launched now, and
+            # their requests are ordered by phi across all of them before the
+            # split, so requests that share a phi fall into one batch.  Within
+            # a batch, positions ordered by phi make those requests read the
+            # same rows of its p(t) table at about the same time (their
+            # row-pass workgroups lie a fixed stride apart, on one XCD), so its
+            # L2 serves the repeats.  No request waits for the grouping, and a
+            # request's result does not depend on its batch or position.
+            nbat = min(len(slots), -(-len(q) // cap))
+            m = min(len(q), nbat * cap)
+            reqs_all = sorted((q.popleft() for _ in range(m)), key=lambda r: r.phi_id)
+            for i in range(nbat):
+                s = slots[i]
+                reqs = reqs_all[i * cap:(i + 1) * cap]
+                handles = [self.phis[(r.phi_id, r.N)].handle for r in reqs]
+                for r in reqs:
+                    r.state = "running"
+                token = self.backend.launch(key, reqs, handles, s)
+                self.inflight.append(_Batch(token, reqs, s, key))
+                self.per_stream[s] += 1
+                self.batch_sizes.append(len(reqs))
+                launched += 1
         return launched
+
+    def _free_slots(self):
+        """Streams with room, in the order batches would be given to them
+        one at a time (least loaded first, up to max_inflight each)."""
+        load = list(self.per_stream)
+        slots = []
+        while True:
+            best = None
+            for s, n in enumerate(load):
+                if n < self.max_inflight and (best is None or n < load[best]):
+                    best = s
+            if best is None:
+                return slots
+            slots.append(best)
+            load[best] += 1
 
     def _poll(self) -> int:
         finished = 0

@@ -333,3 +333,28 @@ def test_caustic_phi_rejected(srv):
     assert good.state == "done"
     assert len([h for h in be.prepared if h[1] == 1.0]) == 1 and len(be.released) == 1
     assert s.used == be.fixed + 5000
+
+
+def test_batches_formed_together_group_requests_by_phi(srv):
+    """A bucket holding more than one batch while several stream slots are
+    free: its batches are formed in the same step and ordered by phi before
+    the split, so the requests of one phi fall into one batch (their p(t)
+    rows are then read once per batch); every request still launches at once
+    and gets its own result."""
+    be = HostBackend(nstreams=2, max_batch={16: 8, 32: 8}, latency_polls=3)
+    s = srv.Server(be, _phi_source, hbm_budget=10 ** 9, max_inflight=1)
+    # 16 requests in arrival order with phi 0, 1, 2, 3, 0, 1, ... (4 each)
+    reqs = [s.submit(_req(srv, 16, 1, i % 4, FWD, i)) for i in range(16)]
+    s.step()
+    assert len(be.launches) == 2                      # both streams, same step
+    assert {t["stream"] for t in be.launches} == {0, 1}
+    assert [len(t["reqs"]) for t in be.launches] == [8, 8]
+    phis = [sorted({r.phi_id for r in t["reqs"]}) for t in be.launches]
+    assert phis == [[0, 1], [2, 3]]                   # each phi in one batch
+    for t in be.launches:                             # positions ordered by phi
+        ids = [r.phi_id for r in t["reqs"]]
+        assert ids == sorted(ids)
+    s.drain()
+    assert all(r.state == "done" for r in reqs)
+    for r in reqs:
+        assert torch.equal(r.out, r.x * (r.phi_id + 1) * r.direction)
