@@ -321,8 +321,8 @@ class Server:
             self.wf_running += 1
             launched += 1
         while True:
-            slots = self._free_slots()
-            if not slots:
+            s = self._free_stream()
+            if s is None:
                 break
             key, q = None, None
             for k, dq in self.buckets.items():
@@ -331,30 +331,47 @@ class Server:
             if q is None:
                 break
             cap = self.backend.max_batch(key)
+            # batch positions ordered by phi: requests that share a phi then
+            # read the same rows of its p(t) table at about the same time
+            # (their row-pass workgroups lie a fixed stride apart, on one
+            # XCD), so its L2 serves the repeats; a request's result does not
+            # depend on its batch or position
+            if len(q) <= cap:
+                reqs = sorted((q.popleft() for _ in range(len(q))), key=lambda r: r.phi_id)
+                launched += self._start(key, reqs, s)
+                continue
             # a bucket holding more than one batch while several slots are
-            # free: its next batches are formed together and launched now, and
-            # their requests are ordered by phi across all of them before the
-            # split, so requests that share a phi fall into one batch.  Within
-            # a batch, positions ordered by phi make those requests read the
-            # same rows of its p(t) table at about the same time (their
-            # row-pass workgroups lie a fixed stride apart, on one XCD), so its
-            # L2 serves the repeats.  No request waits for the grouping, and a
-            # request's result does not depend on its batch or position.
+            # free: its next batches are formed together and launched now,
+            # ordered by phi across all of them before the split, so requests
+            # that share a phi fall into one batch; no request waits for the
+            # grouping.  (The slot list is built only here: the loop runs on
+            # every scheduler step, and allocating on each one set off garbage
+            # collections that held the interpreter lock and stalled the
+            # Wiener-filter lane's thread.)
+            slots = self._free_slots()
             nbat = min(len(slots), -(-len(q) // cap))
             m = min(len(q), nbat * cap)
             reqs_all = sorted((q.popleft() for _ in range(m)), key=lambda r: r.phi_id)
             for i in range(nbat):
-                s = slots[i]
-                reqs = reqs_all[i * cap:(i + 1) * cap]
-                handles = [self.phis[(r.phi_id, r.N)].handle for r in reqs]
-                for r in reqs:
-                    r.state = "running"
-                token = self.backend.launch(key, reqs, handles, s)
-                self.inflight.append(_Batch(token, reqs, s, key))
-                self.per_stream[s] += 1
-                self.batch_sizes.append(len(reqs))
-                launched += 1
+                launched += self._start(key, reqs_all[i * cap:(i + 1) * cap], slots[i])
         return launched
+
+    def _start(self, key, reqs, s) -> int:
+        handles = [self.phis[(r.phi_id, r.N)].handle for r in reqs]
+        for r in reqs:
+            r.state = "running"
+        token = self.backend.launch(key, reqs, handles, s)
+        self.inflight.append(_Batch(token, reqs, s, key))
+        self.per_stream[s] += 1
+        self.batch_sizes.append(len(reqs))
+        return 1
+
+    def _free_stream(self) -> Optional[int]:
+        best = None
+        for s, n in enumerate(self.per_stream):
+            if n < self.max_inflight and (best is None or n < self.per_stream[best]):
+                best = s
+        return best
 
     def _free_slots(self):
         """Streams with room, in the order batches would be given to them
