@@ -615,7 +615,11 @@ const void* cmbl_phi_derivatives(const cmbl_phi* h) { return h ? h->deriv : null
 namespace {
 
 // Timing: begin_launch/end_launch bracket one kernel launch with events when
-// the plan's timing mode is on; flush_timing waits and accumulates.
+// the plan's timing mode is on; flush_timing waits and accumulates.  The
+// events only measure time, so they are recorded without the system-scope
+// fence (a cache writeback and invalidate per record, which would add to the
+// bracketed interval and slow the next launch).  Nothing reads the bracketed
+// kernels' results through these events.
 struct Timer {
   cmbl_plan* p;
   hipStream_t s;
@@ -625,7 +629,7 @@ struct Timer {
     if (!on || !p->timing) return CMBL_OK;
     while (p->tev.size() < 2 * (used + 1)) {
       hipEvent_t e;
-      HIP_TRY(hipEventCreate(&e));
+      HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
       p->tev.push_back(e);
     }
     if (p->tkind.size() < used + 1) p->tkind.resize(used + 1);

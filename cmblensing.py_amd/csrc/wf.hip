@@ -553,7 +553,11 @@ __device__ __forceinline__ cplx<T> pick(const cplx<T> (&a)[C], int c) {
 //   EDGE 0: packed -> op -> packed, in place (times scale_)
 //   EDGE 1: natural -> op -> packed (times scale_)
 //   EDGE 2: packed -> op -> natural (forward; scale_ unused)
-template <typename T, int N, int KC, int C, int EDGE, typename Op>
+// XP: tiles narrower than a 128-byte line are taken in pairs; blocks b and
+// b + 8 (dispatched to the same XCD, which has its own L2) hold the two tiles
+// of a pair, so each line is fetched from HBM once.  The grid is a multiple
+// of 16 blocks.
+template <typename T, int N, int KC, int C, int EDGE, typename Op, bool XP = false>
 __global__ __launch_bounds__(C * KC * N / 8) void k_col_edge(const cplx<T>* in, cplx<T>* out,
                                                              Op op,
                                                              const cplx<T>* __restrict__ twg,
@@ -565,8 +569,16 @@ __global__ __launch_bounds__(C * KC * N / 8) void k_col_edge(const cplx<T>* in, 
   const int g = threadIdx.x % KC, t = (threadIdx.x / KC) % G, c = threadIdx.x / (KC * G);
   Twiddles<N, T> tw;
   tw.load(twg, t);
-  const long f = blockIdx.x / TILES;
-  const int tile = blockIdx.x % TILES;
+  long f;
+  int tile;
+  if constexpr (XP) {
+    const long b = blockIdx.x, pair = (b / 16) * 8 + b % 8;
+    f = pair / (TILES / 2);
+    tile = (int)(pair % (TILES / 2)) * 2 + (int)((b % 16) / 8);
+  } else {
+    f = blockIdx.x / TILES;
+    tile = blockIdx.x % TILES;
+  }
   const int k = tile * KC + g;
   auto L = [&](int cc, int kx) -> cplx<T>& { return lbuf[cc][g][lds_idx<N, false>(kx)]; };
   cplx<T>* const bufs[1] = {lbuf[c][g]};
@@ -781,10 +793,16 @@ template <typename T, int C, int EDGE, template <typename, int> class OpT>
 static hipError_t col_edge(int N, const cplx<T>* in, cplx<T>* out, const OpT<T, C>& op,
                            long nfields, const cplx<T>* tw, hipStream_t s) {
   CMBL_COL_DISPATCH(N, {
-    constexpr int KC = col_op_kc<T, NN_, C>();
+    constexpr int KC = col_op_kc<T, NN_, C>(), TILES = NN_ / 2 / KC;
+    constexpr bool PAIRABLE = KC * sizeof(cplx<T>) < 128 && TILES % 2 == 0;
     const T sc = T(1) / (T(NN_) * T(NN_));
-    k_col_edge<T, NN_, KC, C, EDGE, OpT<T, C>>
-        <<<(unsigned)(nfields * (NN_ / 2 / KC)), C * KC * NN_ / 8, 0, s>>>(in, out, op, tw, sc);
+    const long nb = nfields * TILES;
+    if (PAIRABLE && nb % 16 == 0)
+      k_col_edge<T, NN_, KC, C, EDGE, OpT<T, C>, PAIRABLE>
+          <<<(unsigned)nb, C * KC * NN_ / 8, 0, s>>>(in, out, op, tw, sc);
+    else
+      k_col_edge<T, NN_, KC, C, EDGE, OpT<T, C>>
+          <<<(unsigned)nb, C * KC * NN_ / 8, 0, s>>>(in, out, op, tw, sc);
   });
   return hipGetLastError();
 }
