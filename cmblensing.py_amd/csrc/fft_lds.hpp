@@ -99,6 +99,30 @@ __device__ __forceinline__ int lds_idx(int i) {
     return i + (i >> 3);
 }
 
+// Index of point i0 + q * step, q < 8, for a step that is a multiple of 8,
+// or 1 with i0 a multiple of 8 (i0 + q then stays in i0's group of 8): in
+// the padded layout the index is then affine in q, lds_idx(i0) +
+// q * lds_idx(step).  Written that way the compiler forms one address and
+// takes the others as immediate offsets of the LDS instructions.
+template <int N, bool SW>
+__device__ __forceinline__ int lds_idx_q(int i0, int q, int step) {
+  if constexpr (!SW)
+    return lds_idx<N, false>(i0) + q * (step + step / 8);
+  else
+    return lds_idx<N, SW>(i0 + q * step);
+}
+
+// Index of point t + q N/8 of a line (thread t's points in the natural
+// order): affine in q when N/8 is a multiple of 8.
+template <int N, bool SW>
+__device__ __forceinline__ int line_idx(int t, int q) {
+  constexpr int G = N / 8;
+  if constexpr (G % 8 == 0)
+    return lds_idx_q<N, SW>(t, q, G);
+  else
+    return lds_idx<N, SW>(t + q * G);
+}
+
 // Stride between the LDS buffers of neighbouring lines in the kernels whose
 // consecutive lanes work on different lines (the column passes, which use
 // the padded layout: lane g of KC consecutive lanes works on line g).  An
@@ -176,6 +200,7 @@ struct Passes {
 template <int N, typename T>
 struct Twiddles {
   static constexpr bool kReload = false;
+  static constexpr bool kPacked = false;
   cplx<T> w[Passes<N>::NTW > 0 ? Passes<N>::NTW : 1];
   // twiddle number wi of this thread (idx: its index in the table)
   __device__ __forceinline__ cplx<T> at(int wi, int) const { return w[wi]; }
@@ -201,20 +226,66 @@ struct Twiddles {
   }
 };
 
-// The same twiddles read from an LDS copy of the table (tab[k] = tw[k],
-// k < N) at each use, for kernels whose occupancy is set by register
-// pressure and which run several transforms: the registers are then taken
-// only while a transform runs.  The table lies in its own __shared__ array, so
-// the compiler could prove its values unchanged from one transform to the
-// next and keep them in registers; fft_lines therefore passes the thread
-// index through an empty asm statement first (kReload), which makes every
-// transform's table addresses opaque to it.
+// The same twiddles read at each use from a table in LDS, for kernels whose
+// occupancy is set by register pressure and which run several transforms:
+// the registers are then taken only while a transform runs.  The table holds
+// each thread group's twiddles side by side -- radix-8 pass p (1 <= p < P8),
+// group ktw < 8^p: W^(q ktw N / 8^(p+1)), q = 1..7, at pass_off(p) + 7 ktw +
+// q - 1; the trailing radix-R pass, group ktw < N/R: W^(q ktw), q = 1..R-1,
+// at kRemOff + (R-1) ktw + q - 1 -- so a pass reads them from one address
+// with immediate offsets (fill with fill_twiddles_lds).  The table lies in
+// its own __shared__ array, so the compiler could prove its values unchanged
+// from one transform to the next and keep them in registers; fft_lines
+// therefore passes the thread index through an empty asm statement first
+// (kReload), which makes every transform's table addresses opaque to it.
 template <int N, typename T>
 struct TwiddlesLds {
+  using P = Passes<N>;
   static constexpr bool kReload = true;
+  static constexpr bool kPacked = true;
+  static constexpr __host__ __device__ int pass_off(int p) {
+    int o = 0, ns = 8;
+    for (int i = 1; i < p; ++i) {
+      o += 7 * ns;
+      ns *= 8;
+    }
+    return o;
+  }
+  static constexpr int kRemOff = pass_off(P::P8);
+  static constexpr int kLen = kRemOff + (P::REM ? (P::R - 1) * (N / P::R) : 0);
   const cplx<T>* tab;
-  __device__ __forceinline__ cplx<T> at(int, int idx) const { return tab[idx]; }
+  __device__ __forceinline__ cplx<T> pass_tw(int p, int ktw, int q) const {
+    return tab[pass_off(p) + 7 * ktw + (q - 1)];
+  }
+  __device__ __forceinline__ cplx<T> rem_tw(int ktw, int q) const {
+    return tab[kRemOff + (P::R - 1) * ktw + (q - 1)];
+  }
 };
+
+// Fill the grouped twiddle table of TwiddlesLds<N, T> (kLen entries) from the
+// table tw[k] = exp(-2 pi i k / N) in global memory; threads tid of nthr.
+template <int N, typename T>
+__device__ __forceinline__ void fill_twiddles_lds(cplx<T>* tab, const cplx<T>* __restrict__ twg,
+                                                  int tid, int nthr) {
+  using TL = TwiddlesLds<N, T>;
+  constexpr int R = Passes<N>::R;
+  for (int e = tid; e < TL::kLen; e += nthr) {
+    int idx;
+    if (e < TL::kRemOff) {
+      int o = 0, ns = 8;
+      while (e >= o + 7 * ns) {
+        o += 7 * ns;
+        ns *= 8;
+      }
+      const int ktw = (e - o) / 7, q = (e - o) % 7 + 1;
+      idx = (q * ktw) * (N / (ns * 8));
+    } else {
+      const int ktw = (e - TL::kRemOff) / (R - 1), q = (e - TL::kRemOff) % (R - 1) + 1;
+      idx = q * ktw;
+    }
+    tab[e] = twg[idx];
+  }
+}
 
 // Synchronisation of an LDS exchange.  Block scope: __syncthreads().  Wave
 // scope (WAVE): for kernels in which the threads of every line lie inside one
@@ -266,7 +337,12 @@ __device__ __forceinline__ void fft_lines(cplx<T> (&v)[NL][8], cplx<T>* const (&
       const int ktw = tt & (Ns - 1);
 #pragma unroll
       for (int q = 1; q < 8; ++q) {
-        const cplx<T> w = tw.at(wi++, (q * ktw) * (N / (Ns * 8)));
+        cplx<T> w;
+        if constexpr (TW::kPacked)
+          w = tw.pass_tw(p, ktw, q);
+        else
+          w = tw.at(wi, (q * ktw) * (N / (Ns * 8)));
+        ++wi;
 #pragma unroll
         for (int l = 0; l < NL; ++l) v[l][q] = cmul_tw<INV>(v[l][q], w);
       }
@@ -275,19 +351,20 @@ __device__ __forceinline__ void fft_lines(cplx<T> (&v)[NL][8], cplx<T>* const (&
     for (int l = 0; l < NL; ++l) dft8<INV>(v[l]);
     const bool last = (p == P8 - 1) && (REM == 0);
     if (!last) {
+      // Ns = 8^p; with Ns == 1 the base 8 t is a multiple of 8
       const int base = (t / Ns) * Ns * 8 + k;
       lds_sync<WAVE>();
 #pragma unroll
       for (int l = 0; l < NL; ++l)
 #pragma unroll
-        for (int q = 0; q < 8; ++q) bufs[l][lds_idx<N, SW>(base + q * Ns)] = v[l][q];
+        for (int q = 0; q < 8; ++q) bufs[l][lds_idx_q<N, SW>(base, q, Ns)] = v[l][q];
       lds_sync<WAVE>();
       Ns *= 8;
       if (p < P8 - 1) {
 #pragma unroll
         for (int l = 0; l < NL; ++l)
 #pragma unroll
-          for (int q = 0; q < 8; ++q) v[l][q] = bufs[l][lds_idx<N, SW>(t + q * G)];
+          for (int q = 0; q < 8; ++q) v[l][q] = bufs[l][line_idx<N, SW>(t, q)];
       }
     }
   }
@@ -299,13 +376,18 @@ __device__ __forceinline__ void fft_lines(cplx<T> (&v)[NL][8], cplx<T>* const (&
 #pragma unroll
     for (int l = 0; l < NL; ++l)
 #pragma unroll
-      for (int m = 0; m < 8; ++m) v[l][m] = bufs[l][lds_idx<N, SW>(t + m * G)];
+      for (int m = 0; m < 8; ++m) v[l][m] = bufs[l][line_idx<N, SW>(t, m)];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const int ktw = (tt + b * G) & (Ns - 1);  // Ns * R == N
 #pragma unroll
       for (int q = 1; q < R; ++q) {
-        const cplx<T> w = tw.at(wi++, q * ktw);
+        cplx<T> w;
+        if constexpr (TW::kPacked)
+          w = tw.rem_tw(ktw, q);
+        else
+          w = tw.at(wi, q * ktw);
+        ++wi;
 #pragma unroll
         for (int l = 0; l < NL; ++l) v[l][b + q * NB] = cmul_tw<INV>(v[l][b + q * NB], w);
       }

@@ -526,12 +526,19 @@ __global__ __launch_bounds__(KC* N / 8) void k_col_lf_dx(const cplx<T>* src, cpl
                                                          const cplx<T>* __restrict__ twg, T dk) {
   constexpr int G = N / 8, H = N / 2;
   constexpr bool TWL = col_tw_lds<T, N>();
-  __shared__ cplx<T> lbuf[KC][line_stride<N>()];
-  __shared__ cplx<T> twl[TWL ? N : 1];
+  // the twiddle table first: at the bottom of LDS its entries are reached
+  // with the immediate offsets of the LDS reads
+  struct alignas(16) Lds {
+    alignas(16) cplx<T> tw[TWL ? TwiddlesLds<N, T>::kLen : 1];
+    alignas(16) cplx<T> line[KC][line_stride<N>()];
+  };
+  __shared__ Lds lds;
+  auto& twl = lds.tw;
+  auto& lbuf = lds.line;
   const int g = threadIdx.x % KC, t = threadIdx.x / KC;
   std::conditional_t<TWL, TwiddlesLds<N, T>, Twiddles<N, T>> tw;
   if constexpr (TWL) {
-    for (int i = threadIdx.x; i < N; i += KC * G) twl[i] = twg[i];
+    fill_twiddles_lds<N>(twl, twg, threadIdx.x, KC * G);
     tw.tab = twl;  // the __syncthreads() below orders these stores
   } else {
     tw.load(twg, t);
@@ -539,24 +546,39 @@ __global__ __launch_bounds__(KC* N / 8) void k_col_lf_dx(const cplx<T>* src, cpl
   long m;
   int k;
   col_tile<N, KC, XP>(blockIdx.x, g, m, k);
-  const cplx<T>* s0 = src + m * N * H + k;
-  cplx<T>* d0 = dst + m * N * H + k;
+  // the map's base is uniform over the workgroup; the lane's point offsets
+  // fit 32 bits (N * N/2 points per map), so the accesses take the
+  // scalar-base form with one 32-bit offset register each
+  const char* s0 = reinterpret_cast<const char*>(src + m * N * H);
+  char* d0 = reinterpret_cast<char*>(dst + m * N * H);
+  constexpr unsigned QB = (unsigned)(G * H * sizeof(cplx<T>));  // bytes per q
+  const unsigned o = (unsigned)((t * H + k) * sizeof(cplx<T>));
+  unsigned oq[8];  // byte offsets of the lane's points
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    oq[q] = o + q * QB;
+    // opaque to the optimiser, which would otherwise fold the constant into
+    // a 64-bit address instead of the 32-bit offset register
+    __asm__ volatile("" : "+v"(oq[q]));
+  }
   cplx<T> v[1][8];
 #pragma unroll
-  for (int q = 0; q < 8; ++q) v[0][q] = s0[(long)(t + q * G) * H];
+  for (int q = 0; q < 8; ++q) v[0][q] = *reinterpret_cast<const cplx<T>*>(s0 + oq[q]);
   __syncthreads();
   cplx<T>* const bufs[1] = {lbuf[g]};
   fft_lines<N, false, 1, false, false>(v, bufs, tw, t);
   const T sc = T(1) / (T(N) * T(N));
+  const T tf = (T)t;
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
-    const int kx = t + q * G;
-    const T kxv = (T)kx_signed<N>(kx) * dk * sc;
-    v[0][q] = (k == 0 && kx == N / 2) ? mk(T(0), T(0)) : imul(v[0][q], kxv);
+    // signed frequency of row t + q N/8: t + q N/8 below N/2, else that - N
+    // (t < N/8); integers, exact in T
+    const T kxv = (tf + T((q < 4 ? q : q - 8) * G)) * dk * sc;
+    v[0][q] = (k == 0 && q == 4 && t == 0) ? mk(T(0), T(0)) : imul(v[0][q], kxv);
   }
   fft_lines<N, true, 1, false, false>(v, bufs, tw, t);
 #pragma unroll
-  for (int q = 0; q < 8; ++q) d0[(long)(t + q * G) * H] = v[0][q];
+  for (int q = 0; q < 8; ++q) *reinterpret_cast<cplx<T>*>(d0 + oq[q]) = v[0][q];
 }
 
 // Classic RK4 bookkeeping of one stage (util.py:184-199), per pixel, with
@@ -618,12 +640,19 @@ __global__ __launch_bounds__(BS) void k_row_lf_std(cplx<T>* __restrict__ S0,
   static_assert(BS % G == 0 && BS >= G, "a block holds whole lines");
   constexpr long NN = (long)N * N;
   constexpr bool TWL = row_tw_lds<T, N, RK>();
-  __shared__ cplx<T> lbuf[PAIRS][2][lds_len<N>()];
-  __shared__ cplx<T> twl[TWL ? N : 1];
+  // the twiddle table first: at the bottom of LDS its entries are reached
+  // with the immediate offsets of the LDS reads
+  struct alignas(16) Lds {
+    alignas(16) cplx<T> tw[TWL ? TwiddlesLds<N, T>::kLen : 1];
+    alignas(16) cplx<T> line[PAIRS][2][lds_len<N>()];
+  };
+  __shared__ Lds lds;
+  auto& twl = lds.tw;
+  auto& lbuf = lds.line;
   const int grp = threadIdx.x / G, t = threadIdx.x % G;
   std::conditional_t<TWL, TwiddlesLds<N, T>, Twiddles<N, T>> tw;
   if constexpr (TWL) {
-    for (int i = threadIdx.x; i < N; i += BS) twl[i] = twg[i];
+    fill_twiddles_lds<N>(twl, twg, threadIdx.x, BS);
     __syncthreads();
     tw.tab = twl;
   } else {
