@@ -118,6 +118,9 @@ struct cmbl_plan {
   // small FP32 maps (N <= 64) lensed by the one-launch resident kernel
   // (k_lf_resident) instead of the two-pass stage sequence
   bool resident = true;
+  // streaming (non-temporal) accesses to the RK4 accumulator and state of
+  // FP32 batches whose stage buffers outgrow half the last-level cache
+  bool stream_acc = true;
   // optional per-kernel timing (HIP events around every stage launch)
   bool timing = false;
   std::vector<hipEvent_t> tev;   // pool, 2 per launch
@@ -475,6 +478,9 @@ cmbl_status cmbl_plan_create(cmbl_plan** out, int N, double theta_pix_arcmin, in
     // CMBL_RESIDENT=0 runs small FP32 maps through the two-pass sequence too
     const char* rs_env = std::getenv("CMBL_RESIDENT");
     p->resident = !(rs_env && rs_env[0] == '0');
+    // CMBL_STREAM_ACC=0 keeps the accumulator and state accesses cached
+    const char* sa_env = std::getenv("CMBL_STREAM_ACC");
+    p->stream_acc = !(sa_env && sa_env[0] == '0');
   }
   // set-up of the allocation-free entry point on this device
   void* sp = nullptr;
@@ -687,6 +693,16 @@ struct LfPart {
   hipStream_t s;
 };
 
+// FP32 calls whose stage buffers (S0, S1, y, acc: 4 N^2 s per map) take at
+// least this many bytes -- half the 256 MB last-level cache; a serving lane
+// of 64 256^2 QU requests takes exactly that, beside a second lane -- access
+// the RK4 accumulator and the state as streaming accesses: each is next read
+// two or more launches later, after more traffic than the cache holds, and
+// caching it only evicts the stage spectra and the shared p(t) rows.  (256^2
+// QU batch 128: +1.6-1.9 % direct; smaller batches, which stay in the cache,
+// lost ~1 % with it.)
+constexpr long kStreamAccMinBytes = 128L << 20;
+
 // Batches of at least this many map pixels (nmaps * N^2) run as two halves on
 // the caller's stream and the plan's second stream, launch by launch: one
 // half's column pass (few large blocks at N = 512, bound by its
@@ -721,6 +737,8 @@ cmbl_status run_lenseflow(cmbl_plan* p, const void* const* ptabs, T* y, long nma
   };
   static const int stage_half[5] = {0, 0, 1, 1, 2};
   const bool adj = adjoint != 0;
+  const bool nt = std::is_same<T, float>::value && p->stream_acc &&
+                  nmaps * NN * (long)sizeof(T) * 4 >= kStreamAccMinBytes;
   Timer tm{p, s, bufs == nullptr};
   // small FP32 maps, standard rate: the whole LenseFlow in one launch with the
   // stage state of each component in one workgroup's LDS
@@ -790,11 +808,11 @@ cmbl_status run_lenseflow(cmbl_plan* p, const void* const* ptabs, T* y, long nma
           // the S1 stack holds the stage input z_s as a real map
           TIMED(1, cmbl::launch_lf_row_adj_ls<T>(N, q.S0, (T*)q.S1, q.y, q.acc, q.pt, tw, C,
                                                  q.nmaps, st, h, tindex(i, stage_half[st]), tidx,
-                                                 last ? 0 : 1, 0, (T)p->dk, q.s));
+                                                 last ? 0 : 1, 0, (T)p->dk, q.s, nt));
         } else {
           TIMED(1, cmbl::launch_lf_row<T>(N, adj, q.S0, q.S1, q.y, q.acc, q.pt, tw, C, q.nmaps,
                                           st, h, tidx, last ? 0 : 1, 0, (T)p->dk, q.s, nullptr,
-                                          nullptr, (last && !adj) ? outp : nullptr));
+                                          nullptr, (last && !adj) ? outp : nullptr, nt));
         }
       }
     }

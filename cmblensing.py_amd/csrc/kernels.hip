@@ -35,6 +35,25 @@ namespace cmbl {
 // ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
+// Accesses of the RK4 accumulator and state that are next read two or more
+// launches later: with NT (a batch whose stage buffers outgrow half the
+// last-level cache) they are streaming (non-temporal) accesses, which leave
+// the cache to the stage spectra and the shared p(t) rows.
+template <bool NT, typename T>
+__device__ __forceinline__ void st_stream(T* p, T v) {
+  if constexpr (NT)
+    __builtin_nontemporal_store(v, p);
+  else
+    *p = v;
+}
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_stream(const T* p) {
+  if constexpr (NT)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
+}
+
 template <int N>
 __device__ __forceinline__ int kx_signed(int kx) {
   return kx < N / 2 ? kx : kx - N;
@@ -628,7 +647,8 @@ __device__ __forceinline__ void add_sources(T* S, long NN, T cs, T tval, T fx, T
 // OUTP: the last stage of the serving executor; the new state y goes to the
 // per-request outputs outp[b] + (component N + row) N (NULL: a padding slot)
 // instead of the staging state.
-template <typename T, int N, int C, bool GRAD, int BS, int RK = 0, bool OUTP = false>
+template <typename T, int N, int C, bool GRAD, int BS, int RK = 0, bool OUTP = false,
+          bool NT = false>
 __global__ __launch_bounds__(BS) void k_row_lf_std(cplx<T>* __restrict__ S0,
                                                     const cplx<T>* __restrict__ S1,
                                                     T* __restrict__ y, T* __restrict__ acc,
@@ -717,7 +737,7 @@ __global__ __launch_bounds__(BS) void k_row_lf_std(cplx<T>* __restrict__ S0,
       const long row = C == 2 ? (long)r * N + rp.ia : (r ? rp.ib : rp.ia);
       if (o) o[row * N + j] = v;
     } else {
-      y[rr * N + j] = v;
+      st_stream<NT>(&y[rr * N + j], v);
     }
   };
   // rate, RK4 update and next stage input of row r from its gradient g
@@ -789,14 +809,14 @@ __global__ __launch_bounds__(BS) void k_row_lf_std(cplx<T>* __restrict__ S0,
           }
           if (active) {
 #pragma unroll
-            for (int m = 0; m < 8; ++m) acc[rr * N + t + m * G] = av[m];
+            for (int m = 0; m < 8; ++m) st_stream<NT>(&acc[rr * N + t + m * G], av[m]);
           }
         } else {
           const T h6 = h / T(6);
 #pragma unroll
           for (int m = 0; m < 8; ++m) {
             const T zin = (r ? zr[0][m].y : zr[0][m].x) * inv_n;
-            zz[m] = acc[rr * N + t + m * G] + (c13 * (zin - yv[m]) + h6 * zz[m]);
+            zz[m] = ld_stream<NT>(&acc[rr * N + t + m * G]) + (c13 * (zin - yv[m]) + h6 * zz[m]);
           }
           if (active) {
 #pragma unroll
@@ -1035,7 +1055,7 @@ __global__ __launch_bounds__(BS) void k_row_lf_adj(cplx<T>* __restrict__ S0,
 // and, unless this is the very last stage, u = p_x(t_next) z_next is row-R2C'd
 // (two rows per line) into S0.  init != 0: no rate, U of z_1 = y only.
 // Per launch (5.25 C + 2) N^2 s against the classic kernel's (6.5 C + 2).
-template <typename T, int N, int C, int BS>
+template <typename T, int N, int C, int BS, bool NT = false>
 __global__ __launch_bounds__(BS) void k_row_lf_adj_ls(cplx<T>* __restrict__ S0,
                                                        T* __restrict__ Z, T* __restrict__ y,
                                                        T* __restrict__ acc,
@@ -1122,8 +1142,8 @@ __global__ __launch_bounds__(BS) void k_row_lf_adj_ls(cplx<T>* __restrict__ S0,
       if (active) {
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
-          acc[rp.ra * N + t + m * G] = aa[m];
-          acc[rp.rb * N + t + m * G] = ab[m];
+          st_stream<NT>(&acc[rp.ra * N + t + m * G], aa[m]);
+          st_stream<NT>(&acc[rp.rb * N + t + m * G], ab[m]);
         }
       }
     } else {
@@ -1132,16 +1152,23 @@ __global__ __launch_bounds__(BS) void k_row_lf_adj_ls(cplx<T>* __restrict__ S0,
       for (int m = 0; m < 8; ++m) {
         const int j = t + m * G;
         const T ya = y[rp.ra * N + j], yb = y[rp.rb * N + j];
-        za[m] = acc[rp.ra * N + j] + (c13 * (za[m] - ya) + h6 * w[0][m].x);
-        zb[m] = acc[rp.rb * N + j] + (c13 * (zb[m] - yb) + h6 * w[0][m].y);
+        za[m] = ld_stream<NT>(&acc[rp.ra * N + j]) + (c13 * (za[m] - ya) + h6 * w[0][m].x);
+        zb[m] = ld_stream<NT>(&acc[rp.rb * N + j]) + (c13 * (zb[m] - yb) + h6 * w[0][m].y);
       }
     }
     if (active) {
-      T* zdst = stage == 4 ? y : Z;
+      if (stage == 4) {  // the new state: next read at the next step's first stage
 #pragma unroll
-      for (int m = 0; m < 8; ++m) {
-        zdst[rp.ra * N + t + m * G] = za[m];
-        zdst[rp.rb * N + t + m * G] = zb[m];
+        for (int m = 0; m < 8; ++m) {
+          st_stream<NT>(&y[rp.ra * N + t + m * G], za[m]);
+          st_stream<NT>(&y[rp.rb * N + t + m * G], zb[m]);
+        }
+      } else {  // the stage input: read by the next row launch
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+          Z[rp.ra * N + t + m * G] = za[m];
+          Z[rp.rb * N + t + m * G] = zb[m];
+        }
       }
     }
   }
@@ -1536,7 +1563,7 @@ template <typename T, int N, int BS>
 void row_lf_launch(bool adjoint, int C, const GradArgs<T>* ga, cplx<T>* S0, cplx<T>* S1, T* y,
                    T* acc, const T* const* ptabs, const cplx<T>* tw, long npairs, int stage, T h,
                    int tidx, int do_r2c, int init, T cy, T* zstore, hipStream_t s,
-                   T* const* outp) {
+                   T* const* outp, bool nt) {
   const unsigned nb = blocks_for(npairs, BS / (N / 8));
   const GradArgs<T> g = ga ? *ga : GradArgs<T>{};
   if (outp) {  // last stage of the executor (standard rate only)
@@ -1571,12 +1598,19 @@ void row_lf_launch(bool adjoint, int C, const GradArgs<T>* ga, cplx<T>* S0, cplx
       k_row_lf_std<T, N, 1, false, BS><<<nb, BS, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs, stage,
                                                          h, tidx, do_r2c, cy, g, nullptr);
   } else if (stage == 2 || stage == 4) {
-    if (C == 2)
-      k_row_lf_std<T, N, 2, false, BS, 2><<<nb, BS, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs,
-                                                            stage, h, tidx, do_r2c, cy, g, nullptr);
+    auto go = [&](auto ntc) {
+      constexpr bool NT = decltype(ntc)::value;
+      if (C == 2)
+        k_row_lf_std<T, N, 2, false, BS, 2, false, NT><<<nb, BS, 0, s>>>(
+            S0, S1, y, acc, ptabs, tw, npairs, stage, h, tidx, do_r2c, cy, g, nullptr);
+      else
+        k_row_lf_std<T, N, 1, false, BS, 2, false, NT><<<nb, BS, 0, s>>>(
+            S0, S1, y, acc, ptabs, tw, npairs, stage, h, tidx, do_r2c, cy, g, nullptr);
+    };
+    if (sizeof(T) == 4 && nt)  // streaming accesses: FP32 only (no FP64 variants)
+      go(std::integral_constant<bool, sizeof(T) == 4>{});
     else
-      k_row_lf_std<T, N, 1, false, BS, 2><<<nb, BS, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs,
-                                                            stage, h, tidx, do_r2c, cy, g, nullptr);
+      go(std::false_type{});
   } else {
     if (C == 2)
       k_row_lf_std<T, N, 2, false, BS, 1><<<nb, BS, 0, s>>>(S0, S1, y, acc, ptabs, tw, npairs,
@@ -1591,17 +1625,17 @@ template <typename T>
 hipError_t launch_lf_row(int N, bool adjoint, cplx<T>* S0, cplx<T>* S1, T* y, T* acc,
                          const T* const* ptabs, const cplx<T>* tw, int C, long nmaps, int stage,
                          T h, int tidx, int do_r2c, int init, T dk, hipStream_t s, T* zstore,
-                         const GradArgs<T>* ga, T* const* outp) {
+                         const GradArgs<T>* ga, T* const* outp, bool nt) {
   CMBL_DISPATCH_N(N, {
     const long npairs = nmaps * NN_ / 2;
     const T cy = dk / T(NN_);
     constexpr int BS_SMALL = NN_ / 8 > 64 ? NN_ / 8 : 64;
     if (blocks_for(npairs, 256 / (NN_ / 8)) < CMBL_SMALL_GRID)
       row_lf_launch<T, NN_, BS_SMALL>(adjoint, C, ga, S0, S1, y, acc, ptabs, tw, npairs, stage, h,
-                                      tidx, do_r2c, init, cy, zstore, s, outp);
+                                      tidx, do_r2c, init, cy, zstore, s, outp, nt);
     else
       row_lf_launch<T, NN_, 256>(adjoint, C, ga, S0, S1, y, acc, ptabs, tw, npairs, stage, h,
-                                 tidx, do_r2c, init, cy, zstore, s, outp);
+                                 tidx, do_r2c, init, cy, zstore, s, outp, nt);
   });
   return hipGetLastError();
 }
@@ -1609,25 +1643,29 @@ hipError_t launch_lf_row(int N, bool adjoint, cplx<T>* S0, cplx<T>* S1, T* y, T*
 template <typename T>
 hipError_t launch_lf_row_adj_ls(int N, cplx<T>* S0, T* Z, T* y, T* acc, const T* const* ptabs,
                                 const cplx<T>* tw, int C, long nmaps, int stage, T h, int tidx,
-                                int tidx_next, int do_r2c, int init, T dk, hipStream_t s) {
+                                int tidx_next, int do_r2c, int init, T dk, hipStream_t s,
+                                bool nt) {
   CMBL_DISPATCH_N(N, {
     const long npairs = nmaps * NN_ / 2;
     const T cy = dk / T(NN_);
     constexpr int BS_SMALL = NN_ / 8 > 64 ? NN_ / 8 : 64;
-    auto go = [&](auto bs) {
+    auto go = [&](auto bs, auto ntc) {
       constexpr int BS = decltype(bs)::value;
+      constexpr bool NT = decltype(ntc)::value;
       const unsigned nb = blocks_for(npairs, BS / (NN_ / 8));
       if (C == 2)
-        k_row_lf_adj_ls<T, NN_, 2, BS><<<nb, BS, 0, s>>>(S0, Z, y, acc, ptabs, tw, npairs, stage,
-                                                         h, tidx, tidx_next, do_r2c, init, cy);
+        k_row_lf_adj_ls<T, NN_, 2, BS, NT><<<nb, BS, 0, s>>>(
+            S0, Z, y, acc, ptabs, tw, npairs, stage, h, tidx, tidx_next, do_r2c, init, cy);
       else
-        k_row_lf_adj_ls<T, NN_, 1, BS><<<nb, BS, 0, s>>>(S0, Z, y, acc, ptabs, tw, npairs, stage,
-                                                         h, tidx, tidx_next, do_r2c, init, cy);
+        k_row_lf_adj_ls<T, NN_, 1, BS, NT><<<nb, BS, 0, s>>>(
+            S0, Z, y, acc, ptabs, tw, npairs, stage, h, tidx, tidx_next, do_r2c, init, cy);
     };
     if (blocks_for(npairs, 256 / (NN_ / 8)) < CMBL_SMALL_GRID)
-      go(std::integral_constant<int, BS_SMALL>{});
+      go(std::integral_constant<int, BS_SMALL>{}, std::false_type{});
+    else if (sizeof(T) == 4 && nt)  // streaming accesses: FP32 only (no FP64 variants)
+      go(std::integral_constant<int, 256>{}, std::integral_constant<bool, sizeof(T) == 4>{});
     else
-      go(std::integral_constant<int, 256>{});
+      go(std::integral_constant<int, 256>{}, std::false_type{});
   });
   return hipGetLastError();
 }
@@ -1704,14 +1742,14 @@ hipError_t launch_scatter(const T* src, T* const* dst, long len, int count, hipS
                                        hipStream_t);                                            \
   template hipError_t launch_lf_row<T>(int, bool, cplx<T>*, cplx<T>*, T*, T*, const T* const*,  \
                                        const cplx<T>*, int, long, int, T, int, int, int, T,     \
-                                       hipStream_t, T*, const GradArgs<T>*, T* const*);         \
+                                       hipStream_t, T*, const GradArgs<T>*, T* const*, bool);   \
   template hipError_t launch_row_c2r<T>(int, const cplx<T>*, T*, long, const cplx<T>*, const T*, \
                                         const T*, int, hipStream_t);                            \
   template hipError_t launch_lf_row_r2c_gather<T>(int, const T* const*, int, T*, cplx<T>*,      \
                                                   long, const cplx<T>*, hipStream_t);           \
   template hipError_t launch_lf_row_adj_ls<T>(int, cplx<T>*, T*, T*, T*, const T* const*,       \
                                               const cplx<T>*, int, long, int, T, int, int, int, \
-                                              int, T, hipStream_t);                             \
+                                              int, T, hipStream_t, bool);                       \
   template hipError_t launch_lf_resident<T>(int, const T* const*, T* const*, T*, int,          \
                                             const T* const*, const cplx<T>*, long, int, int, T, \
                                             T, hipStream_t);                                    \

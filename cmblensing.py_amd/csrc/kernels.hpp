@@ -53,21 +53,25 @@ struct GradArgs {
 // receives the next stage input.  Standard rate with ga != NULL: also
 // accumulates the delta-phi sources of the gradient ODE.
 // outp != NULL (standard rate, the last stage of the serving executor): the
-// new state goes to the per-request maps outp[b] instead of y.
+// new state goes to the per-request maps outp[b] instead of y.  nt (FP32,
+// low-storage stages 2 and 4): the accumulator and the new state are written
+// and read as streaming accesses (batches larger than the last-level cache).
 template <typename T>
 hipError_t launch_lf_row(int N, bool adjoint, cplx<T>* S0, cplx<T>* S1, T* y, T* acc,
                          const T* const* ptabs, const cplx<T>* tw, int C, long nmaps, int stage,
                          T h, int tidx, int do_r2c, int init, T dk, hipStream_t s,
                          T* zstore = nullptr, const GradArgs<T>* ga = nullptr,
-                         T* const* outp = nullptr);
+                         T* const* outp = nullptr, bool nt = false);
 // Row pass of one adjoint-rate stage with the low-storage RK4 bookkeeping:
 // S0 = U (x-derivative after the column pass), Z = the stage input z_s as a
 // real map [nmaps][N][N] (stage 1 reads z_1 = y).  tidx: time index of this
 // stage (p_y), tidx_next: of the next stage (p_x of u).  init: U of y only.
+// nt: as for launch_lf_row.
 template <typename T>
 hipError_t launch_lf_row_adj_ls(int N, cplx<T>* S0, T* Z, T* y, T* acc, const T* const* ptabs,
                                 const cplx<T>* tw, int C, long nmaps, int stage, T h, int tidx,
-                                int tidx_next, int do_r2c, int init, T dk, hipStream_t s);
+                                int tidx_next, int do_r2c, int init, T dk, hipStream_t s,
+                                bool nt = false);
 // The whole standard-rate LenseFlow of small FP32 maps (N <= 64) in one
 // launch, one workgroup per component with its stage state in LDS.  src /
 // outp: request pointers of the serving executor (else y in place).
