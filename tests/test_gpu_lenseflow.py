@@ -448,3 +448,50 @@ def test_resident_small_maps(pkg, dev, sims, monkeypatch, N, C):
                 h.close()
     for p in plans.values():
         p.close()
+
+
+def test_streaming_accumulator_accesses(pkg, dev, sims, monkeypatch):
+    """FP32 calls whose stage buffers (S0, S1, y, acc) take at least 128 MiB
+    read and write the RK4 accumulator and the state as streaming
+    (non-temporal) accesses, a cache policy only: forward and adjoint outputs
+    are bit-identical to those of a plan with it off (CMBL_STREAM_ACC=0, read
+    when a plan is created), the split into two halves included, and agree
+    with the FP64 oracle (1e-5)."""
+    from cmblensing_amd import _lib
+    N, th, n, C, B = 256, 2.0, 7, 2, 64
+    assert B * C * N * N * 4 * 4 >= 128 << 20  # the size that takes the streaming kernels
+    _case(f"streaming accumulator N={N} QU batch {B} fp32")
+    pw = olf.pix_width_from_theta(th)
+    fs = np.stack([sims.qu_map(N, th, s % 8) for s in range(B)])
+    phis = [sims.phi_map(N, th, 100 + k) for k in range(4)]
+    x = torch.as_tensor(fs, dtype=torch.float32, device=dev)
+    plans = {}
+    for on in (True, False):
+        if on:
+            monkeypatch.delenv("CMBL_STREAM_ACC", raising=False)
+        else:
+            monkeypatch.setenv("CMBL_STREAM_ACC", "0")
+        plans[on] = _lib.Plan(N, th, torch.float32, B * C, dev)
+    monkeypatch.delenv("CMBL_STREAM_ACC", raising=False)
+    hs = {k: [_lib.PhiHandle(p, torch.as_tensor(ph, device=dev), n) for ph in phis]
+          for k, p in plans.items()}
+    for direction, adjoint in ((1, False), (1, True)):
+        got = {}
+        for k, plan in plans.items():
+            out = torch.empty_like(x)
+            _lib.lenseflow(plan, [hs[k][b % 4] for b in range(B)], x, out, B, C, n, direction,
+                           adjoint)
+            got[k] = out
+        torch.cuda.synchronize()
+        assert torch.equal(got[True], got[False]), (direction, adjoint)
+        for b in (0, 37):
+            ref = olf.lense_flow(fs[b], phis[b % 4], pw, n, direction, adjoint)
+            e = rel(got[True][b].double().cpu().numpy(), ref)
+            print(f"[err] streaming dir={direction} adj={adjoint} request {b}: {e:.2e}",
+                  flush=True)
+            assert e < 1e-5
+    for k in hs:
+        for h in hs[k]:
+            h.close()
+    for p in plans.values():
+        p.close()
