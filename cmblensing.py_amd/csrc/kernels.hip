@@ -36,7 +36,7 @@ namespace cmbl {
 // helpers
 // ---------------------------------------------------------------------------
 // Accesses of the RK4 accumulator and state that are next read two or more
-// launches later: with NT (a batch whose stage buffers outgrow half the
+// launches later: with NT (a call whose stage buffers take at least half the
 // last-level cache) they are streaming (non-temporal) accesses, which leave
 // the cache to the stage spectra and the shared p(t) rows.
 template <bool NT, typename T>
