@@ -47,6 +47,31 @@ def test_vjp_matches_oracle(pkg, dev, sims, C, basis, dtype, tol):
     assert e_phi < tol
 
 
+@pytest.mark.parametrize("n", [1, 10])
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-10), (torch.float32, 1e-5)])
+def test_vjp_other_step_counts(pkg, dev, sims, n, dtype, tol):
+    """The gradient ODE at step counts other than n = 7 (the reference's
+    default n = 10, lense_flow.py:10-51, and a single step): its stage times
+    run backwards over the same 2n+1-point tables."""
+    from cmblensing_amd.lense_flow import lense_flow_vjp_matrices
+    N, th = 64, 2.0
+    _case(f"vjp n={n} N={N} QU {dtype}")
+    pw = olf.pix_width_from_theta(th)
+    f = sims.qu_map(N, th, 21)
+    phi = sims.phi_map(N, th, 22)
+    y = olf.lense_flow(f, phi, pw, n)
+    cot = np.random.default_rng(23).standard_normal(f.shape)
+    _, dfs, dphi = olf.lensing_operator_gradients(phi, list(y), list(cot), -1, n, pw)
+    T = lambda a: torch.as_tensor(a, device=dev)
+    d_field, d_phi = lense_flow_vjp_matrices(T(np.asarray(y)[None]).to(dtype),
+                                             T(cot[None]).to(dtype), T(phi), th, n, 1)
+    e_field = rel(d_field[0].double().cpu().numpy(), np.stack(dfs))
+    e_phi = rel(np.fft.irfft2(d_phi[0].cpu().numpy()), dphi)
+    print(f"[err] n={n}: d_field {e_field:.2e} d_phi {e_phi:.2e}", flush=True)
+    assert e_field < tol
+    assert e_phi < tol
+
+
 def test_autograd_matches_finite_differences(pkg, dev, sims):
     """torch.autograd through LenseFlowFunction: d/dx exact (linear), d/dphi
     within the continuous-adjoint gap of a central difference."""

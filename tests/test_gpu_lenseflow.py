@@ -495,3 +495,33 @@ def test_streaming_accumulator_accesses(pkg, dev, sims, monkeypatch):
             h.close()
     for p in plans.values():
         p.close()
+
+
+@pytest.mark.parametrize("n", [1, 3, 10, 16])
+@pytest.mark.parametrize("N,C,dtype,tol", [(64, 2, torch.float64, 1e-11),
+                                           (32, 1, torch.float32, 1e-5),
+                                           (128, 2, torch.float32, 1e-5)])
+def test_other_step_counts(pkg, dev, sims, n, N, C, dtype, tol):
+    """Step counts other than the workloads' n = 7 (the reference's default
+    is n = 10, lense_flow.py:289): the phi tables' 2n+1 time points and each
+    stage's time index, in all four modes, through the two-pass stage
+    sequence (64^2 QU FP64, 128^2 QU FP32) and the resident kernel (32^2 T
+    FP32: the standard rate), against the FP64 oracle."""
+    th = 2.0
+    _case(f"step count n={n} N={N} C={C} {dtype}")
+    pw = olf.pix_width_from_theta(th)
+    f = sims.qu_map(N, th, 11) if C == 2 else sims.t_map(N, th, 11)
+    phi = sims.phi_map(N, th, 12)
+    T = lambda a: torch.as_tensor(a, dtype=dtype, device=dev)
+    if C == 2:
+        field = pkg.FlatS2(polar_matrix_1=T(f[0]), polar_matrix_2=T(f[1]), theta_pix=th)
+    else:
+        field = pkg.FlatS0(scalar_matrix=T(f[0]), theta_pix=th)
+    phif = pkg.FlatS0(scalar_matrix=torch.as_tensor(phi, device=dev), theta_pix=th)
+    for direction, adjoint in MODES:
+        out = pkg.lense_flow(field, phif, n=n, direction=direction, adjoint=adjoint)
+        got = np.stack([getattr(out, nm).double().cpu().numpy() for nm in out._matrix_names()])
+        ref = np.asarray(olf.lense_flow(f if C == 2 else f[0], phi, pw, n, direction, adjoint))
+        e = rel(got.reshape(ref.shape), ref)
+        print(f"[err] n={n} dir={direction} adj={adjoint}: {e:.2e}", flush=True)
+        assert e < tol
