@@ -34,11 +34,21 @@ def _complex_dtype(t: torch.Tensor) -> torch.dtype:
     return {torch.float32: torch.complex64, torch.float64: torch.complex128}[t.dtype]
 
 
+_WEIGHTS = {}
+
+
 def fourier_weights_for(N: int, device=None, dtype=torch.float64) -> torch.Tensor:
-    """[1, 2, ..., 2, 1] along the half axis (util.py:90-96)."""
-    w = torch.full((N // 2 + 1,), 2.0, dtype=dtype, device=device)
-    w[0] = 1.0
-    w[-1] = 1.0
+    """[1, 2, ..., 2, 1] along the half axis (util.py:90-96).  One tensor per
+    (N, device, dtype) is built and shared: it is read-only (every consumer
+    reads it; none writes it), and building it anew launched three small
+    kernels for each dot product and operator of a sampler's inner loop."""
+    key = (N, str(torch.device(device)) if device is not None else "cpu", dtype)
+    w = _WEIGHTS.get(key)
+    if w is None:
+        w = torch.full((N // 2 + 1,), 2.0, dtype=dtype, device=device)
+        w[0] = 1.0
+        w[-1] = 1.0
+        _WEIGHTS[key] = w
     return w
 
 

@@ -81,6 +81,10 @@ def mixed_grad_phi_logpdf(mixed_field, mixed_phi, data, noise_covariance, phi_co
     # both gradients from one forward LenseFlow and one gradient ODE
     gf, gphi = grad_logpdf(field, phi, data, noise_covariance, phi_covariance, field_covariance,
                            mask, beam, n=n)
+    # its inverse LenseFlow is not the unmixing's: that one lenses by map(phi)
+    # (mixing.py:32), this one by the FOURIER phi (gradients.py:69-85, which
+    # keeps the imaginary parts of the DC and Nyquist columns), and the two
+    # differ once phi has such parts (a momentum step gives it them)
     chain = mixing_jacobian_phi_component(mixed_field, phi, gf, mixing_d, n)
     out = gphi.replace(scalar_matrix=gphi.scalar_matrix + chain.to(gphi.scalar_matrix.dtype))
     return pinv(mixing_g) * out
