@@ -450,7 +450,8 @@ def test_resident_small_maps(pkg, dev, sims, monkeypatch, N, C):
         p.close()
 
 
-def test_streaming_accumulator_accesses(pkg, dev, sims, monkeypatch):
+@pytest.mark.parametrize("N,B", [(256, 64), (512, 16)])
+def test_streaming_accumulator_accesses(pkg, dev, sims, monkeypatch, N, B):
     """FP32 calls whose stage buffers (S0, S1, y, acc) take at least 128 MiB
     read and write the RK4 accumulator and the state as streaming
     (non-temporal) accesses, a cache policy only: forward and adjoint outputs
@@ -458,7 +459,7 @@ def test_streaming_accumulator_accesses(pkg, dev, sims, monkeypatch):
     when a plan is created), the split into two halves included, and agree
     with the FP64 oracle (1e-5)."""
     from cmblensing_amd import _lib
-    N, th, n, C, B = 256, 2.0, 7, 2, 64
+    th, n, C = 2.0, 7, 2
     assert B * C * N * N * 4 * 4 >= 128 << 20  # the size that takes the streaming kernels
     _case(f"streaming accumulator N={N} QU batch {B} fp32")
     pw = olf.pix_width_from_theta(th)
@@ -484,7 +485,7 @@ def test_streaming_accumulator_accesses(pkg, dev, sims, monkeypatch):
             got[k] = out
         torch.cuda.synchronize()
         assert torch.equal(got[True], got[False]), (direction, adjoint)
-        for b in (0, 37):
+        for b in (0, B - 3):
             ref = olf.lense_flow(fs[b], phis[b % 4], pw, n, direction, adjoint)
             e = rel(got[True][b].double().cpu().numpy(), ref)
             print(f"[err] streaming dir={direction} adj={adjoint} request {b}: {e:.2e}",
