@@ -42,10 +42,13 @@ def fourier_weights_for(N: int, device=None, dtype=torch.float64) -> torch.Tenso
     (N, device, dtype) is built and shared: it is read-only (every consumer
     reads it; none writes it), and building it anew launched three small
     kernels for each dot product and operator of a sampler's inner loop."""
-    key = (N, str(torch.device(device)) if device is not None else "cpu", dtype)
+    dev = torch.device(device) if device is not None else torch.device("cpu")
+    if dev.type == "cuda" and dev.index is None:  # "cuda" means the current device
+        dev = torch.device("cuda", torch.cuda.current_device())
+    key = (N, str(dev), dtype)
     w = _WEIGHTS.get(key)
     if w is None:
-        w = torch.full((N // 2 + 1,), 2.0, dtype=dtype, device=device)
+        w = torch.full((N // 2 + 1,), 2.0, dtype=dtype, device=dev)
         w[0] = 1.0
         w[-1] = 1.0
         _WEIGHTS[key] = w
